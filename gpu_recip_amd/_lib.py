@@ -1,0 +1,117 @@
+"""ctypes binding of libpme_hip.so (the C ABI declared in include/pme_recip.h).
+
+The library is loaded from this package directory only.  There is no
+fallback: if the HIP library is missing or fails to load, every call raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpme_hip.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "pme_recip.h")
+
+PME_OK = 0
+STATUS = {
+    0: "PME_OK", 1: "PME_ERR_INVALID_ARG", 2: "PME_ERR_UNSUPPORTED", 3: "PME_ERR_OOM",
+    4: "PME_ERR_BUDGET", 5: "PME_ERR_DEVICE", 6: "PME_ERR_STATE", 7: "PME_ERR_NOT_FOUND",
+}
+FFT_COLUMN, FFT_SLAB, FFT_BOX = 0, 1, 2
+ACCUM_FLOAT, ACCUM_INT32, ACCUM_INT64 = 0, 1, 2
+FORCE_FLOAT, FORCE_FLOAT3, FORCE_INT64 = 0, 1, 2
+GRID_REAL, GRID_SPECTRUM = 0, 1
+
+
+class PmeError(RuntimeError):
+    def __init__(self, status: int, func: str, msg: str):
+        self.status = status
+        self.name = STATUS.get(status, str(status))
+        super().__init__(f"{func}: {self.name}: {msg}")
+
+
+_vp = C.c_void_p
+_i = C.c_int
+_d = C.c_double
+_pd = C.POINTER(C.c_double)
+
+# name -> argtypes (restype is pme_status = int unless listed in _RESTYPE)
+SIGNATURES = {
+    "pme_last_error": [],
+    "pme_status_string": [_i],
+    "pme_ev_create": [C.POINTER(_vp)],
+    "pme_ev_destroy": [_vp],
+    "pme_ev_insert": [_vp, C.c_char_p],
+    "pme_ev_clear": [_vp, _vp],
+    "pme_ev_clear_energy": [_vp, _vp],
+    "pme_ev_clear_virial": [_vp, _vp],
+    "pme_ev_copy_to_host": [_vp, _vp],
+    "pme_ev_get_energy": [_vp, C.c_char_p, _pd],
+    "pme_ev_get_virial": [_vp, _pd],
+    "pme_ev_get_sforce": [_vp, _pd],
+    "pme_ev_energy_pointer": [_vp, C.c_char_p, C.POINTER(_vp)],
+    "pme_ev_virial_pointer": [_vp, C.POINTER(_vp)],
+    "pme_recip_create": [_i, _i, _i, _i, _i, _i, _i, _i, _vp, C.c_char_p, C.c_char_p, _vp, C.POINTER(_vp)],
+    "pme_recip_destroy": [_vp],
+    "pme_recip_set_stream": [_vp, _vp],
+    "pme_recip_set_order": [_vp, _i],
+    "pme_recip_get_info": [_vp, C.POINTER(_i), C.POINTER(_i), C.POINTER(_i), C.POINTER(_i)],
+    "pme_recip_spread_charge": [_vp, _vp, _i, _pd],
+    "pme_recip_r2c_fft": [_vp],
+    "pme_recip_scalar_sum": [_vp, _pd, _d, _i, _i],
+    "pme_recip_c2r_fft": [_vp],
+    "pme_recip_gather_force": [_vp, _vp, _i, _pd, _i, _i, _vp],
+    "pme_recip_calc_self_energy": [_vp, _vp, _i, _d],
+    "pme_recip_solve": [_vp, _vp, _i, _pd, _d, _i, _i, _i, _i, _vp],
+    "pme_recip_copy_grid": [_vp, _i, _vp, C.c_size_t],
+    "pme_recip_set_timing": [_vp, _i],
+    "pme_recip_get_timing": [_vp, C.POINTER(C.c_float)],
+    "pme_fill_bspline": [_i, _i, _i, _i, _pd, _vp, _i] + [_vp] * 10 + [_vp],
+    "pme_bsp_mod": [_i, _i, _pd],
+}
+_RESTYPE = {"pme_last_error": C.c_char_p, "pme_status_string": C.c_char_p}
+
+_lib = None
+
+
+def header_symbols() -> list[str]:
+    """Every function the C header declares (pme_* names followed by '(')."""
+    with open(HEADER) as f:
+        txt = f.read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(pme_[a-z0-9_]+)\s*\(", txt)))
+
+
+def load() -> C.CDLL:
+    """Load libpme_hip.so and bind every declared symbol.  Raises if missing."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} not built: run `python -m gpu_recip_amd.build` "
+                           "(there is no CPU fallback)")
+    lib = C.CDLL(LIB_PATH)
+    for name, argt in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.argtypes = argt
+        fn.restype = _RESTYPE.get(name, C.c_int)
+    _lib = lib
+    return lib
+
+
+def check(status: int, func: str) -> None:
+    if status != PME_OK:
+        msg = load().pme_last_error()
+        raise PmeError(status, func, msg.decode() if msg else "")
+
+
+def call(name: str, *args) -> None:
+    check(getattr(load(), name)(*args), name)
+
+
+def dbl9(recip) -> C.Array:
+    arr = (C.c_double * 9)()
+    for i, v in enumerate(recip):
+        arr[i] = float(v)
+    return arr

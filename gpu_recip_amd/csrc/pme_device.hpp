@@ -1,0 +1,296 @@
+// pme_device.hpp -- device helpers shared by the PME kernels (gfx950, wave64).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pme {
+
+constexpr int kBlock = 256;          // threads per workgroup for every kernel here
+constexpr double kCcelec = 332.0716; // CudaPMERecip.h:27
+constexpr float kForceScale = 1099511627776.0f;  // 2^40, gpu_utils.h:9
+
+// Grid index of one coordinate, CudaPMERecip.cu:512-529.  The reference is
+// built with -fmad=true, which fuses x*recip + 2.0f into one FMA; nothing else
+// in the expression can fuse.  Contraction is switched off here so the
+// compiler cannot fuse anything beyond the explicit fmaf.
+__device__ __forceinline__ void grid_index(float x, float r, int n, int& i, float& w) {
+#pragma clang fp contract(off)
+  const float t = __builtin_fmaf(x, r, 2.0f);
+  const float fr = (float)n * (t - (floorf(t + 0.5f) - 0.5f));
+  i = (int)fr;
+  w = fr - (float)i;
+}
+
+// Grid index brought into [0, n) before it addresses memory.  For finite
+// coordinates of any sane size the formula above already gives 0 <= i < n
+// (n*(1-ulp) never rounds up to n in FP32); non-finite or astronomically large
+// coordinates can give anything, and this keeps every access in bounds.
+__device__ __forceinline__ int safe_index(int i, int n) {
+  if ((unsigned)i < (unsigned)n) return i;
+  i %= n;
+  return i < 0 ? i + n : i;
+}
+
+// theta[j] = M_p(w + p-1-j): the cardinal B-spline recursion of
+// CudaPMERecip.cu:181-208 (same operation order, FP32).
+template <int P>
+__device__ __forceinline__ void bspline_theta(float w, float* th) {
+  th[P - 1] = 0.0f;
+  th[1] = w;
+  th[0] = 1.0f - w;
+#pragma unroll
+  for (int k = 3; k <= P - 1; k++) {
+    const float div = 1.0f / (float)(k - 1);
+    th[k - 1] = div * w * th[k - 2];
+#pragma unroll
+    for (int j = 1; j <= k - 2; j++)
+      th[k - j - 1] = div * ((w + j) * th[k - j - 2] + (k - j - w) * th[k - j - 1]);
+    th[0] = div * (1.0f - w) * th[0];
+  }
+  const float div = 1.0f / (float)(P - 1);
+  th[P - 1] = div * w * th[P - 2];
+#pragma unroll
+  for (int j = 1; j <= P - 2; j++)
+    th[P - j - 1] = div * ((w + j) * th[P - j - 2] + (P - j - w) * th[P - j - 1]);
+  th[0] = div * (1.0f - w) * th[0];
+}
+
+// theta and dtheta = d theta/dw, CudaPMERecip.cu:120-176.
+template <int P>
+__device__ __forceinline__ void bspline_theta_dtheta(float w, float* th, float* dth) {
+  th[P - 1] = 0.0f;
+  th[1] = w;
+  th[0] = 1.0f - w;
+#pragma unroll
+  for (int k = 3; k <= P - 1; k++) {
+    const float div = 1.0f / (float)(k - 1);
+    th[k - 1] = div * w * th[k - 2];
+#pragma unroll
+    for (int j = 1; j <= k - 2; j++)
+      th[k - j - 1] = div * ((w + j) * th[k - j - 2] + (k - j - w) * th[k - j - 1]);
+    th[0] = div * (1.0f - w) * th[0];
+  }
+  dth[0] = -th[0];
+#pragma unroll
+  for (int j = 2; j <= P; j++) dth[j - 1] = th[j - 2] - th[j - 1];
+  const float div = 1.0f / (float)(P - 1);
+  th[P - 1] = div * w * th[P - 2];
+#pragma unroll
+  for (int j = 1; j <= P - 2; j++)
+    th[P - j - 1] = div * ((w + j) * th[P - j - 2] + (P - j - w) * th[P - j - 1]);
+  th[0] = div * (1.0f - w) * th[0];
+}
+
+// Brick decomposition of one grid axis into nb nearly equal bricks:
+// brick b covers [floor(b n/nb), floor((b+1) n/nb)).
+__device__ __forceinline__ int brick_lo(int b, int n, int nb) { return (b * n) / nb; }
+// brick holding grid point i (0 <= i < n): the unique b with b n < (i+1) nb <= (b+1) n
+__device__ __forceinline__ int brick_of(int i, int n, int nb) { return ((i + 1) * nb - 1) / n; }
+
+// ---- complex helpers -----------------------------------------------------
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
+__device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+// multiply by s*i (s = +1 or -1)
+__device__ __forceinline__ float2 cmul_si(float2 a, float s) { return make_float2(-s * a.y, s * a.x); }
+
+// Small DFTs y_k = sum_j a_j e^{S 2 pi i jk/R}, S = -1 forward, +1 inverse.
+template <int R, int S> struct Dft;
+
+template <int S> struct Dft<2, S> {
+  static __device__ __forceinline__ void run(float2* a) {
+    float2 t = a[1];
+    a[1] = csub(a[0], t);
+    a[0] = cadd(a[0], t);
+  }
+};
+
+template <int S> struct Dft<3, S> {
+  static __device__ __forceinline__ void run(float2* a) {
+    const float c = -0.5f, s3 = 0.866025403784438647f;  // cos(2pi/3), sin(2pi/3)
+    float2 t = cadd(a[1], a[2]);
+    float2 d = csub(a[1], a[2]);
+    float2 m = make_float2(a[0].x + c * t.x, a[0].y + c * t.y);
+    float2 r = cmul_si(cscale(d, s3), (float)S);
+    a[0] = cadd(a[0], t);
+    a[1] = cadd(m, r);
+    a[2] = csub(m, r);
+  }
+};
+
+template <int S> struct Dft<4, S> {
+  static __device__ __forceinline__ void run(float2* a) {
+    float2 t0 = cadd(a[0], a[2]);
+    float2 t1 = csub(a[0], a[2]);
+    float2 t2 = cadd(a[1], a[3]);
+    float2 t3 = cmul_si(csub(a[1], a[3]), (float)S);
+    a[0] = cadd(t0, t2);
+    a[2] = csub(t0, t2);
+    a[1] = cadd(t1, t3);
+    a[3] = csub(t1, t3);
+  }
+};
+
+template <int S> struct Dft<5, S> {
+  static __device__ __forceinline__ void run(float2* a) {
+    const float c1 = 0.309016994374947424f;   // cos(2pi/5)
+    const float c2 = -0.809016994374947424f;  // cos(4pi/5)
+    const float s1 = 0.951056516295153572f;   // sin(2pi/5)
+    const float s2 = 0.587785252292473129f;   // sin(4pi/5)
+    float2 t1 = cadd(a[1], a[4]), d1 = csub(a[1], a[4]);
+    float2 t2 = cadd(a[2], a[3]), d2 = csub(a[2], a[3]);
+    float2 m1 = make_float2(a[0].x + c1 * t1.x + c2 * t2.x, a[0].y + c1 * t1.y + c2 * t2.y);
+    float2 m2 = make_float2(a[0].x + c2 * t1.x + c1 * t2.x, a[0].y + c2 * t1.y + c1 * t2.y);
+    float2 r1 = cmul_si(make_float2(s1 * d1.x + s2 * d2.x, s1 * d1.y + s2 * d2.y), (float)S);
+    float2 r2 = cmul_si(make_float2(s2 * d1.x - s1 * d2.x, s2 * d1.y - s1 * d2.y), (float)S);
+    a[0] = cadd(a[0], cadd(t1, t2));
+    a[1] = cadd(m1, r1);
+    a[4] = csub(m1, r1);
+    a[2] = cadd(m2, r2);
+    a[3] = csub(m2, r2);
+  }
+};
+
+template <int S> struct Dft<8, S> {
+  static __device__ __forceinline__ void run(float2* a) {
+    // radix-2 split: evens and odds as DFT4, odds twiddled by e^{S 2 pi i k/8}
+    float2 e[4] = {a[0], a[2], a[4], a[6]};
+    float2 o[4] = {a[1], a[3], a[5], a[7]};
+    Dft<4, S>::run(e);
+    Dft<4, S>::run(o);
+    const float h = 0.707106781186547524f;
+    // k=1: (h, S h); k=2: S i; k=3: (-h, S h)
+    float2 o1 = cmul(o[1], make_float2(h, S * h));
+    float2 o2 = cmul_si(o[2], (float)S);
+    float2 o3 = cmul(o[3], make_float2(-h, S * h));
+    a[0] = cadd(e[0], o[0]);
+    a[4] = csub(e[0], o[0]);
+    a[1] = cadd(e[1], o1);
+    a[5] = csub(e[1], o1);
+    a[2] = cadd(e[2], o2);
+    a[6] = csub(e[2], o2);
+    a[3] = cadd(e[3], o3);
+    a[7] = csub(e[3], o3);
+  }
+};
+
+// Radix plan of one transform length (Stockham autosort, mixed radix).
+struct RadixPlan {
+  int n;         // transform length
+  int nrad;      // number of stages
+  int rad[12];   // radix of each stage (2, 3, 4, 5, 8)
+  int tstride;   // twiddle table stride: table length / n
+};
+
+// One Stockham stage of radix R over nlines interleaved lines.
+// Element j of line l lives at buf[j*ls + l].  Stage with p = product of the
+// previous radices: butterfly i reads i + r T (T = n/R), twiddles input r by
+// w^{r k n/(pR)} (k = i mod p) and writes (i/p) p R + k + r p.
+template <int R, int S>
+__device__ __forceinline__ void stockham_stage(const float2* __restrict__ src, float2* __restrict__ dst,
+                                               const float2* __restrict__ tw, int n, int p, int tstride,
+                                               int nlines, int ls, int tid, int nthr) {
+  const int T = n / R;
+  const int nbf = T * nlines;
+  const int tscale = (n / (p * R)) * tstride;
+  for (int bf = tid; bf < nbf; bf += nthr) {
+    const int i = bf / nlines;
+    const int l = bf - i * nlines;
+    const int k = i % p;
+    float2 u[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) u[r] = src[(i + r * T) * ls + l];
+    if (p > 1) {
+#pragma unroll
+      for (int r = 1; r < R; r++) {
+        float2 w = tw[r * k * tscale];
+        if (S > 0) w.y = -w.y;
+        u[r] = cmul(u[r], w);
+      }
+    }
+    Dft<R, S>::run(u);
+    const int j = (i / p) * p * R + k;
+#pragma unroll
+    for (int r = 0; r < R; r++) dst[(j + r * p) * ls + l] = u[r];
+  }
+}
+
+// Full transform of nlines lines held in LDS (buf0 -> result).  Returns the
+// buffer holding the result (buf0 or buf1).  Ends with a barrier.
+template <int S>
+__device__ float2* fft_lines(float2* buf0, float2* buf1, const float2* tw, const RadixPlan& rp,
+                             int nlines, int ls, int tid, int nthr) {
+  float2* src = buf0;
+  float2* dst = buf1;
+  int p = 1;
+  for (int s = 0; s < rp.nrad; s++) {
+    const int R = rp.rad[s];
+    switch (R) {
+      case 2: stockham_stage<2, S>(src, dst, tw, rp.n, p, rp.tstride, nlines, ls, tid, nthr); break;
+      case 3: stockham_stage<3, S>(src, dst, tw, rp.n, p, rp.tstride, nlines, ls, tid, nthr); break;
+      case 4: stockham_stage<4, S>(src, dst, tw, rp.n, p, rp.tstride, nlines, ls, tid, nthr); break;
+      case 5: stockham_stage<5, S>(src, dst, tw, rp.n, p, rp.tstride, nlines, ls, tid, nthr); break;
+      default: stockham_stage<8, S>(src, dst, tw, rp.n, p, rp.tstride, nlines, ls, tid, nthr); break;
+    }
+    __syncthreads();
+    float2* t = src;
+    src = dst;
+    dst = t;
+    p *= R;
+  }
+  return src;
+}
+
+// ---- reductions ------------------------------------------------------------
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Block-wide sum of NV doubles (blockDim.x == kBlock).  The result is valid in
+// thread 0.  sh must hold NV * (kBlock/64) doubles.  Fixed reduction order.
+template <int NV>
+__device__ __forceinline__ void block_sum(double* v, double* sh) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int c = 0; c < NV; c++) v[c] = wave_sum(v[c]);
+  if (lane == 0) {
+#pragma unroll
+    for (int c = 0; c < NV; c++) sh[wid * NV + c] = v[c];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int c = 0; c < NV; c++) {
+      double s = 0.0;
+      for (int w = 0; w < kBlock / 64; w++) s += sh[w * NV + c];
+      v[c] = s;
+    }
+  }
+  __syncthreads();
+}
+
+// "Last workgroup done" ticket.  Every workgroup calls it once at its end;
+// returns true in all threads of the workgroup that finished last.  That
+// workgroup resets the ticket, so the kernel needs no host-side clear.
+__device__ __forceinline__ bool last_block_done(unsigned int* ticket, int* sh_flag) {
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned int prev = atomicAdd(ticket, 1u);
+    const bool last = (prev == gridDim.x * gridDim.y * gridDim.z - 1);
+    *sh_flag = last ? 1 : 0;
+    if (last) *ticket = 0u;
+  }
+  __syncthreads();
+  const bool last = (*sh_flag != 0);
+  if (last) __threadfence();
+  return last;
+}
+
+}  // namespace pme

@@ -1,0 +1,332 @@
+// pme_fft.hip -- 3-D real-to-complex FFT of the PME grid and the reciprocal
+// convolution (scalar sum), hand-written for CDNA4.
+//
+// Layouts (the reference's BOX layout, CudaPMERecip.cu:1383, 1399-1400):
+//   real grid     float [nz][ny][nx]            (x fastest, packed)
+//   half spectrum float2[nz][ny][nx/2+1]        (x fastest)
+// Forward transforms use e^{-2 pi i}, inverse e^{+2 pi i}, both unnormalised.
+//
+// Passes: x (R2C / C2R over rows, even nx through a half-length complex FFT),
+// y and z over strided columns, tiled as TX consecutive kx so every HBM access
+// is a contiguous run.  Every 1-D transform runs in LDS as mixed-radix
+// (2/3/4/5/8) Stockham stages.  The z pass can run forward FFT -> influence
+// function (energy/virial in FP64) -> inverse FFT in one kernel, which is the
+// reference's r2c z-stage + scalar_sum + c2r z-stage in one read and one write
+// of the spectrum.
+#include "pme_internal.hpp"
+
+namespace pme {
+
+// ---------------------------------------------------------------- x pass ----
+
+__global__ __launch_bounds__(kBlock) void k_fft_x_r2c(const float* __restrict__ grid, float2* __restrict__ cg, int nx,
+                                                      int nrows, int NR, AxisArgs ax) {
+  extern __shared__ __align__(16) float2 sm[];
+  const int tid = threadIdx.x;
+  const int nf = ax.rp.n;
+  const int LS = NR + 1;
+  float2* tw = sm;
+  float2* b0 = sm + ax.ntw;
+  float2* b1 = b0 + nf * LS;
+  for (int i = tid; i < ax.ntw; i += kBlock) tw[i] = ax.tw[i];
+  const int row0 = blockIdx.x * NR;
+  const int nr = min(NR, nrows - row0);
+  const int hx = nx / 2 + 1;
+  const bool even = (nx & 1) == 0;
+  if (even) {
+    const float2* src = reinterpret_cast<const float2*>(grid + (size_t)row0 * nx);
+    for (int e = tid; e < NR * nf; e += kBlock) {
+      const int row = e / nf;
+      const int m = e - row * nf;
+      b0[m * LS + row] = (row < nr) ? src[(size_t)row * nf + m] : make_float2(0.f, 0.f);
+    }
+  } else {
+    for (int e = tid; e < NR * nx; e += kBlock) {
+      const int row = e / nx;
+      const int m = e - row * nx;
+      b0[m * LS + row] = make_float2((row < nr) ? grid[(size_t)(row0 + row) * nx + m] : 0.f, 0.f);
+    }
+  }
+  __syncthreads();
+  float2* res = fft_lines<-1>(b0, b1, tw, ax.rp, NR, LS, tid, kBlock);
+  if (even) {
+    // X[k] = E[k] + w^k O[k], E = (Z[k] + Z*[h-k])/2, O = (Z[k] - Z*[h-k])/(2i)
+    const int h = nf;
+    for (int e = tid; e < nr * hx; e += kBlock) {
+      const int row = e / hx;
+      const int k = e - row * hx;
+      const float2 zk = res[(k == h ? 0 : k) * LS + row];
+      const float2 zc = cconj(res[(k == 0 ? 0 : h - k) * LS + row]);
+      const float2 E = cscale(cadd(zk, zc), 0.5f);
+      const float2 D = csub(zk, zc);
+      const float2 O = make_float2(0.5f * D.y, -0.5f * D.x);
+      cg[(size_t)(row0 + row) * hx + k] = cadd(E, cmul(tw[k], O));
+    }
+  } else {
+    for (int e = tid; e < nr * hx; e += kBlock) {
+      const int row = e / hx;
+      const int k = e - row * hx;
+      cg[(size_t)(row0 + row) * hx + k] = res[k * LS + row];
+    }
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_fft_x_c2r(const float2* __restrict__ cg, float* __restrict__ grid, int nx,
+                                                      int nrows, int NR, AxisArgs ax) {
+  extern __shared__ __align__(16) float2 sm[];
+  const int tid = threadIdx.x;
+  const int nf = ax.rp.n;
+  const int LS = NR + 1;
+  float2* tw = sm;
+  float2* b0 = sm + ax.ntw;
+  float2* b1 = b0 + nf * LS;
+  for (int i = tid; i < ax.ntw; i += kBlock) tw[i] = ax.tw[i];
+  __syncthreads();
+  const int row0 = blockIdx.x * NR;
+  const int nr = min(NR, nrows - row0);
+  const int hx = nx / 2 + 1;
+  const bool even = (nx & 1) == 0;
+  if (even) {
+    // Z[k] = (X[k] + X*[h-k]) + i w^{-k} (X[k] - X*[h-k]),  k < h
+    const int h = nf;
+    for (int e = tid; e < NR * h; e += kBlock) {
+      const int row = e / h;
+      const int k = e - row * h;
+      float2 z = make_float2(0.f, 0.f);
+      if (row < nr) {
+        const float2* x = cg + (size_t)(row0 + row) * hx;
+        const float2 xk = x[k];
+        const float2 xc = cconj(x[h - k]);
+        const float2 A = cadd(xk, xc);
+        const float2 B = cmul(cconj(tw[k]), csub(xk, xc));
+        z = make_float2(A.x - B.y, A.y + B.x);
+      }
+      b0[k * LS + row] = z;
+    }
+  } else {
+    for (int e = tid; e < NR * nx; e += kBlock) {
+      const int row = e / nx;
+      const int k = e - row * nx;
+      float2 z = make_float2(0.f, 0.f);
+      if (row < nr) {
+        const float2* x = cg + (size_t)(row0 + row) * hx;
+        z = (k < hx) ? x[k] : cconj(x[nx - k]);
+      }
+      b0[k * LS + row] = z;
+    }
+  }
+  __syncthreads();
+  float2* res = fft_lines<+1>(b0, b1, tw, ax.rp, NR, LS, tid, kBlock);
+  if (even) {
+    float2* dst = reinterpret_cast<float2*>(grid + (size_t)row0 * nx);
+    for (int e = tid; e < nr * nf; e += kBlock) {
+      const int row = e / nf;
+      const int m = e - row * nf;
+      dst[(size_t)row * nf + m] = res[m * LS + row];
+    }
+  } else {
+    for (int e = tid; e < nr * nx; e += kBlock) {
+      const int row = e / nx;
+      const int m = e - row * nx;
+      grid[(size_t)(row0 + row) * nx + m] = res[m * LS + row].x;
+    }
+  }
+}
+
+// ------------------------------------------------------- influence function --
+
+// eterm at (k1, k2, k3) and the energy/virial terms of that point,
+// scalar_sum_ortho_kernel CudaPMERecip.cu:671-704
+__device__ __forceinline__ float eterm_at(const SolveArgs& s, int k1, int k2, int k3, float& mh1, float& mh2,
+                                          float& mh3, float& msq_inv) {
+  const int m1 = k1 >= s.nf1 ? k1 - s.n1 : k1;
+  const int m2 = k2 >= s.nf2 ? k2 - s.n2 : k2;
+  const int m3 = k3 >= s.nf3 ? k3 - s.n3 : k3;
+  mh1 = s.r1 * (float)m1;
+  mh2 = s.r2 * (float)m2;
+  mh3 = s.r3 * (float)m3;
+  const float msq = mh1 * mh1 + mh2 * mh2 + mh3 * mh3;
+  msq_inv = 1.0f / msq;
+  return expf(-s.fac * msq) * s.piv_inv * s.pf1[k1] * s.pf2[k2] * s.pf3[k3] * msq_inv;
+}
+
+__device__ __forceinline__ void ev_accumulate(float eterm, float q2, float fac, float mh1, float mh2, float mh3,
+                                              float msq_inv, double* acc) {
+  const float tmp1 = eterm * q2;
+  const float vterm = 2.0f * (fac + msq_inv);
+  const float tmp2 = tmp1 * vterm;
+  acc[0] += (double)tmp1;
+  acc[1] += (double)(tmp1 * (vterm * mh1 * mh1 - 1.0f));
+  acc[2] += (double)(tmp2 * mh1 * mh2);
+  acc[3] += (double)(tmp2 * mh1 * mh3);
+  acc[4] += (double)(tmp1 * (vterm * mh2 * mh2 - 1.0f));
+  acc[5] += (double)(tmp2 * mh2 * mh3);
+  acc[6] += (double)(tmp1 * (vterm * mh3 * mh3 - 1.0f));
+}
+
+// Multiply one half-spectrum point by eterm; accumulate energy/virial with
+// the reference's weights (mirror term for every k1 >= 1, :708-741).
+__device__ __forceinline__ float2 solve_point(const SolveArgs& s, float2 q, int k1, int k2, int k3, double* acc) {
+  if ((k1 | k2 | k3) == 0) return make_float2(0.f, 0.f);
+  float mh1, mh2, mh3, msq_inv;
+  const float eterm = eterm_at(s, k1, k2, k3, mh1, mh2, mh3, msq_inv);
+  if (s.calc_ev) {
+    const float q2 = q.x * q.x + q.y * q.y;
+    ev_accumulate(eterm, q2, s.fac, mh1, mh2, mh3, msq_inv, acc);
+    if (k1 >= 1) {
+      const int k1s = s.n1 - k1;
+      const int k2s = (s.n2 - k2) % s.n2;
+      const int k3s = (s.n3 - k3) % s.n3;
+      float ms1, ms2, ms3, msqs_inv;
+      const float eterms = eterm_at(s, k1s, k2s, k3s, ms1, ms2, ms3, msqs_inv);
+      ev_accumulate(eterms, q2, s.fac, ms1, ms2, ms3, msqs_inv, acc);
+    }
+  }
+  return cscale(q, eterm);
+}
+
+// Per-workgroup partial -> last workgroup adds the total into the energy slot
+// and virmat (fixed summation order: deterministic for a given launch shape).
+__device__ __forceinline__ void ev_finish(const SolveArgs& s, double* acc, double* sh, int* sh_flag, int blk,
+                                          int nblk) {
+  block_sum<7>(acc, sh);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int c = 0; c < 7; c++) s.partial[(size_t)blk * 7 + c] = acc[c];
+  }
+  if (last_block_done(s.ticket, sh_flag)) {
+    double v[7] = {0, 0, 0, 0, 0, 0, 0};
+    for (int i = threadIdx.x; i < nblk; i += kBlock)
+#pragma unroll
+      for (int c = 0; c < 7; c++) v[c] += s.partial[(size_t)i * 7 + c];
+    block_sum<7>(v, sh);
+    if (threadIdx.x == 0) {
+      const double hc = 0.5 * kCcelec;
+      s.energy[0] += v[0] * hc;
+      const double v0 = -hc * v[1], v1 = -hc * v[2], v2 = -hc * v[3];
+      const double v3 = -hc * v[4], v4 = -hc * v[5], v5 = -hc * v[6];
+      s.virmat[0] += v0; s.virmat[1] += v1; s.virmat[2] += v2;
+      s.virmat[3] += v1; s.virmat[4] += v3; s.virmat[5] += v4;
+      s.virmat[6] += v2; s.virmat[7] += v4; s.virmat[8] += v5;
+    }
+  }
+}
+
+// ------------------------------------------------------------ y / z passes --
+
+template <int MODE>
+__global__ __launch_bounds__(kBlock) void k_fft_cols(float2* __restrict__ cg, int n, int hx, int TX,
+                                                     long long pt_stride, long long outer_stride, AxisArgs ax,
+                                                     SolveArgs sa) {
+  extern __shared__ __align__(16) float2 sm[];
+  __shared__ double sh_red[7 * (kBlock / 64)];
+  __shared__ int sh_flag;
+  const int tid = threadIdx.x;
+  float2* tw = sm;
+  float2* b0 = sm + ax.ntw;
+  float2* b1 = b0 + n * TX;
+  for (int i = tid; i < ax.ntw; i += kBlock) tw[i] = ax.tw[i];
+  const int kx0 = blockIdx.x * TX;
+  const int nl = min(TX, hx - kx0);
+  const size_t base = (size_t)kx0 + (size_t)blockIdx.y * (size_t)outer_stride;
+  for (int e = tid; e < n * TX; e += kBlock) {
+    const int j = e / TX;
+    const int l = e - j * TX;
+    b0[e] = (l < nl) ? cg[base + (size_t)j * pt_stride + l] : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  float2* res;
+  if (MODE == COL_INV) {
+    res = fft_lines<+1>(b0, b1, tw, ax.rp, TX, TX, tid, kBlock);
+  } else {
+    res = fft_lines<-1>(b0, b1, tw, ax.rp, TX, TX, tid, kBlock);
+  }
+  double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+  if (MODE == COL_FWD_SOLVE_INV) {
+    const int k2 = blockIdx.y;
+    for (int e = tid; e < n * TX; e += kBlock) {
+      const int j = e / TX;
+      const int l = e - j * TX;
+      if (l < nl) res[e] = solve_point(sa, res[e], kx0 + l, k2, j, acc);
+    }
+    __syncthreads();
+  }
+  if (MODE == COL_FWD_SOLVE_INV || MODE == COL_FWD_INV) {
+    res = fft_lines<+1>(res, res == b0 ? b1 : b0, tw, ax.rp, TX, TX, tid, kBlock);
+  }
+  for (int e = tid; e < n * TX; e += kBlock) {
+    const int j = e / TX;
+    const int l = e - j * TX;
+    if (l < nl) cg[base + (size_t)j * pt_stride + l] = res[e];
+  }
+  if (MODE == COL_FWD_SOLVE_INV && sa.calc_ev) {
+    ev_finish(sa, acc, sh_red, &sh_flag, blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
+  }
+}
+
+// Stand-alone scalar sum over the whole half spectrum (used when the spectrum
+// has been materialised, e.g. scalar_sum called twice).
+__global__ __launch_bounds__(kBlock) void k_scalar_sum(float2* __restrict__ cg, int hx, int total, SolveArgs sa) {
+  __shared__ double sh_red[7 * (kBlock / 64)];
+  __shared__ int sh_flag;
+  double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+  const int idx = blockIdx.x * kBlock + threadIdx.x;
+  if (idx < total) {
+    const int k1 = idx % hx;
+    const int t = idx / hx;
+    const int k2 = t % sa.n2;
+    const int k3 = t / sa.n2;
+    cg[idx] = solve_point(sa, cg[idx], k1, k2, k3, acc);
+  }
+  if (sa.calc_ev) ev_finish(sa, acc, sh_red, &sh_flag, blockIdx.x, gridDim.x);
+}
+
+// ----------------------------------------------------------------- launch --
+
+size_t fft_x_lds_bytes(int nf, int NR, int ntw) { return sizeof(float2) * ((size_t)ntw + 2 * (size_t)nf * (NR + 1)); }
+size_t fft_cols_lds_bytes(int n, int TX) { return sizeof(float2) * ((size_t)n + 2 * (size_t)n * TX); }
+
+hipError_t launch_fft_x_r2c(const float* grid, float2* cg, int nx, int nrows, int NR, const AxisArgs& ax,
+                            hipStream_t s) {
+  const size_t lds = fft_x_lds_bytes(ax.rp.n, NR, ax.ntw);
+  hipLaunchKernelGGL(k_fft_x_r2c, dim3((nrows + NR - 1) / NR), dim3(kBlock), lds, s, grid, cg, nx, nrows, NR, ax);
+  return hipGetLastError();
+}
+
+hipError_t launch_fft_x_c2r(const float2* cg, float* grid, int nx, int nrows, int NR, const AxisArgs& ax,
+                            hipStream_t s) {
+  const size_t lds = fft_x_lds_bytes(ax.rp.n, NR, ax.ntw);
+  hipLaunchKernelGGL(k_fft_x_c2r, dim3((nrows + NR - 1) / NR), dim3(kBlock), lds, s, cg, grid, nx, nrows, NR, ax);
+  return hipGetLastError();
+}
+
+hipError_t launch_fft_cols(int mode, float2* cg, int n, int hx, int TX, long long pt_stride, int nouter,
+                           long long outer_stride, const AxisArgs& ax, const SolveArgs& sa, hipStream_t s) {
+  const size_t lds = fft_cols_lds_bytes(n, TX);
+  const dim3 grid((hx + TX - 1) / TX, nouter);
+  switch (mode) {
+    case COL_FWD:
+      hipLaunchKernelGGL(k_fft_cols<COL_FWD>, grid, dim3(kBlock), lds, s, cg, n, hx, TX, pt_stride, outer_stride, ax, sa);
+      break;
+    case COL_INV:
+      hipLaunchKernelGGL(k_fft_cols<COL_INV>, grid, dim3(kBlock), lds, s, cg, n, hx, TX, pt_stride, outer_stride, ax, sa);
+      break;
+    case COL_FWD_SOLVE_INV:
+      hipLaunchKernelGGL(k_fft_cols<COL_FWD_SOLVE_INV>, grid, dim3(kBlock), lds, s, cg, n, hx, TX, pt_stride, outer_stride, ax, sa);
+      break;
+    case COL_FWD_INV:
+      hipLaunchKernelGGL(k_fft_cols<COL_FWD_INV>, grid, dim3(kBlock), lds, s, cg, n, hx, TX, pt_stride, outer_stride, ax, sa);
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_scalar_sum(float2* cg, int hx, const SolveArgs& sa, hipStream_t s) {
+  const int total = hx * sa.n2 * sa.n3;
+  hipLaunchKernelGGL(k_scalar_sum, dim3((total + kBlock - 1) / kBlock), dim3(kBlock), 0, s, cg, hx, total, sa);
+  return hipGetLastError();
+}
+
+}  // namespace pme

@@ -1,0 +1,775 @@
+// pme_host.cpp -- host side of the PME reciprocal engine: the C ABI of
+// include/pme_recip.h, per-handle plans and buffers, and the stage state
+// machine that lets the reference's five-call sequence run as fused kernels.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/pme_recip.h"
+#include "pme_internal.hpp"
+
+using namespace pme;
+
+// ------------------------------------------------------------------ errors --
+static thread_local std::string g_err;
+
+static pme_status fail(pme_status s, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return s;
+}
+
+#define HIP_TRY(expr)                                                                     \
+  do {                                                                                    \
+    hipError_t e_ = (expr);                                                               \
+    if (e_ != hipSuccess)                                                                 \
+      return fail(e_ == hipErrorOutOfMemory ? PME_ERR_OOM : PME_ERR_DEVICE, "%s: %s (%s:%d)", #expr, \
+                  hipGetErrorString(e_), __FILE__, __LINE__);                             \
+  } while (0)
+
+#define PME_TRY(expr)             \
+  do {                            \
+    pme_status s_ = (expr);       \
+    if (s_ != PME_OK) return s_;  \
+  } while (0)
+
+extern "C" const char* pme_last_error(void) { return g_err.c_str(); }
+
+extern "C" const char* pme_status_string(pme_status s) {
+  switch (s) {
+    case PME_OK: return "ok";
+    case PME_ERR_INVALID_ARG: return "invalid argument";
+    case PME_ERR_UNSUPPORTED: return "unsupported";
+    case PME_ERR_OOM: return "out of memory";
+    case PME_ERR_BUDGET: return "over budget";
+    case PME_ERR_DEVICE: return "device error";
+    case PME_ERR_STATE: return "stage called out of order";
+    case PME_ERR_NOT_FOUND: return "not found";
+  }
+  return "unknown";
+}
+
+// ---------------------------------------------------------- energy/virial --
+// Virial_t layout, CudaEnergyVirial.h:13-18
+static const size_t kOffSforceDp = 0;
+static const size_t kOffSforceFp = 81 * sizeof(double);
+static const size_t kOffVirmat = kOffSforceFp + 81 * sizeof(long long);
+static const size_t kOffEnergy = kOffVirmat + 9 * sizeof(double);
+static const int kMaxTerms = 64;
+static const size_t kEvBytes = kOffEnergy + kMaxTerms * sizeof(double);
+
+struct pme_energy_virial_s {
+  std::map<std::string, int> index;
+  int n = 0;
+  char* d_buf = nullptr;
+  char* h_buf = nullptr;
+};
+
+extern "C" pme_status pme_ev_create(pme_ev_t* out) {
+  if (!out) return fail(PME_ERR_INVALID_ARG, "pme_ev_create: null out");
+  pme_ev_t ev = new pme_energy_virial_s();
+  hipError_t e = hipMalloc(&ev->d_buf, kEvBytes);
+  if (e != hipSuccess) {
+    delete ev;
+    return fail(PME_ERR_OOM, "pme_ev_create: hipMalloc: %s", hipGetErrorString(e));
+  }
+  e = hipHostMalloc(&ev->h_buf, kEvBytes, hipHostMallocDefault);
+  if (e != hipSuccess) {
+    (void)hipFree(ev->d_buf);
+    delete ev;
+    return fail(PME_ERR_OOM, "pme_ev_create: hipHostMalloc: %s", hipGetErrorString(e));
+  }
+  std::memset(ev->h_buf, 0, kEvBytes);
+  e = hipMemset(ev->d_buf, 0, kEvBytes);
+  if (e != hipSuccess) return fail(PME_ERR_DEVICE, "pme_ev_create: hipMemset: %s", hipGetErrorString(e));
+  *out = ev;
+  return PME_OK;
+}
+
+extern "C" pme_status pme_ev_destroy(pme_ev_t ev) {
+  if (!ev) return PME_OK;
+  (void)hipFree(ev->d_buf);
+  (void)hipHostFree(ev->h_buf);
+  delete ev;
+  return PME_OK;
+}
+
+extern "C" pme_status pme_ev_insert(pme_ev_t ev, const char* name) {
+  if (!ev || !name) return fail(PME_ERR_INVALID_ARG, "pme_ev_insert: null argument");
+  if (ev->index.count(name)) return PME_OK;  // only added if new, EnergyVirial.cpp:16-21
+  if (ev->n >= kMaxTerms) return fail(PME_ERR_UNSUPPORTED, "pme_ev_insert: more than %d energy terms", kMaxTerms);
+  ev->index[name] = ev->n++;
+  return PME_OK;
+}
+
+static pme_status ev_slot(pme_ev_t ev, const char* name, int* slot) {
+  if (!ev || !name) return fail(PME_ERR_INVALID_ARG, "energy term: null argument");
+  auto it = ev->index.find(name);
+  if (it == ev->index.end()) return fail(PME_ERR_NOT_FOUND, "energy term %s not found", name);
+  *slot = it->second;
+  return PME_OK;
+}
+
+extern "C" pme_status pme_ev_clear(pme_ev_t ev, pme_stream_t stream) {
+  if (!ev) return fail(PME_ERR_INVALID_ARG, "pme_ev_clear: null ev");
+  HIP_TRY(launch_fill_zero(ev->d_buf, kEvBytes, (hipStream_t)stream));
+  std::memset(ev->h_buf, 0, kEvBytes);
+  return PME_OK;
+}
+
+extern "C" pme_status pme_ev_clear_energy(pme_ev_t ev, pme_stream_t stream) {
+  if (!ev) return fail(PME_ERR_INVALID_ARG, "pme_ev_clear_energy: null ev");
+  HIP_TRY(launch_fill_zero(ev->d_buf + kOffEnergy, kMaxTerms * sizeof(double), (hipStream_t)stream));
+  std::memset(ev->h_buf + kOffEnergy, 0, kMaxTerms * sizeof(double));
+  return PME_OK;
+}
+
+extern "C" pme_status pme_ev_clear_virial(pme_ev_t ev, pme_stream_t stream) {
+  if (!ev) return fail(PME_ERR_INVALID_ARG, "pme_ev_clear_virial: null ev");
+  HIP_TRY(launch_fill_zero(ev->d_buf, kOffEnergy, (hipStream_t)stream));
+  std::memset(ev->h_buf, 0, kOffEnergy);
+  return PME_OK;
+}
+
+extern "C" pme_status pme_ev_copy_to_host(pme_ev_t ev, pme_stream_t stream) {
+  if (!ev) return fail(PME_ERR_INVALID_ARG, "pme_ev_copy_to_host: null ev");
+  HIP_TRY(hipMemcpyAsync(ev->h_buf, ev->d_buf, kEvBytes, hipMemcpyDeviceToHost, (hipStream_t)stream));
+  return PME_OK;
+}
+
+extern "C" pme_status pme_ev_get_energy(pme_ev_t ev, const char* name, double* out) {
+  int slot;
+  PME_TRY(ev_slot(ev, name, &slot));
+  if (!out) return fail(PME_ERR_INVALID_ARG, "pme_ev_get_energy: null out");
+  std::memcpy(out, ev->h_buf + kOffEnergy + slot * sizeof(double), sizeof(double));
+  return PME_OK;
+}
+
+extern "C" pme_status pme_ev_get_virial(pme_ev_t ev, double virmat[9]) {
+  if (!ev || !virmat) return fail(PME_ERR_INVALID_ARG, "pme_ev_get_virial: null argument");
+  std::memcpy(virmat, ev->h_buf + kOffVirmat, 9 * sizeof(double));
+  return PME_OK;
+}
+
+extern "C" pme_status pme_ev_get_sforce(pme_ev_t ev, double sforce[81]) {
+  if (!ev || !sforce) return fail(PME_ERR_INVALID_ARG, "pme_ev_get_sforce: null argument");
+  const double* dp = reinterpret_cast<const double*>(ev->h_buf + kOffSforceDp);
+  const long long* fp = reinterpret_cast<const long long*>(ev->h_buf + kOffSforceFp);
+  for (int i = 0; i < 81; i++) sforce[i] = dp[i] + (double)fp[i] / (double)(1ll << 30);
+  return PME_OK;
+}
+
+extern "C" pme_status pme_ev_energy_pointer(pme_ev_t ev, const char* name, double** dptr) {
+  int slot;
+  PME_TRY(ev_slot(ev, name, &slot));
+  if (!dptr) return fail(PME_ERR_INVALID_ARG, "pme_ev_energy_pointer: null out");
+  *dptr = reinterpret_cast<double*>(ev->d_buf + kOffEnergy) + slot;
+  return PME_OK;
+}
+
+extern "C" pme_status pme_ev_virial_pointer(pme_ev_t ev, double** dptr) {
+  if (!ev || !dptr) return fail(PME_ERR_INVALID_ARG, "pme_ev_virial_pointer: null argument");
+  *dptr = reinterpret_cast<double*>(ev->d_buf + kOffVirmat);
+  return PME_OK;
+}
+
+// -------------------------------------------------------------- host math --
+static const double kPi = 3.14159265358979323846;
+
+// B-spline values at w (FP64), fill_bspline_host CudaPMERecip.cu:2507-2537
+static void bspline_host(int order, double w, double* th) {
+  th[order - 1] = 0.0;
+  th[1] = w;
+  th[0] = 1.0 - w;
+  for (int k = 3; k <= order; k++) {
+    const double div = 1.0 / (double)(k - 1);
+    th[k - 1] = div * w * th[k - 2];
+    for (int j = 1; j <= k - 2; j++) th[k - j - 1] = div * ((w + j) * th[k - j - 2] + (k - j - w) * th[k - j - 1]);
+    th[0] = div * (1.0 - w) * th[0];
+  }
+}
+
+// bsp_mod[k] = 1/|sum_{j=0}^{n-2} arr[j] e^{2 pi i jk/n}|^2 with arr[1..p] the
+// B-spline at the integer nodes, values < 1e-10 replaced by the neighbour
+// mean; dftmod + calc_prefac, CudaPMERecip.cu:2481-2505, 2544-2579.  FP64.
+static void bsp_mod_host(int order, int n, double* out) {
+  std::vector<double> arr(std::max(n, order + 1), 0.0), th(order);
+  bspline_host(order, 0.0, th.data());
+  for (int i = 2; i <= order + 1; i++) arr[i - 1] = th[i - 2];
+  for (int k = 0; k < n; k++) {
+    double s1 = 0.0, s2 = 0.0;
+    for (int j = 0; j < n - 1; j++) {
+      const long long jk = ((long long)j * k) % n;
+      const double arg = 2.0 * kPi * (double)jk / (double)n;
+      s1 += arr[j] * std::cos(arg);
+      s2 += arr[j] * std::sin(arg);
+    }
+    out[k] = s1 * s1 + s2 * s2;
+  }
+  for (int k = 0; k < n; k++)
+    if (out[k] < 1.0e-10) out[k] = 0.5 * (out[(k + n - 1) % n] + out[(k + 1) % n]);
+  for (int k = 0; k < n; k++) out[k] = 1.0 / out[k];
+}
+
+extern "C" pme_status pme_bsp_mod(int order, int n, double* out) {
+  if (order < 3 || order > 16 || n < 2 || !out) return fail(PME_ERR_INVALID_ARG, "pme_bsp_mod: bad arguments");
+  bsp_mod_host(order, n, out);
+  return PME_OK;
+}
+
+static bool smooth235(int n) {
+  if (n < 1) return false;
+  for (int f : {2, 3, 5})
+    while (n % f == 0) n /= f;
+  return n == 1;
+}
+
+// Stockham radices for length n: 8s, then 4, 2, 5, 3 (fewest stages)
+static bool radix_plan(int n, int tstride, RadixPlan* rp) {
+  rp->n = n;
+  rp->tstride = tstride;
+  rp->nrad = 0;
+  int m = n;
+  while (m % 8 == 0 && m > 8) { rp->rad[rp->nrad++] = 8; m /= 8; }
+  while (m % 4 == 0) { rp->rad[rp->nrad++] = 4; m /= 4; }
+  while (m % 2 == 0) { rp->rad[rp->nrad++] = 2; m /= 2; }
+  while (m % 8 == 0) { rp->rad[rp->nrad++] = 8; m /= 8; }
+  while (m % 5 == 0) { rp->rad[rp->nrad++] = 5; m /= 5; }
+  while (m % 3 == 0) { rp->rad[rp->nrad++] = 3; m /= 3; }
+  if (rp->nrad > 12) return false;
+  return m == 1;
+}
+
+// ------------------------------------------------------------------ handle --
+enum GridState {
+  ST_EMPTY = 0,      // nothing computed yet
+  ST_CHARGE,         // real grid holds the charge grid
+  ST_XY,             // spectrum transformed along x and y only
+  ST_SPECTRUM,       // full half spectrum (possibly solved)
+  ST_SOLVED_ZINV,    // solved spectrum, z already inverse transformed
+  ST_POTENTIAL       // real grid holds the potential
+};
+
+struct pme_recip_s {
+  int nx = 0, ny = 0, nz = 0, order = 0, fft_type = PME_FFT_BOX, accum = PME_ACCUM_FLOAT;
+  hipStream_t stream = nullptr;
+  pme_ev_t ev = nullptr;
+  int slot_recip = 0, slot_self = 0;
+
+  float* grid = nullptr;      // real grid nx*ny*nz
+  float2* cgrid = nullptr;    // half spectrum (nx/2+1)*ny*nz
+  float* prefac[3] = {nullptr, nullptr, nullptr};
+  float2* tw[3] = {nullptr, nullptr, nullptr};
+  RadixPlan rp[3];
+  int NR = 16, TXy = 16, TXz = 16;
+
+  // spatial bins
+  int nb[3] = {1, 1, 1}, wmax[3] = {0, 0, 0}, nbins = 0, cap = 0, bins_for = -1;
+  int* bin_count = nullptr;
+  int* bin_saved = nullptr;
+  int* bin_idx = nullptr;
+  int* ovf = nullptr;
+  int* ovf_idx = nullptr;
+  int ovf_cap = 0;
+  unsigned int* tickets = nullptr;  // [0] spread, [1] solve, [2] self energy
+  double* partial = nullptr;
+  int partial_cap = 0;
+
+  int state = ST_EMPTY;
+  const void* bin_key_ptr = nullptr;
+  int bin_key_n = -1;
+  float bin_key_r[3] = {0, 0, 0};
+
+  bool timing = false;
+  hipEvent_t tev[9] = {};
+  float tms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+};
+
+static void free_handle(pme_recip_t h) {
+  if (!h) return;
+  (void)hipFree(h->grid);
+  (void)hipFree(h->cgrid);
+  for (int i = 0; i < 3; i++) {
+    (void)hipFree(h->prefac[i]);
+    (void)hipFree(h->tw[i]);
+  }
+  (void)hipFree(h->bin_count);
+  (void)hipFree(h->bin_saved);
+  (void)hipFree(h->bin_idx);
+  (void)hipFree(h->ovf);
+  (void)hipFree(h->ovf_idx);
+  (void)hipFree(h->tickets);
+  (void)hipFree(h->partial);
+  for (auto& e : h->tev)
+    if (e) (void)hipEventDestroy(e);
+  delete h;
+}
+
+static pme_status upload_prefac(pme_recip_t h) {
+  const int n[3] = {h->nx, h->ny, h->nz};
+  for (int a = 0; a < 3; a++) {
+    std::vector<double> m(n[a]);
+    bsp_mod_host(h->order, n[a], m.data());
+    std::vector<float> f(n[a]);
+    for (int i = 0; i < n[a]; i++) f[i] = (float)m[i];
+    HIP_TRY(hipMemcpy(h->prefac[a], f.data(), sizeof(float) * n[a], hipMemcpyHostToDevice));
+  }
+  return PME_OK;
+}
+
+static const size_t kLdsBudget = 64 * 1024;
+
+// Brick plan: about 16 points per brick edge, every brick at least order-1
+// wide (a stencil then spans at most two bricks per axis), and the spread and
+// gather LDS footprints within kLdsBudget (split the widest axis further).
+static bool plan_bricks(pme_recip_t h) {
+  const int n[3] = {h->nx, h->ny, h->nz};
+  const int target = 16;
+  for (int a = 0; a < 3; a++) {
+    int nb = (n[a] + target / 2) / target;
+    if (nb < 1) nb = 1;
+    while (nb > 1 && n[a] / nb < h->order - 1) nb--;
+    h->nb[a] = nb;
+    h->wmax[a] = (n[a] + nb - 1) / nb;
+  }
+  while (spread_lds_bytes(h->order, h->wmax) > kLdsBudget || gather_lds_bytes(h->order, h->wmax) > kLdsBudget) {
+    int best = -1;
+    for (int a = 0; a < 3; a++)
+      if (n[a] / (h->nb[a] + 1) >= h->order - 1 && (best < 0 || h->wmax[a] > h->wmax[best])) best = a;
+    if (best < 0) return false;
+    h->nb[best]++;
+    h->wmax[best] = (n[best] + h->nb[best] - 1) / h->nb[best];
+  }
+  h->nbins = h->nb[0] * h->nb[1] * h->nb[2];
+  return true;
+}
+
+static pme_status alloc_bin_tables(pme_recip_t h) {
+  (void)hipFree(h->bin_count);
+  (void)hipFree(h->bin_saved);
+  h->bin_count = h->bin_saved = nullptr;
+  HIP_TRY(hipMalloc(&h->bin_count, sizeof(int) * h->nbins));
+  HIP_TRY(hipMalloc(&h->bin_saved, sizeof(int) * h->nbins));
+  HIP_TRY(hipMemset(h->bin_count, 0, sizeof(int) * h->nbins));
+  HIP_TRY(hipMemset(h->bin_saved, 0, sizeof(int) * h->nbins));
+  h->bins_for = -1;  // bin_idx must be re-sized for the new bin count
+  return PME_OK;
+}
+
+// Size the per-bin atom slots for n atoms: mean + 6 sigma (triplet-correlated
+// counts) + 64, rounded up to 64.  Atoms beyond go to the overflow list.
+static pme_status ensure_bins(pme_recip_t h, int n) {
+  if (n <= h->bins_for) return PME_OK;
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  const double mean = (double)n / h->nbins;
+  int cap = (int)(mean + 6.0 * std::sqrt(3.0 * mean + 1.0) + 64.0);
+  cap = (cap + 63) / 64 * 64;
+  (void)hipFree(h->bin_idx);
+  (void)hipFree(h->ovf_idx);
+  h->bin_idx = h->ovf_idx = nullptr;
+  HIP_TRY(hipMalloc(&h->bin_idx, sizeof(int) * (size_t)h->nbins * cap));
+  HIP_TRY(hipMalloc(&h->ovf_idx, sizeof(int) * (size_t)std::max(n, 1)));
+  h->cap = cap;
+  h->ovf_cap = std::max(n, 1);
+  h->bins_for = n;
+  return PME_OK;
+}
+
+static BinArgs bin_args(pme_recip_t h, const double* recip) {
+  BinArgs a;
+  a.nx = h->nx;
+  a.ny = h->ny;
+  a.nz = h->nz;
+  a.r1 = (float)recip[0];
+  a.r2 = (float)recip[4];
+  a.r3 = (float)recip[8];
+  a.nbx = h->nb[0];
+  a.nby = h->nb[1];
+  a.nbz = h->nb[2];
+  a.cap = h->cap;
+  a.bin_count = h->bin_count;
+  a.bin_saved = h->bin_saved;
+  a.bin_idx = h->bin_idx;
+  a.ovf = h->ovf;
+  a.ovf_idx = h->ovf_idx;
+  a.ovf_cap = h->ovf_cap;
+  a.ticket = h->tickets + 0;
+  return a;
+}
+
+static AxisArgs axis_args(pme_recip_t h, int a) {
+  AxisArgs ax;
+  ax.rp = h->rp[a];
+  ax.tw = h->tw[a];
+  ax.ntw = a == 0 ? h->nx : (a == 1 ? h->ny : h->nz);
+  return ax;
+}
+
+static bool is_ortho(const double* r) {
+  return r[1] == 0.0 && r[2] == 0.0 && r[3] == 0.0 && r[5] == 0.0 && r[6] == 0.0 && r[7] == 0.0;
+}
+
+static SolveArgs solve_args(pme_recip_t h, const double* recip, double kappa, bool calc_ev) {
+  SolveArgs s;
+  s.n1 = h->nx;
+  s.n2 = h->ny;
+  s.n3 = h->nz;
+  s.nf1 = h->nx / 2 + (h->nx % 2);
+  s.nf2 = h->ny / 2 + (h->ny % 2);
+  s.nf3 = h->nz / 2 + (h->nz % 2);
+  s.r1 = (float)recip[0];
+  s.r2 = (float)recip[4];
+  s.r3 = (float)recip[8];
+  // CudaPMERecip.cu:2024-2026: inv_vol in FP64, then cast
+  const double inv_vol = recip[0] * recip[4] * recip[8];
+  s.piv_inv = (float)(inv_vol / kPi);
+  s.fac = (float)(kPi * kPi / (kappa * kappa));
+  s.pf1 = h->prefac[0];
+  s.pf2 = h->prefac[1];
+  s.pf3 = h->prefac[2];
+  s.calc_ev = calc_ev ? 1 : 0;
+  s.partial = h->partial;
+  s.ticket = h->tickets + 1;
+  s.energy = reinterpret_cast<double*>(h->ev->d_buf + kOffEnergy) + h->slot_recip;
+  s.virmat = reinterpret_cast<double*>(h->ev->d_buf + kOffVirmat);
+  return s;
+}
+
+static void pick_tiles(pme_recip_t h) {
+  const int hx = h->nx / 2 + 1;
+  const int nf = h->rp[0].n;
+  h->NR = nf <= 64 ? 16 : (nf <= 128 ? 8 : (nf <= 256 ? 4 : 2));
+  while (h->NR > 1 && fft_x_lds_bytes(nf, h->NR, h->nx) > kLdsBudget) h->NR /= 2;
+  auto tile = [&](int n) {
+    int tx = 16;
+    while (tx > 1 && fft_cols_lds_bytes(n, tx) > kLdsBudget) tx /= 2;
+    if (tx > hx) tx = hx;
+    return tx;
+  };
+  h->TXy = tile(h->ny);
+  h->TXz = tile(h->nz);
+}
+
+extern "C" pme_status pme_recip_create(int nfftx, int nffty, int nfftz, int order, int fft_type, int nnode,
+                                       int mynode, int accum_type, pme_ev_t ev, const char* name_recip,
+                                       const char* name_self, pme_stream_t stream, pme_recip_t* out) {
+  if (!out || !ev || !name_recip || !name_self) return fail(PME_ERR_INVALID_ARG, "pme_recip_create: null argument");
+  *out = nullptr;
+  if (fft_type != PME_FFT_BOX && fft_type != PME_FFT_SLAB && fft_type != PME_FFT_COLUMN)
+    return fail(PME_ERR_INVALID_ARG, "pme_recip_create: fft_type %d", fft_type);
+  if (nnode != 1 || mynode != 0)
+    return fail(PME_ERR_UNSUPPORTED, "pme_recip_create: nnode=%d: one GPU per solver", nnode);
+  if (accum_type != PME_ACCUM_FLOAT)
+    return fail(PME_ERR_UNSUPPORTED, "pme_recip_create: accum_type %d not available (float only)", accum_type);
+  if (order != 4 && order != 6 && order != 8)
+    return fail(PME_ERR_UNSUPPORTED, "pme_recip_create: order %d (4, 6 or 8)", order);
+  const int n[3] = {nfftx, nffty, nfftz};
+  for (int a = 0; a < 3; a++) {
+    if (n[a] < order || n[a] > 512 || !smooth235(n[a]))
+      return fail(PME_ERR_UNSUPPORTED, "pme_recip_create: grid edge %d (need order <= n <= 512, 2/3/5-smooth)", n[a]);
+  }
+  pme_recip_t h = new pme_recip_s();
+  h->nx = nfftx;
+  h->ny = nffty;
+  h->nz = nfftz;
+  h->order = order;
+  h->fft_type = fft_type;
+  h->accum = accum_type;
+  h->stream = (hipStream_t)stream;
+  h->ev = ev;
+  // CudaPMERecip.cu:1426-1429
+  pme_status st = pme_ev_insert(ev, name_recip);
+  if (st == PME_OK) st = pme_ev_insert(ev, name_self);
+  if (st == PME_OK) st = ev_slot(ev, name_recip, &h->slot_recip);
+  if (st == PME_OK) st = ev_slot(ev, name_self, &h->slot_self);
+  if (st != PME_OK) {
+    free_handle(h);
+    return st;
+  }
+  auto init = [&]() -> pme_status {
+    const size_t G = (size_t)h->nx * h->ny * h->nz;
+    const size_t Gc = (size_t)(h->nx / 2 + 1) * h->ny * h->nz;
+    HIP_TRY(hipMalloc(&h->grid, sizeof(float) * G));
+    HIP_TRY(hipMalloc(&h->cgrid, sizeof(float2) * Gc));
+    for (int a = 0; a < 3; a++) {
+      HIP_TRY(hipMalloc(&h->prefac[a], sizeof(float) * n[a]));
+      HIP_TRY(hipMalloc(&h->tw[a], sizeof(float2) * n[a]));
+      std::vector<float2> t(n[a]);
+      for (int k = 0; k < n[a]; k++) {
+        const double ang = 2.0 * kPi * (double)k / (double)n[a];
+        t[k] = make_float2((float)std::cos(ang), (float)(-std::sin(ang)));
+      }
+      HIP_TRY(hipMemcpy(h->tw[a], t.data(), sizeof(float2) * n[a], hipMemcpyHostToDevice));
+    }
+    const bool even = (h->nx % 2) == 0;
+    if (!radix_plan(even ? h->nx / 2 : h->nx, even ? 2 : 1, &h->rp[0]) || !radix_plan(h->ny, 1, &h->rp[1]) ||
+        !radix_plan(h->nz, 1, &h->rp[2]))
+      return fail(PME_ERR_UNSUPPORTED, "pme_recip_create: no radix plan");
+    pick_tiles(h);
+    PME_TRY(upload_prefac(h));
+    if (!plan_bricks(h)) return fail(PME_ERR_UNSUPPORTED, "pme_recip_create: no brick plan fits the LDS budget");
+    PME_TRY(alloc_bin_tables(h));
+    HIP_TRY(hipMalloc(&h->ovf, sizeof(int) * 2));
+    HIP_TRY(hipMemset(h->ovf, 0, sizeof(int) * 2));
+    HIP_TRY(hipMalloc(&h->tickets, sizeof(unsigned int) * 8));
+    HIP_TRY(hipMemset(h->tickets, 0, sizeof(unsigned int) * 8));
+    const int hx = h->nx / 2 + 1;
+    const int zblocks = ((hx + h->TXz - 1) / h->TXz) * h->ny;
+    const int sblocks = (int)((Gc + kBlock - 1) / kBlock);
+    h->partial_cap = std::max(std::max(zblocks, sblocks), 1024);
+    HIP_TRY(hipMalloc(&h->partial, sizeof(double) * 7 * (size_t)h->partial_cap));
+    for (auto& e : h->tev) HIP_TRY(hipEventCreate(&e));
+    return PME_OK;
+  };
+  st = init();
+  if (st != PME_OK) {
+    free_handle(h);
+    return st;
+  }
+  *out = h;
+  return PME_OK;
+}
+
+extern "C" pme_status pme_recip_destroy(pme_recip_t h) {
+  if (h) (void)hipStreamSynchronize(h->stream);
+  free_handle(h);
+  return PME_OK;
+}
+
+extern "C" pme_status pme_recip_set_stream(pme_recip_t h, pme_stream_t stream) {
+  if (!h) return fail(PME_ERR_INVALID_ARG, "pme_recip_set_stream: null handle");
+  h->stream = (hipStream_t)stream;
+  return PME_OK;
+}
+
+extern "C" pme_status pme_recip_set_order(pme_recip_t h, int order) {
+  if (!h) return fail(PME_ERR_INVALID_ARG, "pme_recip_set_order: null handle");
+  if (order != 4 && order != 6 && order != 8) return fail(PME_ERR_UNSUPPORTED, "pme_recip_set_order: order %d", order);
+  if (h->nx < order || h->ny < order || h->nz < order)
+    return fail(PME_ERR_UNSUPPORTED, "pme_recip_set_order: grid smaller than order %d", order);
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  h->order = order;
+  PME_TRY(upload_prefac(h));
+  const int old_nbins = h->nbins;
+  if (!plan_bricks(h)) return fail(PME_ERR_UNSUPPORTED, "pme_recip_set_order: no brick plan fits the LDS budget");
+  if (h->nbins != old_nbins) PME_TRY(alloc_bin_tables(h));
+  h->bin_key_n = -1;
+  h->state = ST_EMPTY;
+  return PME_OK;
+}
+
+extern "C" pme_status pme_recip_get_info(pme_recip_t h, int* nx, int* ny, int* nz, int* order) {
+  if (!h) return fail(PME_ERR_INVALID_ARG, "pme_recip_get_info: null handle");
+  if (nx) *nx = h->nx;
+  if (ny) *ny = h->ny;
+  if (nz) *nz = h->nz;
+  if (order) *order = h->order;
+  return PME_OK;
+}
+
+static void mark(pme_recip_t h, int i) {
+  if (h->timing) (void)hipEventRecord(h->tev[i], h->stream);
+}
+
+static pme_status do_bin(pme_recip_t h, const void* xyzq, int n, const double* recip, bool finalize) {
+  PME_TRY(ensure_bins(h, n));
+  BinArgs a = bin_args(h, recip);
+  HIP_TRY(launch_bin_atoms((const float4*)xyzq, n, a, h->stream));
+  if (finalize) HIP_TRY(launch_bins_finalize(a, h->stream));
+  h->bin_key_ptr = xyzq;
+  h->bin_key_n = n;
+  h->bin_key_r[0] = a.r1;
+  h->bin_key_r[1] = a.r2;
+  h->bin_key_r[2] = a.r3;
+  return PME_OK;
+}
+
+extern "C" pme_status pme_recip_spread_charge(pme_recip_t h, const void* d_xyzq, int ncoord, const double recip[9]) {
+  if (!h || !recip || (ncoord > 0 && !d_xyzq)) return fail(PME_ERR_INVALID_ARG, "pme_recip_spread_charge: null argument");
+  if (ncoord < 0) return fail(PME_ERR_INVALID_ARG, "pme_recip_spread_charge: ncoord %d", ncoord);
+  mark(h, 0);
+  PME_TRY(do_bin(h, d_xyzq, ncoord, recip, false));
+  mark(h, 1);
+  BinArgs a = bin_args(h, recip);
+  HIP_TRY(launch_spread(h->order, (const float4*)d_xyzq, a, h->wmax, h->grid, h->stream));
+  h->state = ST_CHARGE;
+  return PME_OK;
+}
+
+extern "C" pme_status pme_recip_r2c_fft(pme_recip_t h) {
+  if (!h) return fail(PME_ERR_INVALID_ARG, "pme_recip_r2c_fft: null handle");
+  if (h->state != ST_CHARGE && h->state != ST_POTENTIAL)
+    return fail(PME_ERR_STATE, "pme_recip_r2c_fft: no real grid (call spread_charge first)");
+  const int hx = h->nx / 2 + 1;
+  SolveArgs dummy{};
+  mark(h, 2);
+  HIP_TRY(launch_fft_x_r2c(h->grid, h->cgrid, h->nx, h->ny * h->nz, h->NR, axis_args(h, 0), h->stream));
+  mark(h, 3);
+  HIP_TRY(launch_fft_cols(COL_FWD, h->cgrid, h->ny, hx, h->TXy, hx, h->nz, (long long)hx * h->ny, axis_args(h, 1),
+                          dummy, h->stream));
+  h->state = ST_XY;
+  return PME_OK;
+}
+
+static pme_status z_pass(pme_recip_t h, int mode, const SolveArgs& sa) {
+  const int hx = h->nx / 2 + 1;
+  HIP_TRY(launch_fft_cols(mode, h->cgrid, h->nz, hx, h->TXz, (long long)hx * h->ny, h->ny, hx, axis_args(h, 2), sa,
+                          h->stream));
+  return PME_OK;
+}
+
+extern "C" pme_status pme_recip_scalar_sum(pme_recip_t h, const double recip[9], double kappa, int calc_energy,
+                                           int calc_virial) {
+  if (!h || !recip) return fail(PME_ERR_INVALID_ARG, "pme_recip_scalar_sum: null argument");
+  if (!is_ortho(recip))
+    return fail(PME_ERR_UNSUPPORTED, "pme_recip_scalar_sum: only orthorhombic boxes are supported");
+  if (!(kappa > 0.0)) return fail(PME_ERR_INVALID_ARG, "pme_recip_scalar_sum: kappa %g", kappa);
+  const bool ev = calc_energy || calc_virial;
+  SolveArgs sa = solve_args(h, recip, kappa, ev);
+  const int hx = h->nx / 2 + 1;
+  mark(h, 4);
+  if (h->state == ST_XY) {
+    PME_TRY(z_pass(h, COL_FWD_SOLVE_INV, sa));
+    h->state = ST_SOLVED_ZINV;
+  } else if (h->state == ST_SPECTRUM || h->state == ST_SOLVED_ZINV) {
+    if (h->state == ST_SOLVED_ZINV) PME_TRY(z_pass(h, COL_FWD, sa));
+    HIP_TRY(launch_scalar_sum(h->cgrid, hx, sa, h->stream));
+    h->state = ST_SPECTRUM;
+  } else {
+    return fail(PME_ERR_STATE, "pme_recip_scalar_sum: no spectrum (call r2c_fft first)");
+  }
+  return PME_OK;
+}
+
+extern "C" pme_status pme_recip_c2r_fft(pme_recip_t h) {
+  if (!h) return fail(PME_ERR_INVALID_ARG, "pme_recip_c2r_fft: null handle");
+  const int hx = h->nx / 2 + 1;
+  SolveArgs dummy{};
+  if (h->state == ST_XY) {
+    PME_TRY(z_pass(h, COL_FWD_INV, dummy));
+  } else if (h->state == ST_SPECTRUM) {
+    PME_TRY(z_pass(h, COL_INV, dummy));
+  } else if (h->state != ST_SOLVED_ZINV) {
+    return fail(PME_ERR_STATE, "pme_recip_c2r_fft: no spectrum (call r2c_fft first)");
+  }
+  mark(h, 5);
+  HIP_TRY(launch_fft_cols(COL_INV, h->cgrid, h->ny, hx, h->TXy, hx, h->nz, (long long)hx * h->ny, axis_args(h, 1),
+                          dummy, h->stream));
+  mark(h, 6);
+  HIP_TRY(launch_fft_x_c2r(h->cgrid, h->grid, h->nx, h->ny * h->nz, h->NR, axis_args(h, 0), h->stream));
+  h->state = ST_POTENTIAL;
+  return PME_OK;
+}
+
+extern "C" pme_status pme_recip_gather_force(pme_recip_t h, const void* d_xyzq, int ncoord, const double recip[9],
+                                             int stride, int force_type, void* d_force) {
+  if (!h || !recip || (ncoord > 0 && (!d_xyzq || !d_force)))
+    return fail(PME_ERR_INVALID_ARG, "pme_recip_gather_force: null argument");
+  if (ncoord < 0) return fail(PME_ERR_INVALID_ARG, "pme_recip_gather_force: ncoord %d", ncoord);
+  if (force_type < PME_FORCE_FLOAT || force_type > PME_FORCE_INT64)
+    return fail(PME_ERR_INVALID_ARG, "pme_recip_gather_force: force_type %d", force_type);
+  if (force_type != PME_FORCE_FLOAT3 && stride < ncoord)
+    return fail(PME_ERR_INVALID_ARG, "pme_recip_gather_force: stride %d < ncoord %d", stride, ncoord);
+  if (!is_ortho(recip))
+    return fail(PME_ERR_UNSUPPORTED, "pme_recip_gather_force: only orthorhombic boxes are supported");
+  if (h->state != ST_POTENTIAL) return fail(PME_ERR_STATE, "pme_recip_gather_force: no potential (call c2r_fft first)");
+  if (ncoord == 0) return PME_OK;
+  const float r[3] = {(float)recip[0], (float)recip[4], (float)recip[8]};
+  if (d_xyzq != h->bin_key_ptr || ncoord != h->bin_key_n || r[0] != h->bin_key_r[0] || r[1] != h->bin_key_r[1] ||
+      r[2] != h->bin_key_r[2]) {
+    PME_TRY(do_bin(h, d_xyzq, ncoord, recip, true));
+  }
+  mark(h, 7);
+  BinArgs a = bin_args(h, recip);
+  HIP_TRY(launch_gather(h->order, force_type, (const float4*)d_xyzq, a, h->wmax, h->grid, stride, d_force, h->stream));
+  mark(h, 8);
+  return PME_OK;
+}
+
+extern "C" pme_status pme_recip_calc_self_energy(pme_recip_t h, const void* d_xyzq, int ncoord, double kappa) {
+  if (!h || (ncoord > 0 && !d_xyzq)) return fail(PME_ERR_INVALID_ARG, "pme_recip_calc_self_energy: null argument");
+  if (ncoord < 0) return fail(PME_ERR_INVALID_ARG, "pme_recip_calc_self_energy: ncoord %d", ncoord);
+  if (ncoord == 0) return PME_OK;
+  double* slot = reinterpret_cast<double*>(h->ev->d_buf + kOffEnergy) + h->slot_self;
+  const double kcs = kappa * kCcelec / std::sqrt(kPi);
+  HIP_TRY(launch_self_energy((const float4*)d_xyzq, ncoord, kcs, h->partial, h->tickets + 2, slot, h->stream));
+  return PME_OK;
+}
+
+extern "C" pme_status pme_recip_solve(pme_recip_t h, const void* d_xyzq, int ncoord, const double recip[9],
+                                      double kappa, int calc_energy, int calc_virial, int stride, int force_type,
+                                      void* d_force) {
+  if (!h || !recip) return fail(PME_ERR_INVALID_ARG, "pme_recip_solve: null argument");
+  if (!is_ortho(recip)) return fail(PME_ERR_UNSUPPORTED, "pme_recip_solve: only orthorhombic boxes are supported");
+  PME_TRY(pme_recip_spread_charge(h, d_xyzq, ncoord, recip));
+  PME_TRY(pme_recip_r2c_fft(h));
+  PME_TRY(pme_recip_scalar_sum(h, recip, kappa, calc_energy, calc_virial));
+  PME_TRY(pme_recip_c2r_fft(h));
+  PME_TRY(pme_recip_gather_force(h, d_xyzq, ncoord, recip, stride, force_type, d_force));
+  return PME_OK;
+}
+
+extern "C" pme_status pme_recip_copy_grid(pme_recip_t h, int which, void* d_dst, size_t bytes) {
+  if (!h || !d_dst) return fail(PME_ERR_INVALID_ARG, "pme_recip_copy_grid: null argument");
+  const size_t G = (size_t)h->nx * h->ny * h->nz;
+  const size_t Gc = (size_t)(h->nx / 2 + 1) * h->ny * h->nz;
+  if (which == PME_GRID_REAL) {
+    if (h->state != ST_CHARGE && h->state != ST_POTENTIAL)
+      return fail(PME_ERR_STATE, "pme_recip_copy_grid: no real grid");
+    if (bytes < G * sizeof(float)) return fail(PME_ERR_INVALID_ARG, "pme_recip_copy_grid: buffer too small");
+    HIP_TRY(hipMemcpyAsync(d_dst, h->grid, G * sizeof(float), hipMemcpyDeviceToDevice, h->stream));
+  } else if (which == PME_GRID_SPECTRUM) {
+    if (h->state == ST_XY || h->state == ST_SOLVED_ZINV) {
+      SolveArgs dummy{};
+      PME_TRY(z_pass(h, COL_FWD, dummy));
+      h->state = ST_SPECTRUM;
+    }
+    if (h->state != ST_SPECTRUM) return fail(PME_ERR_STATE, "pme_recip_copy_grid: no spectrum");
+    if (bytes < Gc * sizeof(float2)) return fail(PME_ERR_INVALID_ARG, "pme_recip_copy_grid: buffer too small");
+    HIP_TRY(hipMemcpyAsync(d_dst, h->cgrid, Gc * sizeof(float2), hipMemcpyDeviceToDevice, h->stream));
+  } else {
+    return fail(PME_ERR_INVALID_ARG, "pme_recip_copy_grid: which %d", which);
+  }
+  return PME_OK;
+}
+
+extern "C" pme_status pme_recip_set_timing(pme_recip_t h, int enable) {
+  if (!h) return fail(PME_ERR_INVALID_ARG, "pme_recip_set_timing: null handle");
+  h->timing = enable != 0;
+  return PME_OK;
+}
+
+extern "C" pme_status pme_recip_get_timing(pme_recip_t h, float ms[8]) {
+  if (!h || !ms) return fail(PME_ERR_INVALID_ARG, "pme_recip_get_timing: null argument");
+  if (!h->timing) return fail(PME_ERR_STATE, "pme_recip_get_timing: timing not enabled");
+  HIP_TRY(hipEventSynchronize(h->tev[8]));
+  for (int i = 0; i < 8; i++) HIP_TRY(hipEventElapsedTime(&ms[i], h->tev[i], h->tev[i + 1]));
+  return PME_OK;
+}
+
+extern "C" pme_status pme_fill_bspline(int order, int nfftx, int nffty, int nfftz, const double recip[9],
+                                       const void* d_xyzq, int ncoord, int* d_gix, int* d_giy, int* d_giz,
+                                       float* d_charge, float* d_thetax, float* d_thetay, float* d_thetaz,
+                                       float* d_dthetax, float* d_dthetay, float* d_dthetaz, pme_stream_t stream) {
+  if (!recip || ncoord < 0) return fail(PME_ERR_INVALID_ARG, "pme_fill_bspline: bad arguments");
+  if (order != 4 && order != 6 && order != 8) return fail(PME_ERR_UNSUPPORTED, "pme_fill_bspline: order %d", order);
+  if (nfftx < 1 || nffty < 1 || nfftz < 1) return fail(PME_ERR_INVALID_ARG, "pme_fill_bspline: grid");
+  if (ncoord == 0) return PME_OK;
+  if (!d_xyzq || !d_gix || !d_giy || !d_giz || !d_charge || !d_thetax || !d_thetay || !d_thetaz || !d_dthetax ||
+      !d_dthetay || !d_dthetaz)
+    return fail(PME_ERR_INVALID_ARG, "pme_fill_bspline: null device pointer");
+  HIP_TRY(launch_fill_bspline(order, (const float4*)d_xyzq, ncoord, (float)recip[0], (float)recip[4], (float)recip[8],
+                              nfftx, nffty, nfftz, d_gix, d_giy, d_giz, d_charge, d_thetax, d_thetay, d_thetaz,
+                              d_dthetax, d_dthetay, d_dthetaz, (hipStream_t)stream));
+  return PME_OK;
+}

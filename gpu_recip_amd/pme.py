@@ -1,0 +1,153 @@
+"""Python mirror of the reference's CudaPMERecip / CudaEnergyVirial interface
+(CudaPMERecip.h:166-203, CudaEnergyVirial.h:20-53) over the C ABI.
+
+Device buffers are torch tensors on a ROCm device (plumbing only); every
+compute call goes through libpme_hip.so.  Method names and argument meaning
+follow the reference; errors raise PmeError instead of exit(1).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+from . import _lib as L
+
+
+def _ptr(t) -> int:
+    return 0 if t is None else int(t.data_ptr())
+
+
+def _stream_ptr(stream) -> int:
+    if stream is None:
+        return 0
+    if isinstance(stream, int):
+        return stream
+    return int(stream.cuda_stream)
+
+
+class CudaEnergyVirial:
+    """Named FP64 energy slots + 3x3 virial (CudaEnergyVirial.h:20-53)."""
+
+    def __init__(self):
+        h = C.c_void_p()
+        L.call("pme_ev_create", C.byref(h))
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None) is not None and L._lib is not None:
+            L._lib.pme_ev_destroy(self.h)
+            self.h = None
+
+    def insert(self, name: str) -> None:
+        L.call("pme_ev_insert", self.h, name.encode())
+
+    def clear(self, stream=None) -> None:
+        L.call("pme_ev_clear", self.h, _stream_ptr(stream))
+
+    def clearEnergy(self, stream=None) -> None:
+        L.call("pme_ev_clear_energy", self.h, _stream_ptr(stream))
+
+    def clearVirial(self, stream=None) -> None:
+        L.call("pme_ev_clear_virial", self.h, _stream_ptr(stream))
+
+    def copyToHost(self, stream=None) -> None:
+        L.call("pme_ev_copy_to_host", self.h, _stream_ptr(stream))
+
+    def getEnergy(self, name: str) -> float:
+        out = C.c_double()
+        L.call("pme_ev_get_energy", self.h, name.encode(), C.byref(out))
+        return out.value
+
+    def getVirial(self) -> list[float]:
+        v = (C.c_double * 9)()
+        L.call("pme_ev_get_virial", self.h, v)
+        return list(v)
+
+
+class CudaPMERecip:
+    """The reciprocal-space solver (CudaPMERecip<AT,CT,CT2>).
+
+    xyzq: device float32 tensor (ncoord, 4); recip: 9 floats (row-major,
+    orthorhombic); forces: device tensors laid out per force_type.
+    """
+
+    def __init__(self, nfftx, nffty, nfftz, order, energyVirial: CudaEnergyVirial,
+                 nameRecip="recip", nameSelf="self", fft_type=L.FFT_BOX, nnode=1, mynode=0,
+                 accum_type=L.ACCUM_FLOAT, stream=None):
+        h = C.c_void_p()
+        L.call("pme_recip_create", nfftx, nffty, nfftz, order, fft_type, nnode, mynode, accum_type,
+               energyVirial.h, nameRecip.encode(), nameSelf.encode(), _stream_ptr(stream), C.byref(h))
+        self.h = h
+        self.ev = energyVirial  # keep alive
+        self.nfft = (nfftx, nffty, nfftz)
+
+    def __del__(self):
+        if getattr(self, "h", None) is not None and L._lib is not None:
+            L._lib.pme_recip_destroy(self.h)
+            self.h = None
+
+    def set_stream(self, stream) -> None:
+        L.call("pme_recip_set_stream", self.h, _stream_ptr(stream))
+
+    def set_order(self, order: int) -> None:
+        L.call("pme_recip_set_order", self.h, order)
+
+    def get_info(self):
+        v = [C.c_int() for _ in range(4)]
+        L.call("pme_recip_get_info", self.h, *[C.byref(x) for x in v])
+        return tuple(x.value for x in v)
+
+    def spread_charge(self, xyzq, ncoord, recip) -> None:
+        L.call("pme_recip_spread_charge", self.h, _ptr(xyzq), int(ncoord), L.dbl9(recip))
+
+    def r2c_fft(self) -> None:
+        L.call("pme_recip_r2c_fft", self.h)
+
+    def scalar_sum(self, recip, kappa, calc_energy, calc_virial) -> None:
+        L.call("pme_recip_scalar_sum", self.h, L.dbl9(recip), float(kappa), int(bool(calc_energy)),
+               int(bool(calc_virial)))
+
+    def c2r_fft(self) -> None:
+        L.call("pme_recip_c2r_fft", self.h)
+
+    def gather_force(self, xyzq, ncoord, recip, stride, force, force_type=L.FORCE_FLOAT) -> None:
+        L.call("pme_recip_gather_force", self.h, _ptr(xyzq), int(ncoord), L.dbl9(recip), int(stride),
+               int(force_type), _ptr(force))
+
+    def calc_self_energy(self, xyzq, ncoord, kappa) -> None:
+        L.call("pme_recip_calc_self_energy", self.h, _ptr(xyzq), int(ncoord), float(kappa))
+
+    def solve(self, xyzq, ncoord, recip, kappa, calc_energy, calc_virial, stride, force,
+              force_type=L.FORCE_FLOAT) -> None:
+        L.call("pme_recip_solve", self.h, _ptr(xyzq), int(ncoord), L.dbl9(recip), float(kappa),
+               int(bool(calc_energy)), int(bool(calc_virial)), int(stride), int(force_type), _ptr(force))
+
+    def copy_grid(self, which, dst) -> None:
+        L.call("pme_recip_copy_grid", self.h, int(which), _ptr(dst), int(dst.numel() * dst.element_size()))
+
+    def set_timing(self, enable: bool) -> None:
+        L.call("pme_recip_set_timing", self.h, int(bool(enable)))
+
+    def get_timing(self):
+        ms = (C.c_float * 8)()
+        L.call("pme_recip_get_timing", self.h, ms)
+        return list(ms)
+
+
+def force_stride(ncoord: int, elem_size: int = 4) -> int:
+    """Force<T>::realloc stride (Force.h:63-69): aligned to 256-byte boundaries."""
+    return (((ncoord - 1 + 32) * elem_size - 1) // 256 + 1) * 256 // elem_size
+
+
+def fill_bspline(order, nfft, recip, xyzq, ncoord, stream=None):
+    """Bspline<float>::fill_bspline (Bspline.cu:203-231): device arrays
+    (gix, giy, giz, charge, thetax, thetay, thetaz, dthetax, dthetay, dthetaz)."""
+    import torch
+
+    dev = xyzq.device
+    gi = [torch.empty(ncoord, dtype=torch.int32, device=dev) for _ in range(3)]
+    q = torch.empty(ncoord, dtype=torch.float32, device=dev)
+    th = [torch.empty(ncoord * order, dtype=torch.float32, device=dev) for _ in range(6)]
+    L.call("pme_fill_bspline", int(order), int(nfft[0]), int(nfft[1]), int(nfft[2]), L.dbl9(recip),
+           _ptr(xyzq), int(ncoord), *[_ptr(t) for t in gi], _ptr(q), *[_ptr(t) for t in th],
+           _stream_ptr(stream))
+    return (*gi, q, *th)

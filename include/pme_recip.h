@@ -1,0 +1,154 @@
+/*
+ * pme_recip.h -- C ABI of the MI355X (gfx950) PME reciprocal-space engine.
+ *
+ * Drop-in boundary for the CudaPMERecip<AT,CT,CT2> / CudaEnergyVirial call
+ * surface of ap-hynninen/gpu_recip.  Each entry point names the reference
+ * member it replaces (file:line in that repository).  Plain pointers and sizes
+ * only: device pointers are HIP device addresses, streams are hipStream_t
+ * passed as void*.
+ *
+ * Deliberate deviations from the reference (which prints and calls exit(1) on
+ * every error): every function returns a pme_status and pme_last_error()
+ * gives the message of the calling thread's last failure.
+ *
+ * Threading: one handle = one stream, not re-entrant (as the reference).
+ */
+#ifndef PME_RECIP_H
+#define PME_RECIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum pme_status {
+  PME_OK = 0,
+  PME_ERR_INVALID_ARG = 1,   /* null pointer, negative count, bad enum        */
+  PME_ERR_UNSUPPORTED = 2,   /* order/grid/box the engine does not handle     */
+  PME_ERR_OOM = 3,           /* device or host allocation failed              */
+  PME_ERR_BUDGET = 4,        /* request exceeds the configured HBM budget     */
+  PME_ERR_DEVICE = 5,        /* HIP runtime error                             */
+  PME_ERR_STATE = 6,         /* stage called out of order (e.g. gather before c2r) */
+  PME_ERR_NOT_FOUND = 7      /* unknown energy term name                      */
+} pme_status;
+
+typedef void* pme_stream_t;                       /* hipStream_t */
+typedef struct pme_energy_virial_s* pme_ev_t;     /* CudaEnergyVirial */
+typedef struct pme_recip_s* pme_recip_t;          /* CudaPMERecip<AT,CT,CT2> */
+
+/* FFTtype, CudaPMERecip.h:30.  Single GPU: all three give the same results;
+ * intermediate grid views are always returned in the BOX layout. */
+enum { PME_FFT_COLUMN = 0, PME_FFT_SLAB = 1, PME_FFT_BOX = 2 };
+
+/* AT template argument (accumulation type of the charge grid),
+ * CudaPMERecip.cu:2584-2600. */
+enum { PME_ACCUM_FLOAT = 0, PME_ACCUM_INT32 = 1, PME_ACCUM_INT64 = 2 };
+
+/* FT template argument of gather_force, CudaPMERecip.cu:985-1023 */
+enum {
+  PME_FORCE_FLOAT = 0,   /* float*: strided SoA store  f[i], f[i+stride], f[i+2 stride] */
+  PME_FORCE_FLOAT3 = 1,  /* float3*: AoS store (stride ignored)                          */
+  PME_FORCE_INT64 = 2    /* long long*: fixed point 2^40, ADDED atomically, strided SoA */
+};
+
+/* grid views, CudaPMERecip.h:41-64 (charge_grid / fft_grid / solved_grid) */
+enum { PME_GRID_REAL = 0, PME_GRID_SPECTRUM = 1 };
+
+const char* pme_last_error(void);
+const char* pme_status_string(pme_status s);
+
+/* ---------------- energy / virial buffer: CudaEnergyVirial ----------------
+ * Device blob laid out as the reference's Virial_t (CudaEnergyVirial.h:13-18):
+ * double sforce_dp[27][3]; long long sforce_fp[27][3]; double virmat[9];
+ * then one double per energy term in insertion order. */
+pme_status pme_ev_create(pme_ev_t* out);                                  /* CudaEnergyVirial.cu:139-145 */
+pme_status pme_ev_destroy(pme_ev_t ev);                                   /* :150-153 */
+pme_status pme_ev_insert(pme_ev_t ev, const char* name);                  /* EnergyVirial.cpp:16-27 */
+pme_status pme_ev_clear(pme_ev_t ev, pme_stream_t stream);                /* CudaEnergyVirial.cu:172-176 */
+pme_status pme_ev_clear_energy(pme_ev_t ev, pme_stream_t stream);         /* :181-187 */
+pme_status pme_ev_clear_virial(pme_ev_t ev, pme_stream_t stream);         /* :192-197 */
+pme_status pme_ev_copy_to_host(pme_ev_t ev, pme_stream_t stream);         /* :223-226 (async) */
+pme_status pme_ev_get_energy(pme_ev_t ev, const char* name, double* out); /* :245-254 (host copy) */
+pme_status pme_ev_get_virial(pme_ev_t ev, double virmat[9]);              /* :261-265 (host copy) */
+pme_status pme_ev_get_sforce(pme_ev_t ev, double sforce[81]);             /* :270-278 */
+pme_status pme_ev_energy_pointer(pme_ev_t ev, const char* name, double** dptr); /* :231-234 */
+pme_status pme_ev_virial_pointer(pme_ev_t ev, double** dptr);             /* :239-242 */
+
+/* ---------------- the reciprocal solver: CudaPMERecip ---------------- */
+
+/* CudaPMERecip(nfftx, nffty, nfftz, order, fft_type, nnode, mynode,
+ *              energyVirial, nameRecip, nameSelf, stream), CudaPMERecip.cu:1412-1493.
+ * order in {4, 6, 8}; every grid edge >= order and 2/3/5-smooth, <= 512;
+ * nnode must be 1 (one GPU). */
+pme_status pme_recip_create(int nfftx, int nffty, int nfftz, int order, int fft_type,
+                            int nnode, int mynode, int accum_type, pme_ev_t ev,
+                            const char* name_recip, const char* name_self,
+                            pme_stream_t stream, pme_recip_t* out);
+pme_status pme_recip_destroy(pme_recip_t h);                              /* :1645-1707 */
+pme_status pme_recip_set_stream(pme_recip_t h, pme_stream_t stream);      /* :1618-1638 */
+pme_status pme_recip_set_order(pme_recip_t h, int order);                 /* :2476-2479 */
+pme_status pme_recip_get_info(pme_recip_t h, int* nfftx, int* nffty, int* nfftz,
+                              int* order);                                /* CudaPMERecip.h:199-202 */
+
+/* spread_charge(const float4* xyzq, int ncoord, const double* recip),
+ * CudaPMERecip.cu:1797-1923.  d_xyzq: device float4[ncoord]; recip: host
+ * double[9], row-major, must be diagonal (orthorhombic). */
+pme_status pme_recip_spread_charge(pme_recip_t h, const void* d_xyzq, int ncoord,
+                                   const double recip[9]);
+pme_status pme_recip_r2c_fft(pme_recip_t h);                              /* :2352-2427 */
+/* scalar_sum(recip, kappa, calc_energy, calc_virial), :1929-2047.  Energy is
+ * ADDED to the name_recip slot, the virial ADDED into virmat. */
+pme_status pme_recip_scalar_sum(pme_recip_t h, const double recip[9], double kappa,
+                                int calc_energy, int calc_virial);
+pme_status pme_recip_c2r_fft(pme_recip_t h);                              /* :2433-2470 */
+/* gather_force<FT>(xyzq, ncoord, recip, stride, force), :2136-2252.  Output is
+ * the gradient +dE/dr.  force_type: PME_FORCE_*. */
+pme_status pme_recip_gather_force(pme_recip_t h, const void* d_xyzq, int ncoord,
+                                  const double recip[9], int stride, int force_type,
+                                  void* d_force);
+/* calc_self_energy(xyzq, ncoord, kappa), :2053-2061: ADDS -kappa*ccelec/sqrt(pi)*sum q^2
+ * to the name_self slot. */
+pme_status pme_recip_calc_self_energy(pme_recip_t h, const void* d_xyzq, int ncoord,
+                                      double kappa);
+
+/* Whole solve in one call: spread -> r2c -> scalar_sum -> c2r -> gather
+ * (the sequence of CudaDomdecRecip::calc, CudaDomdecRecip.h:50-57, without
+ * the self energy).  Same results as the five calls. */
+pme_status pme_recip_solve(pme_recip_t h, const void* d_xyzq, int ncoord,
+                           const double recip[9], double kappa, int calc_energy,
+                           int calc_virial, int stride, int force_type, void* d_force);
+
+/* Copy a grid view (device to device, on the handle's stream).
+ * PME_GRID_REAL: charge grid after spread_charge, potential after c2r_fft,
+ *   float[nfftz][nffty][nfftx].
+ * PME_GRID_SPECTRUM: half spectrum after r2c_fft / scalar_sum,
+ *   float2[nfftz][nffty][nfftx/2+1]. */
+pme_status pme_recip_copy_grid(pme_recip_t h, int which, void* d_dst, size_t bytes);
+
+/* Per-stage device time of the last solve stages (ms, from HIP events) if
+ * enabled with pme_recip_set_timing(h, 1).  Order: bin, spread, fft_x_r2c,
+ * fft_y_fwd, fft_z_solve, fft_y_inv, fft_x_c2r, gather. */
+pme_status pme_recip_set_timing(pme_recip_t h, int enable);
+pme_status pme_recip_get_timing(pme_recip_t h, float ms[8]);
+
+/* ---------------- B-spline precompute: Bspline<float> ----------------
+ * fill_bspline(xyzq, ncoord), Bspline.cu:203-231 (the reference has order 4
+ * only): grid indices and theta/dtheta per atom, with the exact index formula
+ * the spread and gather kernels use (CudaPMERecip.cu:512-529).
+ * Outputs are device arrays: gi{x,y,z}[ncoord], charge[ncoord],
+ * theta{x,y,z}[ncoord*order], dtheta{x,y,z}[ncoord*order] (atom-major). */
+pme_status pme_fill_bspline(int order, int nfftx, int nffty, int nfftz,
+                            const double recip[9], const void* d_xyzq, int ncoord,
+                            int* d_gix, int* d_giy, int* d_giz, float* d_charge,
+                            float* d_thetax, float* d_thetay, float* d_thetaz,
+                            float* d_dthetax, float* d_dthetay, float* d_dthetaz,
+                            pme_stream_t stream);
+
+/* Host FP64 bsp_mod table (calc_prefac/dftmod, CudaPMERecip.cu:2481-2579). */
+pme_status pme_bsp_mod(int order, int n, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PME_RECIP_H */
