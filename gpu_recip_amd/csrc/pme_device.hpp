@@ -6,6 +6,7 @@
 namespace pme {
 
 constexpr int kBlock = 256;          // threads per workgroup for every kernel here
+constexpr int kStageU = 8;           // global loads kept in flight per thread when staging to LDS
 constexpr double kCcelec = 332.0716; // CudaPMERecip.h:27
 constexpr float kForceScale = 1099511627776.0f;  // 2^40, gpu_utils.h:9
 
@@ -79,6 +80,15 @@ __device__ __forceinline__ void bspline_theta_dtheta(float w, float* th, float* 
   for (int j = 1; j <= P - 2; j++)
     th[P - j - 1] = div * ((w + j) * th[P - j - 2] + (P - j - w) * th[P - j - 1]);
   th[0] = div * (1.0f - w) * th[0];
+}
+
+// 64-bit fixed point with scale 2^40 (gpu_utils.h:9): the product by 2^40 is
+// exact in FP32, then rounded to the nearest integer (ties to even), as
+// lliroundf(FORCE_SCALE*val) in write_grid<long long> (CudaPMERecip.cu:51-58).
+__device__ __forceinline__ long long to_fixed40(float v) { return (long long)rintf(v * kForceScale); }
+// back to FP32: ((float)val)*2^-40 as reduce_force<long long, float> (reduce.cu:66-78)
+__device__ __forceinline__ float from_fixed40(unsigned long long u) {
+  return (float)((double)(long long)u * (1.0 / 1099511627776.0));
 }
 
 // Brick decomposition of one grid axis into nb nearly equal bricks:
@@ -252,9 +262,9 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-// Block-wide sum of NV doubles (blockDim.x == kBlock).  The result is valid in
-// thread 0.  sh must hold NV * (kBlock/64) doubles.  Fixed reduction order.
-template <int NV>
+// Block-wide sum of NV doubles over NT threads (NT == blockDim.x).  The result
+// is valid in thread 0.  sh must hold NV * (NT/64) doubles.  Fixed order.
+template <int NV, int NT = kBlock>
 __device__ __forceinline__ void block_sum(double* v, double* sh) {
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
@@ -268,29 +278,45 @@ __device__ __forceinline__ void block_sum(double* v, double* sh) {
 #pragma unroll
     for (int c = 0; c < NV; c++) {
       double s = 0.0;
-      for (int w = 0; w < kBlock / 64; w++) s += sh[w * NV + c];
+      for (int w = 0; w < NT / 64; w++) s += sh[w * NV + c];
       v[c] = s;
     }
   }
   __syncthreads();
 }
 
-// "Last workgroup done" ticket.  Every workgroup calls it once at its end;
-// returns true in all threads of the workgroup that finished last.  That
-// workgroup resets the ticket, so the kernel needs no host-side clear.
-__device__ __forceinline__ bool last_block_done(unsigned int* ticket, int* sh_flag) {
-  __threadfence();
+// Block-wide exclusive scan of data[0, n) in place over NT threads (NT ==
+// blockDim.x).  Returns the total in every thread.  sh: NT/64 ints of LDS.
+template <int NT = kBlock>
+__device__ __forceinline__ int block_exclusive_scan(int* data, int n, int* sh) {
+  const int tid = threadIdx.x;
+  const int per = (n + NT - 1) / NT;
+  const int lo = min(tid * per, n), hi = min(lo + per, n);
+  int s = 0;
+  for (int i = lo; i < hi; i++) s += data[i];
+  const int lane = tid & 63, wid = tid >> 6;
+  int v = s;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int t = __shfl_up(v, off, 64);
+    if (lane >= off) v += t;
+  }
+  if (lane == 63) sh[wid] = v;
   __syncthreads();
-  if (threadIdx.x == 0) {
-    const unsigned int prev = atomicAdd(ticket, 1u);
-    const bool last = (prev == gridDim.x * gridDim.y * gridDim.z - 1);
-    *sh_flag = last ? 1 : 0;
-    if (last) *ticket = 0u;
+  int wbase = 0, total = 0;
+#pragma unroll
+  for (int k = 0; k < NT / 64; k++) {
+    if (k < wid) wbase += sh[k];
+    total += sh[k];
+  }
+  int run = wbase + v - s;
+  for (int i = lo; i < hi; i++) {
+    const int c = data[i];
+    data[i] = run;
+    run += c;
   }
   __syncthreads();
-  const bool last = (*sh_flag != 0);
-  if (last) __threadfence();
-  return last;
+  return total;
 }
 
 }  // namespace pme

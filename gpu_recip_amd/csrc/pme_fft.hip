@@ -34,17 +34,43 @@ __global__ __launch_bounds__(kBlock) void k_fft_x_r2c(const float* __restrict__ 
   const int hx = nx / 2 + 1;
   const bool even = (nx & 1) == 0;
   if (even) {
+    // the tile's rows are contiguous in HBM: keep U loads in flight per thread
     const float2* src = reinterpret_cast<const float2*>(grid + (size_t)row0 * nx);
-    for (int e = tid; e < NR * nf; e += kBlock) {
-      const int row = e / nf;
-      const int m = e - row * nf;
-      b0[m * LS + row] = (row < nr) ? src[(size_t)row * nf + m] : make_float2(0.f, 0.f);
+    const int ntile = NR * nf, nvalid = nr * nf;
+    for (int e0 = tid; e0 < ntile; e0 += kStageU * kBlock) {
+      float2 v[kStageU];
+#pragma unroll
+      for (int u = 0; u < kStageU; u++) {
+        const int e = e0 + u * kBlock;
+        v[u] = (e < nvalid) ? src[e] : make_float2(0.f, 0.f);
+      }
+#pragma unroll
+      for (int u = 0; u < kStageU; u++) {
+        const int e = e0 + u * kBlock;
+        if (e < ntile) {
+          const int row = e / nf;
+          b0[(e - row * nf) * LS + row] = v[u];
+        }
+      }
     }
   } else {
-    for (int e = tid; e < NR * nx; e += kBlock) {
-      const int row = e / nx;
-      const int m = e - row * nx;
-      b0[m * LS + row] = make_float2((row < nr) ? grid[(size_t)(row0 + row) * nx + m] : 0.f, 0.f);
+    const float* src = grid + (size_t)row0 * nx;
+    const int ntile = NR * nx, nvalid = nr * nx;
+    for (int e0 = tid; e0 < ntile; e0 += kStageU * kBlock) {
+      float v[kStageU];
+#pragma unroll
+      for (int u = 0; u < kStageU; u++) {
+        const int e = e0 + u * kBlock;
+        v[u] = (e < nvalid) ? src[e] : 0.f;
+      }
+#pragma unroll
+      for (int u = 0; u < kStageU; u++) {
+        const int e = e0 + u * kBlock;
+        if (e < ntile) {
+          const int row = e / nx;
+          b0[(e - row * nx) * LS + row] = make_float2(v[u], 0.f);
+        }
+      }
     }
   }
   __syncthreads();
@@ -89,19 +115,31 @@ __global__ __launch_bounds__(kBlock) void k_fft_x_c2r(const float2* __restrict__
   if (even) {
     // Z[k] = (X[k] + X*[h-k]) + i w^{-k} (X[k] - X*[h-k]),  k < h
     const int h = nf;
-    for (int e = tid; e < NR * h; e += kBlock) {
-      const int row = e / h;
-      const int k = e - row * h;
-      float2 z = make_float2(0.f, 0.f);
-      if (row < nr) {
-        const float2* x = cg + (size_t)(row0 + row) * hx;
-        const float2 xk = x[k];
-        const float2 xc = cconj(x[h - k]);
-        const float2 A = cadd(xk, xc);
-        const float2 B = cmul(cconj(tw[k]), csub(xk, xc));
-        z = make_float2(A.x - B.y, A.y + B.x);
+    for (int e0 = tid; e0 < NR * h; e0 += kStageU * kBlock) {
+      float2 xk[kStageU], xc[kStageU];
+#pragma unroll
+      for (int u = 0; u < kStageU; u++) {
+        const int e = e0 + u * kBlock;
+        const int row = e / h;
+        const int k = e - row * h;
+        xk[u] = xc[u] = make_float2(0.f, 0.f);
+        if (row < nr) {
+          const float2* x = cg + (size_t)(row0 + row) * hx;
+          xk[u] = x[k];
+          xc[u] = cconj(x[h - k]);
+        }
       }
-      b0[k * LS + row] = z;
+#pragma unroll
+      for (int u = 0; u < kStageU; u++) {
+        const int e = e0 + u * kBlock;
+        if (e < NR * h) {
+          const int row = e / h;
+          const int k = e - row * h;
+          const float2 A = cadd(xk[u], xc[u]);
+          const float2 B = cmul(cconj(tw[k]), csub(xk[u], xc[u]));
+          b0[k * LS + row] = make_float2(A.x - B.y, A.y + B.x);
+        }
+      }
     }
   } else {
     for (int e = tid; e < NR * nx; e += kBlock) {
@@ -185,30 +223,13 @@ __device__ __forceinline__ float2 solve_point(const SolveArgs& s, float2 q, int 
   return cscale(q, eterm);
 }
 
-// Per-workgroup partial -> last workgroup adds the total into the energy slot
-// and virmat (fixed summation order: deterministic for a given launch shape).
-__device__ __forceinline__ void ev_finish(const SolveArgs& s, double* acc, double* sh, int* sh_flag, int blk,
-                                          int nblk) {
+// Per-workgroup energy/virial partial (fixed summation order inside the
+// workgroup).  The total is formed later by ev_reduce_block in one workgroup.
+__device__ __forceinline__ void ev_partial(const SolveArgs& s, double* acc, double* sh, int blk) {
   block_sum<7>(acc, sh);
   if (threadIdx.x == 0) {
 #pragma unroll
     for (int c = 0; c < 7; c++) s.partial[(size_t)blk * 7 + c] = acc[c];
-  }
-  if (last_block_done(s.ticket, sh_flag)) {
-    double v[7] = {0, 0, 0, 0, 0, 0, 0};
-    for (int i = threadIdx.x; i < nblk; i += kBlock)
-#pragma unroll
-      for (int c = 0; c < 7; c++) v[c] += s.partial[(size_t)i * 7 + c];
-    block_sum<7>(v, sh);
-    if (threadIdx.x == 0) {
-      const double hc = 0.5 * kCcelec;
-      s.energy[0] += v[0] * hc;
-      const double v0 = -hc * v[1], v1 = -hc * v[2], v2 = -hc * v[3];
-      const double v3 = -hc * v[4], v4 = -hc * v[5], v5 = -hc * v[6];
-      s.virmat[0] += v0; s.virmat[1] += v1; s.virmat[2] += v2;
-      s.virmat[3] += v1; s.virmat[4] += v3; s.virmat[5] += v4;
-      s.virmat[6] += v2; s.virmat[7] += v4; s.virmat[8] += v5;
-    }
   }
 }
 
@@ -217,11 +238,11 @@ __device__ __forceinline__ void ev_finish(const SolveArgs& s, double* acc, doubl
 template <int MODE>
 __global__ __launch_bounds__(kBlock) void k_fft_cols(float2* __restrict__ cg, int n, int hx, int TX,
                                                      long long pt_stride, long long outer_stride, AxisArgs ax,
-                                                     SolveArgs sa) {
+                                                     SolveArgs sa, EvReduce red) {
   extern __shared__ __align__(16) float2 sm[];
   __shared__ double sh_red[7 * (kBlock / 64)];
-  __shared__ int sh_flag;
   const int tid = threadIdx.x;
+  if ((red.nsolve > 0 || red.nself > 0) && blockIdx.x == 0 && blockIdx.y == 0) ev_reduce_block(red);
   float2* tw = sm;
   float2* b0 = sm + ax.ntw;
   float2* b1 = b0 + n * TX;
@@ -229,10 +250,20 @@ __global__ __launch_bounds__(kBlock) void k_fft_cols(float2* __restrict__ cg, in
   const int kx0 = blockIdx.x * TX;
   const int nl = min(TX, hx - kx0);
   const size_t base = (size_t)kx0 + (size_t)blockIdx.y * (size_t)outer_stride;
-  for (int e = tid; e < n * TX; e += kBlock) {
-    const int j = e / TX;
-    const int l = e - j * TX;
-    b0[e] = (l < nl) ? cg[base + (size_t)j * pt_stride + l] : make_float2(0.f, 0.f);
+  for (int e0 = tid; e0 < n * TX; e0 += kStageU * kBlock) {
+    float2 v[kStageU];
+#pragma unroll
+    for (int u = 0; u < kStageU; u++) {
+      const int e = e0 + u * kBlock;
+      const int j = e / TX;
+      const int l = e - j * TX;
+      v[u] = (e < n * TX && l < nl) ? cg[base + (size_t)j * pt_stride + l] : make_float2(0.f, 0.f);
+    }
+#pragma unroll
+    for (int u = 0; u < kStageU; u++) {
+      const int e = e0 + u * kBlock;
+      if (e < n * TX) b0[e] = v[u];
+    }
   }
   __syncthreads();
   float2* res;
@@ -259,16 +290,13 @@ __global__ __launch_bounds__(kBlock) void k_fft_cols(float2* __restrict__ cg, in
     const int l = e - j * TX;
     if (l < nl) cg[base + (size_t)j * pt_stride + l] = res[e];
   }
-  if (MODE == COL_FWD_SOLVE_INV && sa.calc_ev) {
-    ev_finish(sa, acc, sh_red, &sh_flag, blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
-  }
+  if (MODE == COL_FWD_SOLVE_INV && sa.calc_ev) ev_partial(sa, acc, sh_red, blockIdx.x + gridDim.x * blockIdx.y);
 }
 
 // Stand-alone scalar sum over the whole half spectrum (used when the spectrum
 // has been materialised, e.g. scalar_sum called twice).
 __global__ __launch_bounds__(kBlock) void k_scalar_sum(float2* __restrict__ cg, int hx, int total, SolveArgs sa) {
   __shared__ double sh_red[7 * (kBlock / 64)];
-  __shared__ int sh_flag;
   double acc[7] = {0, 0, 0, 0, 0, 0, 0};
   const int idx = blockIdx.x * kBlock + threadIdx.x;
   if (idx < total) {
@@ -278,7 +306,7 @@ __global__ __launch_bounds__(kBlock) void k_scalar_sum(float2* __restrict__ cg, 
     const int k3 = t / sa.n2;
     cg[idx] = solve_point(sa, cg[idx], k1, k2, k3, acc);
   }
-  if (sa.calc_ev) ev_finish(sa, acc, sh_red, &sh_flag, blockIdx.x, gridDim.x);
+  if (sa.calc_ev) ev_partial(sa, acc, sh_red, blockIdx.x);
 }
 
 // ----------------------------------------------------------------- launch --
@@ -300,28 +328,33 @@ hipError_t launch_fft_x_c2r(const float2* cg, float* grid, int nx, int nrows, in
   return hipGetLastError();
 }
 
+int cols_blocks(int hx, int TX, int nouter) { return ((hx + TX - 1) / TX) * nouter; }
+
 hipError_t launch_fft_cols(int mode, float2* cg, int n, int hx, int TX, long long pt_stride, int nouter,
-                           long long outer_stride, const AxisArgs& ax, const SolveArgs& sa, hipStream_t s) {
+                           long long outer_stride, const AxisArgs& ax, const SolveArgs& sa, const EvReduce& red,
+                           hipStream_t s) {
   const size_t lds = fft_cols_lds_bytes(n, TX);
   const dim3 grid((hx + TX - 1) / TX, nouter);
   switch (mode) {
     case COL_FWD:
-      hipLaunchKernelGGL(k_fft_cols<COL_FWD>, grid, dim3(kBlock), lds, s, cg, n, hx, TX, pt_stride, outer_stride, ax, sa);
+      hipLaunchKernelGGL(k_fft_cols<COL_FWD>, grid, dim3(kBlock), lds, s, cg, n, hx, TX, pt_stride, outer_stride, ax, sa, red);
       break;
     case COL_INV:
-      hipLaunchKernelGGL(k_fft_cols<COL_INV>, grid, dim3(kBlock), lds, s, cg, n, hx, TX, pt_stride, outer_stride, ax, sa);
+      hipLaunchKernelGGL(k_fft_cols<COL_INV>, grid, dim3(kBlock), lds, s, cg, n, hx, TX, pt_stride, outer_stride, ax, sa, red);
       break;
     case COL_FWD_SOLVE_INV:
-      hipLaunchKernelGGL(k_fft_cols<COL_FWD_SOLVE_INV>, grid, dim3(kBlock), lds, s, cg, n, hx, TX, pt_stride, outer_stride, ax, sa);
+      hipLaunchKernelGGL(k_fft_cols<COL_FWD_SOLVE_INV>, grid, dim3(kBlock), lds, s, cg, n, hx, TX, pt_stride, outer_stride, ax, sa, red);
       break;
     case COL_FWD_INV:
-      hipLaunchKernelGGL(k_fft_cols<COL_FWD_INV>, grid, dim3(kBlock), lds, s, cg, n, hx, TX, pt_stride, outer_stride, ax, sa);
+      hipLaunchKernelGGL(k_fft_cols<COL_FWD_INV>, grid, dim3(kBlock), lds, s, cg, n, hx, TX, pt_stride, outer_stride, ax, sa, red);
       break;
     default:
       return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }
+
+int scalar_sum_blocks(int total) { return (total + kBlock - 1) / kBlock; }
 
 hipError_t launch_scalar_sum(float2* cg, int hx, const SolveArgs& sa, hipStream_t s) {
   const int total = hx * sa.n2 * sa.n3;

@@ -59,6 +59,8 @@ extern "C" const char* pme_status_string(pme_status s) {
   return "unknown";
 }
 
+struct pme_recip_s;
+
 // ---------------------------------------------------------- energy/virial --
 // Virial_t layout, CudaEnergyVirial.h:13-18
 static const size_t kOffSforceDp = 0;
@@ -73,7 +75,17 @@ struct pme_energy_virial_s {
   int n = 0;
   char* d_buf = nullptr;
   char* h_buf = nullptr;
+  std::vector<pme_recip_t> solvers;  // solvers writing into this buffer
 };
+
+// A solver may hold per-workgroup energy/virial partials whose total is not
+// yet in the device buffer (it is normally folded into the next kernel);
+// every operation that reads or clears the buffer flushes them first.
+static pme_status flush_pending(pme_recip_t h, hipStream_t target, bool order_target);
+static pme_status ev_flush(pme_ev_t ev, hipStream_t target, bool order_target = true) {
+  for (pme_recip_t h : ev->solvers) PME_TRY(flush_pending(h, target, order_target));
+  return PME_OK;
+}
 
 extern "C" pme_status pme_ev_create(pme_ev_t* out) {
   if (!out) return fail(PME_ERR_INVALID_ARG, "pme_ev_create: null out");
@@ -122,6 +134,7 @@ static pme_status ev_slot(pme_ev_t ev, const char* name, int* slot) {
 
 extern "C" pme_status pme_ev_clear(pme_ev_t ev, pme_stream_t stream) {
   if (!ev) return fail(PME_ERR_INVALID_ARG, "pme_ev_clear: null ev");
+  PME_TRY(ev_flush(ev, (hipStream_t)stream));
   HIP_TRY(launch_fill_zero(ev->d_buf, kEvBytes, (hipStream_t)stream));
   std::memset(ev->h_buf, 0, kEvBytes);
   return PME_OK;
@@ -129,6 +142,7 @@ extern "C" pme_status pme_ev_clear(pme_ev_t ev, pme_stream_t stream) {
 
 extern "C" pme_status pme_ev_clear_energy(pme_ev_t ev, pme_stream_t stream) {
   if (!ev) return fail(PME_ERR_INVALID_ARG, "pme_ev_clear_energy: null ev");
+  PME_TRY(ev_flush(ev, (hipStream_t)stream));
   HIP_TRY(launch_fill_zero(ev->d_buf + kOffEnergy, kMaxTerms * sizeof(double), (hipStream_t)stream));
   std::memset(ev->h_buf + kOffEnergy, 0, kMaxTerms * sizeof(double));
   return PME_OK;
@@ -136,6 +150,7 @@ extern "C" pme_status pme_ev_clear_energy(pme_ev_t ev, pme_stream_t stream) {
 
 extern "C" pme_status pme_ev_clear_virial(pme_ev_t ev, pme_stream_t stream) {
   if (!ev) return fail(PME_ERR_INVALID_ARG, "pme_ev_clear_virial: null ev");
+  PME_TRY(ev_flush(ev, (hipStream_t)stream));
   HIP_TRY(launch_fill_zero(ev->d_buf, kOffEnergy, (hipStream_t)stream));
   std::memset(ev->h_buf, 0, kOffEnergy);
   return PME_OK;
@@ -143,6 +158,7 @@ extern "C" pme_status pme_ev_clear_virial(pme_ev_t ev, pme_stream_t stream) {
 
 extern "C" pme_status pme_ev_copy_to_host(pme_ev_t ev, pme_stream_t stream) {
   if (!ev) return fail(PME_ERR_INVALID_ARG, "pme_ev_copy_to_host: null ev");
+  PME_TRY(ev_flush(ev, (hipStream_t)stream));
   HIP_TRY(hipMemcpyAsync(ev->h_buf, ev->d_buf, kEvBytes, hipMemcpyDeviceToHost, (hipStream_t)stream));
   return PME_OK;
 }
@@ -173,12 +189,14 @@ extern "C" pme_status pme_ev_energy_pointer(pme_ev_t ev, const char* name, doubl
   int slot;
   PME_TRY(ev_slot(ev, name, &slot));
   if (!dptr) return fail(PME_ERR_INVALID_ARG, "pme_ev_energy_pointer: null out");
+  PME_TRY(ev_flush(ev, nullptr, false));
   *dptr = reinterpret_cast<double*>(ev->d_buf + kOffEnergy) + slot;
   return PME_OK;
 }
 
 extern "C" pme_status pme_ev_virial_pointer(pme_ev_t ev, double** dptr) {
   if (!ev || !dptr) return fail(PME_ERR_INVALID_ARG, "pme_ev_virial_pointer: null argument");
+  PME_TRY(ev_flush(ev, nullptr, false));
   *dptr = reinterpret_cast<double*>(ev->d_buf + kOffVirmat);
   return PME_OK;
 }
@@ -260,6 +278,10 @@ enum GridState {
   ST_POTENTIAL       // real grid holds the potential
 };
 
+static const size_t kLdsBudget = 64 * 1024;        // FFT kernels (several workgroups per CU)
+static const size_t kBrickLdsBudget = 150 * 1024;  // spread/gather: one 1024-thread workgroup per CU (160 KB LDS)
+static const int kMaxBins = 16384;   // LDS histogram of the binning kernel
+
 struct pme_recip_s {
   int nx = 0, ny = 0, nz = 0, order = 0, fft_type = PME_FFT_BOX, accum = PME_ACCUM_FLOAT;
   hipStream_t stream = nullptr;
@@ -273,17 +295,21 @@ struct pme_recip_s {
   RadixPlan rp[3];
   int NR = 16, TXy = 16, TXz = 16;
 
-  // spatial bins
-  int nb[3] = {1, 1, 1}, wmax[3] = {0, 0, 0}, nbins = 0, cap = 0, bins_for = -1;
-  int* bin_count = nullptr;
-  int* bin_saved = nullptr;
-  int* bin_idx = nullptr;
-  int* ovf = nullptr;
-  int* ovf_idx = nullptr;
-  int ovf_cap = 0;
-  unsigned int* tickets = nullptr;  // [0] spread, [1] solve, [2] self energy
-  double* partial = nullptr;
+  // bricks and the chunked record store of the binning
+  int nb[3] = {1, 1, 1}, wmax[3] = {0, 0, 0}, nbins = 0;
+  int bins_for = -1;          // atom capacity of the record store
+  int chunk = 0, nchunk = 0, nchunk_cap = 0;
+  int* seg_off = nullptr;     // [nchunk_cap][nbins + 1]
+  float4* rec_pos = nullptr;  // [nchunk_cap * 8 * chunk]
+  int* rec_idx = nullptr;
+
+  // energy/virial partials and the reduction still pending on them
+  double* partial = nullptr;       // [partial_cap * 7] scalar-sum partials
   int partial_cap = 0;
+  double* self_partial = nullptr;  // [1024]
+  int pend_solve = 0, pend_self = 0;
+  double pend_kcs = 0.0;
+  hipEvent_t flush_ev = nullptr;
 
   int state = ST_EMPTY;
   const void* bin_key_ptr = nullptr;
@@ -292,27 +318,64 @@ struct pme_recip_s {
 
   bool timing = false;
   hipEvent_t tev[9] = {};
-  float tms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 };
 
 static void free_handle(pme_recip_t h) {
   if (!h) return;
+  if (h->ev) {
+    auto& v = h->ev->solvers;
+    for (size_t i = 0; i < v.size(); i++)
+      if (v[i] == h) {
+        v.erase(v.begin() + i);
+        break;
+      }
+  }
   (void)hipFree(h->grid);
   (void)hipFree(h->cgrid);
   for (int i = 0; i < 3; i++) {
     (void)hipFree(h->prefac[i]);
     (void)hipFree(h->tw[i]);
   }
-  (void)hipFree(h->bin_count);
-  (void)hipFree(h->bin_saved);
-  (void)hipFree(h->bin_idx);
-  (void)hipFree(h->ovf);
-  (void)hipFree(h->ovf_idx);
-  (void)hipFree(h->tickets);
+  (void)hipFree(h->seg_off);
+  (void)hipFree(h->rec_pos);
+  (void)hipFree(h->rec_idx);
   (void)hipFree(h->partial);
+  (void)hipFree(h->self_partial);
+  if (h->flush_ev) (void)hipEventDestroy(h->flush_ev);
   for (auto& e : h->tev)
     if (e) (void)hipEventDestroy(e);
   delete h;
+}
+
+static EvReduce pending_reduce(pme_recip_t h) {
+  EvReduce r;
+  r.partial = h->partial;
+  r.nsolve = h->pend_solve;
+  r.energy = reinterpret_cast<double*>(h->ev->d_buf + kOffEnergy) + h->slot_recip;
+  r.virmat = reinterpret_cast<double*>(h->ev->d_buf + kOffVirmat);
+  r.self_partial = h->self_partial;
+  r.nself = h->pend_self;
+  r.self_slot = reinterpret_cast<double*>(h->ev->d_buf + kOffEnergy) + h->slot_self;
+  r.kcs = h->pend_kcs;
+  return r;
+}
+
+static EvReduce no_reduce() {
+  EvReduce r{};
+  return r;
+}
+
+// Launch the pending energy/virial reduction on the solver's stream; order
+// `target` after it when it is another stream.
+static pme_status flush_pending(pme_recip_t h, hipStream_t target, bool order_target) {
+  if (h->pend_solve == 0 && h->pend_self == 0) return PME_OK;
+  HIP_TRY(launch_ev_reduce(pending_reduce(h), h->stream));
+  h->pend_solve = h->pend_self = 0;
+  if (order_target && target != h->stream) {
+    HIP_TRY(hipEventRecord(h->flush_ev, h->stream));
+    HIP_TRY(hipStreamWaitEvent(target, h->flush_ev, 0));
+  }
+  return PME_OK;
 }
 
 static pme_status upload_prefac(pme_recip_t h) {
@@ -327,11 +390,9 @@ static pme_status upload_prefac(pme_recip_t h) {
   return PME_OK;
 }
 
-static const size_t kLdsBudget = 64 * 1024;
-
 // Brick plan: about 16 points per brick edge, every brick at least order-1
-// wide (a stencil then spans at most two bricks per axis), and the spread and
-// gather LDS footprints within kLdsBudget (split the widest axis further).
+// wide (a stencil then spans at most two bricks per axis), the spread and
+// gather LDS footprints within kLdsBudget and at most kMaxBins bricks.
 static bool plan_bricks(pme_recip_t h) {
   const int n[3] = {h->nx, h->ny, h->nz};
   const int target = 16;
@@ -342,7 +403,8 @@ static bool plan_bricks(pme_recip_t h) {
     h->nb[a] = nb;
     h->wmax[a] = (n[a] + nb - 1) / nb;
   }
-  while (spread_lds_bytes(h->order, h->wmax) > kLdsBudget || gather_lds_bytes(h->order, h->wmax) > kLdsBudget) {
+  while (spread_lds_bytes(h->order, h->wmax, kMaxChunks) > kBrickLdsBudget ||
+         gather_lds_bytes(h->order, h->wmax, kMaxChunks) > kBrickLdsBudget) {
     int best = -1;
     for (int a = 0; a < 3; a++)
       if (n[a] / (h->nb[a] + 1) >= h->order - 1 && (best < 0 || h->wmax[a] > h->wmax[best])) best = a;
@@ -351,37 +413,36 @@ static bool plan_bricks(pme_recip_t h) {
     h->wmax[best] = (n[best] + h->nb[best] - 1) / h->nb[best];
   }
   h->nbins = h->nb[0] * h->nb[1] * h->nb[2];
-  return true;
+  return h->nbins <= kMaxBins;
 }
 
-static pme_status alloc_bin_tables(pme_recip_t h) {
-  (void)hipFree(h->bin_count);
-  (void)hipFree(h->bin_saved);
-  h->bin_count = h->bin_saved = nullptr;
-  HIP_TRY(hipMalloc(&h->bin_count, sizeof(int) * h->nbins));
-  HIP_TRY(hipMalloc(&h->bin_saved, sizeof(int) * h->nbins));
-  HIP_TRY(hipMemset(h->bin_count, 0, sizeof(int) * h->nbins));
-  HIP_TRY(hipMemset(h->bin_saved, 0, sizeof(int) * h->nbins));
-  h->bins_for = -1;  // bin_idx must be re-sized for the new bin count
-  return PME_OK;
-}
-
-// Size the per-bin atom slots for n atoms: mean + 6 sigma (triplet-correlated
-// counts) + 64, rounded up to 64.  Atoms beyond go to the overflow list.
+// Size the record store for n atoms: chunks of kMinChunk atoms (more per
+// chunk beyond kMaxChunks chunks), 8 record slots per atom (a stencil touches
+// at most 8 bricks).
 static pme_status ensure_bins(pme_recip_t h, int n) {
-  if (n <= h->bins_for) return PME_OK;
+  int chunk = kMinChunk;
+  while ((long long)chunk * kMaxChunks < n) chunk *= 2;
+  const int nchunk = std::max(1, (n + chunk - 1) / chunk);
+  if (n <= h->bins_for && chunk == h->chunk && nchunk <= h->nchunk_cap) {
+    h->nchunk = nchunk;
+    return PME_OK;
+  }
   HIP_TRY(hipStreamSynchronize(h->stream));
-  const double mean = (double)n / h->nbins;
-  int cap = (int)(mean + 6.0 * std::sqrt(3.0 * mean + 1.0) + 64.0);
-  cap = (cap + 63) / 64 * 64;
-  (void)hipFree(h->bin_idx);
-  (void)hipFree(h->ovf_idx);
-  h->bin_idx = h->ovf_idx = nullptr;
-  HIP_TRY(hipMalloc(&h->bin_idx, sizeof(int) * (size_t)h->nbins * cap));
-  HIP_TRY(hipMalloc(&h->ovf_idx, sizeof(int) * (size_t)std::max(n, 1)));
-  h->cap = cap;
-  h->ovf_cap = std::max(n, 1);
-  h->bins_for = n;
+  (void)hipFree(h->seg_off);
+  (void)hipFree(h->rec_pos);
+  (void)hipFree(h->rec_idx);
+  h->seg_off = nullptr;
+  h->rec_pos = nullptr;
+  h->rec_idx = nullptr;
+  h->bins_for = -1;
+  const size_t rs = 8 * (size_t)chunk;
+  HIP_TRY(hipMalloc(&h->seg_off, sizeof(int) * (size_t)nchunk * (h->nbins + 1)));
+  HIP_TRY(hipMalloc(&h->rec_pos, sizeof(float4) * rs * nchunk));
+  HIP_TRY(hipMalloc(&h->rec_idx, sizeof(int) * rs * nchunk));
+  h->chunk = chunk;
+  h->nchunk = nchunk;
+  h->nchunk_cap = nchunk;
+  h->bins_for = std::max(n, 1);
   return PME_OK;
 }
 
@@ -396,14 +457,13 @@ static BinArgs bin_args(pme_recip_t h, const double* recip) {
   a.nbx = h->nb[0];
   a.nby = h->nb[1];
   a.nbz = h->nb[2];
-  a.cap = h->cap;
-  a.bin_count = h->bin_count;
-  a.bin_saved = h->bin_saved;
-  a.bin_idx = h->bin_idx;
-  a.ovf = h->ovf;
-  a.ovf_idx = h->ovf_idx;
-  a.ovf_cap = h->ovf_cap;
-  a.ticket = h->tickets + 0;
+  a.order = h->order;
+  a.chunk = h->chunk;
+  a.nchunk = h->nchunk;
+  a.rec_stride = 8 * h->chunk;
+  a.seg_off = h->seg_off;
+  a.rec_pos = h->rec_pos;
+  a.rec_idx = h->rec_idx;
   return a;
 }
 
@@ -439,9 +499,6 @@ static SolveArgs solve_args(pme_recip_t h, const double* recip, double kappa, bo
   s.pf3 = h->prefac[2];
   s.calc_ev = calc_ev ? 1 : 0;
   s.partial = h->partial;
-  s.ticket = h->tickets + 1;
-  s.energy = reinterpret_cast<double*>(h->ev->d_buf + kOffEnergy) + h->slot_recip;
-  s.virmat = reinterpret_cast<double*>(h->ev->d_buf + kOffVirmat);
   return s;
 }
 
@@ -486,14 +543,17 @@ extern "C" pme_status pme_recip_create(int nfftx, int nffty, int nfftz, int orde
   h->fft_type = fft_type;
   h->accum = accum_type;
   h->stream = (hipStream_t)stream;
-  h->ev = ev;
+  if (!plan_bricks(h)) {
+    delete h;
+    return fail(PME_ERR_UNSUPPORTED, "pme_recip_create: no brick plan fits the LDS budget");
+  }
   // CudaPMERecip.cu:1426-1429
   pme_status st = pme_ev_insert(ev, name_recip);
   if (st == PME_OK) st = pme_ev_insert(ev, name_self);
   if (st == PME_OK) st = ev_slot(ev, name_recip, &h->slot_recip);
   if (st == PME_OK) st = ev_slot(ev, name_self, &h->slot_self);
   if (st != PME_OK) {
-    free_handle(h);
+    delete h;
     return st;
   }
   auto init = [&]() -> pme_status {
@@ -517,17 +577,11 @@ extern "C" pme_status pme_recip_create(int nfftx, int nffty, int nfftz, int orde
       return fail(PME_ERR_UNSUPPORTED, "pme_recip_create: no radix plan");
     pick_tiles(h);
     PME_TRY(upload_prefac(h));
-    if (!plan_bricks(h)) return fail(PME_ERR_UNSUPPORTED, "pme_recip_create: no brick plan fits the LDS budget");
-    PME_TRY(alloc_bin_tables(h));
-    HIP_TRY(hipMalloc(&h->ovf, sizeof(int) * 2));
-    HIP_TRY(hipMemset(h->ovf, 0, sizeof(int) * 2));
-    HIP_TRY(hipMalloc(&h->tickets, sizeof(unsigned int) * 8));
-    HIP_TRY(hipMemset(h->tickets, 0, sizeof(unsigned int) * 8));
     const int hx = h->nx / 2 + 1;
-    const int zblocks = ((hx + h->TXz - 1) / h->TXz) * h->ny;
-    const int sblocks = (int)((Gc + kBlock - 1) / kBlock);
-    h->partial_cap = std::max(std::max(zblocks, sblocks), 1024);
+    h->partial_cap = std::max(cols_blocks(hx, h->TXz, h->ny), scalar_sum_blocks((int)Gc));
     HIP_TRY(hipMalloc(&h->partial, sizeof(double) * 7 * (size_t)h->partial_cap));
+    HIP_TRY(hipMalloc(&h->self_partial, sizeof(double) * 1024));
+    HIP_TRY(hipEventCreateWithFlags(&h->flush_ev, hipEventDisableTiming));
     for (auto& e : h->tev) HIP_TRY(hipEventCreate(&e));
     return PME_OK;
   };
@@ -536,18 +590,24 @@ extern "C" pme_status pme_recip_create(int nfftx, int nffty, int nfftz, int orde
     free_handle(h);
     return st;
   }
+  h->ev = ev;
+  ev->solvers.push_back(h);
   *out = h;
   return PME_OK;
 }
 
 extern "C" pme_status pme_recip_destroy(pme_recip_t h) {
-  if (h) (void)hipStreamSynchronize(h->stream);
+  if (!h) return PME_OK;
+  (void)flush_pending(h, h->stream, false);
+  (void)hipStreamSynchronize(h->stream);
   free_handle(h);
   return PME_OK;
 }
 
 extern "C" pme_status pme_recip_set_stream(pme_recip_t h, pme_stream_t stream) {
   if (!h) return fail(PME_ERR_INVALID_ARG, "pme_recip_set_stream: null handle");
+  // partials written on the old stream are reduced there first
+  PME_TRY(flush_pending(h, (hipStream_t)stream, true));
   h->stream = (hipStream_t)stream;
   return PME_OK;
 }
@@ -557,12 +617,27 @@ extern "C" pme_status pme_recip_set_order(pme_recip_t h, int order) {
   if (order != 4 && order != 6 && order != 8) return fail(PME_ERR_UNSUPPORTED, "pme_recip_set_order: order %d", order);
   if (h->nx < order || h->ny < order || h->nz < order)
     return fail(PME_ERR_UNSUPPORTED, "pme_recip_set_order: grid smaller than order %d", order);
+  PME_TRY(flush_pending(h, h->stream, false));
   HIP_TRY(hipStreamSynchronize(h->stream));
+  const int old_order = h->order, old_nbins = h->nbins;
+  int old_nb[3] = {h->nb[0], h->nb[1], h->nb[2]}, old_w[3] = {h->wmax[0], h->wmax[1], h->wmax[2]};
   h->order = order;
+  if (!plan_bricks(h)) {
+    h->order = old_order;
+    h->nbins = old_nbins;
+    for (int a = 0; a < 3; a++) {
+      h->nb[a] = old_nb[a];
+      h->wmax[a] = old_w[a];
+    }
+    return fail(PME_ERR_UNSUPPORTED, "pme_recip_set_order: no brick plan fits the LDS budget");
+  }
   PME_TRY(upload_prefac(h));
-  const int old_nbins = h->nbins;
-  if (!plan_bricks(h)) return fail(PME_ERR_UNSUPPORTED, "pme_recip_set_order: no brick plan fits the LDS budget");
-  if (h->nbins != old_nbins) PME_TRY(alloc_bin_tables(h));
+  if (h->nbins != old_nbins) {  // segment table is sized by the brick count
+    (void)hipFree(h->seg_off);
+    h->seg_off = nullptr;
+    h->bins_for = -1;
+    h->nchunk_cap = 0;
+  }
   h->bin_key_n = -1;
   h->state = ST_EMPTY;
   return PME_OK;
@@ -581,11 +656,10 @@ static void mark(pme_recip_t h, int i) {
   if (h->timing) (void)hipEventRecord(h->tev[i], h->stream);
 }
 
-static pme_status do_bin(pme_recip_t h, const void* xyzq, int n, const double* recip, bool finalize) {
+static pme_status do_bin(pme_recip_t h, const void* xyzq, int n, const double* recip) {
   PME_TRY(ensure_bins(h, n));
   BinArgs a = bin_args(h, recip);
   HIP_TRY(launch_bin_atoms((const float4*)xyzq, n, a, h->stream));
-  if (finalize) HIP_TRY(launch_bins_finalize(a, h->stream));
   h->bin_key_ptr = xyzq;
   h->bin_key_n = n;
   h->bin_key_r[0] = a.r1;
@@ -598,10 +672,10 @@ extern "C" pme_status pme_recip_spread_charge(pme_recip_t h, const void* d_xyzq,
   if (!h || !recip || (ncoord > 0 && !d_xyzq)) return fail(PME_ERR_INVALID_ARG, "pme_recip_spread_charge: null argument");
   if (ncoord < 0) return fail(PME_ERR_INVALID_ARG, "pme_recip_spread_charge: ncoord %d", ncoord);
   mark(h, 0);
-  PME_TRY(do_bin(h, d_xyzq, ncoord, recip, false));
+  PME_TRY(do_bin(h, d_xyzq, ncoord, recip));
   mark(h, 1);
   BinArgs a = bin_args(h, recip);
-  HIP_TRY(launch_spread(h->order, (const float4*)d_xyzq, a, h->wmax, h->grid, h->stream));
+  HIP_TRY(launch_spread(h->order, a, h->wmax, h->grid, h->stream));
   h->state = ST_CHARGE;
   return PME_OK;
 }
@@ -616,7 +690,7 @@ extern "C" pme_status pme_recip_r2c_fft(pme_recip_t h) {
   HIP_TRY(launch_fft_x_r2c(h->grid, h->cgrid, h->nx, h->ny * h->nz, h->NR, axis_args(h, 0), h->stream));
   mark(h, 3);
   HIP_TRY(launch_fft_cols(COL_FWD, h->cgrid, h->ny, hx, h->TXy, hx, h->nz, (long long)hx * h->ny, axis_args(h, 1),
-                          dummy, h->stream));
+                          dummy, no_reduce(), h->stream));
   h->state = ST_XY;
   return PME_OK;
 }
@@ -624,7 +698,7 @@ extern "C" pme_status pme_recip_r2c_fft(pme_recip_t h) {
 static pme_status z_pass(pme_recip_t h, int mode, const SolveArgs& sa) {
   const int hx = h->nx / 2 + 1;
   HIP_TRY(launch_fft_cols(mode, h->cgrid, h->nz, hx, h->TXz, (long long)hx * h->ny, h->ny, hx, axis_args(h, 2), sa,
-                          h->stream));
+                          no_reduce(), h->stream));
   return PME_OK;
 }
 
@@ -634,19 +708,23 @@ extern "C" pme_status pme_recip_scalar_sum(pme_recip_t h, const double recip[9],
   if (!is_ortho(recip))
     return fail(PME_ERR_UNSUPPORTED, "pme_recip_scalar_sum: only orthorhombic boxes are supported");
   if (!(kappa > 0.0)) return fail(PME_ERR_INVALID_ARG, "pme_recip_scalar_sum: kappa %g", kappa);
+  if (h->state != ST_XY && h->state != ST_SPECTRUM && h->state != ST_SOLVED_ZINV)
+    return fail(PME_ERR_STATE, "pme_recip_scalar_sum: no spectrum (call r2c_fft first)");
+  // the partial buffer is about to be rewritten
+  if (h->pend_solve) PME_TRY(flush_pending(h, h->stream, false));
   const bool ev = calc_energy || calc_virial;
   SolveArgs sa = solve_args(h, recip, kappa, ev);
   const int hx = h->nx / 2 + 1;
   mark(h, 4);
   if (h->state == ST_XY) {
     PME_TRY(z_pass(h, COL_FWD_SOLVE_INV, sa));
+    if (ev) h->pend_solve = cols_blocks(hx, h->TXz, h->ny);
     h->state = ST_SOLVED_ZINV;
-  } else if (h->state == ST_SPECTRUM || h->state == ST_SOLVED_ZINV) {
+  } else {
     if (h->state == ST_SOLVED_ZINV) PME_TRY(z_pass(h, COL_FWD, sa));
     HIP_TRY(launch_scalar_sum(h->cgrid, hx, sa, h->stream));
+    if (ev) h->pend_solve = scalar_sum_blocks(hx * h->ny * h->nz);
     h->state = ST_SPECTRUM;
-  } else {
-    return fail(PME_ERR_STATE, "pme_recip_scalar_sum: no spectrum (call r2c_fft first)");
   }
   return PME_OK;
 }
@@ -663,8 +741,10 @@ extern "C" pme_status pme_recip_c2r_fft(pme_recip_t h) {
     return fail(PME_ERR_STATE, "pme_recip_c2r_fft: no spectrum (call r2c_fft first)");
   }
   mark(h, 5);
+  // the y-inverse pass also folds in the pending energy/virial reduction
   HIP_TRY(launch_fft_cols(COL_INV, h->cgrid, h->ny, hx, h->TXy, hx, h->nz, (long long)hx * h->ny, axis_args(h, 1),
-                          dummy, h->stream));
+                          dummy, pending_reduce(h), h->stream));
+  h->pend_solve = h->pend_self = 0;
   mark(h, 6);
   HIP_TRY(launch_fft_x_c2r(h->cgrid, h->grid, h->nx, h->ny * h->nz, h->NR, axis_args(h, 0), h->stream));
   h->state = ST_POTENTIAL;
@@ -685,9 +765,12 @@ extern "C" pme_status pme_recip_gather_force(pme_recip_t h, const void* d_xyzq, 
   if (h->state != ST_POTENTIAL) return fail(PME_ERR_STATE, "pme_recip_gather_force: no potential (call c2r_fft first)");
   if (ncoord == 0) return PME_OK;
   const float r[3] = {(float)recip[0], (float)recip[4], (float)recip[8]};
+  // the records of the last binning say which brick gathers which atom; the
+  // gather itself reads the current coordinates, so stale records only cost
+  // a detour through HBM.  Different atoms or box: bin again.
   if (d_xyzq != h->bin_key_ptr || ncoord != h->bin_key_n || r[0] != h->bin_key_r[0] || r[1] != h->bin_key_r[1] ||
       r[2] != h->bin_key_r[2]) {
-    PME_TRY(do_bin(h, d_xyzq, ncoord, recip, true));
+    PME_TRY(do_bin(h, d_xyzq, ncoord, recip));
   }
   mark(h, 7);
   BinArgs a = bin_args(h, recip);
@@ -700,9 +783,11 @@ extern "C" pme_status pme_recip_calc_self_energy(pme_recip_t h, const void* d_xy
   if (!h || (ncoord > 0 && !d_xyzq)) return fail(PME_ERR_INVALID_ARG, "pme_recip_calc_self_energy: null argument");
   if (ncoord < 0) return fail(PME_ERR_INVALID_ARG, "pme_recip_calc_self_energy: ncoord %d", ncoord);
   if (ncoord == 0) return PME_OK;
-  double* slot = reinterpret_cast<double*>(h->ev->d_buf + kOffEnergy) + h->slot_self;
-  const double kcs = kappa * kCcelec / std::sqrt(kPi);
-  HIP_TRY(launch_self_energy((const float4*)d_xyzq, ncoord, kcs, h->partial, h->tickets + 2, slot, h->stream));
+  if (h->pend_self) PME_TRY(flush_pending(h, h->stream, false));
+  int nblk = 0;
+  HIP_TRY(launch_self_energy((const float4*)d_xyzq, ncoord, h->self_partial, &nblk, h->stream));
+  h->pend_self = nblk;
+  h->pend_kcs = kappa * kCcelec / std::sqrt(kPi);
   return PME_OK;
 }
 

@@ -7,20 +7,37 @@
 
 namespace pme {
 
-// Spatial bins of atoms: one bin per grid brick, holding the atoms whose base
-// grid index (the lowest stencil point) lies in that brick.
+// Atoms sorted into grid bricks by a chunked counting sort.  Binning
+// workgroup w takes atoms [w*chunk, (w+1)*chunk) and writes one record per
+// brick an atom's stencil touches (1 to 8) into its own region of rec_stride
+// slots, grouped by brick; seg_off[w][b] is the start of brick b's records in
+// that region (exclusive prefix, seg_off[w][nbins] = total).  No global
+// atomics: counts and cursors live in LDS.
 struct BinArgs {
   int nx, ny, nz;          // grid
   float r1, r2, r3;        // (float) recip diagonal, as the reference casts it
   int nbx, nby, nbz;       // bricks per axis
-  int cap;                 // atom slots per bin
-  int* bin_count;          // [nbins] live counters, zero between binnings
-  int* bin_saved;          // [nbins] counts of the last binning
-  int* bin_idx;            // [nbins * cap] atom indices
-  int* ovf;                // [0] live overflow count, [1] saved overflow count
-  int* ovf_idx;            // [ovf_cap] atoms that did not fit their bin
-  int ovf_cap;
-  unsigned int* ticket;    // last-workgroup ticket of the spread kernel
+  int order;               // B-spline order (stencil width)
+  int chunk;               // atoms per binning workgroup
+  int nchunk;              // binning workgroups
+  int rec_stride;          // record slots per chunk region (8 * chunk)
+  int* seg_off;            // [nchunk][nbins + 1]
+  float4* rec_pos;         // [nchunk * rec_stride] (x, y, z, q) copies
+  int* rec_idx;            // [nchunk * rec_stride] atom index, bit 31 set on the home copy
+};
+
+// Deferred energy/virial reduction: per-workgroup partials written by the
+// scalar-sum and self-energy kernels are summed in fixed order by one
+// workgroup of a later kernel (no contended atomics).
+struct EvReduce {
+  const double* partial;   // [nsolve * 7] scalar-sum partials
+  int nsolve;              // 0: none pending
+  double* energy;          // recip slot
+  double* virmat;          // virmat[9]
+  const double* self_partial;  // [nself]
+  int nself;               // 0: none pending
+  double* self_slot;
+  double kcs;              // kappa*ccelec/sqrt(pi)
 };
 
 struct AxisArgs {
@@ -41,18 +58,43 @@ struct SolveArgs {
   const float* pf3;
   int calc_ev;             // accumulate energy and virial
   double* partial;         // [nblocks * 7] per-workgroup partial sums
-  unsigned int* ticket;
-  double* energy;          // device energy slot (added to)
-  double* virmat;          // device virmat[9] (added to)
 };
 
 enum ColMode { COL_FWD = 0, COL_INV = 1, COL_FWD_SOLVE_INV = 2, COL_FWD_INV = 3 };
 
+// Segments of brick b in the chunked record store: s_base[w] = first record
+// of b in chunk w's region, s_pre[w] = exclusive prefix of the counts over w
+// (s_pre[nchunk] = total).  Returns the total.  NT == blockDim.x.
+template <int NT>
+__device__ __forceinline__ int brick_segments(const BinArgs& a, int b, int* s_pre, int* s_base, int* sh_scan) {
+  const int nbins = a.nbx * a.nby * a.nbz;
+  for (int w = threadIdx.x; w < a.nchunk; w += NT) {
+    const int* seg = a.seg_off + (size_t)w * (nbins + 1);
+    const int lo = seg[b];
+    s_base[w] = lo;
+    s_pre[w] = seg[b + 1] - lo;
+  }
+  __syncthreads();
+  const int total = block_exclusive_scan<NT>(s_pre, a.nchunk, sh_scan);
+  if (threadIdx.x == 0) s_pre[a.nchunk] = total;
+  __syncthreads();
+  return total;
+}
+
+// record slot of the v-th record of the brick (0 <= v < total)
+__device__ __forceinline__ size_t brick_record(const BinArgs& a, int v, const int* s_pre, const int* s_base) {
+  int lo = 0, hi = a.nchunk;  // s_pre[lo] <= v < s_pre[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (s_pre[mid] <= v) lo = mid;
+    else hi = mid;
+  }
+  return (size_t)lo * a.rec_stride + s_base[lo] + (v - s_pre[lo]);
+}
+
 // ---- launchers (return the HIP error of the launch) ----
 hipError_t launch_bin_atoms(const float4* xyzq, int n, const BinArgs& a, hipStream_t s);
-hipError_t launch_bins_finalize(const BinArgs& a, hipStream_t s);
-hipError_t launch_spread(int order, const float4* xyzq, const BinArgs& a, const int wmax[3],
-                         float* grid, hipStream_t s);
+hipError_t launch_spread(int order, const BinArgs& a, const int wmax[3], float* grid, hipStream_t s);
 hipError_t launch_gather(int order, int force_type, const float4* xyzq, const BinArgs& a,
                          const int wmax[3], const float* phi, int stride, void* force,
                          hipStream_t s);
@@ -64,24 +106,63 @@ hipError_t launch_fft_x_c2r(const float2* cg, float* grid, int nx, int nrows, in
 // lines along one axis of the half spectrum: tile of TX consecutive kx, grid
 // (ceil(hx/TX), nouter); element (line l, point j) at base + j*pt_stride + l,
 // base = kx0 + outer*outer_stride.  For the z pass outer is ky (needed by
-// COL_FWD_SOLVE_INV).
+// COL_FWD_SOLVE_INV, which writes per-workgroup energy/virial partials).
+// Workgroup (0,0) also performs a pending energy/virial reduction if red
+// has one.
 hipError_t launch_fft_cols(int mode, float2* cg, int n, int hx, int TX, long long pt_stride,
                            int nouter, long long outer_stride, const AxisArgs& ax,
-                           const SolveArgs& sa, hipStream_t s);
+                           const SolveArgs& sa, const EvReduce& red, hipStream_t s);
+int cols_blocks(int hx, int TX, int nouter);
 hipError_t launch_scalar_sum(float2* cg, int hx, const SolveArgs& sa, hipStream_t s);
+int scalar_sum_blocks(int total);
 
-hipError_t launch_self_energy(const float4* xyzq, int n, double kappa_ccelec_sqrtpi, double* partial,
-                              unsigned int* ticket, double* slot, hipStream_t s);
+hipError_t launch_self_energy(const float4* xyzq, int n, double* partial, int* nblocks, hipStream_t s);
+hipError_t launch_ev_reduce(const EvReduce& red, hipStream_t s);
 hipError_t launch_fill_zero(void* p, size_t bytes, hipStream_t s);
 hipError_t launch_fill_bspline(int order, const float4* xyzq, int n, float r1, float r2, float r3,
                                int nx, int ny, int nz, int* gix, int* giy, int* giz, float* q,
                                float* thx, float* thy, float* thz, float* dthx, float* dthy,
                                float* dthz, hipStream_t s);
 
+// Sum the pending partials in fixed order (one workgroup) and add them into
+// the energy/virial buffer (CudaPMERecip.cu:871-886 for the scalar sum,
+// :1272-1274 for the self energy).
+__device__ __forceinline__ void ev_reduce_block(const EvReduce& r) {
+  __shared__ double sh[7 * (kBlock / 64)];
+  if (r.nsolve > 0) {
+    double v[7] = {0, 0, 0, 0, 0, 0, 0};
+    for (int i = threadIdx.x; i < r.nsolve; i += kBlock)
+#pragma unroll
+      for (int c = 0; c < 7; c++) v[c] += r.partial[(size_t)i * 7 + c];
+    block_sum<7>(v, sh);
+    if (threadIdx.x == 0) {
+      const double hc = 0.5 * kCcelec;
+      r.energy[0] += v[0] * hc;
+      const double v0 = -hc * v[1], v1 = -hc * v[2], v2 = -hc * v[3];
+      const double v3 = -hc * v[4], v4 = -hc * v[5], v5 = -hc * v[6];
+      r.virmat[0] += v0; r.virmat[1] += v1; r.virmat[2] += v2;
+      r.virmat[3] += v1; r.virmat[4] += v3; r.virmat[5] += v4;
+      r.virmat[6] += v2; r.virmat[7] += v4; r.virmat[8] += v5;
+    }
+  }
+  if (r.nself > 0) {
+    double v[1] = {0.0};
+    for (int i = threadIdx.x; i < r.nself; i += kBlock) v[0] += r.self_partial[i];
+    block_sum<1>(v, sh);
+    if (threadIdx.x == 0) r.self_slot[0] += -v[0] * r.kcs;
+  }
+}
+
 // LDS sizes (bytes) the launchers request, exposed for plan validation
-size_t spread_lds_bytes(int order, const int wmax[3]);
-size_t gather_lds_bytes(int order, const int wmax[3]);
+size_t bin_lds_bytes(int nbins);
+size_t spread_lds_bytes(int order, const int wmax[3], int nchunk);
+size_t gather_lds_bytes(int order, const int wmax[3], int nchunk);
 size_t fft_x_lds_bytes(int nf, int NR, int ntw);
 size_t fft_cols_lds_bytes(int n, int TX);
+
+constexpr int kSpreadBlock = 1024;  // threads per spread / gather workgroup (16 waves per CU)
+constexpr int kGatherBatch = 512;   // candidate records per gather pass (LDS weight cache)
+constexpr int kMaxChunks = 1024;   // binning workgroups per solve (chunk grows with n)
+constexpr int kMinChunk = 1024;    // atoms per binning workgroup
 
 }  // namespace pme

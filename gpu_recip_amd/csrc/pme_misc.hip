@@ -24,13 +24,11 @@ hipError_t launch_fill_zero(void* p, size_t bytes, hipStream_t s) {
   return hipGetLastError();
 }
 
-// calc_self_energy_kernel, CudaPMERecip.cu:1252-1275: q*q in FP32, sum in FP64,
-// -kappa*ccelec/sqrt(pi) * sum added to the "self" slot.  Grid-stride with at
-// most 1024 workgroups; the last workgroup sums the partials in fixed order.
-__global__ __launch_bounds__(kBlock) void k_self_energy(const float4* __restrict__ xyzq, int n, double kcs,
-                                                        double* partial, unsigned int* ticket, double* slot) {
+// calc_self_energy_kernel, CudaPMERecip.cu:1252-1275: q*q in FP32, summed in
+// FP64.  Grid-stride with at most 1024 workgroups writing partials; the sum
+// and the -kappa*ccelec/sqrt(pi) factor are applied by ev_reduce_block.
+__global__ __launch_bounds__(kBlock) void k_self_energy(const float4* __restrict__ xyzq, int n, double* partial) {
   __shared__ double sh[kBlock / 64];
-  __shared__ int sh_flag;
   double v[1] = {0.0};
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
     const float q = xyzq[i].w;
@@ -38,20 +36,21 @@ __global__ __launch_bounds__(kBlock) void k_self_energy(const float4* __restrict
   }
   block_sum<1>(v, sh);
   if (threadIdx.x == 0) partial[blockIdx.x] = v[0];
-  if (last_block_done(ticket, &sh_flag)) {
-    double t[1] = {0.0};
-    for (int i = threadIdx.x; i < (int)gridDim.x; i += kBlock) t[0] += partial[i];
-    block_sum<1>(t, sh);
-    if (threadIdx.x == 0) slot[0] += -t[0] * kcs;
-  }
 }
 
-hipError_t launch_self_energy(const float4* xyzq, int n, double kcs, double* partial, unsigned int* ticket,
-                              double* slot, hipStream_t s) {
+hipError_t launch_self_energy(const float4* xyzq, int n, double* partial, int* nblocks, hipStream_t s) {
   int nblk = (n + kBlock - 1) / kBlock;
   if (nblk < 1) nblk = 1;
   if (nblk > 1024) nblk = 1024;
-  hipLaunchKernelGGL(k_self_energy, dim3(nblk), dim3(kBlock), 0, s, xyzq, n, kcs, partial, ticket, slot);
+  *nblocks = nblk;
+  hipLaunchKernelGGL(k_self_energy, dim3(nblk), dim3(kBlock), 0, s, xyzq, n, partial);
+  return hipGetLastError();
+}
+
+__global__ __launch_bounds__(kBlock) void k_ev_reduce(EvReduce red) { ev_reduce_block(red); }
+
+hipError_t launch_ev_reduce(const EvReduce& red, hipStream_t s) {
+  hipLaunchKernelGGL(k_ev_reduce, dim3(1), dim3(kBlock), 0, s, red);
   return hipGetLastError();
 }
 

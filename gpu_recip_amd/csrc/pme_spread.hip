@@ -1,59 +1,108 @@
 // pme_spread.hip -- charge spreading onto the PME grid (CudaPMERecip.cu:479-580
 // semantics), re-designed for CDNA4:
 //
-//  1. k_bin_atoms: every atom goes to the bin of the brick holding its base
-//     grid index (one int atomic per atom, no float atomics to HBM).
-//  2. k_spread_brick: one workgroup per brick.  It pulls the atoms of its own
-//     bin and of the lower neighbour bins (their stencils reach up to p-1
-//     points), keeps those that touch the brick (wave ballot compaction),
-//     accumulates q*thx*thy*thz with LDS float atomics, and writes the brick
-//     to HBM exactly once with coalesced stores.  No grid memset, no global
-//     float atomics.  The last workgroup saves the bin counts for the gather
-//     and re-zeroes the live counters, so the sequence is graph-capturable.
+//  1. k_bin_chunks: a chunked counting sort.  Workgroup w bins its chunk of
+//     atoms by grid brick with LDS counters only (histogram, block scan,
+//     cursors): an atom gets one record in every brick its p^3 stencil
+//     touches (1 to 8), the copy in the brick of its base index is flagged
+//     "home".  No global atomics, so nothing contends across the 8 XCDs.
+//  2. k_spread_brick: one workgroup per brick.  It reads exactly the records
+//     of its brick (all of them touch it), computes the B-spline weights into
+//     LDS and accumulates q*thx*thy*thz in LDS as 64-bit fixed point with
+//     scale 2^40 -- the reference's `long long` grid accumulation
+//     (CudaPMERecip.cu:51-58, write_grid<long long>; reduce_force, reduce.cu:66-78).
+//     Integer LDS atomics are an order of magnitude faster than float ones
+//     on gfx950 and integer sums do not depend on the order of the adds, so
+//     the charge grid is bitwise reproducible.  The brick is converted to
+//     float and written to HBM once with coalesced stores (no memset, no
+//     global atomics).
 #include "pme_internal.hpp"
 
 namespace pme {
 
-__device__ __forceinline__ void atom_grid_index(const float4& p, const BinArgs& a, int& ix, int& iy,
-                                                int& iz, float& wx, float& wy, float& wz) {
-  grid_index(p.x, a.r1, a.nx, ix, wx);
-  grid_index(p.y, a.r2, a.ny, iy, wy);
-  grid_index(p.z, a.r3, a.nz, iz, wz);
+// Does the stencil i..i+P-1 of an atom in brick b run past the brick's end
+// (into brick b+1, periodic)?  An axis with one brick wraps onto itself.
+__device__ __forceinline__ int spills(int i, int b, int n, int nb, int P) {
+  if (nb == 1) return 0;
+  return (i + P - 1 >= brick_lo(b + 1, n, nb)) ? 1 : 0;
 }
 
-__global__ __launch_bounds__(kBlock) void k_bin_atoms(const float4* __restrict__ xyzq, int n, BinArgs a) {
-  const int atom = blockIdx.x * kBlock + threadIdx.x;
-  if (atom >= n) return;
-  const float4 p = xyzq[atom];
+struct AtomBricks {
+  int bx, by, bz;  // brick of the base index
+  int sx, sy, sz;  // 1 if the stencil spills into the next brick
+};
+
+__device__ __forceinline__ AtomBricks atom_bricks(const float4& p, const BinArgs& a) {
   int ix, iy, iz;
-  float wx, wy, wz;
-  atom_grid_index(p, a, ix, iy, iz, wx, wy, wz);
+  float w;
+  grid_index(p.x, a.r1, a.nx, ix, w);
+  grid_index(p.y, a.r2, a.ny, iy, w);
+  grid_index(p.z, a.r3, a.nz, iz, w);
   ix = safe_index(ix, a.nx);
   iy = safe_index(iy, a.ny);
   iz = safe_index(iz, a.nz);
-  const int b = brick_of(ix, a.nx, a.nbx) + a.nbx * (brick_of(iy, a.ny, a.nby) + a.nby * brick_of(iz, a.nz, a.nbz));
-  const int slot = atomicAdd(&a.bin_count[b], 1);
-  if (slot < a.cap) {
-    a.bin_idx[(size_t)b * a.cap + slot] = atom;
-  } else {
-    const int o = atomicAdd(&a.ovf[0], 1);
-    if (o < a.ovf_cap) a.ovf_idx[o] = atom;
+  AtomBricks r;
+  r.bx = brick_of(ix, a.nx, a.nbx);
+  r.by = brick_of(iy, a.ny, a.nby);
+  r.bz = brick_of(iz, a.nz, a.nbz);
+  r.sx = spills(ix, r.bx, a.nx, a.nbx, a.order);
+  r.sy = spills(iy, r.by, a.ny, a.nby, a.order);
+  r.sz = spills(iz, r.bz, a.nz, a.nbz, a.order);
+  return r;
+}
+
+__device__ __forceinline__ int next_brick(int b, int d, int nb) {
+  b += d;
+  return b >= nb ? b - nb : b;
+}
+
+__global__ __launch_bounds__(kBlock) void k_bin_chunks(const float4* __restrict__ xyzq, int n, BinArgs a) {
+  extern __shared__ int hist[];  // [nbins]: counts, then cursors
+  __shared__ int sh_scan[kBlock / 64];
+  const int nbins = a.nbx * a.nby * a.nbz;
+  const int tid = threadIdx.x;
+  const int w = blockIdx.x;
+  const int a0 = w * a.chunk;
+  const int a1 = min(a0 + a.chunk, n);
+  for (int i = tid; i < nbins; i += kBlock) hist[i] = 0;
+  __syncthreads();
+  for (int atom = a0 + tid; atom < a1; atom += kBlock) {
+    const AtomBricks k = atom_bricks(xyzq[atom], a);
+    for (int dz = 0; dz <= k.sz; dz++)
+      for (int dy = 0; dy <= k.sy; dy++)
+        for (int dx = 0; dx <= k.sx; dx++) {
+          const int b = next_brick(k.bx, dx, a.nbx) + a.nbx * (next_brick(k.by, dy, a.nby) + a.nby * next_brick(k.bz, dz, a.nbz));
+          atomicAdd(&hist[b], 1);
+        }
+  }
+  __syncthreads();
+  const int total = block_exclusive_scan(hist, nbins, sh_scan);
+  int* seg = a.seg_off + (size_t)w * (nbins + 1);
+  for (int i = tid; i < nbins; i += kBlock) seg[i] = hist[i];
+  if (tid == 0) seg[nbins] = total;
+  __syncthreads();
+  float4* rp = a.rec_pos + (size_t)w * a.rec_stride;
+  int* ri = a.rec_idx + (size_t)w * a.rec_stride;
+  for (int atom = a0 + tid; atom < a1; atom += kBlock) {
+    const float4 p = xyzq[atom];
+    const AtomBricks k = atom_bricks(p, a);
+    for (int dz = 0; dz <= k.sz; dz++)
+      for (int dy = 0; dy <= k.sy; dy++)
+        for (int dx = 0; dx <= k.sx; dx++) {
+          const int b = next_brick(k.bx, dx, a.nbx) + a.nbx * (next_brick(k.by, dy, a.nby) + a.nby * next_brick(k.bz, dz, a.nbz));
+          const int pos = atomicAdd(&hist[b], 1);
+          rp[pos] = p;
+          ri[pos] = ((dx | dy | dz) == 0) ? (int)((unsigned)atom | 0x80000000u) : atom;
+        }
   }
 }
 
-__device__ __forceinline__ void finalize_bins(const BinArgs& a, int nbins) {
-  for (int i = threadIdx.x; i < nbins; i += kBlock) {
-    a.bin_saved[i] = a.bin_count[i];
-    a.bin_count[i] = 0;
-  }
-  if (threadIdx.x == 0) {
-    a.ovf[1] = a.ovf[0];
-    a.ovf[0] = 0;
-  }
-}
+size_t bin_lds_bytes(int nbins) { return sizeof(int) * (size_t)nbins; }
 
-__global__ __launch_bounds__(kBlock) void k_bins_finalize(BinArgs a) {
-  finalize_bins(a, a.nbx * a.nby * a.nbz);
+hipError_t launch_bin_atoms(const float4* xyzq, int n, const BinArgs& a, hipStream_t s) {
+  const int nbins = a.nbx * a.nby * a.nbz;
+  hipLaunchKernelGGL(k_bin_chunks, dim3(a.nchunk), dim3(kBlock), bin_lds_bytes(nbins), s, xyzq, n, a);
+  return hipGetLastError();
 }
 
 // local offset of a grid index relative to a brick start x0 of width w:
@@ -65,20 +114,18 @@ __device__ __forceinline__ int local_offset(int i, int x0, int w, int n) {
 }
 
 template <int P>
-__global__ __launch_bounds__(kBlock) void k_spread_brick(const float4* __restrict__ xyzq, BinArgs a,
-                                                         int wxm, int wym, int wzm, float* __restrict__ grid) {
+__global__ __launch_bounds__(kSpreadBlock) void k_spread_brick(BinArgs a, int wxm, int wym, int wzm,
+                                                         float* __restrict__ grid) {
   extern __shared__ __align__(16) unsigned char smem[];
-  __shared__ int s_cnt;
-  __shared__ int s_flag;
-  __shared__ int s_src[8];
-  __shared__ int s_srccnt[8];
-  __shared__ int s_nsrc;
+  __shared__ int sh_scan[kSpreadBlock / 64];
 
   const int bxs = wxm + 1;  // padded x stride of the LDS brick
-  float* acc = reinterpret_cast<float*>(smem);
+  unsigned long long* acc = reinterpret_cast<unsigned long long*>(smem);
   const int acc_n = wzm * wym * bxs;
-  int* s_d = reinterpret_cast<int*>(acc + acc_n);           // [3][kBlock] local offsets
-  float* s_th = reinterpret_cast<float*>(s_d + 3 * kBlock);  // [kBlock][3P]
+  int* s_d = reinterpret_cast<int*>(acc + acc_n);            // [3][kSpreadBlock] local offsets
+  float* s_th = reinterpret_cast<float*>(s_d + 3 * kSpreadBlock);   // [kSpreadBlock][3P]
+  int* s_pre = reinterpret_cast<int*>(s_th + kSpreadBlock * 3 * P);  // [nchunk + 1]
+  int* s_base = s_pre + a.nchunk + 1;                         // [nchunk]
 
   const int tid = threadIdx.x;
   const int b = blockIdx.x;
@@ -88,81 +135,26 @@ __global__ __launch_bounds__(kBlock) void k_spread_brick(const float4* __restric
   const int z0 = brick_lo(bzi, a.nz, a.nbz), wz = brick_lo(bzi + 1, a.nz, a.nbz) - z0;
   const bool wrapx = (a.nbx == 1), wrapy = (a.nby == 1), wrapz = (a.nbz == 1);
 
-  for (int i = tid; i < acc_n; i += kBlock) acc[i] = 0.0f;
+  for (int i = tid; i < acc_n; i += kSpreadBlock) acc[i] = 0ull;
+  const int total = brick_segments<kSpreadBlock>(a, b, s_pre, s_base, sh_scan);
 
-  if (tid == 0) {
-    // own bin and the lower neighbours; an axis with one brick wraps onto itself
-    int ns = 0;
-    const int ox = a.nbx > 1 ? 2 : 1, oy = a.nby > 1 ? 2 : 1, oz = a.nbz > 1 ? 2 : 1;
-    for (int dz = 0; dz < oz; dz++)
-      for (int dy = 0; dy < oy; dy++)
-        for (int dx = 0; dx < ox; dx++) {
-          int sx = bxi - dx, sy = byi - dy, sz = bzi - dz;
-          if (sx < 0) sx += a.nbx;
-          if (sy < 0) sy += a.nby;
-          if (sz < 0) sz += a.nbz;
-          const int s = sx + a.nbx * (sy + a.nby * sz);
-          int c = a.bin_count[s];
-          if (c > a.cap) c = a.cap;
-          s_src[ns] = s;
-          s_srccnt[ns] = c;
-          ns++;
-        }
-    s_nsrc = ns;
-  }
-  __syncthreads();
-  const int nsrc = s_nsrc;
-  int total = 0;
-  for (int k = 0; k < nsrc; k++) total += s_srccnt[k];
-  int novf = a.ovf[0];
-  if (novf > a.ovf_cap) novf = a.ovf_cap;
-  const int ntot = total + novf;
-
-  for (int base = 0; base < ntot; base += kBlock) {
-    if (tid == 0) s_cnt = 0;
-    __syncthreads();
+  for (int base = 0; base < total; base += kSpreadBlock) {
     const int v = base + tid;
-    int atom = -1;
     if (v < total) {
-      int k = 0, pre = 0;
-      while (v >= pre + s_srccnt[k]) {
-        pre += s_srccnt[k];
-        k++;
-      }
-      atom = a.bin_idx[(size_t)s_src[k] * a.cap + (v - pre)];
-    } else if (v < ntot) {
-      atom = a.ovf_idx[v - total];
-    }
-    bool hit = false;
-    int dx = 0, dy = 0, dz = 0;
-    float fx = 0.f, fy = 0.f, fz = 0.f, q = 0.f;
-    if (atom >= 0) {
-      const float4 p = xyzq[atom];
+      const float4 p = a.rec_pos[brick_record(a, v, s_pre, s_base)];
       int ix, iy, iz;
-      atom_grid_index(p, a, ix, iy, iz, fx, fy, fz);
-      ix = safe_index(ix, a.nx);
-      iy = safe_index(iy, a.ny);
-      iz = safe_index(iz, a.nz);
-      dx = local_offset(ix, x0, wx, a.nx);
-      dy = local_offset(iy, y0, wy, a.ny);
-      dz = local_offset(iz, z0, wz, a.nz);
-      hit = (dx > -P) && (dy > -P) && (dz > -P);
-      q = p.w;
-    }
-    const unsigned long long m = __ballot(hit);
-    int wbase = 0;
-    if ((tid & 63) == 0 && m != 0ull) wbase = atomicAdd(&s_cnt, __popcll(m));
-    wbase = __shfl(wbase, 0, 64);
-    if (hit) {
-      const int slot = wbase + __popcll(m & __lanemask_lt());
-      s_d[slot] = dx;
-      s_d[kBlock + slot] = dy;
-      s_d[2 * kBlock + slot] = dz;
+      float fx, fy, fz;
+      grid_index(p.x, a.r1, a.nx, ix, fx);
+      grid_index(p.y, a.r2, a.ny, iy, fy);
+      grid_index(p.z, a.r3, a.nz, iz, fz);
+      s_d[tid] = local_offset(safe_index(ix, a.nx), x0, wx, a.nx);
+      s_d[kSpreadBlock + tid] = local_offset(safe_index(iy, a.ny), y0, wy, a.ny);
+      s_d[2 * kSpreadBlock + tid] = local_offset(safe_index(iz, a.nz), z0, wz, a.nz);
       float th[P];
-      float* o = s_th + slot * 3 * P;
+      float* o = s_th + tid * 3 * P;
       bspline_theta<P>(fx, th);
 #pragma unroll
-      for (int j = 0; j < P; j++) o[j] = q * th[j];
+      for (int j = 0; j < P; j++) o[j] = p.w * th[j];
       bspline_theta<P>(fy, th);
 #pragma unroll
       for (int j = 0; j < P; j++) o[P + j] = th[j];
@@ -171,76 +163,66 @@ __global__ __launch_bounds__(kBlock) void k_spread_brick(const float4* __restric
       for (int j = 0; j < P; j++) o[2 * P + j] = th[j];
     }
     __syncthreads();
-    const int na = s_cnt;
+    const int na = min(kSpreadBlock, total - base);
     const int nrows = na * P * P;
-    for (int r = tid; r < nrows; r += kBlock) {
+    for (int r = tid; r < nrows; r += kSpreadBlock) {
       const int ai = r / (P * P);
       const int yz = r - ai * (P * P);
       const int jz = yz / P;
       const int jy = yz - jz * P;
-      int ly = s_d[kBlock + ai] + jy;
-      int lz = s_d[2 * kBlock + ai] + jz;
-      if (wrapy && ly >= wy) ly -= a.ny;
-      if (wrapz && lz >= wz) lz -= a.nz;
-      if (ly < 0 || ly >= wy || lz < 0 || lz >= wz) continue;
+      // every LDS read of the row up front (one wait), then up to P LDS
+      // atomics without waits (exec-masked outside the brick)
       const float* th = s_th + ai * 3 * P;
-      const float c = th[P + jy] * th[2 * P + jz];
-      float* row = acc + (lz * wym + ly) * bxs;
       const int dxa = s_d[ai];
+      int ly = s_d[kSpreadBlock + ai] + jy;
+      int lz = s_d[2 * kSpreadBlock + ai] + jz;
+      float tx[P];
+#pragma unroll
+      for (int jx = 0; jx < P; jx++) tx[jx] = th[jx];
+      const float c = th[P + jy] * th[2 * P + jz];
+      ly -= (wrapy && ly >= wy) ? a.ny : 0;
+      lz -= (wrapz && lz >= wz) ? a.nz : 0;
+      if (ly < 0 || ly >= wy || lz < 0 || lz >= wz) continue;
+      unsigned long long* row = acc + (lz * wym + ly) * bxs;
 #pragma unroll
       for (int jx = 0; jx < P; jx++) {
         int lx = dxa + jx;
-        if (wrapx && lx >= wx) lx -= a.nx;
-        if (lx >= 0 && lx < wx) atomicAdd(&row[lx], th[jx] * c);
+        lx -= (wrapx && lx >= wx) ? a.nx : 0;
+        if (lx >= 0 && lx < wx) atomicAdd(&row[lx], (unsigned long long)to_fixed40(tx[jx] * c));
       }
     }
     __syncthreads();
   }
-  __syncthreads();
 
   // the brick is complete: one coalesced store per grid point, no memset needed
   const int nb = wx * wy * wz;
-  for (int i = tid; i < nb; i += kBlock) {
+  for (int i = tid; i < nb; i += kSpreadBlock) {
     const int lx = i % wx;
     const int t = i / wx;
     const int ly = t % wy;
     const int lz = t / wy;
     grid[(size_t)(x0 + lx) + (size_t)a.nx * ((size_t)(y0 + ly) + (size_t)a.ny * (z0 + lz))] =
-        acc[(lz * wym + ly) * bxs + lx];
+        from_fixed40(acc[(lz * wym + ly) * bxs + lx]);
   }
-
-  if (last_block_done(a.ticket, &s_flag)) finalize_bins(a, a.nbx * a.nby * a.nbz);
 }
 
-size_t spread_lds_bytes(int order, const int wmax[3]) {
-  return sizeof(float) * (size_t)wmax[2] * wmax[1] * (wmax[0] + 1) + sizeof(int) * 3 * kBlock +
-         sizeof(float) * (size_t)kBlock * 3 * order;
+size_t spread_lds_bytes(int order, const int wmax[3], int nchunk) {
+  return sizeof(unsigned long long) * (size_t)wmax[2] * wmax[1] * (wmax[0] + 1) + sizeof(int) * 3 * kSpreadBlock +
+         sizeof(float) * (size_t)kSpreadBlock * 3 * order + sizeof(int) * (2 * (size_t)nchunk + 1);
 }
 
-hipError_t launch_bin_atoms(const float4* xyzq, int n, const BinArgs& a, hipStream_t s) {
-  if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_bin_atoms, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, xyzq, n, a);
-  return hipGetLastError();
-}
-
-hipError_t launch_bins_finalize(const BinArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL(k_bins_finalize, dim3(1), dim3(kBlock), 0, s, a);
-  return hipGetLastError();
-}
-
-hipError_t launch_spread(int order, const float4* xyzq, const BinArgs& a, const int wmax[3], float* grid,
-                         hipStream_t s) {
+hipError_t launch_spread(int order, const BinArgs& a, const int wmax[3], float* grid, hipStream_t s) {
   const int nbins = a.nbx * a.nby * a.nbz;
-  const size_t lds = spread_lds_bytes(order, wmax);
+  const size_t lds = spread_lds_bytes(order, wmax, a.nchunk);
   switch (order) {
     case 4:
-      hipLaunchKernelGGL(k_spread_brick<4>, dim3(nbins), dim3(kBlock), lds, s, xyzq, a, wmax[0], wmax[1], wmax[2], grid);
+      hipLaunchKernelGGL(k_spread_brick<4>, dim3(nbins), dim3(kSpreadBlock), lds, s, a, wmax[0], wmax[1], wmax[2], grid);
       break;
     case 6:
-      hipLaunchKernelGGL(k_spread_brick<6>, dim3(nbins), dim3(kBlock), lds, s, xyzq, a, wmax[0], wmax[1], wmax[2], grid);
+      hipLaunchKernelGGL(k_spread_brick<6>, dim3(nbins), dim3(kSpreadBlock), lds, s, a, wmax[0], wmax[1], wmax[2], grid);
       break;
     case 8:
-      hipLaunchKernelGGL(k_spread_brick<8>, dim3(nbins), dim3(kBlock), lds, s, xyzq, a, wmax[0], wmax[1], wmax[2], grid);
+      hipLaunchKernelGGL(k_spread_brick<8>, dim3(nbins), dim3(kSpreadBlock), lds, s, a, wmax[0], wmax[1], wmax[2], grid);
       break;
     default:
       return hipErrorInvalidValue;

@@ -67,7 +67,10 @@ def run_solve(nfft, order, kappa, box, xyzq, calc_ev=True, force_type=L.FORCE_FL
 
 
 def rel_rms(a, b):
-    return float(np.sqrt(((a - b) ** 2).sum() / max((b ** 2).sum(), 1e-300)))
+    """||a - b|| / ||b|| (real or complex)."""
+    a = np.asarray(a)
+    b = np.asarray(b)
+    return float(np.sqrt((np.abs(a - b) ** 2).sum() / max((np.abs(b) ** 2).sum(), 1e-300)))
 
 
 def rel_err(a, b):
