@@ -36,6 +36,21 @@ _i = C.c_int
 _d = C.c_double
 _pd = C.POINTER(C.c_double)
 
+
+
+class SystemDesc(C.Structure):
+    """pme_system_desc (include/pme_recip.h)."""
+    _fields_ = [
+        ("nfftx", C.c_int), ("nffty", C.c_int), ("nfftz", C.c_int), ("order", C.c_int),
+        ("kappa", C.c_double), ("recip", C.c_double * 9), ("ncoord", C.c_int),
+        ("d_xyzq", C.c_void_p), ("d_force", C.c_void_p), ("force_type", C.c_int),
+        ("stride", C.c_int), ("calc_energy_virial", C.c_int),
+    ]
+
+
+_pi = C.POINTER(C.c_int)
+_sz = C.c_size_t
+
 # name -> argtypes (restype is pme_status = int unless listed in _RESTYPE)
 SIGNATURES = {
     "pme_last_error": [],
@@ -69,6 +84,18 @@ SIGNATURES = {
     "pme_recip_get_timing": [_vp, C.POINTER(C.c_float)],
     "pme_fill_bspline": [_i, _i, _i, _i, _pd, _vp, _i] + [_vp] * 10 + [_vp],
     "pme_bsp_mod": [_i, _i, _pd],
+    "pme_batch_create": [_sz, _vp, C.POINTER(_vp)],
+    "pme_batch_destroy": [_vp],
+    "pme_batch_add": [_vp, C.POINTER(SystemDesc), _pi],
+    "pme_batch_update": [_vp, _i, C.POINTER(SystemDesc)],
+    "pme_batch_remove": [_vp, _i],
+    "pme_batch_solve": [_vp, _i, _pi],
+    "pme_batch_results": [_vp, _i, _pi, _pd, _pd],
+    "pme_batch_set_graph": [_vp, _i],
+    "pme_batch_set_tuning": [_vp, _i, _i, _i],
+    "pme_batch_get_tuning": [_vp, _pi, _pi, _pi],
+    "pme_batch_autotune": [_vp, C.c_char_p, C.POINTER(C.c_float)],
+    "pme_batch_memory": [_vp, C.POINTER(_sz), C.POINTER(_sz)],
 }
 _RESTYPE = {"pme_last_error": C.c_char_p, "pme_status_string": C.c_char_p}
 

@@ -71,6 +71,22 @@ def build(verbose: bool = False) -> str:
     return LIB
 
 
+DROPIN_SRC = os.path.join(ROOT, "tests", "cpp", "gpu_recip_dropin.cpp")
+DROPIN_BIN = os.path.join(ROOT, "tests", "cpp", "gpu_recip_dropin")
+
+
+def build_dropin_test() -> str:
+    """Compile the C++ drop-in test program (host code over include/pme_recip.hpp,
+    linked against libpme_hip.so with an rpath to this package)."""
+    cmd = [HIPCC, "-std=c++17", "-O2", f"-I{INCLUDE}", DROPIN_SRC, "-o", DROPIN_BIN, f"-L{HERE}", "-lpme_hip",
+           f"-Wl,-rpath,{HERE}", "-Wl,-rpath,$ORIGIN/../../gpu_recip_amd"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"drop-in test build failed:\n{r.stdout}\n{r.stderr}")
+    return DROPIN_BIN
+
+
 if __name__ == "__main__":
     build(verbose=True)
+    build_dropin_test()
     sys.exit(0)

@@ -19,9 +19,9 @@ namespace pme {
 
 // ---------------------------------------------------------------- x pass ----
 
-__global__ __launch_bounds__(kBlock) void k_fft_x_r2c(const float* __restrict__ grid, float2* __restrict__ cg, int nx,
-                                                      int nrows, int NR, AxisArgs ax) {
-  extern __shared__ __align__(16) float2 sm[];
+// Row tile `tile` (NR rows) of one grid; sm: dynamic LDS (fft_x_lds_bytes).
+__device__ __forceinline__ void fft_x_r2c_body(const float* __restrict__ grid, float2* __restrict__ cg, int nx,
+                                               int nrows, int NR, const AxisArgs& ax, int tile, float2* sm) {
   const int tid = threadIdx.x;
   const int nf = ax.rp.n;
   const int LS = NR + 1;
@@ -29,7 +29,7 @@ __global__ __launch_bounds__(kBlock) void k_fft_x_r2c(const float* __restrict__ 
   float2* b0 = sm + ax.ntw;
   float2* b1 = b0 + nf * LS;
   for (int i = tid; i < ax.ntw; i += kBlock) tw[i] = ax.tw[i];
-  const int row0 = blockIdx.x * NR;
+  const int row0 = tile * NR;
   const int nr = min(NR, nrows - row0);
   const int hx = nx / 2 + 1;
   const bool even = (nx & 1) == 0;
@@ -97,9 +97,14 @@ __global__ __launch_bounds__(kBlock) void k_fft_x_r2c(const float* __restrict__ 
   }
 }
 
-__global__ __launch_bounds__(kBlock) void k_fft_x_c2r(const float2* __restrict__ cg, float* __restrict__ grid, int nx,
+__global__ __launch_bounds__(kBlock) void k_fft_x_r2c(const float* __restrict__ grid, float2* __restrict__ cg, int nx,
                                                       int nrows, int NR, AxisArgs ax) {
   extern __shared__ __align__(16) float2 sm[];
+  fft_x_r2c_body(grid, cg, nx, nrows, NR, ax, blockIdx.x, sm);
+}
+
+__device__ __forceinline__ void fft_x_c2r_body(const float2* __restrict__ cg, float* __restrict__ grid, int nx,
+                                               int nrows, int NR, const AxisArgs& ax, int tile, float2* sm) {
   const int tid = threadIdx.x;
   const int nf = ax.rp.n;
   const int LS = NR + 1;
@@ -108,7 +113,7 @@ __global__ __launch_bounds__(kBlock) void k_fft_x_c2r(const float2* __restrict__
   float2* b1 = b0 + nf * LS;
   for (int i = tid; i < ax.ntw; i += kBlock) tw[i] = ax.tw[i];
   __syncthreads();
-  const int row0 = blockIdx.x * NR;
+  const int row0 = tile * NR;
   const int nr = min(NR, nrows - row0);
   const int hx = nx / 2 + 1;
   const bool even = (nx & 1) == 0;
@@ -169,6 +174,12 @@ __global__ __launch_bounds__(kBlock) void k_fft_x_c2r(const float2* __restrict__
       grid[(size_t)(row0 + row) * nx + m] = res[m * LS + row].x;
     }
   }
+}
+
+__global__ __launch_bounds__(kBlock) void k_fft_x_c2r(const float2* __restrict__ cg, float* __restrict__ grid, int nx,
+                                                      int nrows, int NR, AxisArgs ax) {
+  extern __shared__ __align__(16) float2 sm[];
+  fft_x_c2r_body(cg, grid, nx, nrows, NR, ax, blockIdx.x, sm);
 }
 
 // ------------------------------------------------------- influence function --
@@ -235,21 +246,21 @@ __device__ __forceinline__ void ev_partial(const SolveArgs& s, double* acc, doub
 
 // ------------------------------------------------------------ y / z passes --
 
+// Column tile (tx, ty) of one grid: lines kx0 = tx*TX ..., outer index ty
+// (ky for the z pass); blk indexes the energy/virial partial of the tile.
 template <int MODE>
-__global__ __launch_bounds__(kBlock) void k_fft_cols(float2* __restrict__ cg, int n, int hx, int TX,
-                                                     long long pt_stride, long long outer_stride, AxisArgs ax,
-                                                     SolveArgs sa, EvReduce red) {
-  extern __shared__ __align__(16) float2 sm[];
+__device__ __forceinline__ void fft_cols_body(float2* __restrict__ cg, int n, int hx, int TX, long long pt_stride,
+                                              long long outer_stride, const AxisArgs& ax, const SolveArgs& sa,
+                                              int tx, int ty, int blk, float2* sm) {
   __shared__ double sh_red[7 * (kBlock / 64)];
   const int tid = threadIdx.x;
-  if ((red.nsolve > 0 || red.nself > 0) && blockIdx.x == 0 && blockIdx.y == 0) ev_reduce_block(red);
   float2* tw = sm;
   float2* b0 = sm + ax.ntw;
   float2* b1 = b0 + n * TX;
   for (int i = tid; i < ax.ntw; i += kBlock) tw[i] = ax.tw[i];
-  const int kx0 = blockIdx.x * TX;
+  const int kx0 = tx * TX;
   const int nl = min(TX, hx - kx0);
-  const size_t base = (size_t)kx0 + (size_t)blockIdx.y * (size_t)outer_stride;
+  const size_t base = (size_t)kx0 + (size_t)ty * (size_t)outer_stride;
   for (int e0 = tid; e0 < n * TX; e0 += kStageU * kBlock) {
     float2 v[kStageU];
 #pragma unroll
@@ -274,7 +285,7 @@ __global__ __launch_bounds__(kBlock) void k_fft_cols(float2* __restrict__ cg, in
   }
   double acc[7] = {0, 0, 0, 0, 0, 0, 0};
   if (MODE == COL_FWD_SOLVE_INV) {
-    const int k2 = blockIdx.y;
+    const int k2 = ty;
     for (int e = tid; e < n * TX; e += kBlock) {
       const int j = e / TX;
       const int l = e - j * TX;
@@ -290,7 +301,17 @@ __global__ __launch_bounds__(kBlock) void k_fft_cols(float2* __restrict__ cg, in
     const int l = e - j * TX;
     if (l < nl) cg[base + (size_t)j * pt_stride + l] = res[e];
   }
-  if (MODE == COL_FWD_SOLVE_INV && sa.calc_ev) ev_partial(sa, acc, sh_red, blockIdx.x + gridDim.x * blockIdx.y);
+  if (MODE == COL_FWD_SOLVE_INV && sa.calc_ev) ev_partial(sa, acc, sh_red, blk);
+}
+
+template <int MODE>
+__global__ __launch_bounds__(kBlock) void k_fft_cols(float2* __restrict__ cg, int n, int hx, int TX,
+                                                     long long pt_stride, long long outer_stride, AxisArgs ax,
+                                                     SolveArgs sa, EvReduce red) {
+  extern __shared__ __align__(16) float2 sm[];
+  if ((red.nsolve > 0 || red.nself > 0) && blockIdx.x == 0 && blockIdx.y == 0) ev_reduce_block(red);
+  fft_cols_body<MODE>(cg, n, hx, TX, pt_stride, outer_stride, ax, sa, blockIdx.x, blockIdx.y,
+                      blockIdx.x + gridDim.x * blockIdx.y, sm);
 }
 
 // Stand-alone scalar sum over the whole half spectrum (used when the spectrum
@@ -359,6 +380,85 @@ int scalar_sum_blocks(int total) { return (total + kBlock - 1) / kBlock; }
 hipError_t launch_scalar_sum(float2* cg, int hx, const SolveArgs& sa, hipStream_t s) {
   const int total = hx * sa.n2 * sa.n3;
   hipLaunchKernelGGL(k_scalar_sum, dim3((total + kBlock - 1) / kBlock), dim3(kBlock), 0, s, cg, hx, total, sa);
+  return hipGetLastError();
+}
+
+// ---- batched: persistent workgroups over the work items of many systems ----
+
+template <int INV>
+__global__ __launch_bounds__(kBlock) void k_batch_fft_x(const SysDesc* __restrict__ sys,
+                                                        const WorkItem* __restrict__ items, int nitems) {
+  extern __shared__ __align__(16) float2 sm[];
+  for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+    const WorkItem w = items[it];
+    const SysDesc& d = sys[w.sys];
+    const int nrows = d.bin.ny * d.bin.nz;
+    if (INV)
+      fft_x_c2r_body(d.cgrid, d.grid, d.bin.nx, nrows, d.NR, d.ax[0], w.idx, sm);
+    else
+      fft_x_r2c_body(d.grid, d.cgrid, d.bin.nx, nrows, d.NR, d.ax[0], w.idx, sm);
+    __syncthreads();
+  }
+}
+
+// energy/virial of one system from its z-pass partials (fixed order, one
+// workgroup), assigned to ev_out
+__device__ __forceinline__ void batch_ev_reduce(const SysDesc& d) {
+  __shared__ double sh[7 * (kBlock / 64)];
+  double v[7] = {0, 0, 0, 0, 0, 0, 0};
+  for (int i = threadIdx.x; i < d.nzblk; i += kBlock)
+#pragma unroll
+    for (int c = 0; c < 7; c++) v[c] += d.sa.partial[(size_t)i * 7 + c];
+  block_sum<7>(v, sh);
+  if (threadIdx.x == 0) {
+    const double hc = 0.5 * kCcelec;
+    double* o = d.ev_out;
+    o[0] = v[0] * hc;
+    const double v0 = -hc * v[1], v1 = -hc * v[2], v2 = -hc * v[3];
+    const double v3 = -hc * v[4], v4 = -hc * v[5], v5 = -hc * v[6];
+    o[1] = v0; o[2] = v1; o[3] = v2;
+    o[4] = v1; o[5] = v3; o[6] = v4;
+    o[7] = v2; o[8] = v4; o[9] = v5;
+  }
+}
+
+template <int AXIS, int MODE>
+__global__ __launch_bounds__(kBlock) void k_batch_cols(const SysDesc* __restrict__ sys,
+                                                       const WorkItem* __restrict__ items, int nitems, int reduce) {
+  extern __shared__ __align__(16) float2 sm[];
+  for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+    const WorkItem w = items[it];
+    const SysDesc& d = sys[w.sys];
+    if (reduce && w.idx == 0 && d.sa.calc_ev) batch_ev_reduce(d);
+    const int hx = d.bin.nx / 2 + 1;
+    const int n = AXIS == 1 ? d.bin.ny : d.bin.nz;
+    const int TX = AXIS == 1 ? d.TXy : d.TXz;
+    const long long pt = AXIS == 1 ? (long long)hx : (long long)hx * d.bin.ny;
+    const long long outer = AXIS == 1 ? (long long)hx * d.bin.ny : (long long)hx;
+    const int ntx = (hx + TX - 1) / TX;
+    fft_cols_body<MODE>(d.cgrid, n, hx, TX, pt, outer, d.ax[AXIS], d.sa, w.idx % ntx, w.idx / ntx, w.idx, sm);
+    __syncthreads();
+  }
+}
+
+hipError_t launch_batch_fft_x(int inverse, const SysDesc* sys, const WorkItem* items, int nitems, int grid, size_t lds,
+                              hipStream_t s) {
+  if (nitems <= 0) return hipSuccess;
+  if (inverse)
+    hipLaunchKernelGGL(k_batch_fft_x<1>, dim3(grid), dim3(kBlock), lds, s, sys, items, nitems);
+  else
+    hipLaunchKernelGGL(k_batch_fft_x<0>, dim3(grid), dim3(kBlock), lds, s, sys, items, nitems);
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_cols(int axis, int mode, int reduce, const SysDesc* sys, const WorkItem* items, int nitems,
+                             int grid, size_t lds, hipStream_t s) {
+  if (nitems <= 0) return hipSuccess;
+  const dim3 g(grid), b(kBlock);
+  if (axis == 1 && mode == COL_FWD) hipLaunchKernelGGL((k_batch_cols<1, COL_FWD>), g, b, lds, s, sys, items, nitems, reduce);
+  else if (axis == 1 && mode == COL_INV) hipLaunchKernelGGL((k_batch_cols<1, COL_INV>), g, b, lds, s, sys, items, nitems, reduce);
+  else if (axis == 2 && mode == COL_FWD_SOLVE_INV) hipLaunchKernelGGL((k_batch_cols<2, COL_FWD_SOLVE_INV>), g, b, lds, s, sys, items, nitems, reduce);
+  else return hipErrorInvalidValue;
   return hipGetLastError();
 }
 

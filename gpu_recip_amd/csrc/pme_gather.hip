@@ -13,14 +13,13 @@
 
 namespace pme {
 
-template <int FT>
-__device__ __forceinline__ void store_force(void* force, int atom, int stride, float fx, float fy, float fz) {
-  if (FT == 0) {  // float: strided SoA store, CudaPMERecip.cu:1005-1012
+__device__ __forceinline__ void store_force(int ft, void* force, int atom, int stride, float fx, float fy, float fz) {
+  if (ft == 0) {  // float: strided SoA store, CudaPMERecip.cu:1005-1012
     float* f = reinterpret_cast<float*>(force);
     f[atom] = fx;
     f[(size_t)atom + stride] = fy;
     f[(size_t)atom + 2 * (size_t)stride] = fz;
-  } else if (FT == 1) {  // float3: AoS store, :1016-1023
+  } else if (ft == 1) {  // float3: AoS store, :1016-1023
     float* f = reinterpret_cast<float*>(force) + 3 * (size_t)atom;
     f[0] = fx;
     f[1] = fy;
@@ -36,13 +35,14 @@ __device__ __forceinline__ void store_force(void* force, int atom, int stride, f
   }
 }
 
-template <int P, int FT>
-__global__ __launch_bounds__(kSpreadBlock) void k_gather_brick(const float4* __restrict__ xyzq, BinArgs a, int wxm,
-                                                         int wym, int wzm, const float* __restrict__ phi,
-                                                         float ccelec_f, int stride, void* force) {
+// Brick b of one grid (one kSpreadBlock-thread workgroup); smem: the
+// workgroup's dynamic LDS (gather_lds_bytes).  ft: PME_FORCE_* store mode.
+template <int P>
+__device__ __forceinline__ void gather_brick_body(const float4* __restrict__ xyzq, const BinArgs& a, int b, int wxm,
+                                                  int wym, int wzm, const float* __restrict__ phi, float ccelec_f,
+                                                  int ft, int stride, void* force, unsigned char* smem) {
   constexpr int G = (P <= 4) ? 4 : 8;
   constexpr int NT = 6 * P;  // theta x/y/z and dtheta x/y/z per atom
-  extern __shared__ __align__(16) unsigned char smem[];
   __shared__ int sh_scan[kSpreadBlock / 64];
   __shared__ int s_cnt;
   const int hxs = wxm + P - 1, hys = wym + P - 1, hzs = wzm + P - 1;
@@ -54,7 +54,6 @@ __global__ __launch_bounds__(kSpreadBlock) void k_gather_brick(const float4* __r
   int* s_base = s_pre + a.nchunk + 1;                                       // [nchunk]
 
   const int tid = threadIdx.x;
-  const int b = blockIdx.x;
   const int bxi = b % a.nbx, byi = (b / a.nbx) % a.nby, bzi = b / (a.nbx * a.nby);
   const int x0 = brick_lo(bxi, a.nx, a.nbx), wx = brick_lo(bxi + 1, a.nx, a.nbx) - x0;
   const int y0 = brick_lo(byi, a.ny, a.nby), wy = brick_lo(byi + 1, a.ny, a.nby) - y0;
@@ -202,11 +201,19 @@ __global__ __launch_bounds__(kSpreadBlock) void k_gather_brick(const float4* __r
         const float fx = q * a.r1 * f1 * (float)a.nx;
         const float fy = q * a.r2 * f2 * (float)a.ny;
         const float fz = q * a.r3 * f3 * (float)a.nz;
-        store_force<FT>(force, s_meta[ai].w, stride, fx, fy, fz);
+        store_force(ft, force, s_meta[ai].w, stride, fx, fy, fz);
       }
     }
     __syncthreads();
   }
+}
+
+template <int P>
+__global__ __launch_bounds__(kSpreadBlock) void k_gather_brick(const float4* __restrict__ xyzq, BinArgs a, int wxm,
+                                                               int wym, int wzm, const float* __restrict__ phi,
+                                                               float ccelec_f, int ft, int stride, void* force) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  gather_brick_body<P>(xyzq, a, blockIdx.x, wxm, wym, wzm, phi, ccelec_f, ft, stride, force, smem);
 }
 
 size_t gather_lds_bytes(int order, const int wmax[3], int nchunk) {
@@ -220,20 +227,8 @@ static hipError_t launch_gather_p(int ft, const float4* xyzq, const BinArgs& a, 
                                   int stride, void* force, hipStream_t s) {
   const int nbins = a.nbx * a.nby * a.nbz;
   const size_t lds = gather_lds_bytes(P, wmax, a.nchunk);
-  const float cc = (float)kCcelec;
-  switch (ft) {
-    case 0:
-      hipLaunchKernelGGL((k_gather_brick<P, 0>), dim3(nbins), dim3(kSpreadBlock), lds, s, xyzq, a, wmax[0], wmax[1], wmax[2], phi, cc, stride, force);
-      break;
-    case 1:
-      hipLaunchKernelGGL((k_gather_brick<P, 1>), dim3(nbins), dim3(kSpreadBlock), lds, s, xyzq, a, wmax[0], wmax[1], wmax[2], phi, cc, stride, force);
-      break;
-    case 2:
-      hipLaunchKernelGGL((k_gather_brick<P, 2>), dim3(nbins), dim3(kSpreadBlock), lds, s, xyzq, a, wmax[0], wmax[1], wmax[2], phi, cc, stride, force);
-      break;
-    default:
-      return hipErrorInvalidValue;
-  }
+  hipLaunchKernelGGL(k_gather_brick<P>, dim3(nbins), dim3(kSpreadBlock), lds, s, xyzq, a, wmax[0], wmax[1], wmax[2], phi,
+                     (float)kCcelec, ft, stride, force);
   return hipGetLastError();
 }
 
@@ -245,6 +240,33 @@ hipError_t launch_gather(int order, int force_type, const float4* xyzq, const Bi
     case 8: return launch_gather_p<8>(force_type, xyzq, a, wmax, phi, stride, force, s);
     default: return hipErrorInvalidValue;
   }
+}
+
+// ---- batched: persistent workgroups over the work items of many systems ----
+
+template <int P>
+__global__ __launch_bounds__(kSpreadBlock) void k_batch_gather(const SysDesc* __restrict__ sys,
+                                                               const WorkItem* __restrict__ items, int nitems) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+    const WorkItem w = items[it];
+    const SysDesc& d = sys[w.sys];
+    gather_brick_body<P>(d.xyzq, d.bin, w.idx, d.wmax[0], d.wmax[1], d.wmax[2], d.grid, (float)kCcelec, d.force_type,
+                         d.stride, d.force, smem);
+    __syncthreads();
+  }
+}
+
+hipError_t launch_batch_gather(int order, const SysDesc* sys, const WorkItem* items, int nitems, int grid, size_t lds,
+                               hipStream_t s) {
+  if (nitems <= 0) return hipSuccess;
+  switch (order) {
+    case 4: hipLaunchKernelGGL(k_batch_gather<4>, dim3(grid), dim3(kSpreadBlock), lds, s, sys, items, nitems); break;
+    case 6: hipLaunchKernelGGL(k_batch_gather<6>, dim3(grid), dim3(kSpreadBlock), lds, s, sys, items, nitems); break;
+    case 8: hipLaunchKernelGGL(k_batch_gather<8>, dim3(grid), dim3(kSpreadBlock), lds, s, sys, items, nitems); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
 }
 
 }  // namespace pme

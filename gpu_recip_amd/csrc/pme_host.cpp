@@ -13,13 +13,14 @@
 
 #include "../../include/pme_recip.h"
 #include "pme_internal.hpp"
+#include "pme_plan.hpp"
 
 using namespace pme;
 
 // ------------------------------------------------------------------ errors --
 static thread_local std::string g_err;
 
-static pme_status fail(pme_status s, const char* fmt, ...) {
+pme_status pme::set_error(pme_status s, const char* fmt, ...) {
   char buf[512];
   va_list ap;
   va_start(ap, fmt);
@@ -28,6 +29,8 @@ static pme_status fail(pme_status s, const char* fmt, ...) {
   g_err = buf;
   return s;
 }
+
+#define fail pme::set_error
 
 #define HIP_TRY(expr)                                                                     \
   do {                                                                                    \
@@ -204,68 +207,10 @@ extern "C" pme_status pme_ev_virial_pointer(pme_ev_t ev, double** dptr) {
 // -------------------------------------------------------------- host math --
 static const double kPi = 3.14159265358979323846;
 
-// B-spline values at w (FP64), fill_bspline_host CudaPMERecip.cu:2507-2537
-static void bspline_host(int order, double w, double* th) {
-  th[order - 1] = 0.0;
-  th[1] = w;
-  th[0] = 1.0 - w;
-  for (int k = 3; k <= order; k++) {
-    const double div = 1.0 / (double)(k - 1);
-    th[k - 1] = div * w * th[k - 2];
-    for (int j = 1; j <= k - 2; j++) th[k - j - 1] = div * ((w + j) * th[k - j - 2] + (k - j - w) * th[k - j - 1]);
-    th[0] = div * (1.0 - w) * th[0];
-  }
-}
-
-// bsp_mod[k] = 1/|sum_{j=0}^{n-2} arr[j] e^{2 pi i jk/n}|^2 with arr[1..p] the
-// B-spline at the integer nodes, values < 1e-10 replaced by the neighbour
-// mean; dftmod + calc_prefac, CudaPMERecip.cu:2481-2505, 2544-2579.  FP64.
-static void bsp_mod_host(int order, int n, double* out) {
-  std::vector<double> arr(std::max(n, order + 1), 0.0), th(order);
-  bspline_host(order, 0.0, th.data());
-  for (int i = 2; i <= order + 1; i++) arr[i - 1] = th[i - 2];
-  for (int k = 0; k < n; k++) {
-    double s1 = 0.0, s2 = 0.0;
-    for (int j = 0; j < n - 1; j++) {
-      const long long jk = ((long long)j * k) % n;
-      const double arg = 2.0 * kPi * (double)jk / (double)n;
-      s1 += arr[j] * std::cos(arg);
-      s2 += arr[j] * std::sin(arg);
-    }
-    out[k] = s1 * s1 + s2 * s2;
-  }
-  for (int k = 0; k < n; k++)
-    if (out[k] < 1.0e-10) out[k] = 0.5 * (out[(k + n - 1) % n] + out[(k + 1) % n]);
-  for (int k = 0; k < n; k++) out[k] = 1.0 / out[k];
-}
-
 extern "C" pme_status pme_bsp_mod(int order, int n, double* out) {
   if (order < 3 || order > 16 || n < 2 || !out) return fail(PME_ERR_INVALID_ARG, "pme_bsp_mod: bad arguments");
   bsp_mod_host(order, n, out);
   return PME_OK;
-}
-
-static bool smooth235(int n) {
-  if (n < 1) return false;
-  for (int f : {2, 3, 5})
-    while (n % f == 0) n /= f;
-  return n == 1;
-}
-
-// Stockham radices for length n: 8s, then 4, 2, 5, 3 (fewest stages)
-static bool radix_plan(int n, int tstride, RadixPlan* rp) {
-  rp->n = n;
-  rp->tstride = tstride;
-  rp->nrad = 0;
-  int m = n;
-  while (m % 8 == 0 && m > 8) { rp->rad[rp->nrad++] = 8; m /= 8; }
-  while (m % 4 == 0) { rp->rad[rp->nrad++] = 4; m /= 4; }
-  while (m % 2 == 0) { rp->rad[rp->nrad++] = 2; m /= 2; }
-  while (m % 8 == 0) { rp->rad[rp->nrad++] = 8; m /= 8; }
-  while (m % 5 == 0) { rp->rad[rp->nrad++] = 5; m /= 5; }
-  while (m % 3 == 0) { rp->rad[rp->nrad++] = 3; m /= 3; }
-  if (rp->nrad > 12) return false;
-  return m == 1;
 }
 
 // ------------------------------------------------------------------ handle --
@@ -278,9 +223,6 @@ enum GridState {
   ST_POTENTIAL       // real grid holds the potential
 };
 
-static const size_t kLdsBudget = 64 * 1024;        // FFT kernels (several workgroups per CU)
-static const size_t kBrickLdsBudget = 150 * 1024;  // spread/gather: one 1024-thread workgroup per CU (160 KB LDS)
-static const int kMaxBins = 16384;   // LDS histogram of the binning kernel
 
 struct pme_recip_s {
   int nx = 0, ny = 0, nz = 0, order = 0, fft_type = PME_FFT_BOX, accum = PME_ACCUM_FLOAT;
@@ -390,39 +332,24 @@ static pme_status upload_prefac(pme_recip_t h) {
   return PME_OK;
 }
 
-// Brick plan: about 16 points per brick edge, every brick at least order-1
-// wide (a stencil then spans at most two bricks per axis), the spread and
-// gather LDS footprints within kLdsBudget and at most kMaxBins bricks.
 static bool plan_bricks(pme_recip_t h) {
   const int n[3] = {h->nx, h->ny, h->nz};
-  const int target = 16;
+  BrickPlan bp;
+  if (!pme::plan_bricks(n, h->order, kDefaultBrick, &bp)) return false;
   for (int a = 0; a < 3; a++) {
-    int nb = (n[a] + target / 2) / target;
-    if (nb < 1) nb = 1;
-    while (nb > 1 && n[a] / nb < h->order - 1) nb--;
-    h->nb[a] = nb;
-    h->wmax[a] = (n[a] + nb - 1) / nb;
+    h->nb[a] = bp.nb[a];
+    h->wmax[a] = bp.wmax[a];
   }
-  while (spread_lds_bytes(h->order, h->wmax, kMaxChunks) > kBrickLdsBudget ||
-         gather_lds_bytes(h->order, h->wmax, kMaxChunks) > kBrickLdsBudget) {
-    int best = -1;
-    for (int a = 0; a < 3; a++)
-      if (n[a] / (h->nb[a] + 1) >= h->order - 1 && (best < 0 || h->wmax[a] > h->wmax[best])) best = a;
-    if (best < 0) return false;
-    h->nb[best]++;
-    h->wmax[best] = (n[best] + h->nb[best] - 1) / h->nb[best];
-  }
-  h->nbins = h->nb[0] * h->nb[1] * h->nb[2];
-  return h->nbins <= kMaxBins;
+  h->nbins = bp.nbins;
+  return true;
 }
 
 // Size the record store for n atoms: chunks of kMinChunk atoms (more per
 // chunk beyond kMaxChunks chunks), 8 record slots per atom (a stencil touches
 // at most 8 bricks).
 static pme_status ensure_bins(pme_recip_t h, int n) {
-  int chunk = kMinChunk;
-  while ((long long)chunk * kMaxChunks < n) chunk *= 2;
-  const int nchunk = std::max(1, (n + chunk - 1) / chunk);
+  const ChunkPlan cp = plan_chunks(n);
+  const int chunk = cp.chunk, nchunk = cp.nchunk;
   if (n <= h->bins_for && chunk == h->chunk && nchunk <= h->nchunk_cap) {
     h->nchunk = nchunk;
     return PME_OK;
@@ -503,18 +430,10 @@ static SolveArgs solve_args(pme_recip_t h, const double* recip, double kappa, bo
 }
 
 static void pick_tiles(pme_recip_t h) {
-  const int hx = h->nx / 2 + 1;
-  const int nf = h->rp[0].n;
-  h->NR = nf <= 64 ? 16 : (nf <= 128 ? 8 : (nf <= 256 ? 4 : 2));
-  while (h->NR > 1 && fft_x_lds_bytes(nf, h->NR, h->nx) > kLdsBudget) h->NR /= 2;
-  auto tile = [&](int n) {
-    int tx = 16;
-    while (tx > 1 && fft_cols_lds_bytes(n, tx) > kLdsBudget) tx /= 2;
-    if (tx > hx) tx = hx;
-    return tx;
-  };
-  h->TXy = tile(h->ny);
-  h->TXz = tile(h->nz);
+  const FftTiles t = pick_fft_tiles(h->nx, h->ny, h->nz, h->rp[0].n, kDefaultFftTile);
+  h->NR = t.NR;
+  h->TXy = t.TXy;
+  h->TXz = t.TXz;
 }
 
 extern "C" pme_status pme_recip_create(int nfftx, int nffty, int nfftz, int order, int fft_type, int nnode,
@@ -526,8 +445,8 @@ extern "C" pme_status pme_recip_create(int nfftx, int nffty, int nfftz, int orde
     return fail(PME_ERR_INVALID_ARG, "pme_recip_create: fft_type %d", fft_type);
   if (nnode != 1 || mynode != 0)
     return fail(PME_ERR_UNSUPPORTED, "pme_recip_create: nnode=%d: one GPU per solver", nnode);
-  if (accum_type != PME_ACCUM_FLOAT)
-    return fail(PME_ERR_UNSUPPORTED, "pme_recip_create: accum_type %d not available (float only)", accum_type);
+  if (accum_type != PME_ACCUM_FLOAT && accum_type != PME_ACCUM_INT32 && accum_type != PME_ACCUM_INT64)
+    return fail(PME_ERR_INVALID_ARG, "pme_recip_create: accum_type %d", accum_type);
   if (order != 4 && order != 6 && order != 8)
     return fail(PME_ERR_UNSUPPORTED, "pme_recip_create: order %d (4, 6 or 8)", order);
   const int n[3] = {nfftx, nffty, nfftz};
@@ -564,11 +483,7 @@ extern "C" pme_status pme_recip_create(int nfftx, int nffty, int nfftz, int orde
     for (int a = 0; a < 3; a++) {
       HIP_TRY(hipMalloc(&h->prefac[a], sizeof(float) * n[a]));
       HIP_TRY(hipMalloc(&h->tw[a], sizeof(float2) * n[a]));
-      std::vector<float2> t(n[a]);
-      for (int k = 0; k < n[a]; k++) {
-        const double ang = 2.0 * kPi * (double)k / (double)n[a];
-        t[k] = make_float2((float)std::cos(ang), (float)(-std::sin(ang)));
-      }
+      const std::vector<float2> t = twiddles(n[a]);
       HIP_TRY(hipMemcpy(h->tw[a], t.data(), sizeof(float2) * n[a], hipMemcpyHostToDevice));
     }
     const bool even = (h->nx % 2) == 0;

@@ -62,6 +62,34 @@ struct SolveArgs {
 
 enum ColMode { COL_FWD = 0, COL_INV = 1, COL_FWD_SOLVE_INV = 2, COL_FWD_INV = 3 };
 
+// ---- batched path: many independent systems per launch ----------------------
+// Everything the stage bodies need for one system, resident on the device.
+struct SysDesc {
+  BinArgs bin;             // grid, recip, bricks and record store
+  int wmax[3];             // largest brick per axis
+  const float4* xyzq;      // caller's coordinates (x, y, z, q)
+  int natom;
+  float* grid;             // real grid
+  float2* cgrid;           // half spectrum
+  AxisArgs ax[3];          // FFT plans and twiddles
+  int NR, TXy, TXz;        // FFT tiles
+  SolveArgs sa;            // influence function; sa.partial = this system's partials
+  int nzblk;               // z-pass tiles (= partial count)
+  void* force;             // caller's force output
+  int force_type, stride;
+  double* ev_out;          // [10]: energy, virial[9] of the last solve (assigned)
+};
+
+// One workgroup-sized task of a batched stage: system index and the tile of
+// that system (chunk, brick, row tile or column tile).
+struct WorkItem {
+  int sys;
+  int idx;
+};
+
+// Batched stages; each has its own work-item list.
+enum BatchStage { BS_BIN = 0, BS_SPREAD, BS_FFTX, BS_FFTY, BS_FFTZ, BS_GATHER, BS_COUNT };
+
 // Segments of brick b in the chunked record store: s_base[w] = first record
 // of b in chunk w's region, s_pre[w] = exclusive prefix of the counts over w
 // (s_pre[nchunk] = total).  Returns the total.  NT == blockDim.x.
@@ -152,6 +180,19 @@ __device__ __forceinline__ void ev_reduce_block(const EvReduce& r) {
     if (threadIdx.x == 0) r.self_slot[0] += -v[0] * r.kcs;
   }
 }
+
+// batched launchers: `grid` persistent workgroups loop over the work items
+hipError_t launch_batch_bin(const SysDesc* sys, const WorkItem* items, int nitems, int grid, size_t lds, hipStream_t s);
+hipError_t launch_batch_spread(int order, const SysDesc* sys, const WorkItem* items, int nitems, int grid, size_t lds,
+                               hipStream_t s);
+hipError_t launch_batch_gather(int order, const SysDesc* sys, const WorkItem* items, int nitems, int grid, size_t lds,
+                               hipStream_t s);
+hipError_t launch_batch_fft_x(int inverse, const SysDesc* sys, const WorkItem* items, int nitems, int grid, size_t lds,
+                              hipStream_t s);
+// axis 1 (y) or 2 (z); mode COL_*; with reduce != 0 the first tile of every
+// system sums that system's energy/virial partials into ev_out
+hipError_t launch_batch_cols(int axis, int mode, int reduce, const SysDesc* sys, const WorkItem* items, int nitems,
+                             int grid, size_t lds, hipStream_t s);
 
 // LDS sizes (bytes) the launchers request, exposed for plan validation
 size_t bin_lds_bytes(int nbins);

@@ -1,0 +1,119 @@
+// pme_plan.cpp -- host-side planning (see pme_plan.hpp).
+#include "pme_plan.hpp"
+
+#include <algorithm>
+#include <cmath>
+
+namespace pme {
+
+static const double kPi = 3.14159265358979323846;
+
+// B-spline values at w (FP64), fill_bspline_host CudaPMERecip.cu:2507-2537
+static void bspline_host(int order, double w, double* th) {
+  th[order - 1] = 0.0;
+  th[1] = w;
+  th[0] = 1.0 - w;
+  for (int k = 3; k <= order; k++) {
+    const double div = 1.0 / (double)(k - 1);
+    th[k - 1] = div * w * th[k - 2];
+    for (int j = 1; j <= k - 2; j++) th[k - j - 1] = div * ((w + j) * th[k - j - 2] + (k - j - w) * th[k - j - 1]);
+    th[0] = div * (1.0 - w) * th[0];
+  }
+}
+
+void bsp_mod_host(int order, int n, double* out) {
+  std::vector<double> arr(std::max(n, order + 1), 0.0), th(order);
+  bspline_host(order, 0.0, th.data());
+  for (int i = 2; i <= order + 1; i++) arr[i - 1] = th[i - 2];
+  for (int k = 0; k < n; k++) {
+    double s1 = 0.0, s2 = 0.0;
+    for (int j = 0; j < n - 1; j++) {
+      const long long jk = ((long long)j * k) % n;
+      const double arg = 2.0 * kPi * (double)jk / (double)n;
+      s1 += arr[j] * std::cos(arg);
+      s2 += arr[j] * std::sin(arg);
+    }
+    out[k] = s1 * s1 + s2 * s2;
+  }
+  for (int k = 0; k < n; k++)
+    if (out[k] < 1.0e-10) out[k] = 0.5 * (out[(k + n - 1) % n] + out[(k + 1) % n]);
+  for (int k = 0; k < n; k++) out[k] = 1.0 / out[k];
+}
+
+bool smooth235(int n) {
+  if (n < 1) return false;
+  for (int f : {2, 3, 5})
+    while (n % f == 0) n /= f;
+  return n == 1;
+}
+
+bool radix_plan(int n, int tstride, RadixPlan* rp) {
+  rp->n = n;
+  rp->tstride = tstride;
+  rp->nrad = 0;
+  int m = n;
+  while (m % 8 == 0 && m > 8) { rp->rad[rp->nrad++] = 8; m /= 8; }
+  while (m % 4 == 0) { rp->rad[rp->nrad++] = 4; m /= 4; }
+  while (m % 2 == 0) { rp->rad[rp->nrad++] = 2; m /= 2; }
+  while (m % 8 == 0) { rp->rad[rp->nrad++] = 8; m /= 8; }
+  while (m % 5 == 0) { rp->rad[rp->nrad++] = 5; m /= 5; }
+  while (m % 3 == 0) { rp->rad[rp->nrad++] = 3; m /= 3; }
+  if (rp->nrad > 12) return false;
+  return m == 1;
+}
+
+std::vector<float2> twiddles(int n) {
+  std::vector<float2> t(n);
+  for (int k = 0; k < n; k++) {
+    const double ang = 2.0 * kPi * (double)k / (double)n;
+    t[k] = make_float2((float)std::cos(ang), (float)(-std::sin(ang)));
+  }
+  return t;
+}
+
+bool plan_bricks(const int n[3], int order, int target, BrickPlan* bp) {
+  for (int a = 0; a < 3; a++) {
+    int nb = (n[a] + target / 2) / target;
+    if (nb < 1) nb = 1;
+    while (nb > 1 && n[a] / nb < order - 1) nb--;
+    bp->nb[a] = nb;
+    bp->wmax[a] = (n[a] + nb - 1) / nb;
+  }
+  while (spread_lds_bytes(order, bp->wmax, kMaxChunks) > kBrickLdsBudget ||
+         gather_lds_bytes(order, bp->wmax, kMaxChunks) > kBrickLdsBudget) {
+    int best = -1;
+    for (int a = 0; a < 3; a++)
+      if (n[a] / (bp->nb[a] + 1) >= order - 1 && (best < 0 || bp->wmax[a] > bp->wmax[best])) best = a;
+    if (best < 0) return false;
+    bp->nb[best]++;
+    bp->wmax[best] = (n[best] + bp->nb[best] - 1) / bp->nb[best];
+  }
+  bp->nbins = bp->nb[0] * bp->nb[1] * bp->nb[2];
+  return bp->nbins <= kMaxBins && bin_lds_bytes(bp->nbins) <= kLdsBudget;
+}
+
+FftTiles pick_fft_tiles(int nx, int ny, int nz, int nf_x, int tile_target) {
+  FftTiles t;
+  const int hx = nx / 2 + 1;
+  t.NR = nf_x <= 64 ? 16 : (nf_x <= 128 ? 8 : (nf_x <= 256 ? 4 : 2));
+  while (t.NR > 1 && fft_x_lds_bytes(nf_x, t.NR, nx) > kLdsBudget) t.NR /= 2;
+  auto tile = [&](int n) {
+    int tx = tile_target;
+    while (tx > 1 && fft_cols_lds_bytes(n, tx) > kLdsBudget) tx /= 2;
+    if (tx > hx) tx = hx;
+    return tx;
+  };
+  t.TXy = tile(ny);
+  t.TXz = tile(nz);
+  return t;
+}
+
+ChunkPlan plan_chunks(int natom) {
+  ChunkPlan c;
+  c.chunk = kMinChunk;
+  while ((long long)c.chunk * kMaxChunks < natom) c.chunk *= 2;
+  c.nchunk = std::max(1, (natom + c.chunk - 1) / c.chunk);
+  return c;
+}
+
+}  // namespace pme

@@ -56,12 +56,12 @@ __device__ __forceinline__ int next_brick(int b, int d, int nb) {
   return b >= nb ? b - nb : b;
 }
 
-__global__ __launch_bounds__(kBlock) void k_bin_chunks(const float4* __restrict__ xyzq, int n, BinArgs a) {
-  extern __shared__ int hist[];  // [nbins]: counts, then cursors
+// Chunk w of the counting sort (one workgroup).  hist: nbins ints of LDS.
+__device__ __forceinline__ void bin_chunk_body(const float4* __restrict__ xyzq, int n, const BinArgs& a, int w,
+                                               int* hist) {
   __shared__ int sh_scan[kBlock / 64];
   const int nbins = a.nbx * a.nby * a.nbz;
   const int tid = threadIdx.x;
-  const int w = blockIdx.x;
   const int a0 = w * a.chunk;
   const int a1 = min(a0 + a.chunk, n);
   for (int i = tid; i < nbins; i += kBlock) hist[i] = 0;
@@ -97,6 +97,11 @@ __global__ __launch_bounds__(kBlock) void k_bin_chunks(const float4* __restrict_
   }
 }
 
+__global__ __launch_bounds__(kBlock) void k_bin_chunks(const float4* __restrict__ xyzq, int n, BinArgs a) {
+  extern __shared__ int hist[];  // [nbins]: counts, then cursors
+  bin_chunk_body(xyzq, n, a, blockIdx.x, hist);
+}
+
 size_t bin_lds_bytes(int nbins) { return sizeof(int) * (size_t)nbins; }
 
 hipError_t launch_bin_atoms(const float4* xyzq, int n, const BinArgs& a, hipStream_t s) {
@@ -113,10 +118,11 @@ __device__ __forceinline__ int local_offset(int i, int x0, int w, int n) {
   return d;
 }
 
+// Brick b of one grid (one kSpreadBlock-thread workgroup); smem: the
+// workgroup's dynamic LDS (spread_lds_bytes).
 template <int P>
-__global__ __launch_bounds__(kSpreadBlock) void k_spread_brick(BinArgs a, int wxm, int wym, int wzm,
-                                                         float* __restrict__ grid) {
-  extern __shared__ __align__(16) unsigned char smem[];
+__device__ __forceinline__ void spread_brick_body(const BinArgs& a, int b, int wxm, int wym, int wzm,
+                                                  float* __restrict__ grid, unsigned char* smem) {
   __shared__ int sh_scan[kSpreadBlock / 64];
 
   const int bxs = wxm + 1;  // padded x stride of the LDS brick
@@ -128,7 +134,6 @@ __global__ __launch_bounds__(kSpreadBlock) void k_spread_brick(BinArgs a, int wx
   int* s_base = s_pre + a.nchunk + 1;                         // [nchunk]
 
   const int tid = threadIdx.x;
-  const int b = blockIdx.x;
   const int bxi = b % a.nbx, byi = (b / a.nbx) % a.nby, bzi = b / (a.nbx * a.nby);
   const int x0 = brick_lo(bxi, a.nx, a.nbx), wx = brick_lo(bxi + 1, a.nx, a.nbx) - x0;
   const int y0 = brick_lo(byi, a.ny, a.nby), wy = brick_lo(byi + 1, a.ny, a.nby) - y0;
@@ -206,6 +211,13 @@ __global__ __launch_bounds__(kSpreadBlock) void k_spread_brick(BinArgs a, int wx
   }
 }
 
+template <int P>
+__global__ __launch_bounds__(kSpreadBlock) void k_spread_brick(BinArgs a, int wxm, int wym, int wzm,
+                                                               float* __restrict__ grid) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  spread_brick_body<P>(a, blockIdx.x, wxm, wym, wzm, grid, smem);
+}
+
 size_t spread_lds_bytes(int order, const int wmax[3], int nchunk) {
   return sizeof(unsigned long long) * (size_t)wmax[2] * wmax[1] * (wmax[0] + 1) + sizeof(int) * 3 * kSpreadBlock +
          sizeof(float) * (size_t)kSpreadBlock * 3 * order + sizeof(int) * (2 * (size_t)nchunk + 1);
@@ -226,6 +238,50 @@ hipError_t launch_spread(int order, const BinArgs& a, const int wmax[3], float* 
       break;
     default:
       return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+// ---- batched: persistent workgroups over the work items of many systems ----
+
+__global__ __launch_bounds__(kBlock) void k_batch_bin(const SysDesc* __restrict__ sys,
+                                                      const WorkItem* __restrict__ items, int nitems) {
+  extern __shared__ int hist[];
+  for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+    const WorkItem w = items[it];
+    const SysDesc& d = sys[w.sys];
+    bin_chunk_body(d.xyzq, d.natom, d.bin, w.idx, hist);
+    __syncthreads();
+  }
+}
+
+template <int P>
+__global__ __launch_bounds__(kSpreadBlock) void k_batch_spread(const SysDesc* __restrict__ sys,
+                                                               const WorkItem* __restrict__ items, int nitems) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+    const WorkItem w = items[it];
+    const SysDesc& d = sys[w.sys];
+    spread_brick_body<P>(d.bin, w.idx, d.wmax[0], d.wmax[1], d.wmax[2], d.grid, smem);
+    __syncthreads();
+  }
+}
+
+hipError_t launch_batch_bin(const SysDesc* sys, const WorkItem* items, int nitems, int grid, size_t lds,
+                            hipStream_t s) {
+  if (nitems <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_batch_bin, dim3(grid), dim3(kBlock), lds, s, sys, items, nitems);
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_spread(int order, const SysDesc* sys, const WorkItem* items, int nitems, int grid, size_t lds,
+                               hipStream_t s) {
+  if (nitems <= 0) return hipSuccess;
+  switch (order) {
+    case 4: hipLaunchKernelGGL(k_batch_spread<4>, dim3(grid), dim3(kSpreadBlock), lds, s, sys, items, nitems); break;
+    case 6: hipLaunchKernelGGL(k_batch_spread<6>, dim3(grid), dim3(kSpreadBlock), lds, s, sys, items, nitems); break;
+    case 8: hipLaunchKernelGGL(k_batch_spread<8>, dim3(grid), dim3(kSpreadBlock), lds, s, sys, items, nitems); break;
+    default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }

@@ -151,3 +151,93 @@ def fill_bspline(order, nfft, recip, xyzq, ncoord, stream=None):
            _ptr(xyzq), int(ncoord), *[_ptr(t) for t in gi], _ptr(q), *[_ptr(t) for t in th],
            _stream_ptr(stream))
     return (*gi, q, *th)
+
+
+class PmeBatch:
+    """Many independent systems solved per launch (pme_batch_* in the C ABI).
+
+    Systems are admitted with add() against the HBM budget and stay resident;
+    solve() enqueues one solve of a subset (all by default) on the batch's
+    stream, results() waits and returns (energy[n], virial[n, 9]) as lists.
+    """
+
+    def __init__(self, hbm_budget_bytes: int, stream=None):
+        h = C.c_void_p()
+        L.call("pme_batch_create", C.c_size_t(int(hbm_budget_bytes)), _stream_ptr(stream), C.byref(h))
+        self.h = h
+        self._keep = {}  # id -> tensors the descriptor points at
+
+    def __del__(self):
+        if getattr(self, "h", None) is not None and L._lib is not None:
+            L._lib.pme_batch_destroy(self.h)
+            self.h = None
+
+    @staticmethod
+    def desc(nfft, order, kappa, recip, xyzq, ncoord, force, force_type=L.FORCE_FLOAT, stride=None,
+             calc_energy_virial=True) -> "L.SystemDesc":
+        d = L.SystemDesc()
+        d.nfftx, d.nffty, d.nfftz = (int(v) for v in nfft)
+        d.order = int(order)
+        d.kappa = float(kappa)
+        for i, v in enumerate(recip):
+            d.recip[i] = float(v)
+        d.ncoord = int(ncoord)
+        d.d_xyzq = _ptr(xyzq)
+        d.d_force = _ptr(force)
+        d.force_type = int(force_type)
+        d.stride = int(ncoord if stride is None else stride)
+        d.calc_energy_virial = int(bool(calc_energy_virial))
+        return d
+
+    def add(self, desc, keep=()) -> int:
+        i = C.c_int()
+        L.call("pme_batch_add", self.h, C.byref(desc), C.byref(i))
+        self._keep[i.value] = tuple(keep)
+        return i.value
+
+    def update(self, sid: int, desc, keep=()) -> None:
+        L.call("pme_batch_update", self.h, int(sid), C.byref(desc))
+        self._keep[sid] = tuple(keep)
+
+    def remove(self, sid: int) -> None:
+        L.call("pme_batch_remove", self.h, int(sid))
+        self._keep.pop(sid, None)
+
+    @staticmethod
+    def _ids(ids):
+        if ids is None:
+            return 0, None
+        arr = (C.c_int * len(ids))(*[int(i) for i in ids])
+        return len(ids), arr
+
+    def solve(self, ids=None) -> None:
+        n, arr = self._ids(ids)
+        L.call("pme_batch_solve", self.h, n, arr)
+
+    def results(self, ids):
+        n, arr = self._ids(ids)
+        e = (C.c_double * n)()
+        v = (C.c_double * (9 * n))()
+        L.call("pme_batch_results", self.h, n, arr, e, v)
+        return list(e), [list(v[9 * k:9 * k + 9]) for k in range(n)]
+
+    def set_graph(self, enable: bool) -> None:
+        L.call("pme_batch_set_graph", self.h, int(bool(enable)))
+
+    def set_tuning(self, brick_target: int, fft_tile: int, wg_per_cu: int) -> None:
+        L.call("pme_batch_set_tuning", self.h, int(brick_target), int(fft_tile), int(wg_per_cu))
+
+    def get_tuning(self):
+        v = [C.c_int() for _ in range(3)]
+        L.call("pme_batch_get_tuning", self.h, *[C.byref(x) for x in v])
+        return tuple(x.value for x in v)
+
+    def autotune(self, cache_path=None) -> float:
+        ms = C.c_float()
+        L.call("pme_batch_autotune", self.h, None if cache_path is None else str(cache_path).encode(), C.byref(ms))
+        return ms.value
+
+    def memory(self):
+        u, b = C.c_size_t(), C.c_size_t()
+        L.call("pme_batch_memory", self.h, C.byref(u), C.byref(b))
+        return u.value, b.value

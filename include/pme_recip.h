@@ -42,7 +42,11 @@ typedef struct pme_recip_s* pme_recip_t;          /* CudaPMERecip<AT,CT,CT2> */
 enum { PME_FFT_COLUMN = 0, PME_FFT_SLAB = 1, PME_FFT_BOX = 2 };
 
 /* AT template argument (accumulation type of the charge grid),
- * CudaPMERecip.cu:2584-2600. */
+ * CudaPMERecip.cu:2584-2600.  The engine accumulates every charge grid as
+ * 64-bit fixed point with scale 2^40 (the reference's `long long` mode,
+ * CudaPMERecip.cu:51-58), which is order independent and at least as precise
+ * as the reference's float and 2^31 int modes; all three values are accepted
+ * and give the same grid. */
 enum { PME_ACCUM_FLOAT = 0, PME_ACCUM_INT32 = 1, PME_ACCUM_INT64 = 2 };
 
 /* FT template argument of gather_force, CudaPMERecip.cu:985-1023 */
@@ -144,6 +148,59 @@ pme_status pme_fill_bspline(int order, int nfftx, int nffty, int nfftz,
                             float* d_thetax, float* d_thetay, float* d_thetaz,
                             float* d_dthetax, float* d_dthetay, float* d_dthetaz,
                             pme_stream_t stream);
+
+/* ---------------- batched path (no counterpart in the reference) ----------------
+ * Many independent orthorhombic systems solved together: every stage is one
+ * launch over all systems (persistent workgroups over per-stage work lists),
+ * and the whole sequence is captured once in a hipGraph and replayed.  Each
+ * system keeps its grids, FFT plans and binning store resident in HBM
+ * ("slots"); admission is checked against the HBM budget of the batch.
+ * A solve of a system is exactly the five-call sequence of the solver above
+ * (same kernels, same results), energy and virial assigned per system. */
+typedef struct pme_batch_s* pme_batch_t;
+
+typedef struct pme_system_desc {
+  int nfftx, nffty, nfftz;   /* grid (2/3/5-smooth, >= order, <= 512) */
+  int order;                 /* 4, 6 or 8 */
+  double kappa;              /* 1/A */
+  double recip[9];           /* row-major reciprocal box, diagonal */
+  int ncoord;                /* atoms */
+  const void* d_xyzq;        /* device float4[ncoord] */
+  void* d_force;             /* device output, laid out per force_type */
+  int force_type;            /* PME_FORCE_* (INT64 adds, the others store) */
+  int stride;                /* SoA stride for FLOAT / INT64 (>= ncoord) */
+  int calc_energy_virial;    /* energy and virial of this system */
+} pme_system_desc;
+
+pme_status pme_batch_create(size_t hbm_budget_bytes, pme_stream_t stream, pme_batch_t* out);
+pme_status pme_batch_destroy(pme_batch_t b);
+/* Admit a system: plan it and allocate its resident state.  PME_ERR_BUDGET
+ * if it does not fit the remaining HBM budget (nothing is allocated then). */
+pme_status pme_batch_add(pme_batch_t b, const pme_system_desc* sys, int* id);
+/* New coordinates/box/outputs for a resident system (ncoord may grow up to
+ * the capacity it was admitted with; a larger ncoord re-plans the slot). */
+pme_status pme_batch_update(pme_batch_t b, int id, const pme_system_desc* sys);
+pme_status pme_batch_remove(pme_batch_t b, int id);
+/* Enqueue one solve of the listed systems (n = 0: every resident system) on
+ * the batch's stream.  Forces are written on the device; energies/virials
+ * are read with pme_batch_results. */
+pme_status pme_batch_solve(pme_batch_t b, int n, const int* ids);
+/* Wait for the last solve and copy energy[n] and virial[n*9] of the listed
+ * systems (either pointer may be NULL). */
+pme_status pme_batch_results(pme_batch_t b, int n, const int* ids, double* energy, double* virial);
+/* hipGraph capture/replay of the solve sequence (default on). */
+pme_status pme_batch_set_graph(pme_batch_t b, int enable);
+/* Tuning knobs: brick edge target for spread/gather (>= 8), y/z FFT tile
+ * width target (power of two), persistent workgroups per CU (0: one
+ * workgroup per work item).  Re-plans every resident system. */
+pme_status pme_batch_set_tuning(pme_batch_t b, int brick_target, int fft_tile, int wg_per_cu);
+pme_status pme_batch_get_tuning(pme_batch_t b, int* brick_target, int* fft_tile, int* wg_per_cu);
+/* Autotuner: time the current resident set over a grid of tuning choices,
+ * keep the fastest, and persist it in a JSON cache keyed by the set's
+ * signature and the GPU architecture (cache_path may be NULL).  Returns the
+ * best time per solve (ms). */
+pme_status pme_batch_autotune(pme_batch_t b, const char* cache_path, float* best_ms);
+pme_status pme_batch_memory(pme_batch_t b, size_t* used_bytes, size_t* budget_bytes);
 
 /* Host FP64 bsp_mod table (calc_prefac/dftmod, CudaPMERecip.cu:2481-2579). */
 pme_status pme_bsp_mod(int order, int n, double* out);

@@ -1,0 +1,157 @@
+// pme_recip.hpp -- C++ drop-in for the reference's CudaPMERecip /
+// CudaEnergyVirial classes (ap-hynninen/gpu_recip CudaPMERecip.h:37-205,
+// CudaEnergyVirial.h:20-53), header-only over the C ABI of pme_recip.h.
+//
+// Same class names, member names and argument meaning:
+//   CudaEnergyVirial ev;
+//   CudaPMERecip<float, float, float2> pme(nfftx, nffty, nfftz, order, BOX, 1, 0, ev, "recip", "self");
+//   ev.clear();
+//   pme.spread_charge(d_xyzq, ncoord, recip);
+//   pme.r2c_fft();
+//   pme.scalar_sum(recip, kappa, true, true);
+//   pme.c2r_fft();
+//   pme.gather_force(d_xyzq, ncoord, recip, stride, d_force);   // FT = float, float3 or long long
+//   ev.copyToHost(); hipDeviceSynchronize(); ev.getEnergy("recip"); ev.getVirial(vir);
+//
+// Error behaviour: the reference prints and calls exit(1); this header throws
+// pme::Error (a std::runtime_error carrying the pme_status) instead.
+#ifndef PME_RECIP_HPP
+#define PME_RECIP_HPP
+
+#include <hip/hip_runtime.h>
+
+#include <stdexcept>
+#include <string>
+#include <type_traits>
+
+#include "pme_recip.h"
+
+namespace pme {
+
+class Error : public std::runtime_error {
+ public:
+  Error(pme_status s, const std::string& where)
+      : std::runtime_error(where + ": " + pme_status_string(s) + ": " + pme_last_error()), status(s) {}
+  pme_status status;
+};
+
+inline void check(pme_status s, const char* where) {
+  if (s != PME_OK) throw Error(s, where);
+}
+
+}  // namespace pme
+
+// FFTtype, CudaPMERecip.h:30
+enum FFTtype { COLUMN = PME_FFT_COLUMN, SLAB = PME_FFT_SLAB, BOX = PME_FFT_BOX };
+
+// CudaEnergyVirial: named FP64 energy terms + 3x3 virial in one device blob
+class CudaEnergyVirial {
+ public:
+  CudaEnergyVirial() { pme::check(pme_ev_create(&h_), "CudaEnergyVirial"); }
+  ~CudaEnergyVirial() { pme_ev_destroy(h_); }
+  CudaEnergyVirial(const CudaEnergyVirial&) = delete;
+  CudaEnergyVirial& operator=(const CudaEnergyVirial&) = delete;
+
+  void insert(const char* name) { pme::check(pme_ev_insert(h_, name), "insert"); }
+  void insert(const std::string& name) { insert(name.c_str()); }
+  void clear(hipStream_t stream = 0) { pme::check(pme_ev_clear(h_, stream), "clear"); }
+  void clearEnergy(hipStream_t stream = 0) { pme::check(pme_ev_clear_energy(h_, stream), "clearEnergy"); }
+  void clearVirial(hipStream_t stream = 0) { pme::check(pme_ev_clear_virial(h_, stream), "clearVirial"); }
+  void copyToHost(hipStream_t stream = 0) { pme::check(pme_ev_copy_to_host(h_, stream), "copyToHost"); }
+  double* getEnergyPointer(const std::string& name) {
+    double* p = nullptr;
+    pme::check(pme_ev_energy_pointer(h_, name.c_str(), &p), "getEnergyPointer");
+    return p;
+  }
+  double* getVirialPointer() {
+    double* p = nullptr;
+    pme::check(pme_ev_virial_pointer(h_, &p), "getVirialPointer");
+    return p;
+  }
+  double getEnergy(const char* name) {
+    double e = 0.0;
+    pme::check(pme_ev_get_energy(h_, name, &e), "getEnergy");
+    return e;
+  }
+  double getEnergy(const std::string& name) { return getEnergy(name.c_str()); }
+  void getVirial(double* virmat) { pme::check(pme_ev_get_virial(h_, virmat), "getVirial"); }
+  void getSforce(double* sforce) { pme::check(pme_ev_get_sforce(h_, sforce), "getSforce"); }
+
+  pme_ev_t handle() const { return h_; }
+
+ private:
+  pme_ev_t h_ = nullptr;
+};
+
+// CudaPMERecip<AT, CT, CT2>: AT = accumulation type of the charge grid
+// (float, int or long long: every one accumulates as 2^40 fixed point, see
+// PME_ACCUM_* in pme_recip.h), CT/CT2 = float/float2.
+template <typename AT, typename CT, typename CT2>
+class CudaPMERecip {
+  static_assert(std::is_same<CT, float>::value && std::is_same<CT2, float2>::value,
+                "the MI355X engine computes in FP32 (CT = float, CT2 = float2)");
+
+ public:
+  CudaPMERecip(int nfftx, int nffty, int nfftz, int order, FFTtype fft_type, int nnode, int mynode,
+               CudaEnergyVirial& energyVirial, const char* nameRecip, const char* nameSelf,
+               hipStream_t stream = 0)
+      : nfftx_(nfftx), nffty_(nffty), nfftz_(nfftz), order_(order) {
+    pme::check(pme_recip_create(nfftx, nffty, nfftz, order, (int)fft_type, nnode, mynode, accum_type(),
+                                energyVirial.handle(), nameRecip, nameSelf, stream, &h_),
+               "CudaPMERecip");
+  }
+  ~CudaPMERecip() { pme_recip_destroy(h_); }
+  CudaPMERecip(const CudaPMERecip&) = delete;
+  CudaPMERecip& operator=(const CudaPMERecip&) = delete;
+
+  void set_stream(hipStream_t stream) { pme::check(pme_recip_set_stream(h_, stream), "set_stream"); }
+
+  void spread_charge(const float4* xyzq, const int ncoord, const double* recip) {
+    pme::check(pme_recip_spread_charge(h_, xyzq, ncoord, recip), "spread_charge");
+  }
+  void r2c_fft() { pme::check(pme_recip_r2c_fft(h_), "r2c_fft"); }
+  void scalar_sum(const double* recip, const double kappa, const bool calc_energy, const bool calc_virial) {
+    pme::check(pme_recip_scalar_sum(h_, recip, kappa, calc_energy, calc_virial), "scalar_sum");
+  }
+  void c2r_fft() { pme::check(pme_recip_c2r_fft(h_), "c2r_fft"); }
+  void calc_self_energy(const float4* xyzq, const int ncoord, const double kappa) {
+    pme::check(pme_recip_calc_self_energy(h_, xyzq, ncoord, kappa), "calc_self_energy");
+  }
+  // FT = float (strided SoA store), float3 (AoS store), long long (2^40 fixed-point add)
+  template <typename FT>
+  void gather_force(const float4* xyzq, const int ncoord, const double* recip, const int stride, FT* force) {
+    pme::check(pme_recip_gather_force(h_, xyzq, ncoord, recip, stride, force_type<FT>(), force), "gather_force");
+  }
+
+  int get_nfftx() { return nfftx_; }
+  int get_nffty() { return nffty_; }
+  int get_nfftz() { return nfftz_; }
+  int get_order() { return order_; }
+  void set_order(int order) {
+    pme::check(pme_recip_set_order(h_, order), "set_order");
+    order_ = order;
+  }
+
+  pme_recip_t handle() const { return h_; }
+
+ private:
+  static int accum_type() {
+    if (std::is_same<AT, float>::value) return PME_ACCUM_FLOAT;
+    if (std::is_same<AT, long long>::value || std::is_same<AT, long long int>::value) return PME_ACCUM_INT64;
+    return PME_ACCUM_INT32;
+  }
+  template <typename FT>
+  static int force_type() {
+    if (std::is_same<FT, float>::value) return PME_FORCE_FLOAT;
+    if (std::is_same<FT, float3>::value) return PME_FORCE_FLOAT3;
+    static_assert(std::is_same<FT, float>::value || std::is_same<FT, float3>::value ||
+                      std::is_same<FT, long long>::value,
+                  "gather_force<FT>: FT is float, float3 or long long");
+    return PME_FORCE_INT64;
+  }
+
+  pme_recip_t h_ = nullptr;
+  int nfftx_, nffty_, nfftz_, order_;
+};
+
+#endif  // PME_RECIP_HPP

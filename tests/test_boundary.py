@@ -72,7 +72,7 @@ def _create(lib, nx=48, ny=48, nz=48, order=4, fft=2, nnode=1, accum=0, ev=1):
     (dict(nx=77), 2),            # 7*11: not 2/3/5-smooth
     (dict(ny=1024), 2),          # above the 512 edge limit
     (dict(nz=4, order=6), 2),    # grid edge below the order
-    (dict(accum=1), 2),
+    (dict(accum=3), 1),
     (dict(fft=7), 1),
     (dict(ev=0), 1),
 ])

@@ -1,0 +1,243 @@
+"""GPU tests of the batched path (pme_batch_*): every system of a batch gives
+the single-system solver's result (same kernels: forces bit-identical, energy
+and virial to FP64 summation order) and agrees with the FP64 oracle; hipGraph
+replays match eager solves; admission honours the HBM budget; subsets,
+updates, removals and the autotuner keep results unchanged."""
+import json
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+from gpu_recip_amd import _lib as L
+from gpu_recip_amd import systems
+from gpu_recip_amd.pme import PmeBatch, force_stride
+from gpu_util import rel_err, rel_fro, rel_rms, run_solve, say, to_dev, torch_dev
+
+pytestmark = pytest.mark.gpu
+
+
+def _mixed_systems():
+    """Heterogeneous set: orders 4/6/8, odd and even grids, 1 to 30k atoms."""
+    out = []
+    specs = [
+        (4, (48, 48, 48), (40.0, 40.0, 40.0), 2667),
+        (6, (45, 50, 54), (42.0, 47.0, 50.0), 3000),
+        (8, (36, 40, 32), (33.0, 37.0, 30.0), 1500),
+        (4, (60, 64, 72), (55.0, 60.0, 66.0), 10000),
+        (6, (25, 27, 30), (22.0, 24.0, 27.0), 1),
+        (4, (80, 75, 81), (72.0, 68.0, 74.0), 5),
+    ]
+    for i, (order, nfft, box, ntrip) in enumerate(specs):
+        x = systems.triplets(ntrip, box, systems.SEED_BASE + 500 + i)
+        out.append(systems.PmeSystem(f"mix{i}", x, box, nfft, order, 0.34))
+    return out
+
+
+def _add(b, s, torch, dev, force_type=L.FORCE_FLOAT, calc_ev=True):
+    n = s.natom
+    x = to_dev(s.xyzq)
+    stride = force_stride(n)
+    if force_type == L.FORCE_INT64:
+        f = torch.zeros(3 * stride, dtype=torch.int64, device=dev)
+    elif force_type == L.FORCE_FLOAT3:
+        f = torch.zeros(3 * n, dtype=torch.float32, device=dev)
+    else:
+        f = torch.zeros(3 * stride, dtype=torch.float32, device=dev)
+    d = PmeBatch.desc(s.nfft, s.order, s.kappa, s.recip, x, n, f, force_type, stride, calc_ev)
+    sid = b.add(d, keep=(x, f))
+    return sid, x, f, stride
+
+
+def _forces(f, n, stride, force_type=L.FORCE_FLOAT):
+    fh = f.cpu().numpy()
+    if force_type == L.FORCE_FLOAT3:
+        return fh.reshape(n, 3).astype(np.float64)
+    out = np.stack([fh[:n], fh[stride:stride + n], fh[2 * stride:2 * stride + n]], 1).astype(np.float64)
+    return out / 2.0 ** 40 if force_type == L.FORCE_INT64 else out
+
+
+@pytest.mark.parametrize("graph", [True, False])
+def test_batch_matches_single_and_oracle(graph):
+    torch, dev = torch_dev()
+    sl = _mixed_systems()
+    b = PmeBatch(8 << 30)
+    b.set_graph(graph)
+    reg = [_add(b, s, torch, dev) for s in sl]
+    ids = [r[0] for r in reg]
+    say(f"batch solve of {len(ids)} systems, graph={graph}")
+    b.solve()
+    e, v = b.results(ids)
+    for s, (sid, x, f, stride), ek, vk in zip(sl, reg, e, v):
+        fb = _forces(f, s.natom, stride)
+        one = run_solve(s.nfft, s.order, s.kappa, s.box, s.xyzq)
+        assert np.array_equal(fb, one["force"]), s.name
+        assert rel_err(ek, one["energy"]) < 1e-12, s.name
+        assert rel_fro(vk, one["virial"]) < 1e-12, s.name
+        o = O.solve(O.OracleParams(*s.nfft, s.order, s.kappa, s.box, index_mode=O.INDEX_F32), s.xyzq)
+        frms = rel_rms(fb, o["force"])
+        say(f"  {s.name}: force rel-RMS vs oracle {frms:.2e} energy rel {rel_err(ek, o['energy']):.2e}")
+        assert frms < 1e-5 and rel_err(ek, o["energy"]) < 1e-5 and rel_fro(vk, o["virial"]) < 1e-5
+
+
+def test_batched_config_vs_oracle():
+    """The 64-system batched workload: every system against the single-system
+    solver, eight of them against the FP64 oracle (exact coordinates)."""
+    torch, dev = torch_dev()
+    sl = systems.config_batched()
+    b = PmeBatch(16 << 30)
+    reg = [_add(b, s, torch, dev) for s in sl]
+    ids = [r[0] for r in reg]
+    for _ in range(3):  # graph replays
+        b.solve()
+    e, v = b.results(ids)
+    worst = 0.0
+    for k, (s, (sid, x, f, stride)) in enumerate(zip(sl, reg)):
+        fb = _forces(f, s.natom, stride)
+        one = run_solve(s.nfft, s.order, s.kappa, s.box, s.xyzq)
+        assert np.array_equal(fb, one["force"]), s.name
+        assert rel_err(e[k], one["energy"]) < 1e-12
+        if k % 8 == 0:
+            o = O.solve(O.OracleParams(*s.nfft, s.order, s.kappa, s.box), s.xyzq)
+            frms = rel_rms(fb, o["force"])
+            worst = max(worst, frms)
+            assert frms < 1e-4, (s.name, frms)
+            assert rel_err(e[k], o["energy"]) < 1e-4
+    say(f"  worst force rel-RMS vs FP64 oracle (exact index): {worst:.2e}")
+
+
+def test_graph_replays_match_eager():
+    """Replays of the captured graph, with eager work and coordinate updates
+    between them, each agree with an eager (uncaptured) solve."""
+    torch, dev = torch_dev()
+    sl = _mixed_systems()[:4]
+    bg = PmeBatch(4 << 30)
+    be = PmeBatch(4 << 30)
+    be.set_graph(False)
+    rg = [_add(bg, s, torch, dev) for s in sl]
+    re_ = [_add(be, s, torch, dev) for s in sl]
+    rng = np.random.default_rng(7)
+    for rep in range(4):
+        # move the atoms of system rep % 4 in place (the descriptor is unchanged)
+        k = rep % len(sl)
+        dx = torch.from_numpy(rng.normal(0, 0.3, (sl[k].natom, 4)).astype(np.float32)).to(dev)
+        dx[:, 3] = 0
+        rg[k][1].add_(dx)
+        re_[k][1].add_(dx)
+        _ = torch.ones(1024, device=dev).sum().item()  # eager work between replays
+        bg.solve()
+        be.solve()
+        eg, vg = bg.results([r[0] for r in rg])
+        ee, ve = be.results([r[0] for r in re_])
+        for s, a, c in zip(sl, rg, re_):
+            assert np.array_equal(_forces(a[2], s.natom, a[3]), _forces(c[2], s.natom, c[3])), (rep, s.name)
+        assert np.array_equal(np.array(eg), np.array(ee)) and np.array_equal(np.array(vg), np.array(ve))
+        say(f"  replay {rep}: graph == eager")
+
+
+def test_subset_update_remove():
+    torch, dev = torch_dev()
+    sl = _mixed_systems()
+    b = PmeBatch(8 << 30)
+    reg = [_add(b, s, torch, dev) for s in sl]
+    # solve a subset: the others' outputs stay untouched (zero)
+    sub = [reg[0][0], reg[3][0]]
+    b.solve(sub)
+    e, _ = b.results(sub)
+    for k, r in enumerate(reg):
+        fb = _forces(r[2], sl[k].natom, r[3])
+        if r[0] in sub:
+            assert np.abs(fb).sum() > 0
+        else:
+            assert not fb.any()
+    # grow system 1 beyond its admitted capacity: the slot is re-planned
+    s1 = sl[1]
+    big = systems.triplets(4000, s1.box, 99)
+    x = to_dev(big)
+    stride = force_stride(len(big))
+    f = torch.zeros(3 * stride, dtype=torch.float32, device=dev)
+    b.update(reg[1][0], PmeBatch.desc(s1.nfft, s1.order, s1.kappa, s1.recip, x, len(big), f, L.FORCE_FLOAT, stride),
+             keep=(x, f))
+    # remove system 2 and admit another in its place
+    b.remove(reg[2][0])
+    s_new = systems.PmeSystem("new", systems.triplets(700, (30.0, 31.0, 32.0), 5), (30.0, 31.0, 32.0), (36, 36, 36), 6, 0.3)
+    nid, nx_, nf, nstride = _add(b, s_new, torch, dev)
+    assert nid == reg[2][0]  # the freed slot is reused
+    ids = [reg[1][0], nid, reg[4][0]]
+    b.solve(ids)
+    e, v = b.results(ids)
+    for sysm, xx, ff, st, ek in [(systems.PmeSystem("g", big, s1.box, s1.nfft, s1.order, s1.kappa), x, f, stride, e[0]),
+                                 (s_new, nx_, nf, nstride, e[1])]:
+        one = run_solve(sysm.nfft, sysm.order, sysm.kappa, sysm.box, sysm.xyzq)
+        assert np.array_equal(_forces(ff, sysm.natom, st), one["force"])
+        assert rel_err(ek, one["energy"]) < 1e-12
+    with pytest.raises(L.PmeError):
+        b.solve([reg[2][0] + 100])
+
+
+def test_budget_admission():
+    torch, dev = torch_dev()
+    s = _mixed_systems()[3]
+    b = PmeBatch(1 << 20)  # 1 MiB: the 60x64x72 grid alone needs more
+    used0, budget = b.memory()
+    assert budget == 1 << 20
+    with pytest.raises(L.PmeError) as ei:
+        _add(b, s, torch, dev)
+    assert ei.value.status == 4  # PME_ERR_BUDGET
+    assert b.memory()[0] == used0
+    # a tiny system fits
+    small = systems.PmeSystem("tiny", systems.triplets(10, (20.0, 20.0, 20.0), 3), (20.0, 20.0, 20.0), (16, 16, 16), 4, 0.4)
+    sid = _add(b, small, torch, dev)[0]
+    assert b.memory()[0] > used0
+    b.solve([sid])
+    b.results([sid])
+
+
+def test_force_types_and_accumulation():
+    torch, dev = torch_dev()
+    s = _mixed_systems()[0]
+    b = PmeBatch(2 << 30)
+    r_float = _add(b, s, torch, dev, L.FORCE_FLOAT)
+    r_f3 = _add(b, s, torch, dev, L.FORCE_FLOAT3)
+    r_i64 = _add(b, s, torch, dev, L.FORCE_INT64, calc_ev=False)
+    b.solve()
+    b.solve()  # INT64 accumulates: twice the force
+    e, _ = b.results([r_float[0], r_i64[0]])
+    assert e[1] == 0.0
+    ff = _forces(r_float[2], s.natom, r_float[3])
+    f3 = _forces(r_f3[2], s.natom, r_f3[3], L.FORCE_FLOAT3)
+    fi = _forces(r_i64[2], s.natom, r_i64[3], L.FORCE_INT64)
+    assert np.array_equal(ff, f3)
+    assert rel_rms(fi, 2 * ff) < 1e-6
+
+
+def test_autotune_cache(tmp_path):
+    torch, dev = torch_dev()
+    sl = systems.config_batched(8)
+    b = PmeBatch(8 << 30)
+    reg = [_add(b, s, torch, dev) for s in sl]
+    ids = [r[0] for r in reg]
+    b.solve()
+    e0, _ = b.results(ids)
+    f0 = [_forces(r[2], s.natom, r[3]) for s, r in zip(sl, reg)]
+    cache = tmp_path / "tune.json"
+    ms = b.autotune(str(cache))
+    tune = b.get_tuning()
+    say(f"  autotune: {ms:.3f} ms/solve with (brick, tile, wg/cu) = {tune}")
+    assert ms > 0 and cache.exists()
+    entry = list(json.loads(cache.read_text()).values())[0]
+    assert tuple(int(v) for v in entry[:3]) == tune
+    # results with the tuned plan: forces to FP32 rounding (tiles change the
+    # FFT's summation order), energies to 1e-6
+    b.solve()
+    e1, _ = b.results(ids)
+    for s, r, fa, ea, eb in zip(sl, reg, f0, e0, e1):
+        assert rel_rms(_forces(r[2], s.natom, r[3]), fa) < 1e-5
+        assert rel_err(eb, ea) < 1e-6
+    # a second batch with the same set reads the cache
+    b2 = PmeBatch(8 << 30)
+    for s in sl:
+        _add(b2, s, torch, dev)
+    ms2 = b2.autotune(str(cache))
+    assert b2.get_tuning() == tune and math.isclose(ms2, ms, rel_tol=1e-6)
