@@ -2,17 +2,21 @@
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
 
-Workload: the production plain solve -- 96,000 atoms (32,000 TIP3P-like
-triplets, seeded), box 100 x 96 x 90 A, grid 108 x 100 x 96, B-spline order 6,
-kappa 0.32, FP32.  A step is one complete solve through the C ABI:
-energy/virial clear, spread_charge, r2c_fft, scalar_sum (energy + virial),
-c2r_fft, gather_force, energy/virial copy to host and a stream sync.  Inputs
-are resident in HBM before the timed region.
+Headline workload: the batched set -- 64 independent orthorhombic systems
+(atoms log-uniform in 2,000-40,000 as seeded TIP3P-like triplets, box edges
+U(30, 75) A, grid edge = smallest 2/3/5-smooth >= edge/0.9, order 4, kappa
+0.34, FP32).  A step is one batched solve of all 64 systems (one hipGraph
+replay: binning, spread, 3-D R2C FFT, influence function with energy and
+virial, 3-D C2R FFT, gather) plus the energy/virial copy to the host, so
+value = 64 x steps / seconds.  Beside it, the same 64 systems solved one at
+a time through the plain five-call path (the comparison the batched path is
+built to beat) and the production plain solve (96,000 atoms, 108x100x96,
+order 6).  Inputs are resident in HBM before every timed region.
 
-Prints ONE JSON line.  The roofline object is for the kernel with the largest
-average duration, timed with HIP events recorded on the solver's stream
-during the timed steps.  The CPU baseline is the FP64 host oracle (oracle/)
-timed on this host's cores on the same workload.
+Prints ONE JSON line.  The roofline object is for the batched kernel with
+the largest average duration, timed with HIP events recorded on the batch's
+stream.  The CPU baseline is the oracle (oracle/, FP64 C with OpenMP) timed on
+this host's cores over a sample of the same 64 systems.
 """
 import argparse
 import json
@@ -27,11 +31,12 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
 STAGES = ["bin", "spread", "fft_x_r2c", "fft_y_fwd", "fft_z_solve", "fft_y_inv", "fft_x_c2r", "gather"]
+METRIC = "PME recip solves/s + atom-steps/s, batched/served vs plain; p99 latency @1e-4 err"
 
 
 def stage_bytes(s):
-    """Algorithmic HBM bytes per launch of each stage kernel (FP32 data):
-    every kernel reads its inputs and writes its outputs once."""
+    """Algorithmic HBM bytes per launch of each stage kernel for one system
+    (FP32 data): every kernel reads its inputs and writes its outputs once."""
     N = s.natom
     nx, ny, nz = s.nfft
     G = nx * ny * nz
@@ -48,25 +53,30 @@ def stage_bytes(s):
     }
 
 
-def cpu_baseline(s, budget_s=12.0):
+def forces_of(f, n, stride):
+    fh = f.cpu().numpy()
+    return np.stack([fh[:n], fh[stride:stride + n], fh[2 * stride:2 * stride + n]], 1).astype(np.float64)
+
+
+def cpu_baseline(sl, budget_s=12.0):
+    """The oracle on the host cores over the 64 systems in turn (bounded sample)."""
     import oracle as O
 
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1
-    p = O.OracleParams(*s.nfft, s.order, s.kappa, s.box, index_mode=O.INDEX_F32, nthreads=threads)
-    O.solve(p, s.xyzq)  # warm-up (page in, thread pool)
-    n = 0
+    O.solve(O.OracleParams(*sl[0].nfft, sl[0].order, sl[0].kappa, sl[0].box, nthreads=threads), sl[0].xyzq)
+    n = atoms = 0
     t0 = time.perf_counter()
-    while True:
-        O.solve(p, s.xyzq)
+    while time.perf_counter() - t0 < budget_s:
+        s = sl[n % len(sl)]
+        O.solve(O.OracleParams(*s.nfft, s.order, s.kappa, s.box, index_mode=O.INDEX_F32, nthreads=threads), s.xyzq)
         n += 1
-        if time.perf_counter() - t0 > budget_s:
-            break
+        atoms += s.natom
     dt = time.perf_counter() - t0
     return {
         "value": n / dt, "unit": "solves/s", "cores": threads, "kind": "port",
-        "sample": f"{n} full solves of the same 96,000-atom 108x100x96 order-6 system by the FP64 C oracle "
-                  f"(OpenMP, {threads} threads), {dt:.1f} s",
-        "atom_steps_per_s": n * s.natom / dt,
+        "sample": f"{n} full solves of the batched systems taken in turn (systems 0..{min(n, len(sl)) - 1}) "
+                  f"by the FP64 C oracle with OpenMP on {threads} threads, {dt:.1f} s",
+        "atom_steps_per_s": atoms / dt,
     }
 
 
@@ -76,6 +86,9 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run 5 device copies of 256 MiB (scale for the HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this PME engine runs on one GPU; whole systems shard across GPUs "
@@ -86,96 +99,188 @@ def main():
 
     from gpu_recip_amd import _lib as L
     from gpu_recip_amd import systems
-    from gpu_recip_amd.pme import CudaEnergyVirial, CudaPMERecip, force_stride
+    from gpu_recip_amd.pme import CudaEnergyVirial, CudaPMERecip, PmeBatch, force_stride
 
     dev = torch.device("cuda:0")
-    s = systems.config_production()
-    n = s.natom
-    ev = CudaEnergyVirial()
-    pme = CudaPMERecip(*s.nfft, s.order, ev)
-    x = torch.from_numpy(s.xyzq).to(dev)
-    stride = force_stride(n)
-    f = torch.zeros(3 * stride, dtype=torch.float32, device=dev)
-    recip = s.recip
-
-    def step():
-        ev.clear()
-        pme.spread_charge(x, n, recip)
-        pme.r2c_fft()
-        pme.scalar_sum(recip, s.kappa, True, True)
-        pme.c2r_fft()
-        pme.gather_force(x, n, recip, stride, f, L.FORCE_FLOAT)
-        ev.copyToHost()
+    if args.copy_calibration:
+        a = torch.ones(64 << 20, dtype=torch.float32, device=dev)
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
         torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration": {"bytes_read": a.numel() * 4, "bytes_written": b.numel() * 4,
+                                               "launches": 5}}))
+        return
+
+    # ---------------- batched: 64 systems, one graph replay per step ----------
+    sl = systems.config_batched()
+    batch = PmeBatch(32 << 30)
+    reg = []
+    for s in sl:
+        x = torch.from_numpy(s.xyzq).to(dev)
+        stride = force_stride(s.natom)
+        f = torch.zeros(3 * stride, dtype=torch.float32, device=dev)
+        sid = batch.add(PmeBatch.desc(s.nfft, s.order, s.kappa, s.recip, x, s.natom, f, L.FORCE_FLOAT, stride),
+                        keep=(x, f))
+        reg.append((sid, x, f, stride))
+    ids = [r[0] for r in reg]
+    tune_ms = None if args.no_autotune else batch.autotune(None)
+    tuning = batch.get_tuning()
+
+    def bstep():
+        batch.solve()
+        return batch.results(ids)
 
     for _ in range(args.warmup):
-        step()
-    energy = ev.getEnergy("recip")
-
-    # timed region: plain steps
+        bstep()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        step()
+        e_b, v_b = bstep()
     torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
+    dt_b = time.perf_counter() - t0
+    nsolve = len(sl)
+    atoms_b = sum(s.natom for s in sl)
 
-    # per-kernel durations from HIP events on the solver's stream
-    pme.set_timing(True)
+    # per-stage kernel times of the batched solve (HIP events on its stream)
+    batch.set_timing(True)
     stage_ms = np.zeros(len(STAGES))
-    for _ in range(args.steps):
-        step()
-        stage_ms += np.array(pme.get_timing())
-    pme.set_timing(False)
-    stage_ms /= args.steps
-
-    sb = stage_bytes(s)
+    nt = max(10, args.steps // 4)
+    for _ in range(nt):
+        bstep()
+        stage_ms += np.array(batch.get_timing())
+    batch.set_timing(False)
+    stage_ms /= nt
+    sb = {k: sum(stage_bytes(s)[k] for s in sl) for k in STAGES}
     k = int(np.argmax(stage_ms))
     kname = STAGES[k]
     achieved = sb[kname] / (stage_ms[k] * 1e-3) / 1e9
+    bytes_b = sum(s.algorithmic_bytes() for s in sl)
+    ms_b = dt_b / args.steps * 1e3
 
-    # accuracy of the benchmarked result vs the FP64 oracle
+    # ---------------- plain: the same 64 systems one at a time ----------------
+    plain = []
+    for s, (sid, x, f, stride) in zip(sl, reg):
+        ev = CudaEnergyVirial()
+        pme = CudaPMERecip(*s.nfft, s.order, ev)
+        fp = torch.zeros_like(f)
+        plain.append((s, ev, pme, x, fp, stride))
+
+    def pstep():
+        for s, ev, pme, x, fp, stride in plain:
+            r = s.recip
+            ev.clear()
+            pme.spread_charge(x, s.natom, r)
+            pme.r2c_fft()
+            pme.scalar_sum(r, s.kappa, True, True)
+            pme.c2r_fft()
+            pme.gather_force(x, s.natom, r, stride, fp, L.FORCE_FLOAT)
+            ev.copyToHost()
+            torch.cuda.synchronize()
+
+    psteps = max(5, args.steps // 10)
+    for _ in range(max(2, args.warmup // 5)):
+        pstep()
+    t0 = time.perf_counter()
+    for _ in range(psteps):
+        pstep()
+    dt_p = time.perf_counter() - t0
+    plain_c3 = nsolve * psteps / dt_p
+
+    # ---------------- plain production solve (96,000 atoms) -------------------
+    s2 = systems.config_production()
+    ev2 = CudaEnergyVirial()
+    pme2 = CudaPMERecip(*s2.nfft, s2.order, ev2)
+    x2 = torch.from_numpy(s2.xyzq).to(dev)
+    st2 = force_stride(s2.natom)
+    f2 = torch.zeros(3 * st2, dtype=torch.float32, device=dev)
+
+    def p2step():
+        r = s2.recip
+        ev2.clear()
+        pme2.spread_charge(x2, s2.natom, r)
+        pme2.r2c_fft()
+        pme2.scalar_sum(r, s2.kappa, True, True)
+        pme2.c2r_fft()
+        pme2.gather_force(x2, s2.natom, r, st2, f2, L.FORCE_FLOAT)
+        ev2.copyToHost()
+        torch.cuda.synchronize()
+
+    for _ in range(args.warmup):
+        p2step()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        p2step()
+    dt_2 = time.perf_counter() - t0
+    pme2.set_timing(True)
+    st_ms2 = np.zeros(len(STAGES))
+    for _ in range(nt):
+        p2step()
+        st_ms2 += np.array(pme2.get_timing())
+    pme2.set_timing(False)
+    st_ms2 /= nt
+
+    # ---------------- accuracy vs the FP64 oracle (the checker) ---------------
     acc = {}
     try:
         import oracle as O
 
-        o = O.solve(O.OracleParams(*s.nfft, s.order, s.kappa, s.box, index_mode=O.INDEX_F64), s.xyzq)
-        fh = f.cpu().numpy()
-        force = np.stack([fh[:n], fh[stride:stride + n], fh[2 * stride:2 * stride + n]], 1).astype(np.float64)
+        worst_f = worst_e = 0.0
+        for i in range(0, nsolve, 8):
+            s = sl[i]
+            o = O.solve(O.OracleParams(*s.nfft, s.order, s.kappa, s.box, index_mode=O.INDEX_F64), s.xyzq)
+            fb = forces_of(reg[i][2], s.natom, reg[i][3])
+            worst_f = max(worst_f, float(np.sqrt(((fb - o["force"]) ** 2).sum() / (o["force"] ** 2).sum())))
+            worst_e = max(worst_e, float(abs(e_b[i] - o["energy"]) / abs(o["energy"])))
+        o2 = O.solve(O.OracleParams(*s2.nfft, s2.order, s2.kappa, s2.box, index_mode=O.INDEX_F64), s2.xyzq)
+        fp2 = forces_of(f2, s2.natom, st2)
         acc = {
-            "force_rel_rms_vs_fp64": float(np.sqrt(((force - o["force"]) ** 2).sum() / (o["force"] ** 2).sum())),
-            "energy_rel_vs_fp64": float(abs(energy - o["energy"]) / abs(o["energy"])),
+            "batched_worst_force_rel_rms_vs_fp64": worst_f,
+            "batched_worst_energy_rel_vs_fp64": worst_e,
+            "batched_systems_checked": list(range(0, nsolve, 8)),
+            "production_force_rel_rms_vs_fp64": float(np.sqrt(((fp2 - o2["force"]) ** 2).sum() / (o2["force"] ** 2).sum())),
+            "production_energy_rel_vs_fp64": float(abs(ev2.getEnergy("recip") - o2["energy"]) / abs(o2["energy"])),
         }
     except Exception as e:  # the oracle is the checker; its absence must not hide the GPU number
         acc = {"error": str(e)}
 
-    ms = dt / args.steps * 1e3
     out = {
-        "metric": "PME recip solves/s (plain single-system path)",
-        "value": args.steps / dt,
+        "metric": METRIC,
+        "value": nsolve * args.steps / dt_b,
         "unit": "solves/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms,
+        "ms_per_step": ms_b,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded TIP3P-like triplets)",
-        "config": {"workload": s.name, "natom": n, "nfft": list(s.nfft), "order": s.order, "kappa": s.kappa,
-                   "box": list(s.box), "path": "plain: 5 reference stage calls + energy/virial readback per solve"},
-        "atom_steps_per_s": n * args.steps / dt,
+        "config": {"workload": "batched-64-systems-2k-40k-atoms-p4", "systems": nsolve, "atoms_total": atoms_b,
+                   "grid_edges": [min(min(s.nfft) for s in sl), max(max(s.nfft) for s in sl)], "order": 4,
+                   "kappa": 0.34, "step": "one batched solve of all 64 systems (hipGraph replay) + energy/virial "
+                                          "readback", "tuning": {"brick_target": tuning[0], "fft_tile": tuning[1],
+                                                                 "wg_per_cu": tuning[2], "autotune_ms": tune_ms}},
+        "atom_steps_per_s": atoms_b * args.steps / dt_b,
+        "plain_same_systems": {"solves_per_s": plain_c3, "atom_steps_per_s": atoms_b * psteps / dt_p,
+                               "path": "the 64 systems one at a time: 5 stage calls + readback + sync each"},
+        "speedup_batched_vs_plain": (nsolve * args.steps / dt_b) / plain_c3,
+        "production_plain": {"workload": s2.name, "solves_per_s": args.steps / dt_2,
+                             "atom_steps_per_s": s2.natom * args.steps / dt_2,
+                             "ms_per_solve": dt_2 / args.steps * 1e3,
+                             "stage_ms": {STAGES[i]: float(st_ms2[i]) for i in range(len(STAGES))},
+                             "solve_roofline_frac": s2.algorithmic_bytes() / (dt_2 / args.steps) / (HBM_PEAK_GBS * 1e9)},
         "stage_ms": {STAGES[i]: float(stage_ms[i]) for i in range(len(STAGES))},
-        "solve_algorithmic_bytes": s.algorithmic_bytes(),
-        "solve_roofline_frac": s.algorithmic_bytes() / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9),
+        "solve_algorithmic_bytes": bytes_b,
+        "solve_roofline_frac": bytes_b / (ms_b * 1e-3) / (HBM_PEAK_GBS * 1e9),
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "algorithmic_bytes_per_launch": sb[kname], "avg_launch_ms": float(stage_ms[k])},
         "accuracy": acc,
     }
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(s)
+        out["cpu_baseline"] = cpu_baseline(sl)
     print(json.dumps(out))
 
 

@@ -102,6 +102,10 @@ struct pme_batch_s {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   bool graph_valid = false;
+  // per-stage timing (eager launches bracketed by events; graph bypassed)
+  bool timing = false;
+  hipEvent_t tev[9] = {};
+  float stage_ms[8] = {};
 };
 
 static void drop_graph(pme_batch_t b) {
@@ -376,6 +380,8 @@ extern "C" pme_status pme_batch_destroy(pme_batch_t b) {
   (void)hipFree(b->d_ev);
   (void)hipHostFree(b->h_ev);
   (void)hipFree(b->d_items);
+  for (hipEvent_t e : b->tev)
+    if (e) (void)hipEventDestroy(e);
   if (b->ev_in) (void)hipEventDestroy(b->ev_in);
   if (b->ev_done) (void)hipEventDestroy(b->ev_done);
   if (b->own_stream) (void)hipStreamDestroy(b->stream);
@@ -535,40 +541,45 @@ static int grid_for(pme_batch_t b, int count, bool big_block) {
   return std::max(1, std::min(count, b->ncu * per_cu));
 }
 
+// Stage order: bin, spread, x forward, y forward, z forward + influence
+// function + z inverse, y inverse (with the per-system energy/virial
+// reduction), x inverse, gather.  With timing on, event k is recorded before
+// stage k and event 8 after the last.
 static pme_status enqueue(pme_batch_t b) {
-  const Launch* fx = nullptr;
-  for (const Launch& L : b->launches)
-    if (L.stage == BS_FFTX) fx = &L;
-  // forward: bin, spread, x, y, z (+ influence function), then y inverse
-  // (with the per-system energy/virial reduction), x inverse, gather
-  for (const Launch& L : b->launches) {
-    const WorkItem* it = b->d_items + L.offset;
-    switch (L.stage) {
-      case BS_BIN: BHIP(launch_batch_bin(b->d_sys, it, L.count, grid_for(b, L.count, false), L.lds, b->stream)); break;
-      case BS_SPREAD:
-        BHIP(launch_batch_spread(L.order, b->d_sys, it, L.count, grid_for(b, L.count, true), L.lds, b->stream));
-        break;
-      case BS_FFTX: BHIP(launch_batch_fft_x(0, b->d_sys, it, L.count, grid_for(b, L.count, false), L.lds, b->stream)); break;
-      case BS_FFTY:
-        BHIP(launch_batch_cols(1, COL_FWD, 0, b->d_sys, it, L.count, grid_for(b, L.count, false), L.lds, b->stream));
-        break;
-      case BS_FFTZ:
-        BHIP(launch_batch_cols(2, COL_FWD_SOLVE_INV, 0, b->d_sys, it, L.count, grid_for(b, L.count, false), L.lds,
-                               b->stream));
-        break;
-      default: break;
+  auto mark = [&](int k) -> pme_status {
+    if (b->timing) BHIP(hipEventRecord(b->tev[k], b->stream));
+    return PME_OK;
+  };
+  auto run = [&](int stage, bool inverse) -> pme_status {
+    for (const Launch& L : b->launches) {
+      if (L.stage != stage) continue;
+      const WorkItem* it = b->d_items + L.offset;
+      const bool big = stage == BS_SPREAD || stage == BS_GATHER;
+      const int grid = grid_for(b, L.count, big);
+      switch (stage) {
+        case BS_BIN: BHIP(launch_batch_bin(b->d_sys, it, L.count, grid, L.lds, b->stream)); break;
+        case BS_SPREAD: BHIP(launch_batch_spread(L.order, b->d_sys, it, L.count, grid, L.lds, b->stream)); break;
+        case BS_FFTX: BHIP(launch_batch_fft_x(inverse ? 1 : 0, b->d_sys, it, L.count, grid, L.lds, b->stream)); break;
+        case BS_FFTY:
+          BHIP(launch_batch_cols(1, inverse ? COL_INV : COL_FWD, inverse ? 1 : 0, b->d_sys, it, L.count, grid, L.lds,
+                                 b->stream));
+          break;
+        case BS_FFTZ:
+          BHIP(launch_batch_cols(2, COL_FWD_SOLVE_INV, 0, b->d_sys, it, L.count, grid, L.lds, b->stream));
+          break;
+        case BS_GATHER: BHIP(launch_batch_gather(L.order, b->d_sys, it, L.count, grid, L.lds, b->stream)); break;
+        default: break;
+      }
     }
+    return PME_OK;
+  };
+  const int seq[8][2] = {{BS_BIN, 0}, {BS_SPREAD, 0}, {BS_FFTX, 0}, {BS_FFTY, 0},
+                         {BS_FFTZ, 0}, {BS_FFTY, 1}, {BS_FFTX, 1}, {BS_GATHER, 0}};
+  for (int k = 0; k < 8; k++) {
+    BTRY(mark(k));
+    BTRY(run(seq[k][0], seq[k][1] != 0));
   }
-  for (const Launch& L : b->launches)
-    if (L.stage == BS_FFTY)
-      BHIP(launch_batch_cols(1, COL_INV, 1, b->d_sys, b->d_items + L.offset, L.count, grid_for(b, L.count, false), L.lds,
-                             b->stream));
-  if (fx) BHIP(launch_batch_fft_x(1, b->d_sys, b->d_items + fx->offset, fx->count, grid_for(b, fx->count, false), fx->lds, b->stream));
-  for (const Launch& L : b->launches)
-    if (L.stage == BS_GATHER)
-      BHIP(launch_batch_gather(L.order, b->d_sys, b->d_items + L.offset, L.count, grid_for(b, L.count, true), L.lds,
-                               b->stream));
-  return PME_OK;
+  return mark(8);
 }
 
 static pme_status run_solve(pme_batch_t b);
@@ -611,7 +622,7 @@ extern "C" pme_status pme_batch_solve(pme_batch_t b, int n, const int* ids) {
 }
 
 static pme_status run_solve(pme_batch_t b) {
-  if (!b->use_graph) return enqueue(b);
+  if (!b->use_graph || b->timing) return enqueue(b);
   if (!b->graph_valid) {
     BHIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
     pme_status st = enqueue(b);
@@ -641,6 +652,8 @@ extern "C" pme_status pme_batch_results(pme_batch_t b, int n, const int* ids, do
     hi = std::max(hi, id + 1);
   }
   if (hi > 0) BHIP(hipMemcpy(b->h_ev, b->d_ev, sizeof(double) * 10 * hi, hipMemcpyDeviceToHost));
+  if (b->timing && b->lists_valid)
+    for (int k = 0; k < 8; k++) BHIP(hipEventElapsedTime(&b->stage_ms[k], b->tev[k], b->tev[k + 1]));
   for (size_t k = 0; k < v.size(); k++) {
     const double* e = b->h_ev + 10 * (size_t)v[k];
     const bool on = b->slots[v[k]].in.calc_energy_virial != 0;
@@ -687,6 +700,26 @@ static pme_status replan_all(pme_batch_t b) {
     }
     BTRY(upload_desc(b, (int)i));
   }
+  return PME_OK;
+}
+
+extern "C" pme_status pme_batch_set_timing(pme_batch_t b, int enable) {
+  if (!b) return set_error(PME_ERR_INVALID_ARG, "pme_batch_set_timing: null batch");
+  BHIP(hipStreamSynchronize(b->stream));
+  if (enable && !b->tev[0])
+    for (int k = 0; k < 9; k++) BHIP(hipEventCreate(&b->tev[k]));
+  b->timing = enable != 0;
+  for (float& v : b->stage_ms) v = 0.f;
+  return PME_OK;
+}
+
+extern "C" pme_status pme_batch_get_timing(pme_batch_t b, float* ms) {
+  if (!b || !ms) return set_error(PME_ERR_INVALID_ARG, "pme_batch_get_timing: null argument");
+  if (!b->timing) return set_error(PME_ERR_STATE, "pme_batch_get_timing: timing is off");
+  BHIP(hipStreamSynchronize(b->stream));
+  if (b->lists_valid)
+    for (int k = 0; k < 8; k++) BHIP(hipEventElapsedTime(&b->stage_ms[k], b->tev[k], b->tev[k + 1]));
+  for (int k = 0; k < 8; k++) ms[k] = b->stage_ms[k];
   return PME_OK;
 }
 
@@ -783,6 +816,7 @@ static std::map<std::string, std::vector<double>> read_cache(const char* path) {
 static void write_cache(const char* path, const std::map<std::string, std::vector<double>>& m) {
   if (!path) return;
   std::ofstream f(path);
+  f.precision(9);
   f << "{\n";
   size_t k = 0;
   for (auto& kv : m) {

@@ -237,6 +237,14 @@ class PmeBatch:
         L.call("pme_batch_autotune", self.h, None if cache_path is None else str(cache_path).encode(), C.byref(ms))
         return ms.value
 
+    def set_timing(self, enable: bool) -> None:
+        L.call("pme_batch_set_timing", self.h, int(bool(enable)))
+
+    def get_timing(self):
+        ms = (C.c_float * 8)()
+        L.call("pme_batch_get_timing", self.h, ms)
+        return list(ms)
+
     def memory(self):
         u, b = C.c_size_t(), C.c_size_t()
         L.call("pme_batch_memory", self.h, C.byref(u), C.byref(b))

@@ -201,6 +201,12 @@ pme_status pme_batch_get_tuning(pme_batch_t b, int* brick_target, int* fft_tile,
  * best time per solve (ms). */
 pme_status pme_batch_autotune(pme_batch_t b, const char* cache_path, float* best_ms);
 pme_status pme_batch_memory(pme_batch_t b, size_t* used_bytes, size_t* budget_bytes);
+/* Per-stage kernel times of the last solve (HIP events on the batch's
+ * stream; solves run eagerly while timing is on).  ms[8]: bin, spread,
+ * x forward, y forward, z forward+solve+inverse, y inverse, x inverse,
+ * gather -- the order of pme_recip_get_timing. */
+pme_status pme_batch_set_timing(pme_batch_t b, int enable);
+pme_status pme_batch_get_timing(pme_batch_t b, float* ms);
 
 /* Host FP64 bsp_mod table (calc_prefac/dftmod, CudaPMERecip.cu:2481-2579). */
 pme_status pme_bsp_mod(int order, int n, double* out);

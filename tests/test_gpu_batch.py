@@ -240,4 +240,24 @@ def test_autotune_cache(tmp_path):
     for s in sl:
         _add(b2, s, torch, dev)
     ms2 = b2.autotune(str(cache))
-    assert b2.get_tuning() == tune and math.isclose(ms2, ms, rel_tol=1e-6)
+    assert b2.get_tuning() == tune and math.isclose(ms2, ms, rel_tol=1e-5)
+
+
+def test_stage_timing():
+    torch, dev = torch_dev()
+    sl = _mixed_systems()[:3]
+    b = PmeBatch(4 << 30)
+    reg = [_add(b, s, torch, dev) for s in sl]
+    ids = [r[0] for r in reg]
+    b.solve()
+    e0, _ = b.results(ids)
+    b.set_timing(True)
+    b.solve()
+    e1, _ = b.results(ids)
+    ms = b.get_timing()
+    say(f"  batch stage ms: {[round(v, 4) for v in ms]}")
+    assert len(ms) == 8 and all(v > 0 for v in ms)
+    assert e0 == e1
+    b.set_timing(False)
+    with pytest.raises(L.PmeError):
+        b.get_timing()
