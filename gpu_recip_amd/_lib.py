@@ -22,6 +22,7 @@ FFT_COLUMN, FFT_SLAB, FFT_BOX = 0, 1, 2
 ACCUM_FLOAT, ACCUM_INT32, ACCUM_INT64 = 0, 1, 2
 FORCE_FLOAT, FORCE_FLOAT3, FORCE_INT64 = 0, 1, 2
 GRID_REAL, GRID_SPECTRUM = 0, 1
+PREC_FP32, PREC_BF16X3, PREC_FP16X3 = 0, 1, 2
 
 
 class PmeError(RuntimeError):
@@ -44,11 +45,26 @@ class SystemDesc(C.Structure):
         ("nfftx", C.c_int), ("nffty", C.c_int), ("nfftz", C.c_int), ("order", C.c_int),
         ("kappa", C.c_double), ("recip", C.c_double * 9), ("ncoord", C.c_int),
         ("d_xyzq", C.c_void_p), ("d_force", C.c_void_p), ("force_type", C.c_int),
-        ("stride", C.c_int), ("calc_energy_virial", C.c_int),
+        ("stride", C.c_int), ("calc_energy_virial", C.c_int), ("precision", C.c_int),
     ]
 
 
+class ServerStats(C.Structure):
+    """pme_server_stats (include/pme_recip.h)."""
+    _fields_ = [
+        ("submitted", C.c_ulonglong), ("completed", C.c_ulonglong), ("failed", C.c_ulonglong),
+        ("batches", C.c_ulonglong), ("mean_batch", C.c_double), ("max_batch", C.c_int),
+        ("evictions", C.c_ulonglong), ("admitted_new", C.c_ulonglong), ("resident", C.c_int),
+        ("arena_used", C.c_size_t), ("arena_budget", C.c_size_t),
+        ("p50_ms", C.c_double), ("p99_ms", C.c_double), ("mean_ms", C.c_double), ("max_ms", C.c_double),
+    ]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
 _pi = C.POINTER(C.c_int)
+_pu64 = C.POINTER(C.c_ulonglong)
 _sz = C.c_size_t
 
 # name -> argtypes (restype is pme_status = int unless listed in _RESTYPE)
@@ -71,6 +87,7 @@ SIGNATURES = {
     "pme_recip_destroy": [_vp],
     "pme_recip_set_stream": [_vp, _vp],
     "pme_recip_set_order": [_vp, _i],
+    "pme_recip_set_precision": [_vp, _i],
     "pme_recip_get_info": [_vp, C.POINTER(_i), C.POINTER(_i), C.POINTER(_i), C.POINTER(_i)],
     "pme_recip_spread_charge": [_vp, _vp, _i, _pd],
     "pme_recip_r2c_fft": [_vp],
@@ -97,6 +114,16 @@ SIGNATURES = {
     "pme_batch_autotune": [_vp, C.c_char_p, C.POINTER(C.c_float)],
     "pme_batch_memory": [_vp, C.POINTER(_sz), C.POINTER(_sz)],
     "pme_batch_set_timing": [_vp, _i],
+    "pme_server_create": [_sz, _i, C.POINTER(_vp)],
+    "pme_server_destroy": [_vp],
+    "pme_server_submit": [_vp, _i, C.POINTER(SystemDesc), _vp, _pu64],
+    "pme_server_wait": [_vp, C.c_ulonglong, _pd, _pd, _pd],
+    "pme_server_poll": [_vp, C.c_ulonglong, _pi, _pd, _pd, _pd],
+    "pme_server_drain": [_vp],
+    "pme_server_release": [_vp, _i],
+    "pme_server_stats_get": [_vp, C.POINTER(ServerStats)],
+    "pme_server_stats_reset": [_vp],
+    "pme_server_set_tuning": [_vp, _i, _i, _i],
     "pme_batch_get_timing": [_vp, C.POINTER(C.c_float)],
 }
 _RESTYPE = {"pme_last_error": C.c_char_p, "pme_status_string": C.c_char_p}

@@ -24,6 +24,7 @@
 #include "../../include/pme_recip.h"
 #include "pme_internal.hpp"
 #include "pme_plan.hpp"
+#include "pme_sched.hpp"
 
 using namespace pme;
 
@@ -43,32 +44,14 @@ using namespace pme;
 
 namespace {
 
-const double kPiB = 3.14159265358979323846;
-
 struct Slot {
   bool used = false;
   pme_system_desc in{};
   int cap_atoms = 0;
-  BrickPlan bp{};
-  ChunkPlan cp{};
-  FftTiles ft{};
-  RadixPlan rp[3]{};
+  SystemPlan plan{};
   SysDesc d{};
+  void* region = nullptr;  // one allocation: grid, spectrum, record store, partials
   size_t bytes = 0;
-  float* grid = nullptr;
-  float2* cgrid = nullptr;
-  int* seg_off = nullptr;
-  float4* rec_pos = nullptr;
-  int* rec_idx = nullptr;
-  double* partial = nullptr;
-};
-
-struct Launch {
-  int stage;     // BatchStage
-  int order;     // spread/gather template order (0 otherwise)
-  int offset;    // first item in the concatenated list
-  int count;     // items
-  size_t lds;    // dynamic LDS bytes (max over the launch's systems)
 };
 
 }  // namespace
@@ -81,7 +64,7 @@ struct pme_batch_s {
   bool own_stream = false;
   hipEvent_t ev_in = nullptr, ev_done = nullptr;
   size_t budget = 0, used = 0;
-  int brick_target = kDefaultBrick, fft_tile = kDefaultFftTile, wg_per_cu = 0;
+  Tuning tune;
   bool use_graph = true;
   int ncu = 256;
   std::string arch;
@@ -90,8 +73,7 @@ struct pme_batch_s {
   int d_sys_cap = 0;
   double* d_ev = nullptr;    // [d_sys_cap * 10]
   double* h_ev = nullptr;    // pinned mirror
-  std::map<int, float2*> tw_cache;
-  std::map<std::pair<int, int>, float*> pf_cache;
+  TableCache tables;
   // work lists of the current subset
   std::vector<int> cur_ids;
   bool lists_valid = false;
@@ -122,184 +104,28 @@ static void invalidate(pme_batch_t b) {
 }
 
 static void free_slot_state(pme_batch_t b, Slot& s) {
-  (void)hipFree(s.grid);
-  (void)hipFree(s.cgrid);
-  (void)hipFree(s.seg_off);
-  (void)hipFree(s.rec_pos);
-  (void)hipFree(s.rec_idx);
-  (void)hipFree(s.partial);
-  s.grid = nullptr;
-  s.cgrid = nullptr;
-  s.seg_off = nullptr;
-  s.rec_pos = nullptr;
-  s.rec_idx = nullptr;
-  s.partial = nullptr;
+  (void)hipFree(s.region);
+  s.region = nullptr;
   b->used -= s.bytes;
   s.bytes = 0;
 }
 
-static pme_status shared_tables(pme_batch_t b, int order, int n, const float2** tw, const float** pf) {
-  auto it = b->tw_cache.find(n);
-  if (it == b->tw_cache.end()) {
-    float2* d = nullptr;
-    BHIP(hipMalloc(&d, sizeof(float2) * n));
-    const std::vector<float2> t = twiddles(n);
-    BHIP(hipMemcpy(d, t.data(), sizeof(float2) * n, hipMemcpyHostToDevice));
-    b->used += sizeof(float2) * n;
-    it = b->tw_cache.emplace(n, d).first;
-  }
-  *tw = it->second;
-  auto jt = b->pf_cache.find({order, n});
-  if (jt == b->pf_cache.end()) {
-    std::vector<double> m(n);
-    bsp_mod_host(order, n, m.data());
-    std::vector<float> f(n);
-    for (int i = 0; i < n; i++) f[i] = (float)m[i];
-    float* d = nullptr;
-    BHIP(hipMalloc(&d, sizeof(float) * n));
-    BHIP(hipMemcpy(d, f.data(), sizeof(float) * n, hipMemcpyHostToDevice));
-    b->used += sizeof(float) * n;
-    jt = b->pf_cache.emplace(std::make_pair(order, n), d).first;
-  }
-  *pf = jt->second;
-  return PME_OK;
-}
-
-static pme_status validate(const pme_system_desc* s) {
-  if (!s) return set_error(PME_ERR_INVALID_ARG, "system: null descriptor");
-  if (s->order != 4 && s->order != 6 && s->order != 8) return set_error(PME_ERR_UNSUPPORTED, "system: order %d", s->order);
-  const int n[3] = {s->nfftx, s->nffty, s->nfftz};
-  for (int a = 0; a < 3; a++)
-    if (n[a] < s->order || n[a] > 512 || !smooth235(n[a]))
-      return set_error(PME_ERR_UNSUPPORTED, "system: grid edge %d", n[a]);
-  const double* r = s->recip;
-  if (r[1] != 0.0 || r[2] != 0.0 || r[3] != 0.0 || r[5] != 0.0 || r[6] != 0.0 || r[7] != 0.0)
-    return set_error(PME_ERR_UNSUPPORTED, "system: only orthorhombic boxes are supported");
-  if (!(r[0] > 0.0 && r[4] > 0.0 && r[8] > 0.0)) return set_error(PME_ERR_INVALID_ARG, "system: recip diagonal");
-  if (!(s->kappa > 0.0)) return set_error(PME_ERR_INVALID_ARG, "system: kappa %g", s->kappa);
-  if (s->ncoord < 0) return set_error(PME_ERR_INVALID_ARG, "system: ncoord %d", s->ncoord);
-  if (s->ncoord > 0 && (!s->d_xyzq || !s->d_force)) return set_error(PME_ERR_INVALID_ARG, "system: null device pointer");
-  if (s->force_type < PME_FORCE_FLOAT || s->force_type > PME_FORCE_INT64)
-    return set_error(PME_ERR_INVALID_ARG, "system: force_type %d", s->force_type);
-  if (s->force_type != PME_FORCE_FLOAT3 && s->stride < s->ncoord)
-    return set_error(PME_ERR_INVALID_ARG, "system: stride %d < ncoord %d", s->stride, s->ncoord);
-  return PME_OK;
-}
-
-// Values of the descriptor that change with coordinates, box and outputs.
-static void fill_dynamic(Slot& s) {
-  const pme_system_desc& in = s.in;
-  SysDesc& d = s.d;
-  d.bin.r1 = (float)in.recip[0];
-  d.bin.r2 = (float)in.recip[4];
-  d.bin.r3 = (float)in.recip[8];
-  d.bin.nchunk = std::max(1, (in.ncoord + s.cp.chunk - 1) / s.cp.chunk);
-  d.xyzq = (const float4*)in.d_xyzq;
-  d.natom = in.ncoord;
-  d.force = in.d_force;
-  d.force_type = in.force_type;
-  d.stride = in.stride;
-  d.sa.r1 = d.bin.r1;
-  d.sa.r2 = d.bin.r2;
-  d.sa.r3 = d.bin.r3;
-  const double inv_vol = in.recip[0] * in.recip[4] * in.recip[8];
-  d.sa.piv_inv = (float)(inv_vol / kPiB);
-  d.sa.fac = (float)(kPiB * kPiB / (in.kappa * in.kappa));
-  d.sa.calc_ev = in.calc_energy_virial ? 1 : 0;
-}
-
-// Plan one system under the batch's tuning; returns the bytes its resident
-// state needs.  Touches no device memory.
-static pme_status plan_only(pme_batch_t b, Slot& s, size_t* bytes) {
-  const pme_system_desc& in = s.in;
-  const int n[3] = {in.nfftx, in.nffty, in.nfftz};
-  if (!plan_bricks(n, in.order, b->brick_target, &s.bp))
-    return set_error(PME_ERR_UNSUPPORTED, "system: no brick plan fits the LDS budget");
-  s.cp = plan_chunks(s.cap_atoms);
-  const bool even = (n[0] % 2) == 0;
-  if (!radix_plan(even ? n[0] / 2 : n[0], even ? 2 : 1, &s.rp[0]) || !radix_plan(n[1], 1, &s.rp[1]) ||
-      !radix_plan(n[2], 1, &s.rp[2]))
-    return set_error(PME_ERR_UNSUPPORTED, "system: no radix plan");
-  s.ft = pick_fft_tiles(n[0], n[1], n[2], s.rp[0].n, b->fft_tile);
-  const int hx = n[0] / 2 + 1;
-  const int nzblk = cols_blocks(hx, s.ft.TXz, n[1]);
-  const size_t G = (size_t)n[0] * n[1] * n[2];
-  const size_t Gc = (size_t)hx * n[1] * n[2];
-  const size_t rs = 8 * (size_t)s.cp.chunk;
-  *bytes = sizeof(float) * G + sizeof(float2) * Gc + sizeof(int) * (size_t)s.cp.nchunk * (s.bp.nbins + 1) +
-           (sizeof(float4) + sizeof(int)) * rs * s.cp.nchunk + sizeof(double) * 7 * (size_t)nzblk;
-  return PME_OK;
-}
-
-// Allocate the planned state of slot `slot_index` and fill its descriptor.
-static pme_status alloc_slot(pme_batch_t b, Slot& s, int slot_index, size_t bytes) {
-  const pme_system_desc& in = s.in;
-  const int n[3] = {in.nfftx, in.nffty, in.nfftz};
-  const int hx = n[0] / 2 + 1;
-  const int nzblk = cols_blocks(hx, s.ft.TXz, n[1]);
-  const size_t G = (size_t)n[0] * n[1] * n[2];
-  const size_t Gc = (size_t)hx * n[1] * n[2];
-  const size_t rs = 8 * (size_t)s.cp.chunk;
-  s.bytes = bytes;
-  b->used += bytes;
-  BHIP(hipMalloc(&s.grid, sizeof(float) * G));
-  BHIP(hipMalloc(&s.cgrid, sizeof(float2) * Gc));
-  BHIP(hipMalloc(&s.seg_off, sizeof(int) * (size_t)s.cp.nchunk * (s.bp.nbins + 1)));
-  BHIP(hipMalloc(&s.rec_pos, sizeof(float4) * rs * s.cp.nchunk));
-  BHIP(hipMalloc(&s.rec_idx, sizeof(int) * rs * s.cp.nchunk));
-  BHIP(hipMalloc(&s.partial, sizeof(double) * 7 * (size_t)nzblk));
-
-  SysDesc& d = s.d;
-  std::memset(&d, 0, sizeof(d));
-  d.bin.nx = n[0];
-  d.bin.ny = n[1];
-  d.bin.nz = n[2];
-  d.bin.nbx = s.bp.nb[0];
-  d.bin.nby = s.bp.nb[1];
-  d.bin.nbz = s.bp.nb[2];
-  d.bin.order = in.order;
-  d.bin.chunk = s.cp.chunk;
-  d.bin.rec_stride = (int)rs;
-  d.bin.seg_off = s.seg_off;
-  d.bin.rec_pos = s.rec_pos;
-  d.bin.rec_idx = s.rec_idx;
-  for (int a = 0; a < 3; a++) d.wmax[a] = s.bp.wmax[a];
-  d.grid = s.grid;
-  d.cgrid = s.cgrid;
-  for (int a = 0; a < 3; a++) {
-    const float2* tw = nullptr;
-    const float* pf = nullptr;
-    BTRY(shared_tables(b, in.order, n[a], &tw, &pf));
-    d.ax[a].rp = s.rp[a];
-    d.ax[a].tw = tw;
-    d.ax[a].ntw = n[a];
-    if (a == 0) d.sa.pf1 = pf;
-    if (a == 1) d.sa.pf2 = pf;
-    if (a == 2) d.sa.pf3 = pf;
-  }
-  d.NR = s.ft.NR;
-  d.TXy = s.ft.TXy;
-  d.TXz = s.ft.TXz;
-  d.sa.n1 = n[0];
-  d.sa.n2 = n[1];
-  d.sa.n3 = n[2];
-  d.sa.nf1 = n[0] / 2 + (n[0] % 2);
-  d.sa.nf2 = n[1] / 2 + (n[1] % 2);
-  d.sa.nf3 = n[2] / 2 + (n[2] % 2);
-  d.sa.partial = s.partial;
-  d.nzblk = nzblk;
-  d.ev_out = b->d_ev + 10 * (size_t)slot_index;
-  fill_dynamic(s);
+// Allocate the planned region of slot `slot_index` and fill its descriptor.
+static pme_status alloc_slot(pme_batch_t b, Slot& s, int slot_index) {
+  BHIP(hipMalloc(&s.region, s.plan.total));
+  s.bytes = s.plan.total;
+  b->used += s.bytes;
+  BTRY(fill_desc(s.d, s.in, s.plan, s.region, b->tables));
+  s.d.ev_out = b->d_ev + 10 * (size_t)slot_index;
   return PME_OK;
 }
 
 // Plan, check the budget (nothing allocated when it does not fit), allocate.
 static pme_status plan_slot(pme_batch_t b, Slot& s, int slot_index) {
-  size_t bytes = 0;
-  BTRY(plan_only(b, s, &bytes));
-  if (b->used + bytes > b->budget)
-    return set_error(PME_ERR_BUDGET, "system needs %zu bytes, %zu of %zu in use", bytes, b->used, b->budget);
-  return alloc_slot(b, s, slot_index, bytes);
+  BTRY(plan_system(s.in, s.cap_atoms, b->tune, &s.plan));
+  if (b->used + s.plan.total > b->budget)
+    return set_error(PME_ERR_BUDGET, "system needs %zu bytes, %zu of %zu in use", s.plan.total, b->used, b->budget);
+  return alloc_slot(b, s, slot_index);
 }
 
 static pme_status upload_desc(pme_batch_t b, int id) {
@@ -374,8 +200,7 @@ extern "C" pme_status pme_batch_destroy(pme_batch_t b) {
   drop_graph(b);
   for (auto& s : b->slots)
     if (s.used) free_slot_state(b, s);
-  for (auto& kv : b->tw_cache) (void)hipFree(kv.second);
-  for (auto& kv : b->pf_cache) (void)hipFree(kv.second);
+  b->tables.release();
   (void)hipFree(b->d_sys);
   (void)hipFree(b->d_ev);
   (void)hipHostFree(b->h_ev);
@@ -391,7 +216,7 @@ extern "C" pme_status pme_batch_destroy(pme_batch_t b) {
 
 extern "C" pme_status pme_batch_add(pme_batch_t b, const pme_system_desc* sys, int* id) {
   if (!b || !id) return set_error(PME_ERR_INVALID_ARG, "pme_batch_add: null argument");
-  BTRY(validate(sys));
+  BTRY(validate_system(sys));
   int k = -1;
   for (size_t i = 0; i < b->slots.size(); i++)
     if (!b->slots[i].used) {
@@ -423,16 +248,16 @@ extern "C" pme_status pme_batch_add(pme_batch_t b, const pme_system_desc* sys, i
 extern "C" pme_status pme_batch_update(pme_batch_t b, int id, const pme_system_desc* sys) {
   if (!b || id < 0 || id >= (int)b->slots.size() || !b->slots[id].used)
     return set_error(PME_ERR_INVALID_ARG, "pme_batch_update: unknown id %d", id);
-  BTRY(validate(sys));
+  BTRY(validate_system(sys));
   Slot& s = b->slots[id];
   const pme_system_desc& o = s.in;
   const bool same_plan = sys->nfftx == o.nfftx && sys->nffty == o.nffty && sys->nfftz == o.nfftz &&
-                         sys->order == o.order && sys->ncoord <= s.cap_atoms;
+                         sys->order == o.order && sys->precision == o.precision && sys->ncoord <= s.cap_atoms;
   const int old_nchunk = s.d.bin.nchunk;
   BTRY(hipStreamSynchronize(b->stream) == hipSuccess ? PME_OK : set_error(PME_ERR_DEVICE, "sync"));
   if (same_plan) {
     s.in = *sys;
-    fill_dynamic(s);
+    fill_dynamic(s.d, s.in, s.plan);
     if (s.d.bin.nchunk != old_nchunk) invalidate(b);
   } else {
     free_slot_state(b, s);
@@ -461,66 +286,25 @@ extern "C" pme_status pme_batch_remove(pme_batch_t b, int id) {
 }
 
 // Work lists of the subset: per stage (and per order for spread/gather) one
-// contiguous run of items, systems in decreasing size so the largest tiles
-// are dispatched first.
+// contiguous run of items, systems in decreasing cost so the largest tiles
+// are dispatched first.  Items index the slot table.
 static pme_status build_lists(pme_batch_t b, const std::vector<int>& ids) {
   std::vector<int> order_ids = ids;
   std::sort(order_ids.begin(), order_ids.end(), [&](int x, int y) {
-    const Slot& a = b->slots[x];
-    const Slot& c = b->slots[y];
-    const double ca = (double)a.in.ncoord * a.in.order * a.in.order * a.in.order + (double)a.in.nfftx * a.in.nffty * a.in.nfftz;
-    const double cc = (double)c.in.ncoord * c.in.order * c.in.order * c.in.order + (double)c.in.nfftx * c.in.nffty * c.in.nfftz;
+    const double ca = system_cost(b->slots[x].in), cc = system_cost(b->slots[y].in);
     return ca > cc || (ca == cc && x < y);
   });
+  std::vector<const SysDesc*> descs;
+  std::vector<const SystemPlan*> plans;
+  std::vector<int> orders;
+  for (int id : order_ids) {
+    descs.push_back(&b->slots[id].d);
+    plans.push_back(&b->slots[id].plan);
+    orders.push_back(b->slots[id].in.order);
+  }
   std::vector<WorkItem> items;
-  b->launches.clear();
-  auto add_stage = [&](int stage, int order) {
-    Launch L{stage, order, (int)items.size(), 0, 0};
-    for (int id : order_ids) {
-      const Slot& s = b->slots[id];
-      const SysDesc& d = s.d;
-      if ((stage == BS_SPREAD || stage == BS_GATHER) && s.in.order != order) continue;
-      int cnt = 0;
-      size_t lds = 0;
-      const int hx = d.bin.nx / 2 + 1;
-      switch (stage) {
-        case BS_BIN:
-          cnt = d.bin.nchunk;
-          lds = bin_lds_bytes(s.bp.nbins);
-          break;
-        case BS_SPREAD:
-          cnt = s.bp.nbins;
-          lds = spread_lds_bytes(s.in.order, s.bp.wmax, d.bin.nchunk);
-          break;
-        case BS_GATHER:
-          cnt = s.bp.nbins;
-          lds = gather_lds_bytes(s.in.order, s.bp.wmax, d.bin.nchunk);
-          break;
-        case BS_FFTX:
-          cnt = (d.bin.ny * d.bin.nz + d.NR - 1) / d.NR;
-          lds = fft_x_lds_bytes(d.ax[0].rp.n, d.NR, d.bin.nx);
-          break;
-        case BS_FFTY:
-          cnt = cols_blocks(hx, d.TXy, d.bin.nz);
-          lds = fft_cols_lds_bytes(d.bin.ny, d.TXy);
-          break;
-        case BS_FFTZ:
-          cnt = cols_blocks(hx, d.TXz, d.bin.ny);
-          lds = fft_cols_lds_bytes(d.bin.nz, d.TXz);
-          break;
-      }
-      for (int i = 0; i < cnt; i++) items.push_back(WorkItem{id, i});
-      L.count += cnt;
-      L.lds = std::max(L.lds, lds);
-    }
-    if (L.count > 0) b->launches.push_back(L);
-  };
-  add_stage(BS_BIN, 0);
-  for (int o : {4, 6, 8}) add_stage(BS_SPREAD, o);
-  add_stage(BS_FFTX, 0);
-  add_stage(BS_FFTY, 0);
-  add_stage(BS_FFTZ, 0);
-  for (int o : {4, 6, 8}) add_stage(BS_GATHER, o);
+  build_items(descs.data(), plans.data(), orders.data(), (int)order_ids.size(), items, b->launches);
+  for (WorkItem& w : items) w.sys = order_ids[w.sys];
   BHIP(hipStreamSynchronize(b->stream));
   if (items.size() > b->items_cap) {
     (void)hipFree(b->d_items);
@@ -535,51 +319,8 @@ static pme_status build_lists(pme_batch_t b, const std::vector<int>& ids) {
   return PME_OK;
 }
 
-static int grid_for(pme_batch_t b, int count, bool big_block) {
-  if (b->wg_per_cu <= 0) return count;
-  const int per_cu = big_block ? 1 : b->wg_per_cu;
-  return std::max(1, std::min(count, b->ncu * per_cu));
-}
-
-// Stage order: bin, spread, x forward, y forward, z forward + influence
-// function + z inverse, y inverse (with the per-system energy/virial
-// reduction), x inverse, gather.  With timing on, event k is recorded before
-// stage k and event 8 after the last.
 static pme_status enqueue(pme_batch_t b) {
-  auto mark = [&](int k) -> pme_status {
-    if (b->timing) BHIP(hipEventRecord(b->tev[k], b->stream));
-    return PME_OK;
-  };
-  auto run = [&](int stage, bool inverse) -> pme_status {
-    for (const Launch& L : b->launches) {
-      if (L.stage != stage) continue;
-      const WorkItem* it = b->d_items + L.offset;
-      const bool big = stage == BS_SPREAD || stage == BS_GATHER;
-      const int grid = grid_for(b, L.count, big);
-      switch (stage) {
-        case BS_BIN: BHIP(launch_batch_bin(b->d_sys, it, L.count, grid, L.lds, b->stream)); break;
-        case BS_SPREAD: BHIP(launch_batch_spread(L.order, b->d_sys, it, L.count, grid, L.lds, b->stream)); break;
-        case BS_FFTX: BHIP(launch_batch_fft_x(inverse ? 1 : 0, b->d_sys, it, L.count, grid, L.lds, b->stream)); break;
-        case BS_FFTY:
-          BHIP(launch_batch_cols(1, inverse ? COL_INV : COL_FWD, inverse ? 1 : 0, b->d_sys, it, L.count, grid, L.lds,
-                                 b->stream));
-          break;
-        case BS_FFTZ:
-          BHIP(launch_batch_cols(2, COL_FWD_SOLVE_INV, 0, b->d_sys, it, L.count, grid, L.lds, b->stream));
-          break;
-        case BS_GATHER: BHIP(launch_batch_gather(L.order, b->d_sys, it, L.count, grid, L.lds, b->stream)); break;
-        default: break;
-      }
-    }
-    return PME_OK;
-  };
-  const int seq[8][2] = {{BS_BIN, 0}, {BS_SPREAD, 0}, {BS_FFTX, 0}, {BS_FFTY, 0},
-                         {BS_FFTZ, 0}, {BS_FFTY, 1}, {BS_FFTX, 1}, {BS_GATHER, 0}};
-  for (int k = 0; k < 8; k++) {
-    BTRY(mark(k));
-    BTRY(run(seq[k][0], seq[k][1] != 0));
-  }
-  return mark(8);
+  return enqueue_stages(b->launches, b->d_sys, b->d_items, b->tune, b->ncu, b->stream, b->timing ? b->tev : nullptr);
 }
 
 static pme_status run_solve(pme_batch_t b);
@@ -678,10 +419,9 @@ static pme_status replan_all(pme_batch_t b) {
   size_t total = b->used;
   for (Slot& s : b->slots) {
     if (!s.used) continue;
-    Slot t = s;
-    size_t bytes = 0;
-    BTRY(plan_only(b, t, &bytes));
-    total = total - s.bytes + bytes;
+    SystemPlan np;
+    BTRY(plan_system(s.in, s.cap_atoms, b->tune, &np));
+    total = total - s.bytes + np.total;
   }
   if (total > b->budget) return set_error(PME_ERR_BUDGET, "re-plan needs %zu of %zu bytes", total, b->budget);
   BHIP(hipStreamSynchronize(b->stream));
@@ -690,9 +430,8 @@ static pme_status replan_all(pme_batch_t b) {
     Slot& s = b->slots[i];
     if (!s.used) continue;
     free_slot_state(b, s);
-    size_t bytes = 0;
-    pme_status st = plan_only(b, s, &bytes);
-    if (st == PME_OK) st = alloc_slot(b, s, (int)i, bytes);
+    pme_status st = plan_system(s.in, s.cap_atoms, b->tune, &s.plan);
+    if (st == PME_OK) st = alloc_slot(b, s, (int)i);
     if (st != PME_OK) {
       free_slot_state(b, s);
       s.used = false;
@@ -729,18 +468,15 @@ extern "C" pme_status pme_batch_set_tuning(pme_batch_t b, int brick_target, int 
       wg_per_cu < 0 || wg_per_cu > 64)
     return set_error(PME_ERR_INVALID_ARG, "pme_batch_set_tuning: brick %d tile %d wg/cu %d", brick_target, fft_tile,
                      wg_per_cu);
-  const bool replan = brick_target != b->brick_target || fft_tile != b->fft_tile;
-  const int ob = b->brick_target, ot = b->fft_tile;
-  b->brick_target = brick_target;
-  b->fft_tile = fft_tile;
-  b->wg_per_cu = wg_per_cu;
+  const bool replan = brick_target != b->tune.brick_target || fft_tile != b->tune.fft_tile;
+  const Tuning old = b->tune;
+  b->tune.brick_target = brick_target;
+  b->tune.fft_tile = fft_tile;
+  b->tune.wg_per_cu = wg_per_cu;
   drop_graph(b);
   if (replan) {
     pme_status st = replan_all(b);
-    if (st == PME_ERR_BUDGET || st == PME_ERR_UNSUPPORTED) {  // nothing was freed
-      b->brick_target = ob;
-      b->fft_tile = ot;
-    }
+    if (st == PME_ERR_BUDGET || st == PME_ERR_UNSUPPORTED) b->tune = old;  // nothing was freed
     return st;
   }
   return PME_OK;
@@ -748,9 +484,9 @@ extern "C" pme_status pme_batch_set_tuning(pme_batch_t b, int brick_target, int 
 
 extern "C" pme_status pme_batch_get_tuning(pme_batch_t b, int* brick_target, int* fft_tile, int* wg_per_cu) {
   if (!b) return set_error(PME_ERR_INVALID_ARG, "pme_batch_get_tuning: null batch");
-  if (brick_target) *brick_target = b->brick_target;
-  if (fft_tile) *fft_tile = b->fft_tile;
-  if (wg_per_cu) *wg_per_cu = b->wg_per_cu;
+  if (brick_target) *brick_target = b->tune.brick_target;
+  if (fft_tile) *fft_tile = b->tune.fft_tile;
+  if (wg_per_cu) *wg_per_cu = b->tune.wg_per_cu;
   return PME_OK;
 }
 
@@ -858,7 +594,7 @@ extern "C" pme_status pme_batch_autotune(pme_batch_t b, const char* cache_path, 
     if (s.used && s.in.force_type == PME_FORCE_INT64)
       return set_error(PME_ERR_STATE, "pme_batch_autotune: resident INT64 (accumulating) outputs");
   float best = 1e30f;
-  int bb = b->brick_target, bt = b->fft_tile, bw = b->wg_per_cu;
+  int bb = b->tune.brick_target, bt = b->tune.fft_tile, bw = b->tune.wg_per_cu;
   for (int brick : {12, 16, 20, 24})
     for (int tile : {8, 16, 32})
       for (int wpc : {0, 4}) {

@@ -229,15 +229,169 @@ __device__ __forceinline__ void stockham_stage(const float2* __restrict__ src, f
   }
 }
 
+// ---- split-precision radix-16 stage on the matrix cores ---------------------
+// Precision of the radix-16 first stage: FP32 (no MFMA stage), BF16 hi-lo
+// with three products, FP16 hi-lo with three products and a per-column
+// power-of-two scale (FP16's range).  Accumulation is FP32 in both.
+enum { PREC_FP32 = 0, PREC_BF16X3 = 1, PREC_FP16X3 = 2 };
+
+typedef short v4s16 __attribute__((ext_vector_type(4)));
+typedef _Float16 v4f16 __attribute__((ext_vector_type(4)));
+typedef float v16f32 __attribute__((ext_vector_type(16)));
+
+// round-to-nearest-even FP32 -> BF16 bits (finite inputs)
+__device__ __forceinline__ unsigned short bf16_bits(float x) {
+  const unsigned u = __float_as_uint(x);
+  return (unsigned short)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+}
+__device__ __forceinline__ float bf16_value(unsigned short b) { return __uint_as_float((unsigned)b << 16); }
+
+// x = hi + lo in the chosen 16-bit format (hi, lo as raw 16-bit patterns)
+template <int PREC>
+__device__ __forceinline__ void split16(float x, short& hi, short& lo) {
+  if (PREC == PREC_BF16X3) {
+    const unsigned short h = bf16_bits(x);
+    hi = (short)h;
+    lo = (short)bf16_bits(x - bf16_value(h));
+  } else {
+    const _Float16 h = (_Float16)x;
+    const _Float16 l = (_Float16)(x - (float)h);
+    hi = __builtin_bit_cast(short, h);
+    lo = __builtin_bit_cast(short, l);
+  }
+}
+
+template <int PREC>
+__device__ __forceinline__ v16f32 mfma32x32x8(v4s16 a, v4s16 b, v16f32 c) {
+  if (PREC == PREC_BF16X3) return __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, c, 0, 0, 0);
+  return __builtin_amdgcn_mfma_f32_32x32x8f16(__builtin_bit_cast(v4f16, a), __builtin_bit_cast(v4f16, b), c, 0, 0, 0);
+}
+
+// First Stockham stage with radix 16 (p = 1, no twiddles) as a matrix
+// product on the matrix cores: for every column c = i*nlines + l
+// (i < T = n/16), y[r] = sum_m W16^(S r m) x[m] with x[m] = src[(i + m T) ls + l]
+// and y[r] -> dst[(16 i + r) ls + l].  In real arithmetic this is the 32x32
+// matrix [[Fr, -Fi], [Fi, Fr]] times the 32-row column [Re x; Im x], done per
+// 32 columns by one wave with v_mfma_f32_32x32x8 (K = 8, four K steps).
+// Operand layouts of the 32x32x8 instruction: lane L = 32h + j holds
+// A[j][8q + 4h + t] and B[8q + 4h + t][j] (t < 4) and returns D[8(r/4) + 4h +
+// r%4][j] in accumulator register r.  hi-lo: D = Ah Bh + Ah Bl + Al Bh.
+template <int S, int PREC>
+__device__ __forceinline__ void mfma_dft16_stage(const float2* __restrict__ src, float2* __restrict__ dst, int n,
+                                                 int nlines, int ls, int tid, int nthr) {
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int nwave = nthr >> 6;
+  const int h = lane >> 5;
+  const int j = lane & 31;
+  const int T = n >> 4;
+  const int ncols = T * nlines;
+  // A operand (the DFT matrix in real form) for the four K steps
+  const float c16[16] = {1.0f, 0.92387953251128674f, 0.70710678118654752f, 0.38268343236508977f, 0.0f,
+                         -0.38268343236508977f, -0.70710678118654752f, -0.92387953251128674f, -1.0f,
+                         -0.92387953251128674f, -0.70710678118654752f, -0.38268343236508977f, 0.0f,
+                         0.38268343236508977f, 0.70710678118654752f, 0.92387953251128674f};
+  v4s16 ah[4], al[4];
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+      const int k = 8 * q + 4 * h + t;
+      const int r = j & 15, m = k & 15;
+      const int e = (r * m) & 15;
+      const float cr = c16[e];
+      const float si = (float)S * c16[(e + 12) & 15];  // S * sin(2 pi e / 16)
+      float a;
+      if (j < 16) a = (k < 16) ? cr : -si;
+      else a = (k < 16) ? si : cr;
+      short hi, lo;
+      split16<PREC>(a, hi, lo);
+      ah[q][t] = hi;
+      al[q][t] = lo;
+    }
+  }
+  const int nblk = (ncols + 31) >> 5;
+  for (int blk = wave; blk < nblk; blk += nwave) {
+    const int c = blk * 32 + j;
+    const bool valid = c < ncols;
+    const int i = valid ? c / nlines : 0;
+    const int l = valid ? c - i * nlines : 0;
+    float2 x[8];  // x[4h + t] and x[8 + 4h + t]
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+      x[t] = valid ? src[(i + (4 * h + t) * T) * ls + l] : make_float2(0.f, 0.f);
+      x[4 + t] = valid ? src[(i + (8 + 4 * h + t) * T) * ls + l] : make_float2(0.f, 0.f);
+    }
+    float scale = 1.0f, unscale = 1.0f;
+    if (PREC == PREC_FP16X3) {
+      float mx = 0.0f;
+#pragma unroll
+      for (int t = 0; t < 8; t++) mx = fmaxf(mx, fmaxf(fabsf(x[t].x), fabsf(x[t].y)));
+      mx = fmaxf(mx, __shfl_xor(mx, 32, 64));
+      if (mx > 0.0f) {
+        const int ex = ilogbf(mx);
+        scale = ldexpf(1.0f, 13 - ex);
+        unscale = ldexpf(1.0f, ex - 13);
+      }
+    }
+    v4s16 bh[4], bl[4];
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+      short hi, lo;
+      split16<PREC>(x[t].x * scale, hi, lo);      // q = 0: Re x[4h + t]
+      bh[0][t] = hi;
+      bl[0][t] = lo;
+      split16<PREC>(x[4 + t].x * scale, hi, lo);  // q = 1: Re x[8 + 4h + t]
+      bh[1][t] = hi;
+      bl[1][t] = lo;
+      split16<PREC>(x[t].y * scale, hi, lo);      // q = 2: Im x[4h + t]
+      bh[2][t] = hi;
+      bl[2][t] = lo;
+      split16<PREC>(x[4 + t].y * scale, hi, lo);  // q = 3: Im x[8 + 4h + t]
+      bh[3][t] = hi;
+      bl[3][t] = lo;
+    }
+    v16f32 acc = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int q = 0; q < 4; q++) acc = mfma32x32x8<PREC>(ah[q], bl[q], acc);
+#pragma unroll
+    for (int q = 0; q < 4; q++) acc = mfma32x32x8<PREC>(al[q], bh[q], acc);
+#pragma unroll
+    for (int q = 0; q < 4; q++) acc = mfma32x32x8<PREC>(ah[q], bh[q], acc);
+    if (valid) {
+#pragma unroll
+      for (int r = 0; r < 8; r++) {
+        const int row = 8 * (r >> 2) + 4 * h + (r & 3);  // 0..15: output point
+        dst[(16 * i + row) * ls + l] = make_float2(acc[r] * unscale, acc[r + 8] * unscale);
+      }
+    }
+  }
+}
+
 // Full transform of nlines lines held in LDS (buf0 -> result).  Returns the
-// buffer holding the result (buf0 or buf1).  Ends with a barrier.
+// buffer holding the result (buf0 or buf1).  Ends with a barrier.  With
+// prec != PREC_FP32 and a plan whose first radix is 16, that stage runs on
+// the matrix cores (mfma_dft16_stage).
 template <int S>
 __device__ float2* fft_lines(float2* buf0, float2* buf1, const float2* tw, const RadixPlan& rp,
-                             int nlines, int ls, int tid, int nthr) {
+                             int nlines, int ls, int tid, int nthr, int prec = PREC_FP32) {
   float2* src = buf0;
   float2* dst = buf1;
   int p = 1;
-  for (int s = 0; s < rp.nrad; s++) {
+  int s0 = 0;
+  if (prec != PREC_FP32 && rp.nrad > 0 && rp.rad[0] == 16) {
+    if (prec == PREC_BF16X3)
+      mfma_dft16_stage<S, PREC_BF16X3>(src, dst, rp.n, nlines, ls, tid, nthr);
+    else
+      mfma_dft16_stage<S, PREC_FP16X3>(src, dst, rp.n, nlines, ls, tid, nthr);
+    __syncthreads();
+    float2* t = src;
+    src = dst;
+    dst = t;
+    p = 16;
+    s0 = 1;
+  }
+  for (int s = s0; s < rp.nrad; s++) {
     const int R = rp.rad[s];
     switch (R) {
       case 2: stockham_stage<2, S>(src, dst, tw, rp.n, p, rp.tstride, nlines, ls, tid, nthr); break;

@@ -279,9 +279,9 @@ __device__ __forceinline__ void fft_cols_body(float2* __restrict__ cg, int n, in
   __syncthreads();
   float2* res;
   if (MODE == COL_INV) {
-    res = fft_lines<+1>(b0, b1, tw, ax.rp, TX, TX, tid, kBlock);
+    res = fft_lines<+1>(b0, b1, tw, ax.rp, TX, TX, tid, kBlock, ax.prec);
   } else {
-    res = fft_lines<-1>(b0, b1, tw, ax.rp, TX, TX, tid, kBlock);
+    res = fft_lines<-1>(b0, b1, tw, ax.rp, TX, TX, tid, kBlock, ax.prec);
   }
   double acc[7] = {0, 0, 0, 0, 0, 0, 0};
   if (MODE == COL_FWD_SOLVE_INV) {
@@ -294,7 +294,7 @@ __device__ __forceinline__ void fft_cols_body(float2* __restrict__ cg, int n, in
     __syncthreads();
   }
   if (MODE == COL_FWD_SOLVE_INV || MODE == COL_FWD_INV) {
-    res = fft_lines<+1>(res, res == b0 ? b1 : b0, tw, ax.rp, TX, TX, tid, kBlock);
+    res = fft_lines<+1>(res, res == b0 ? b1 : b0, tw, ax.rp, TX, TX, tid, kBlock, ax.prec);
   }
   for (int e = tid; e < n * TX; e += kBlock) {
     const int j = e / TX;

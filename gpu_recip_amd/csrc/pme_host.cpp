@@ -235,6 +235,9 @@ struct pme_recip_s {
   float* prefac[3] = {nullptr, nullptr, nullptr};
   float2* tw[3] = {nullptr, nullptr, nullptr};
   RadixPlan rp[3];
+  RadixPlan rp_split[3];      // radix-16 first stage (y, z) for the split-precision modes
+  bool has_split[3] = {false, false, false};
+  int prec = PME_PREC_FP32;
   int NR = 16, TXy = 16, TXz = 16;
 
   // bricks and the chunked record store of the binning
@@ -396,9 +399,11 @@ static BinArgs bin_args(pme_recip_t h, const double* recip) {
 
 static AxisArgs axis_args(pme_recip_t h, int a) {
   AxisArgs ax;
-  ax.rp = h->rp[a];
+  const bool split = h->prec != PME_PREC_FP32 && h->has_split[a];
+  ax.rp = split ? h->rp_split[a] : h->rp[a];
   ax.tw = h->tw[a];
   ax.ntw = a == 0 ? h->nx : (a == 1 ? h->ny : h->nz);
+  ax.prec = split ? h->prec : PME_PREC_FP32;
   return ax;
 }
 
@@ -487,6 +492,8 @@ extern "C" pme_status pme_recip_create(int nfftx, int nffty, int nfftz, int orde
       HIP_TRY(hipMemcpy(h->tw[a], t.data(), sizeof(float2) * n[a], hipMemcpyHostToDevice));
     }
     const bool even = (h->nx % 2) == 0;
+    h->has_split[1] = radix_plan_split(h->ny, 1, &h->rp_split[1]);
+    h->has_split[2] = radix_plan_split(h->nz, 1, &h->rp_split[2]);
     if (!radix_plan(even ? h->nx / 2 : h->nx, even ? 2 : 1, &h->rp[0]) || !radix_plan(h->ny, 1, &h->rp[1]) ||
         !radix_plan(h->nz, 1, &h->rp[2]))
       return fail(PME_ERR_UNSUPPORTED, "pme_recip_create: no radix plan");
@@ -524,6 +531,16 @@ extern "C" pme_status pme_recip_set_stream(pme_recip_t h, pme_stream_t stream) {
   // partials written on the old stream are reduced there first
   PME_TRY(flush_pending(h, (hipStream_t)stream, true));
   h->stream = (hipStream_t)stream;
+  return PME_OK;
+}
+
+extern "C" pme_status pme_recip_set_precision(pme_recip_t h, int precision) {
+  if (!h) return fail(PME_ERR_INVALID_ARG, "pme_recip_set_precision: null handle");
+  if (precision < PME_PREC_FP32 || precision > PME_PREC_FP16X3)
+    return fail(PME_ERR_INVALID_ARG, "pme_recip_set_precision: precision %d", precision);
+  if (h->state != ST_EMPTY && h->state != ST_CHARGE && h->state != ST_POTENTIAL)
+    return fail(PME_ERR_STATE, "pme_recip_set_precision: a transform is in progress");
+  h->prec = precision;
   return PME_OK;
 }
 

@@ -44,6 +44,7 @@ struct AxisArgs {
   RadixPlan rp;            // transform length and radices
   const float2* tw;        // e^{-2 pi i k/ntw}, k < ntw (FP64-computed, stored FP32)
   int ntw;                 // table length (the grid edge)
+  int prec;                // PREC_*: a radix-16 first stage runs on the matrix cores
 };
 
 // Influence function + energy/virial (scalar_sum), CudaPMERecip.cu:594-901

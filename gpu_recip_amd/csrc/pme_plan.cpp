@@ -62,6 +62,22 @@ bool radix_plan(int n, int tstride, RadixPlan* rp) {
   return m == 1;
 }
 
+bool radix_plan_split(int n, int tstride, RadixPlan* rp) {
+  if (n % 16 != 0) return false;
+  RadixPlan rest;
+  if (n > 16) {
+    if (!radix_plan(n / 16, tstride, &rest) || rest.nrad + 1 > 12) return false;
+  } else {
+    rest.nrad = 0;
+  }
+  rp->n = n;
+  rp->tstride = tstride;
+  rp->nrad = rest.nrad + 1;
+  rp->rad[0] = 16;
+  for (int s = 0; s < rest.nrad; s++) rp->rad[s + 1] = rest.rad[s];
+  return true;
+}
+
 std::vector<float2> twiddles(int n) {
   std::vector<float2> t(n);
   for (int k = 0; k < n; k++) {

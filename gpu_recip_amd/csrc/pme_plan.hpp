@@ -28,6 +28,10 @@ void bsp_mod_host(int order, int n, double* out);
 bool smooth235(int n);
 // Stockham radices for length n (8, 4, 2, 5, 3), table stride tstride
 bool radix_plan(int n, int tstride, RadixPlan* rp);
+// Plan whose first stage has radix 16 (run on the matrix cores in the
+// split-precision modes), the rest as radix_plan(n/16).  False if 16 does
+// not divide n.
+bool radix_plan_split(int n, int tstride, RadixPlan* rp);
 // e^{-2 pi i k/n}, k < n, computed in FP64
 std::vector<float2> twiddles(int n);
 

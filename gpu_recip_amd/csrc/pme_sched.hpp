@@ -1,0 +1,81 @@
+// pme_sched.hpp -- multi-system scheduling core shared by the batched path
+// (pme_batch.cpp) and the serving loop (pme_serve.cpp): planning one
+// system's resident state, filling its device descriptor, building the
+// per-stage work lists of a set of systems and enqueueing the stage launches.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <map>
+#include <utility>
+#include <vector>
+
+#include "../../include/pme_recip.h"
+#include "pme_internal.hpp"
+#include "pme_plan.hpp"
+
+namespace pme {
+
+// Resident state of one system under a tuning: plans and the byte size of
+// every buffer (each piece 256-byte aligned inside one region).
+struct SystemPlan {
+  BrickPlan bp;
+  ChunkPlan cp;
+  RadixPlan rp[3];
+  int prec[3];          // PREC_* per axis (the radix-16 stage on the matrix cores)
+  FftTiles ft;
+  int nzblk;            // z-pass tiles = energy/virial partials
+  size_t off[6];        // grid, cgrid, seg_off, rec_pos, rec_idx, partial
+  size_t total;         // bytes of the region
+};
+
+struct Tuning {
+  int brick_target = kDefaultBrick;
+  int fft_tile = kDefaultFftTile;
+  int wg_per_cu = 0;    // 0: one workgroup per work item
+};
+
+pme_status validate_system(const pme_system_desc* s);
+// Plans for up to cap_atoms atoms; touches no device memory.
+pme_status plan_system(const pme_system_desc& in, int cap_atoms, const Tuning& t, SystemPlan* p);
+
+// Device twiddle and prefac tables shared by every system with the same
+// edge (and order).  get() uploads on first use (synchronous copy).
+struct TableCache {
+  std::map<int, float2*> tw;
+  std::map<std::pair<int, int>, float*> pf;
+  size_t bytes = 0;
+  pme_status get(int order, int n, const float2** tw_out, const float** pf_out);
+  void release();
+};
+
+// Descriptor of a system whose region starts at `base`.  ev_out is left for
+// the caller.
+pme_status fill_desc(SysDesc& d, const pme_system_desc& in, const SystemPlan& p, void* base, TableCache& tables);
+// The parts that change with coordinates, box and outputs.
+void fill_dynamic(SysDesc& d, const pme_system_desc& in, const SystemPlan& p);
+
+// One launch of a stage over the items [offset, offset + count).
+struct Launch {
+  int stage;   // BatchStage
+  int order;   // spread/gather template order (0 otherwise)
+  int offset;
+  int count;
+  size_t lds;  // dynamic LDS bytes: max over the launch's systems
+};
+
+// Work lists of systems descs[0..n): items refer to the index in descs.
+// Systems are visited in the given order (callers sort by cost).
+void build_items(const SysDesc* const* descs, const SystemPlan* const* plans, const int* orders, int n,
+                 std::vector<WorkItem>& items, std::vector<Launch>& launches);
+
+// Enqueue the eight stages (bin, spread, x, y, z+solve, y inverse with the
+// energy/virial reduction, x inverse, gather) on `stream`.  With tev != NULL,
+// tev[k] is recorded before stage k and tev[8] after the last.
+pme_status enqueue_stages(const std::vector<Launch>& launches, const SysDesc* d_sys, const WorkItem* d_items,
+                          const Tuning& t, int ncu, hipStream_t stream, hipEvent_t* tev);
+
+// Cost used to order systems (largest first).
+double system_cost(const pme_system_desc& in);
+
+}  // namespace pme

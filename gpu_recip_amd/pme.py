@@ -91,6 +91,10 @@ class CudaPMERecip:
     def set_order(self, order: int) -> None:
         L.call("pme_recip_set_order", self.h, order)
 
+    def set_precision(self, precision: int) -> None:
+        """PREC_FP32, PREC_BF16X3 or PREC_FP16X3 for the y/z FFT passes."""
+        L.call("pme_recip_set_precision", self.h, int(precision))
+
     def get_info(self):
         v = [C.c_int() for _ in range(4)]
         L.call("pme_recip_get_info", self.h, *[C.byref(x) for x in v])
@@ -174,7 +178,7 @@ class PmeBatch:
 
     @staticmethod
     def desc(nfft, order, kappa, recip, xyzq, ncoord, force, force_type=L.FORCE_FLOAT, stride=None,
-             calc_energy_virial=True) -> "L.SystemDesc":
+             calc_energy_virial=True, precision=L.PREC_FP32) -> "L.SystemDesc":
         d = L.SystemDesc()
         d.nfftx, d.nffty, d.nfftz = (int(v) for v in nfft)
         d.order = int(order)
@@ -187,6 +191,7 @@ class PmeBatch:
         d.force_type = int(force_type)
         d.stride = int(ncoord if stride is None else stride)
         d.calc_energy_virial = int(bool(calc_energy_virial))
+        d.precision = int(precision)
         return d
 
     def add(self, desc, keep=()) -> int:
@@ -249,3 +254,70 @@ class PmeBatch:
         u, b = C.c_size_t(), C.c_size_t()
         L.call("pme_batch_memory", self.h, C.byref(u), C.byref(b))
         return u.value, b.value
+
+
+class PmeServer:
+    """Continuous-batching serving loop (pme_server_* in the C ABI).
+
+    submit() queues a solve of system `key` and returns a ticket at once;
+    wait(ticket) blocks for (energy, virial[9], latency_ms).  The caller keeps
+    the descriptor's device buffers alive until the request completes.
+    """
+
+    def __init__(self, hbm_budget_bytes: int, nlanes: int = 2):
+        h = C.c_void_p()
+        L.call("pme_server_create", C.c_size_t(int(hbm_budget_bytes)), int(nlanes), C.byref(h))
+        self.h = h
+        self._keep = {}  # ticket -> buffers of the request
+
+    def close(self) -> None:
+        if getattr(self, "h", None) is not None and L._lib is not None:
+            L._lib.pme_server_destroy(self.h)
+            self.h = None
+            self._keep.clear()
+
+    def __del__(self):
+        self.close()
+
+    def submit(self, key: int, desc, keep=(), ready_event=None) -> int:
+        t = C.c_ulonglong()
+        L.call("pme_server_submit", self.h, int(key), C.byref(desc), ready_event, C.byref(t))
+        if keep:
+            self._keep[t.value] = tuple(keep)
+        return t.value
+
+    def wait(self, ticket: int):
+        e, ms = C.c_double(), C.c_double()
+        v = (C.c_double * 9)()
+        try:
+            L.call("pme_server_wait", self.h, C.c_ulonglong(ticket), C.byref(e), v, C.byref(ms))
+        finally:
+            self._keep.pop(ticket, None)
+        return e.value, list(v), ms.value
+
+    def poll(self, ticket: int):
+        """None while in progress, else (energy, virial, latency_ms)."""
+        d, e, ms = C.c_int(), C.c_double(), C.c_double()
+        v = (C.c_double * 9)()
+        L.call("pme_server_poll", self.h, C.c_ulonglong(ticket), C.byref(d), C.byref(e), v, C.byref(ms))
+        if not d.value:
+            return None
+        self._keep.pop(ticket, None)
+        return e.value, list(v), ms.value
+
+    def drain(self) -> None:
+        L.call("pme_server_drain", self.h)
+
+    def release(self, key: int) -> None:
+        L.call("pme_server_release", self.h, int(key))
+
+    def stats(self) -> dict:
+        st = L.ServerStats()
+        L.call("pme_server_stats_get", self.h, C.byref(st))
+        return st.as_dict()
+
+    def reset_stats(self) -> None:
+        L.call("pme_server_stats_reset", self.h)
+
+    def set_tuning(self, brick_target: int, fft_tile: int, wg_per_cu: int) -> None:
+        L.call("pme_server_set_tuning", self.h, int(brick_target), int(fft_tile), int(wg_per_cu))

@@ -92,6 +92,16 @@ pme_status pme_recip_create(int nfftx, int nffty, int nfftz, int order, int fft_
 pme_status pme_recip_destroy(pme_recip_t h);                              /* :1645-1707 */
 pme_status pme_recip_set_stream(pme_recip_t h, pme_stream_t stream);      /* :1618-1638 */
 pme_status pme_recip_set_order(pme_recip_t h, int order);                 /* :2476-2479 */
+/* Precision of the y and z FFT passes (no counterpart in the reference,
+ * which has FP32 cuFFT only).  PME_PREC_FP32 (default): every stage FP32.
+ * PME_PREC_BF16X3 / PME_PREC_FP16X3: on an axis whose length is a multiple
+ * of 16, the radix-16 DFT stage runs on the matrix cores as a hi-lo split
+ * product (x = hi + lo in BF16 or FP16; Ah*Bh + Ah*Bl + Al*Bh accumulated in
+ * FP32; FP16 columns scaled by a power of two); the remaining stages, the x
+ * pass, the influence function and the spline work stay FP32, and bsp_mod and
+ * the energy/virial sums stay FP64.  Takes effect from the next spread. */
+enum { PME_PREC_FP32 = 0, PME_PREC_BF16X3 = 1, PME_PREC_FP16X3 = 2 };
+pme_status pme_recip_set_precision(pme_recip_t h, int precision);
 pme_status pme_recip_get_info(pme_recip_t h, int* nfftx, int* nffty, int* nfftz,
                               int* order);                                /* CudaPMERecip.h:199-202 */
 
@@ -170,6 +180,7 @@ typedef struct pme_system_desc {
   int force_type;            /* PME_FORCE_* (INT64 adds, the others store) */
   int stride;                /* SoA stride for FLOAT / INT64 (>= ncoord) */
   int calc_energy_virial;    /* energy and virial of this system */
+  int precision;             /* PME_PREC_* of the y/z FFT passes */
 } pme_system_desc;
 
 pme_status pme_batch_create(size_t hbm_budget_bytes, pme_stream_t stream, pme_batch_t* out);
@@ -207,6 +218,57 @@ pme_status pme_batch_memory(pme_batch_t b, size_t* used_bytes, size_t* budget_by
  * gather -- the order of pme_recip_get_timing. */
 pme_status pme_batch_set_timing(pme_batch_t b, int enable);
 pme_status pme_batch_get_timing(pme_batch_t b, float* ms);
+
+/* ---------------- serving loop (no counterpart in the reference) ----------------
+ * Continuous batching of PME solves that arrive one by one.  A worker thread
+ * admits submitted requests into an HBM arena of hbm_budget_bytes (one
+ * allocation at creation), keeps each system's grids, plans and binning
+ * store resident under its key for follow-up requests (idle systems are
+ * evicted least-recently-used first when space is needed), and launches
+ * every admitted request that is not already in flight as one batched
+ * solve on one of `nlanes` streams, so batches overlap on the device.  The
+ * results of a request do not depend on what it was batched with (the
+ * same kernels; integer charge accumulation).
+ * Inputs (d_xyzq) must be complete when the request is submitted, or
+ * ready_event (a hipEvent_t recorded after they were written) is given; the
+ * caller keeps d_xyzq and d_force alive until the request completes. */
+typedef struct pme_server_s* pme_server_t;
+
+typedef struct pme_server_stats {
+  unsigned long long submitted, completed, failed;
+  unsigned long long batches;      /* batched solves launched */
+  double mean_batch;               /* requests per batched solve */
+  int max_batch;
+  unsigned long long evictions;    /* resident systems evicted for space */
+  unsigned long long admitted_new; /* requests that needed a new slot */
+  int resident;                    /* systems resident now */
+  size_t arena_used, arena_budget;
+  double p50_ms, p99_ms, mean_ms, max_ms;  /* submit -> result host-visible */
+} pme_server_stats;
+
+pme_status pme_server_create(size_t hbm_budget_bytes, int nlanes, pme_server_t* out);
+/* Finishes every submitted request, then stops the worker and frees. */
+pme_status pme_server_destroy(pme_server_t s);
+/* key >= 0 names the system: a resident key reuses its state (re-planned
+ * only if the grid or order changes or ncoord exceeds what it was admitted
+ * with).  PME_ERR_BUDGET if the system could never fit the arena. */
+pme_status pme_server_submit(pme_server_t s, int key, const pme_system_desc* sys, void* ready_event,
+                             unsigned long long* ticket);
+/* Block until the request completes; returns its status.  energy/virial
+ * (9)/latency_ms may be NULL.  A ticket's result can be taken once. */
+pme_status pme_server_wait(pme_server_t s, unsigned long long ticket, double* energy, double* virial,
+                           double* latency_ms);
+/* Non-blocking: *done = 0 while in progress (result left in place). */
+pme_status pme_server_poll(pme_server_t s, unsigned long long ticket, int* done, double* energy, double* virial,
+                           double* latency_ms);
+/* Wait until every request submitted so far has completed. */
+pme_status pme_server_drain(pme_server_t s);
+/* Drop a resident system (no-op for an unknown key, or one that is in flight
+ * or queued at the time the worker handles the release). */
+pme_status pme_server_release(pme_server_t s, int key);
+pme_status pme_server_stats_get(pme_server_t s, pme_server_stats* out);
+pme_status pme_server_stats_reset(pme_server_t s);
+pme_status pme_server_set_tuning(pme_server_t s, int brick_target, int fft_tile, int wg_per_cu);
 
 /* Host FP64 bsp_mod table (calc_prefac/dftmod, CudaPMERecip.cu:2481-2579). */
 pme_status pme_bsp_mod(int order, int n, double* out);

@@ -118,3 +118,25 @@ def test_algorithmic_bytes():
     s = systems.config_production()
     assert s.natom == 96000
     assert s.algorithmic_bytes() == 44 * 96000 + 16 * 1036800 + 32 * 528000
+
+
+def test_serving_schedule():
+    from gpu_recip_amd import serving
+
+    reqs = serving.schedule(10.0)
+    assert 2700 < len(reqs) < 3300  # Poisson, 300/s
+    t = np.array([r.t for r in reqs])
+    assert np.all(np.diff(t) > 0) and t[-1] < 10.0
+    fu = [r for r in reqs if r.followup]
+    assert 0.15 < len(fu) / len(reqs) < 0.25
+    for r in fu:
+        b = reqs[r.base]
+        assert not b.followup and b.key == r.key and b.t < r.t and (b.natom, b.nfft, b.order) == (r.natom, r.nfft, r.order)
+    new = [r for r in reqs if not r.followup]
+    assert len({r.key for r in new}) == len(new)
+    n = np.array([r.natom for r in new])
+    assert n.min() >= 4998 and n.max() <= 400002 and np.all(n % 3 == 0)
+    assert 0.4 < np.mean([r.order == 4 for r in new]) < 0.6
+    for r in new[:200]:
+        assert abs(r.natom / r.box ** 3 - 0.1) < 1e-9 and r.nfft >= r.box and systems.smooth235_at_least(r.box) == r.nfft
+    assert len(serving.schedule(1e9, max_requests=7)) == 7

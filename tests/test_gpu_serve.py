@@ -1,0 +1,130 @@
+"""GPU tests of the serving loop (pme_server_*): results are the same alone
+and in the mix (bit for bit), equal the plain single-system path and agree
+with the FP64 oracle; admission under a small HBM arena evicts idle systems
+and follow-ups stay right; an open-loop run completes with latency stats."""
+import numpy as np
+import pytest
+
+import oracle as O
+from gpu_recip_amd import _lib as L
+from gpu_recip_amd import serving
+from gpu_recip_amd.pme import PmeServer
+from gpu_util import rel_err, rel_rms, run_solve, say, torch_dev
+
+pytestmark = pytest.mark.gpu
+
+
+def _forces(r):
+    fh = r.force.cpu().numpy()
+    n, st = r.natom, r.stride
+    return np.stack([fh[:n], fh[st:st + n], fh[2 * st:2 * st + n]], 1).astype(np.float64)
+
+
+def _small_schedule(n=24, nmax=60000, seed=11):
+    reqs = serving.schedule(1e9, rate=300.0, nmin=3000, nmax=nmax, seed=seed, max_requests=n)
+    return reqs
+
+
+def test_alone_vs_mix():
+    torch, dev = torch_dev()
+    reqs = _small_schedule()
+    assert any(r.followup for r in reqs) and len({r.order for r in reqs}) == 2
+    serving.materialise(reqs, dev)
+    srv = PmeServer(4 << 30, nlanes=2)
+    # the mix: everything submitted at once
+    serving.run_burst(srv, reqs)
+    mix = [(_forces(r), r.result[0], r.result[1]) for r in reqs]
+    st = srv.stats()
+    say(f"  mix: {st['batches']} batches, mean batch {st['mean_batch']:.1f}, max {st['max_batch']}")
+    assert st["completed"] == len(reqs) and st["max_batch"] > 1
+    srv.close()
+    # alone: a fresh server, one request at a time
+    srv = PmeServer(4 << 30, nlanes=2)
+    for i, r in enumerate(reqs):
+        r.force.zero_()
+        t = srv.submit(r.key, serving.desc_of(r))
+        e, v, _ = srv.wait(t)
+        f = _forces(r)
+        assert np.array_equal(f, mix[i][0]), i
+        assert e == mix[i][1] and v == mix[i][2], i
+    srv.close()
+
+
+def test_served_equals_plain_and_oracle():
+    torch, dev = torch_dev()
+    reqs = _small_schedule(n=10, nmax=30000, seed=5)
+    serving.materialise(reqs, dev)
+    srv = PmeServer(4 << 30, nlanes=3)
+    serving.run_burst(srv, reqs)
+    for r in reqs:
+        xyzq = r.xyzq.cpu().numpy()
+        box = (r.box,) * 3
+        one = run_solve((r.nfft,) * 3, r.order, r.kappa, box, xyzq)
+        f = _forces(r)
+        assert np.array_equal(f, one["force"])
+        assert rel_err(r.result[0], one["energy"]) < 1e-12
+        if r.natom <= 15000:
+            o = O.solve(O.OracleParams(r.nfft, r.nfft, r.nfft, r.order, r.kappa, box), xyzq)
+            frms = rel_rms(f, o["force"])
+            say(f"  key {r.key} n={r.natom} grid {r.nfft} p{r.order}: force rel-RMS vs FP64 {frms:.2e}")
+            assert frms < 1e-4 and rel_err(r.result[0], o["energy"]) < 1e-4
+    srv.close()
+
+
+def test_eviction_under_small_arena():
+    torch, dev = torch_dev()
+    reqs = _small_schedule(n=30, nmax=40000, seed=3)
+    serving.materialise(reqs, dev)
+    srv = PmeServer(96 << 20, nlanes=2)  # room for a few systems only
+    serving.run_burst(srv, reqs)
+    st = srv.stats()
+    say(f"  small arena: evictions {st['evictions']}, new slots {st['admitted_new']}, resident {st['resident']}")
+    assert st["completed"] == len(reqs) and st["evictions"] > 0
+    assert st["arena_used"] <= st["arena_budget"]
+    for r in reqs[-8:]:
+        one = run_solve((r.nfft,) * 3, r.order, r.kappa, (r.box,) * 3, r.xyzq.cpu().numpy())
+        assert np.array_equal(_forces(r), one["force"])
+    srv.close()
+
+
+def test_rejects_and_release():
+    torch, dev = torch_dev()
+    reqs = _small_schedule(n=2, nmax=6000, seed=9)
+    serving.materialise(reqs, dev)
+    srv = PmeServer(2 << 20, nlanes=1)
+    big = serving.ServeRequest(0.0, 99, 90000, 97.0, 100, 4, 0.34, False)
+    serving.materialise([big], dev)
+    with pytest.raises(L.PmeError) as ei:
+        srv.submit(big.key, serving.desc_of(big))
+    assert ei.value.status == 4  # PME_ERR_BUDGET: never fits the arena
+    srv.close()
+    srv = PmeServer(1 << 30, nlanes=1)
+    r = reqs[0]
+    t = srv.submit(r.key, serving.desc_of(r))
+    srv.wait(t)
+    assert srv.stats()["resident"] == 1
+    srv.release(r.key)
+    t = srv.submit(r.key + 1000, serving.desc_of(r))  # a different key: new slot
+    srv.wait(t)
+    st = srv.stats()
+    assert st["resident"] == 1 and st["admitted_new"] == 2
+    with pytest.raises(L.PmeError):
+        srv.wait(t)  # a result is taken once
+    srv.reset_stats()
+    assert srv.stats()["completed"] == 0
+    srv.close()
+
+
+def test_open_loop_short():
+    torch, dev = torch_dev()
+    reqs = serving.schedule(1.0, rate=300.0, nmin=5000, nmax=100000, seed=21)
+    serving.materialise(reqs, dev)
+    srv = PmeServer(16 << 30, nlanes=2)
+    wall, lag = serving.run_open_loop(srv, reqs)
+    st = srv.stats()
+    say(f"  open loop: {len(reqs)} requests in {wall:.2f} s, p50 {st['p50_ms']:.3f} ms, p99 {st['p99_ms']:.3f} ms, "
+        f"mean batch {st['mean_batch']:.2f}, submit lag max {lag:.2f} ms")
+    assert st["completed"] == len(reqs) and st["failed"] == 0
+    assert 0 < st["p50_ms"] <= st["p99_ms"] <= st["max_ms"]
+    assert all(r.result is not None and np.isfinite(r.result[0]) for r in reqs)
+    srv.close()
