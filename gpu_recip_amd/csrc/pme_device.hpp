@@ -85,7 +85,14 @@ __device__ __forceinline__ void bspline_theta_dtheta(float w, float* th, float* 
 // 64-bit fixed point with scale 2^40 (gpu_utils.h:9): the product by 2^40 is
 // exact in FP32, then rounded to the nearest integer (ties to even), as
 // lliroundf(FORCE_SCALE*val) in write_grid<long long> (CudaPMERecip.cu:51-58).
-__device__ __forceinline__ long long to_fixed40(float v) { return (long long)rintf(v * kForceScale); }
+// round(v * 2^40) to the nearest integer (ties to even), as 64-bit two's
+// complement: v*2^40 + 1.5*2^52 in FP64 puts the rounded integer in the low
+// mantissa bits (exact for |v| < 2^11); subtracting the constant's bit pattern
+// leaves it.  Same value as llrintf(v * 2^40), in 3 instructions.
+__device__ __forceinline__ long long to_fixed40(float v) {
+  const double t = __fma_rn((double)v, 1099511627776.0, 6755399441055744.0);
+  return (long long)((unsigned long long)__double_as_longlong(t) - 0x4338000000000000ull);
+}
 // back to FP32: ((float)val)*2^-40 as reduce_force<long long, float> (reduce.cu:66-78)
 __device__ __forceinline__ float from_fixed40(unsigned long long u) {
   return (float)((double)(long long)u * (1.0 / 1099511627776.0));

@@ -157,17 +157,23 @@ __device__ __forceinline__ void gather_brick_body(const float4* __restrict__ xyz
         if (dx >= wx) dx -= a.nx;
         if (dy >= wy) dy -= a.ny;
         if (dz >= wz) dz -= a.nz;
+        // f1 = dtx sum_z thz sum_y thy phi, f2 = tx sum_z thz sum_y dthy phi,
+        // f3 = tx sum_z dthz sum_y thy phi: two y sums per z plane
+        float g1 = 0.f, g2 = 0.f, g3 = 0.f;
         if (dx >= 0 && dy >= 0 && dz >= 0) {
           const float* col = pl + (dz * hys + dy) * hxs + dx + g;
 #pragma unroll
           for (int jz = 0; jz < P; jz++) {
+            float sy = 0.f, sdy = 0.f;
 #pragma unroll
             for (int jy = 0; jy < P; jy++) {
               const float vv = col[(jz * hys + jy) * hxs];
-              f1 += dtx * thy[jy] * thz[jz] * vv;
-              f2 += tx * dthy[jy] * thz[jz] * vv;
-              f3 += tx * thy[jy] * dthz[jz] * vv;
+              sy = fmaf(thy[jy], vv, sy);
+              sdy = fmaf(dthy[jy], vv, sdy);
             }
+            g1 = fmaf(thz[jz], sy, g1);
+            g2 = fmaf(thz[jz], sdy, g2);
+            g3 = fmaf(dthz[jz], sy, g3);
           }
         } else {
           // coordinates changed since the binning: potential from HBM
@@ -177,17 +183,23 @@ __device__ __forceinline__ void gather_brick_body(const float4* __restrict__ xyz
           for (int jz = 0; jz < P; jz++) {
             int gz = mt.z + jz;
             if (gz >= a.nz) gz -= a.nz;
+            float sy = 0.f, sdy = 0.f;
 #pragma unroll
             for (int jy = 0; jy < P; jy++) {
               int gy = mt.y + jy;
               if (gy >= a.ny) gy -= a.ny;
               const float vv = phi[(size_t)gx + (size_t)a.nx * ((size_t)gy + (size_t)a.ny * gz)];
-              f1 += dtx * thy[jy] * thz[jz] * vv;
-              f2 += tx * dthy[jy] * thz[jz] * vv;
-              f3 += tx * thy[jy] * dthz[jz] * vv;
+              sy = fmaf(thy[jy], vv, sy);
+              sdy = fmaf(dthy[jy], vv, sdy);
             }
+            g1 = fmaf(thz[jz], sy, g1);
+            g2 = fmaf(thz[jz], sdy, g2);
+            g3 = fmaf(dthz[jz], sy, g3);
           }
         }
+        f1 = dtx * g1;
+        f2 = tx * g2;
+        f3 = tx * g3;
       }
 #pragma unroll
       for (int off = 1; off < G; off <<= 1) {

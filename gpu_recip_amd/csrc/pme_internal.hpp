@@ -203,6 +203,7 @@ size_t fft_x_lds_bytes(int nf, int NR, int ntw);
 size_t fft_cols_lds_bytes(int n, int TX);
 
 constexpr int kSpreadBlock = 1024;  // threads per spread / gather workgroup (16 waves per CU)
+constexpr int kBinBlock = 1024;     // threads per binning workgroup (one atom each at the default chunk)
 constexpr int kGatherBatch = 512;   // candidate records per gather pass (LDS weight cache)
 constexpr int kMaxChunks = 1024;   // binning workgroups per solve (chunk grows with n)
 constexpr int kMinChunk = 1024;    // atoms per binning workgroup

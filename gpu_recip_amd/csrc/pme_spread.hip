@@ -56,18 +56,24 @@ __device__ __forceinline__ int next_brick(int b, int d, int nb) {
   return b >= nb ? b - nb : b;
 }
 
-// Chunk w of the counting sort (one workgroup).  hist: nbins ints of LDS.
+// Chunk w of the counting sort (one kBinBlock-thread workgroup).  hist: nbins
+// ints of LDS.  With the default chunk of kBinBlock atoms each thread bins one
+// atom and keeps it in registers between the counting and the placing pass.
 __device__ __forceinline__ void bin_chunk_body(const float4* __restrict__ xyzq, int n, const BinArgs& a, int w,
                                                int* hist) {
-  __shared__ int sh_scan[kBlock / 64];
+  __shared__ int sh_scan[kBinBlock / 64];
   const int nbins = a.nbx * a.nby * a.nbz;
   const int tid = threadIdx.x;
   const int a0 = w * a.chunk;
   const int a1 = min(a0 + a.chunk, n);
-  for (int i = tid; i < nbins; i += kBlock) hist[i] = 0;
+  for (int i = tid; i < nbins; i += kBinBlock) hist[i] = 0;
+  // first atom of this thread, loaded while the histogram is cleared
+  const int first = a0 + tid;
+  float4 p0 = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (first < a1) p0 = xyzq[first];
   __syncthreads();
-  for (int atom = a0 + tid; atom < a1; atom += kBlock) {
-    const AtomBricks k = atom_bricks(xyzq[atom], a);
+  for (int atom = first; atom < a1; atom += kBinBlock) {
+    const AtomBricks k = atom_bricks(atom == first ? p0 : xyzq[atom], a);
     for (int dz = 0; dz <= k.sz; dz++)
       for (int dy = 0; dy <= k.sy; dy++)
         for (int dx = 0; dx <= k.sx; dx++) {
@@ -76,15 +82,15 @@ __device__ __forceinline__ void bin_chunk_body(const float4* __restrict__ xyzq, 
         }
   }
   __syncthreads();
-  const int total = block_exclusive_scan(hist, nbins, sh_scan);
+  const int total = block_exclusive_scan<kBinBlock>(hist, nbins, sh_scan);
   int* seg = a.seg_off + (size_t)w * (nbins + 1);
-  for (int i = tid; i < nbins; i += kBlock) seg[i] = hist[i];
+  for (int i = tid; i < nbins; i += kBinBlock) seg[i] = hist[i];
   if (tid == 0) seg[nbins] = total;
   __syncthreads();
   float4* rp = a.rec_pos + (size_t)w * a.rec_stride;
   int* ri = a.rec_idx + (size_t)w * a.rec_stride;
-  for (int atom = a0 + tid; atom < a1; atom += kBlock) {
-    const float4 p = xyzq[atom];
+  for (int atom = first; atom < a1; atom += kBinBlock) {
+    const float4 p = atom == first ? p0 : xyzq[atom];
     const AtomBricks k = atom_bricks(p, a);
     for (int dz = 0; dz <= k.sz; dz++)
       for (int dy = 0; dy <= k.sy; dy++)
@@ -97,7 +103,7 @@ __device__ __forceinline__ void bin_chunk_body(const float4* __restrict__ xyzq, 
   }
 }
 
-__global__ __launch_bounds__(kBlock) void k_bin_chunks(const float4* __restrict__ xyzq, int n, BinArgs a) {
+__global__ __launch_bounds__(kBinBlock) void k_bin_chunks(const float4* __restrict__ xyzq, int n, BinArgs a) {
   extern __shared__ int hist[];  // [nbins]: counts, then cursors
   bin_chunk_body(xyzq, n, a, blockIdx.x, hist);
 }
@@ -106,7 +112,7 @@ size_t bin_lds_bytes(int nbins) { return sizeof(int) * (size_t)nbins; }
 
 hipError_t launch_bin_atoms(const float4* xyzq, int n, const BinArgs& a, hipStream_t s) {
   const int nbins = a.nbx * a.nby * a.nbz;
-  hipLaunchKernelGGL(k_bin_chunks, dim3(a.nchunk), dim3(kBlock), bin_lds_bytes(nbins), s, xyzq, n, a);
+  hipLaunchKernelGGL(k_bin_chunks, dim3(a.nchunk), dim3(kBinBlock), bin_lds_bytes(nbins), s, xyzq, n, a);
   return hipGetLastError();
 }
 
@@ -118,6 +124,12 @@ __device__ __forceinline__ int local_offset(int i, int x0, int w, int n) {
   return d;
 }
 
+// x stride of the LDS brick: P-1 margin points on either side, so a stencil
+// row never needs a bounds test along x (margin sums are dropped, or folded
+// back when one brick spans the whole periodic axis); odd in 64-bit words so
+// rows of consecutive lanes fall in different banks.
+__host__ __device__ constexpr int spread_xstride(int wxm, int P) { return (wxm + 2 * (P - 1)) | 1; }
+
 // Brick b of one grid (one kSpreadBlock-thread workgroup); smem: the
 // workgroup's dynamic LDS (spread_lds_bytes).
 template <int P>
@@ -125,7 +137,7 @@ __device__ __forceinline__ void spread_brick_body(const BinArgs& a, int b, int w
                                                   float* __restrict__ grid, unsigned char* smem) {
   __shared__ int sh_scan[kSpreadBlock / 64];
 
-  const int bxs = wxm + 1;  // padded x stride of the LDS brick
+  const int bxs = spread_xstride(wxm, P);
   unsigned long long* acc = reinterpret_cast<unsigned long long*>(smem);
   const int acc_n = wzm * wym * bxs;
   int* s_d = reinterpret_cast<int*>(acc + acc_n);            // [3][kSpreadBlock] local offsets
@@ -152,7 +164,8 @@ __device__ __forceinline__ void spread_brick_body(const BinArgs& a, int b, int w
       grid_index(p.x, a.r1, a.nx, ix, fx);
       grid_index(p.y, a.r2, a.ny, iy, fy);
       grid_index(p.z, a.r3, a.nz, iz, fz);
-      s_d[tid] = local_offset(safe_index(ix, a.nx), x0, wx, a.nx);
+      // x offset into the margin-padded row: in [0, wx + 2(P-1)) always
+      s_d[tid] = local_offset(safe_index(ix, a.nx), x0, wx, a.nx) + (P - 1);
       s_d[kSpreadBlock + tid] = local_offset(safe_index(iy, a.ny), y0, wy, a.ny);
       s_d[2 * kSpreadBlock + tid] = local_offset(safe_index(iz, a.nz), z0, wz, a.nz);
       float th[P];
@@ -175,39 +188,36 @@ __device__ __forceinline__ void spread_brick_body(const BinArgs& a, int b, int w
       const int yz = r - ai * (P * P);
       const int jz = yz / P;
       const int jy = yz - jz * P;
-      // every LDS read of the row up front (one wait), then up to P LDS
-      // atomics without waits (exec-masked outside the brick)
       const float* th = s_th + ai * 3 * P;
-      const int dxa = s_d[ai];
       int ly = s_d[kSpreadBlock + ai] + jy;
       int lz = s_d[2 * kSpreadBlock + ai] + jz;
-      float tx[P];
-#pragma unroll
-      for (int jx = 0; jx < P; jx++) tx[jx] = th[jx];
-      const float c = th[P + jy] * th[2 * P + jz];
       ly -= (wrapy && ly >= wy) ? a.ny : 0;
       lz -= (wrapz && lz >= wz) ? a.nz : 0;
       if (ly < 0 || ly >= wy || lz < 0 || lz >= wz) continue;
-      unsigned long long* row = acc + (lz * wym + ly) * bxs;
+      const float c = th[P + jy] * th[2 * P + jz];
+      unsigned long long* row = acc + (lz * wym + ly) * bxs + s_d[ai];
+      float tx[P];
 #pragma unroll
-      for (int jx = 0; jx < P; jx++) {
-        int lx = dxa + jx;
-        lx -= (wrapx && lx >= wx) ? a.nx : 0;
-        if (lx >= 0 && lx < wx) atomicAdd(&row[lx], (unsigned long long)to_fixed40(tx[jx] * c));
-      }
+      for (int jx = 0; jx < P; jx++) tx[jx] = th[jx];
+#pragma unroll
+      for (int jx = 0; jx < P; jx++) atomicAdd(&row[jx], (unsigned long long)to_fixed40(tx[jx] * c));
     }
     __syncthreads();
   }
 
-  // the brick is complete: one coalesced store per grid point, no memset needed
+  // the brick is complete: one coalesced store per grid point, no memset
+  // needed.  With one brick along x the high margin is the periodic image of
+  // the row start.
   const int nb = wx * wy * wz;
   for (int i = tid; i < nb; i += kSpreadBlock) {
     const int lx = i % wx;
     const int t = i / wx;
     const int ly = t % wy;
     const int lz = t / wy;
-    grid[(size_t)(x0 + lx) + (size_t)a.nx * ((size_t)(y0 + ly) + (size_t)a.ny * (z0 + lz))] =
-        from_fixed40(acc[(lz * wym + ly) * bxs + lx]);
+    const unsigned long long* row = acc + (lz * wym + ly) * bxs + (P - 1);
+    unsigned long long v = row[lx];
+    if (wrapx && lx < P - 1) v += row[lx + a.nx];
+    grid[(size_t)(x0 + lx) + (size_t)a.nx * ((size_t)(y0 + ly) + (size_t)a.ny * (z0 + lz))] = from_fixed40(v);
   }
 }
 
@@ -219,7 +229,8 @@ __global__ __launch_bounds__(kSpreadBlock) void k_spread_brick(BinArgs a, int wx
 }
 
 size_t spread_lds_bytes(int order, const int wmax[3], int nchunk) {
-  return sizeof(unsigned long long) * (size_t)wmax[2] * wmax[1] * (wmax[0] + 1) + sizeof(int) * 3 * kSpreadBlock +
+  return sizeof(unsigned long long) * (size_t)wmax[2] * wmax[1] * spread_xstride(wmax[0], order) +
+         sizeof(int) * 3 * kSpreadBlock +
          sizeof(float) * (size_t)kSpreadBlock * 3 * order + sizeof(int) * (2 * (size_t)nchunk + 1);
 }
 
@@ -244,7 +255,7 @@ hipError_t launch_spread(int order, const BinArgs& a, const int wmax[3], float* 
 
 // ---- batched: persistent workgroups over the work items of many systems ----
 
-__global__ __launch_bounds__(kBlock) void k_batch_bin(const SysDesc* __restrict__ sys,
+__global__ __launch_bounds__(kBinBlock) void k_batch_bin(const SysDesc* __restrict__ sys,
                                                       const WorkItem* __restrict__ items, int nitems) {
   extern __shared__ int hist[];
   for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
@@ -270,7 +281,7 @@ __global__ __launch_bounds__(kSpreadBlock) void k_batch_spread(const SysDesc* __
 hipError_t launch_batch_bin(const SysDesc* sys, const WorkItem* items, int nitems, int grid, size_t lds,
                             hipStream_t s) {
   if (nitems <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_batch_bin, dim3(grid), dim3(kBlock), lds, s, sys, items, nitems);
+  hipLaunchKernelGGL(k_batch_bin, dim3(grid), dim3(kBinBlock), lds, s, sys, items, nitems);
   return hipGetLastError();
 }
 

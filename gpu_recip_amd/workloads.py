@@ -1,0 +1,75 @@
+"""Run one benchmark workload for a fixed number of solves (for profilers).
+
+    python -m gpu_recip_amd.workloads production [--solves 20] [--precision fp32|bf16x3|fp16x3]
+    python -m gpu_recip_amd.workloads split      [--solves 20] [--precision ...]
+    python -m gpu_recip_amd.workloads batched    [--solves 20]
+    python -m gpu_recip_amd.workloads small      [--solves 20]
+
+production: 96,000 atoms, 108x100x96, order 6 (plain five-call path)
+split:      250,002 atoms, 144^3, order 4 (plain path, chosen FFT precision)
+batched:    the 64-system set, one batched solve per step
+small:      8,001 atoms, 48^3, order 4 (plain path)
+Inputs are uploaded once; every solve ends with the energy/virial readback.
+"""
+from __future__ import annotations
+
+import argparse
+import time
+
+PRECS = {"fp32": 0, "bf16x3": 1, "fp16x3": 2}
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("workload", choices=["production", "split", "batched", "small"])
+    ap.add_argument("--solves", type=int, default=20)
+    ap.add_argument("--precision", choices=sorted(PRECS), default="fp32")
+    args = ap.parse_args(argv)
+
+    import torch
+
+    from . import _lib as L
+    from . import systems
+    from .pme import CudaEnergyVirial, CudaPMERecip, PmeBatch, force_stride
+
+    dev = torch.device("cuda:0")
+    t0 = time.perf_counter()
+    if args.workload == "batched":
+        b = PmeBatch(32 << 30)
+        ids = []
+        for s in systems.config_batched():
+            x = torch.from_numpy(s.xyzq).to(dev)
+            st = force_stride(s.natom)
+            f = torch.zeros(3 * st, dtype=torch.float32, device=dev)
+            ids.append(b.add(PmeBatch.desc(s.nfft, s.order, s.kappa, s.recip, x, s.natom, f, L.FORCE_FLOAT, st,
+                                           precision=PRECS[args.precision]), keep=(x, f)))
+        t0 = time.perf_counter()
+        for _ in range(args.solves):
+            b.solve()
+            b.results(ids)
+        print(f"batched: {args.solves} x 64 solves in {time.perf_counter() - t0:.3f} s", flush=True)
+        return
+    s = {"production": systems.config_production, "split": systems.config_split_precision,
+         "small": systems.config_plain_small}[args.workload]()
+    ev = CudaEnergyVirial()
+    pme = CudaPMERecip(*s.nfft, s.order, ev)
+    pme.set_precision(PRECS[args.precision])
+    x = torch.from_numpy(s.xyzq).to(dev)
+    st = force_stride(s.natom)
+    f = torch.zeros(3 * st, dtype=torch.float32, device=dev)
+    t0 = time.perf_counter()
+    for _ in range(args.solves):
+        ev.clear()
+        pme.spread_charge(x, s.natom, s.recip)
+        pme.r2c_fft()
+        pme.scalar_sum(s.recip, s.kappa, True, True)
+        pme.c2r_fft()
+        pme.gather_force(x, s.natom, s.recip, st, f, L.FORCE_FLOAT)
+        ev.copyToHost()
+        torch.cuda.synchronize()
+    print(f"{s.name} ({args.precision}): {args.solves} solves in {time.perf_counter() - t0:.3f} s, "
+          f"energy {ev.getEnergy('recip'):.6f}", flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -75,7 +75,7 @@ def test_eviction_under_small_arena():
     torch, dev = torch_dev()
     reqs = _small_schedule(n=30, nmax=40000, seed=3)
     serving.materialise(reqs, dev)
-    srv = PmeServer(96 << 20, nlanes=2)  # room for a few systems only
+    srv = PmeServer(20 << 20, nlanes=2)  # room for a few systems only (the largest needs ~9 MB)
     serving.run_burst(srv, reqs)
     st = srv.stats()
     say(f"  small arena: evictions {st['evictions']}, new slots {st['admitted_new']}, resident {st['resident']}")
