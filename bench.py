@@ -30,26 +30,41 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
-STAGES = ["bin", "spread", "fft_x_r2c", "fft_y_fwd", "fft_z_solve", "fft_y_inv", "fft_x_c2r", "gather"]
+# timing slots of pme_recip_get_timing / pme_batch_get_timing; when a grid's
+# xy plane fits one workgroup the x and y passes run as one plane kernel
+# (slots 2 and 5), and slots 3 and 6 are empty
+STAGES = ["bin", "spread", "fft_fwd_x_or_xy", "fft_fwd_y", "fft_z_solve", "fft_inv_y_or_xy", "fft_inv_x", "gather"]
 METRIC = "PME recip solves/s + atom-steps/s, batched/served vs plain; p99 latency @1e-4 err"
+PLANE_LDS_BUDGET = 150 * 1024
+
+
+def plane_mode(nfft):
+    """Whether the library runs the x and y passes of this grid as one xy
+    plane kernel: the same LDS test as fft_plane_lds_bytes (pme_fft.hip)."""
+    nx, ny = nfft[0], nfft[1]
+    nf = nx // 2 if nx % 2 == 0 else nx
+    odd = lambda n: (n + 1) | 1  # noqa: E731
+    buf = max(nf * odd(ny), ny * odd(nx // 2 + 1))
+    return 8 * (nx + ny + 2 * buf) <= PLANE_LDS_BUDGET
 
 
 def stage_bytes(s):
-    """Algorithmic HBM bytes per launch of each stage kernel for one system
+    """Algorithmic HBM bytes per launch of each timing slot for one system
     (FP32 data): every kernel reads its inputs and writes its outputs once."""
     N = s.natom
     nx, ny, nz = s.nfft
     G = nx * ny * nz
     Gc = (nx // 2 + 1) * ny * nz
+    plane = plane_mode(s.nfft)
     return {
-        "bin": 16 * N + 4 * N,              # xyzq in, atom index into its bin out
-        "spread": 16 * N + 4 * N + 4 * G,    # xyzq + bin index in, charge grid out
-        "fft_x_r2c": 4 * G + 8 * Gc,         # real rows in, half-spectrum rows out
-        "fft_y_fwd": 16 * Gc,                # spectrum in and out
-        "fft_z_solve": 16 * Gc,              # spectrum in, solved + z-inverted spectrum out
-        "fft_y_inv": 16 * Gc,
-        "fft_x_c2r": 8 * Gc + 4 * G,
-        "gather": 16 * N + 4 * N + 4 * G + 12 * N,  # xyzq + bin index + potential in, forces out
+        "bin": 16 * N + 4 * N,                           # xyzq in, atom index into its bin out
+        "spread": 16 * N + 4 * N + 4 * G,                 # xyzq + bin index in, charge grid out
+        "fft_fwd_x_or_xy": 4 * G + 8 * Gc,                # real grid in, half spectrum out (x rows or xy planes)
+        "fft_fwd_y": 0 if plane else 16 * Gc,             # spectrum in and out
+        "fft_z_solve": 16 * Gc,                           # spectrum in, solved + z-inverted spectrum out
+        "fft_inv_y_or_xy": (8 * Gc + 4 * G) if plane else 16 * Gc,
+        "fft_inv_x": 0 if plane else 8 * Gc + 4 * G,
+        "gather": 16 * N + 4 * N + 4 * G + 12 * N,        # xyzq + bin index + potential in, forces out
     }
 
 

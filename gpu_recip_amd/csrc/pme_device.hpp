@@ -199,9 +199,12 @@ template <int S> struct Dft<8, S> {
 struct RadixPlan {
   int n;         // transform length
   int nrad;      // number of stages
-  int rad[12];   // radix of each stage (2, 3, 4, 5, 8)
+  int rad[12];   // radix of each stage (2, 3, 4, 5, 8; 16 for a matrix-core first stage)
   int tstride;   // twiddle table stride: table length / n
+  unsigned long long rcode;  // rad[] packed 5 bits per stage (read by the kernels: no indexed arrays)
 };
+
+__host__ __device__ inline int plan_radix(const RadixPlan& rp, int s) { return (int)((rp.rcode >> (5 * s)) & 31); }
 
 // One Stockham stage of radix R over nlines interleaved lines.
 // Element j of line l lives at buf[j*ls + l].  Stage with p = product of the
@@ -268,6 +271,15 @@ __device__ __forceinline__ void split16(float x, short& hi, short& lo) {
   }
 }
 
+// cos(2 pi e / 16) for 0 <= e < 16, from three constants by symmetry (selects,
+// no table in memory)
+__device__ __forceinline__ float cos16(int e) {
+  const float c1 = 0.92387953251128674f, c2 = 0.70710678118654752f, c3 = 0.38268343236508977f;
+  const int q = e & 7;
+  const float v = q == 0 ? 1.0f : q == 1 ? c1 : q == 2 ? c2 : q == 3 ? c3 : q == 4 ? 0.0f : q == 5 ? -c3 : q == 6 ? -c2 : -c1;
+  return (e & 8) ? -v : v;
+}
+
 template <int PREC>
 __device__ __forceinline__ v16f32 mfma32x32x8(v4s16 a, v4s16 b, v16f32 c) {
   if (PREC == PREC_BF16X3) return __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, c, 0, 0, 0);
@@ -290,14 +302,13 @@ __device__ __forceinline__ void mfma_dft16_stage(const float2* __restrict__ src,
   const int wave = tid >> 6;
   const int nwave = nthr >> 6;
   const int h = lane >> 5;
-  const int j = lane & 31;
+  int j = lane & 31;
+  // opaque to the optimiser: keeps the operand set-up below inside this
+  // stage instead of hoisted (and held in registers) across callers' loops
+  asm volatile("" : "+v"(j));
   const int T = n >> 4;
   const int ncols = T * nlines;
   // A operand (the DFT matrix in real form) for the four K steps
-  const float c16[16] = {1.0f, 0.92387953251128674f, 0.70710678118654752f, 0.38268343236508977f, 0.0f,
-                         -0.38268343236508977f, -0.70710678118654752f, -0.92387953251128674f, -1.0f,
-                         -0.92387953251128674f, -0.70710678118654752f, -0.38268343236508977f, 0.0f,
-                         0.38268343236508977f, 0.70710678118654752f, 0.92387953251128674f};
   v4s16 ah[4], al[4];
 #pragma unroll
   for (int q = 0; q < 4; q++) {
@@ -306,8 +317,8 @@ __device__ __forceinline__ void mfma_dft16_stage(const float2* __restrict__ src,
       const int k = 8 * q + 4 * h + t;
       const int r = j & 15, m = k & 15;
       const int e = (r * m) & 15;
-      const float cr = c16[e];
-      const float si = (float)S * c16[(e + 12) & 15];  // S * sin(2 pi e / 16)
+      const float cr = cos16(e);
+      const float si = (float)S * cos16((e + 12) & 15);  // S * sin(2 pi e / 16)
       float a;
       if (j < 16) a = (k < 16) ? cr : -si;
       else a = (k < 16) ? si : cr;
@@ -380,13 +391,13 @@ __device__ __forceinline__ void mfma_dft16_stage(const float2* __restrict__ src,
 // prec != PREC_FP32 and a plan whose first radix is 16, that stage runs on
 // the matrix cores (mfma_dft16_stage).
 template <int S>
-__device__ float2* fft_lines(float2* buf0, float2* buf1, const float2* tw, const RadixPlan& rp,
+__device__ __forceinline__ float2* fft_lines(float2* buf0, float2* buf1, const float2* tw, const RadixPlan& rp,
                              int nlines, int ls, int tid, int nthr, int prec = PREC_FP32) {
   float2* src = buf0;
   float2* dst = buf1;
   int p = 1;
   int s0 = 0;
-  if (prec != PREC_FP32 && rp.nrad > 0 && rp.rad[0] == 16) {
+  if (prec != PREC_FP32 && rp.nrad > 0 && plan_radix(rp, 0) == 16) {
     if (prec == PREC_BF16X3)
       mfma_dft16_stage<S, PREC_BF16X3>(src, dst, rp.n, nlines, ls, tid, nthr);
     else
@@ -399,7 +410,7 @@ __device__ float2* fft_lines(float2* buf0, float2* buf1, const float2* tw, const
     s0 = 1;
   }
   for (int s = s0; s < rp.nrad; s++) {
-    const int R = rp.rad[s];
+    const int R = plan_radix(rp, s);
     switch (R) {
       case 2: stockham_stage<2, S>(src, dst, tw, rp.n, p, rp.tstride, nlines, ls, tid, nthr); break;
       case 3: stockham_stage<3, S>(src, dst, tw, rp.n, p, rp.tstride, nlines, ls, tid, nthr); break;

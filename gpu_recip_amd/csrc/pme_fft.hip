@@ -182,6 +182,217 @@ __global__ __launch_bounds__(kBlock) void k_fft_x_c2r(const float2* __restrict__
   fft_x_c2r_body(cg, grid, nx, nrows, NR, ax, blockIdx.x, sm);
 }
 
+// ------------------------------------------------------------- xy planes --
+// One workgroup transforms one z plane through x and y in LDS, so the plane is
+// read once and written once with contiguous accesses:
+//   forward: real rows --x R2C--> half-spectrum rows, transposed in LDS to
+//            columns --y FFT--> the plane of the half spectrum;
+//   inverse: the reverse (y inverse, C2R pre-processing, x inverse).
+// LDS: twiddles of x and y, two buffers of plane_buf() points.  x lines
+// (rows) have point stride LSX, y lines (columns kx) point stride LSY, both
+// odd so that 8-byte accesses of consecutive lines hit distinct bank pairs.
+
+__host__ __device__ constexpr int odd_pad(int n) { return (n + 1) | 1; }
+
+__host__ __device__ constexpr int plane_buf(int nx, int ny, int nf) {
+  return (nf * odd_pad(ny) > ny * odd_pad(nx / 2 + 1)) ? nf * odd_pad(ny) : ny * odd_pad(nx / 2 + 1);
+}
+
+size_t fft_plane_lds_bytes(int nx, int ny, int nf_x) {
+  const size_t b = sizeof(float2) * ((size_t)nx + ny + 2 * (size_t)plane_buf(nx, ny, nf_x));
+  return b <= kPlaneLdsBudget ? b : 0;
+}
+
+__device__ __forceinline__ void fft_plane_fwd_body(const float* __restrict__ grid, float2* __restrict__ cg, int nx,
+                                                   int ny, const AxisArgs& axx, const AxisArgs& axy, int z,
+                                                   float2* sm) {
+  const int tid = threadIdx.x;
+  const int nf = axx.rp.n;  // x line length: nx/2 (even nx) or nx
+  const int hx = nx / 2 + 1;
+  const int LSX = odd_pad(ny), LSY = odd_pad(hx);
+  const int bsz = plane_buf(nx, ny, nf);
+  float2* twx = sm;
+  float2* twy = twx + nx;
+  float2* b0 = twy + ny;
+  float2* b1 = b0 + bsz;
+  for (int i = tid; i < nx; i += kPlaneBlock) twx[i] = axx.tw[i];
+  for (int i = tid; i < ny; i += kPlaneBlock) twy[i] = axy.tw[i];
+  const bool even = (nx & 1) == 0;
+  const size_t plane = (size_t)z * nx * ny;
+  if (even) {
+    const float2* src = reinterpret_cast<const float2*>(grid + plane);
+    const int ne = nf * ny;
+    for (int e0 = tid; e0 < ne; e0 += kStageU * kPlaneBlock) {
+      float2 v[kStageU];
+#pragma unroll
+      for (int u = 0; u < kStageU; u++) {
+        const int e = e0 + u * kPlaneBlock;
+        v[u] = e < ne ? src[e] : make_float2(0.f, 0.f);
+      }
+#pragma unroll
+      for (int u = 0; u < kStageU; u++) {
+        const int e = e0 + u * kPlaneBlock;
+        if (e < ne) {
+          const int y = e / nf;
+          b0[(e - y * nf) * LSX + y] = v[u];
+        }
+      }
+    }
+  } else {
+    const float* src = grid + plane;
+    const int ne = nx * ny;
+    for (int e0 = tid; e0 < ne; e0 += kStageU * kPlaneBlock) {
+      float v[kStageU];
+#pragma unroll
+      for (int u = 0; u < kStageU; u++) {
+        const int e = e0 + u * kPlaneBlock;
+        v[u] = e < ne ? src[e] : 0.f;
+      }
+#pragma unroll
+      for (int u = 0; u < kStageU; u++) {
+        const int e = e0 + u * kPlaneBlock;
+        if (e < ne) {
+          const int y = e / nx;
+          b0[(e - y * nx) * LSX + y] = make_float2(v[u], 0.f);
+        }
+      }
+    }
+  }
+  __syncthreads();
+  float2* res = fft_lines<-1>(b0, b1, twx, axx.rp, ny, LSX, tid, kPlaneBlock);
+  float2* q = res == b0 ? b1 : b0;
+  // R2C post-processing (even nx) and transpose to columns:
+  // X[k] = E[k] + w^k O[k], E = (Z[k] + Z*[h-k])/2, O = (Z[k] - Z*[h-k])/(2i)
+  for (int e = tid; e < ny * hx; e += kPlaneBlock) {
+    const int y = e / hx;
+    const int k = e - y * hx;
+    float2 X;
+    if (even) {
+      const int h = nf;
+      const float2 zk = res[(k == h ? 0 : k) * LSX + y];
+      const float2 zc = cconj(res[(k == 0 ? 0 : h - k) * LSX + y]);
+      const float2 E = cscale(cadd(zk, zc), 0.5f);
+      const float2 D = csub(zk, zc);
+      const float2 O = make_float2(0.5f * D.y, -0.5f * D.x);
+      X = cadd(E, cmul(twx[k], O));
+    } else {
+      X = res[k * LSX + y];
+    }
+    q[y * LSY + k] = X;
+  }
+  __syncthreads();
+  float2* r2 = fft_lines<-1>(q, res, twy, axy.rp, hx, LSY, tid, kPlaneBlock, axy.prec);
+  float2* dst = cg + (size_t)z * ny * hx;
+  for (int e = tid; e < ny * hx; e += kPlaneBlock) {
+    const int y = e / hx;
+    dst[e] = r2[y * LSY + (e - y * hx)];
+  }
+}
+
+__device__ __forceinline__ void fft_plane_inv_body(const float2* __restrict__ cg, float* __restrict__ grid, int nx,
+                                                   int ny, const AxisArgs& axx, const AxisArgs& axy, int z,
+                                                   float2* sm) {
+  const int tid = threadIdx.x;
+  const int nf = axx.rp.n;
+  const int hx = nx / 2 + 1;
+  const int LSX = odd_pad(ny), LSY = odd_pad(hx);
+  const int bsz = plane_buf(nx, ny, nf);
+  float2* twx = sm;
+  float2* twy = twx + nx;
+  float2* b0 = twy + ny;
+  float2* b1 = b0 + bsz;
+  for (int i = tid; i < nx; i += kPlaneBlock) twx[i] = axx.tw[i];
+  for (int i = tid; i < ny; i += kPlaneBlock) twy[i] = axy.tw[i];
+  const float2* src = cg + (size_t)z * ny * hx;
+  const int ne = ny * hx;
+  for (int e0 = tid; e0 < ne; e0 += kStageU * kPlaneBlock) {
+    float2 v[kStageU];
+#pragma unroll
+    for (int u = 0; u < kStageU; u++) {
+      const int e = e0 + u * kPlaneBlock;
+      v[u] = e < ne ? src[e] : make_float2(0.f, 0.f);
+    }
+#pragma unroll
+    for (int u = 0; u < kStageU; u++) {
+      const int e = e0 + u * kPlaneBlock;
+      if (e < ne) {
+        const int y = e / hx;
+        b0[y * LSY + (e - y * hx)] = v[u];
+      }
+    }
+  }
+  __syncthreads();
+  float2* r = fft_lines<+1>(b0, b1, twy, axy.rp, hx, LSY, tid, kPlaneBlock, axy.prec);
+  float2* p = r == b0 ? b1 : b0;
+  const bool even = (nx & 1) == 0;
+  if (even) {
+    // Z[k] = (X[k] + X*[h-k]) + i w^{-k} (X[k] - X*[h-k]),  k < h
+    const int h = nf;
+    for (int e = tid; e < ny * h; e += kPlaneBlock) {
+      const int y = e / h;
+      const int k = e - y * h;
+      const float2 xk = r[y * LSY + k];
+      const float2 xc = cconj(r[y * LSY + (h - k)]);
+      const float2 A = cadd(xk, xc);
+      const float2 B = cmul(cconj(twx[k]), csub(xk, xc));
+      p[k * LSX + y] = make_float2(A.x - B.y, A.y + B.x);
+    }
+  } else {
+    // the full Hermitian spectrum of each row
+    for (int e = tid; e < ny * nx; e += kPlaneBlock) {
+      const int y = e / nx;
+      const int k = e - y * nx;
+      p[k * LSX + y] = k < hx ? r[y * LSY + k] : cconj(r[y * LSY + (nx - k)]);
+    }
+  }
+  __syncthreads();
+  float2* s = fft_lines<+1>(p, r, twx, axx.rp, ny, LSX, tid, kPlaneBlock);
+  const size_t plane = (size_t)z * nx * ny;
+  if (even) {
+    float2* dst = reinterpret_cast<float2*>(grid + plane);
+    for (int e = tid; e < ny * nf; e += kPlaneBlock) {
+      const int y = e / nf;
+      dst[e] = s[(e - y * nf) * LSX + y];
+    }
+  } else {
+    float* dst = grid + plane;
+    for (int e = tid; e < ny * nx; e += kPlaneBlock) {
+      const int y = e / nx;
+      dst[e] = s[(e - y * nx) * LSX + y].x;
+    }
+  }
+}
+
+__global__ __launch_bounds__(kPlaneBlock) void k_fft_plane_fwd(const float* __restrict__ grid, float2* __restrict__ cg,
+                                                              int nx, int ny, AxisArgs axx, AxisArgs axy) {
+  extern __shared__ __align__(16) float2 sm[];
+  fft_plane_fwd_body(grid, cg, nx, ny, axx, axy, blockIdx.x, sm);
+}
+
+__global__ __launch_bounds__(kPlaneBlock) void k_fft_plane_inv(const float2* __restrict__ cg, float* __restrict__ grid,
+                                                              int nx, int ny, AxisArgs axx, AxisArgs axy,
+                                                              EvReduce red) {
+  extern __shared__ __align__(16) float2 sm[];
+  if ((red.nsolve > 0 || red.nself > 0) && blockIdx.x == 0) ev_reduce_block<kPlaneBlock>(red);
+  fft_plane_inv_body(cg, grid, nx, ny, axx, axy, blockIdx.x, sm);
+}
+
+hipError_t launch_fft_plane_fwd(const float* grid, float2* cg, int nx, int ny, int nz, const AxisArgs& ax,
+                                const AxisArgs& ay, hipStream_t s) {
+  const size_t lds = fft_plane_lds_bytes(nx, ny, ax.rp.n);
+  if (!lds) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_fft_plane_fwd, dim3(nz), dim3(kPlaneBlock), lds, s, grid, cg, nx, ny, ax, ay);
+  return hipGetLastError();
+}
+
+hipError_t launch_fft_plane_inv(const float2* cg, float* grid, int nx, int ny, int nz, const AxisArgs& ax,
+                                const AxisArgs& ay, const EvReduce& red, hipStream_t s) {
+  const size_t lds = fft_plane_lds_bytes(nx, ny, ax.rp.n);
+  if (!lds) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_fft_plane_inv, dim3(nz), dim3(kPlaneBlock), lds, s, cg, grid, nx, ny, ax, ay, red);
+  return hipGetLastError();
+}
+
 // ------------------------------------------------------- influence function --
 
 // eterm at (k1, k2, k3) and the energy/virial terms of that point,
@@ -309,7 +520,7 @@ __global__ __launch_bounds__(kBlock) void k_fft_cols(float2* __restrict__ cg, in
                                                      long long pt_stride, long long outer_stride, AxisArgs ax,
                                                      SolveArgs sa, EvReduce red) {
   extern __shared__ __align__(16) float2 sm[];
-  if ((red.nsolve > 0 || red.nself > 0) && blockIdx.x == 0 && blockIdx.y == 0) ev_reduce_block(red);
+  if ((red.nsolve > 0 || red.nself > 0) && blockIdx.x == 0 && blockIdx.y == 0) ev_reduce_block<kBlock>(red);
   fft_cols_body<MODE>(cg, n, hx, TX, pt_stride, outer_stride, ax, sa, blockIdx.x, blockIdx.y,
                       blockIdx.x + gridDim.x * blockIdx.y, sm);
 }
@@ -390,7 +601,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_fft_x(const SysDesc* __restric
                                                         const WorkItem* __restrict__ items, int nitems) {
   extern __shared__ __align__(16) float2 sm[];
   for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
-    const WorkItem w = items[it];
+    const WorkItem w = uniform_item(items[it]);
     const SysDesc& d = sys[w.sys];
     const int nrows = d.bin.ny * d.bin.nz;
     if (INV)
@@ -403,13 +614,14 @@ __global__ __launch_bounds__(kBlock) void k_batch_fft_x(const SysDesc* __restric
 
 // energy/virial of one system from its z-pass partials (fixed order, one
 // workgroup), assigned to ev_out
+template <int NT = kBlock>
 __device__ __forceinline__ void batch_ev_reduce(const SysDesc& d) {
-  __shared__ double sh[7 * (kBlock / 64)];
+  __shared__ double sh[7 * (NT / 64)];
   double v[7] = {0, 0, 0, 0, 0, 0, 0};
-  for (int i = threadIdx.x; i < d.nzblk; i += kBlock)
+  for (int i = threadIdx.x; i < d.nzblk; i += NT)
 #pragma unroll
     for (int c = 0; c < 7; c++) v[c] += d.sa.partial[(size_t)i * 7 + c];
-  block_sum<7>(v, sh);
+  block_sum<7, NT>(v, sh);
   if (threadIdx.x == 0) {
     const double hc = 0.5 * kCcelec;
     double* o = d.ev_out;
@@ -427,9 +639,9 @@ __global__ __launch_bounds__(kBlock) void k_batch_cols(const SysDesc* __restrict
                                                        const WorkItem* __restrict__ items, int nitems, int reduce) {
   extern __shared__ __align__(16) float2 sm[];
   for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
-    const WorkItem w = items[it];
+    const WorkItem w = uniform_item(items[it]);
     const SysDesc& d = sys[w.sys];
-    if (reduce && w.idx == 0 && d.sa.calc_ev) batch_ev_reduce(d);
+    if (reduce && w.idx == 0 && d.sa.calc_ev) batch_ev_reduce<kBlock>(d);
     const int hx = d.bin.nx / 2 + 1;
     const int n = AXIS == 1 ? d.bin.ny : d.bin.nz;
     const int TX = AXIS == 1 ? d.TXy : d.TXz;
@@ -439,6 +651,34 @@ __global__ __launch_bounds__(kBlock) void k_batch_cols(const SysDesc* __restrict
     fft_cols_body<MODE>(d.cgrid, n, hx, TX, pt, outer, d.ax[AXIS], d.sa, w.idx % ntx, w.idx / ntx, w.idx, sm);
     __syncthreads();
   }
+}
+
+template <int INV>
+__global__ __launch_bounds__(kPlaneBlock) void k_batch_plane(const SysDesc* __restrict__ sys,
+                                                             const WorkItem* __restrict__ items, int nitems,
+                                                             int reduce) {
+  extern __shared__ __align__(16) float2 sm[];
+  for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+    const WorkItem w = uniform_item(items[it]);
+    const SysDesc& d = sys[w.sys];
+    if (INV) {
+      if (reduce && w.idx == 0 && d.sa.calc_ev) batch_ev_reduce<kPlaneBlock>(d);
+      fft_plane_inv_body(d.cgrid, d.grid, d.bin.nx, d.bin.ny, d.ax[0], d.ax[1], w.idx, sm);
+    } else {
+      fft_plane_fwd_body(d.grid, d.cgrid, d.bin.nx, d.bin.ny, d.ax[0], d.ax[1], w.idx, sm);
+    }
+    __syncthreads();
+  }
+}
+
+hipError_t launch_batch_plane(int inverse, int reduce, const SysDesc* sys, const WorkItem* items, int nitems, int grid,
+                              size_t lds, hipStream_t s) {
+  if (nitems <= 0) return hipSuccess;
+  if (inverse)
+    hipLaunchKernelGGL(k_batch_plane<1>, dim3(grid), dim3(kPlaneBlock), lds, s, sys, items, nitems, reduce);
+  else
+    hipLaunchKernelGGL(k_batch_plane<0>, dim3(grid), dim3(kPlaneBlock), lds, s, sys, items, nitems, reduce);
+  return hipGetLastError();
 }
 
 hipError_t launch_batch_fft_x(int inverse, const SysDesc* sys, const WorkItem* items, int nitems, int grid, size_t lds,

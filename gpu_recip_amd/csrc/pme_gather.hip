@@ -261,7 +261,7 @@ __global__ __launch_bounds__(kSpreadBlock) void k_batch_gather(const SysDesc* __
                                                                const WorkItem* __restrict__ items, int nitems) {
   extern __shared__ __align__(16) unsigned char smem[];
   for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
-    const WorkItem w = items[it];
+    const WorkItem w = uniform_item(items[it]);
     const SysDesc& d = sys[w.sys];
     gather_brick_body<P>(d.xyzq, d.bin, w.idx, d.wmax[0], d.wmax[1], d.wmax[2], d.grid, (float)kCcelec, d.force_type,
                          d.stride, d.force, smem);

@@ -238,6 +238,7 @@ struct pme_recip_s {
   RadixPlan rp_split[3];      // radix-16 first stage (y, z) for the split-precision modes
   bool has_split[3] = {false, false, false};
   int prec = PME_PREC_FP32;
+  bool plane = false;         // x and y passes as one xy plane kernel (the plane fits one workgroup's LDS)
   int NR = 16, TXy = 16, TXz = 16;
 
   // bricks and the chunked record store of the binning
@@ -498,6 +499,7 @@ extern "C" pme_status pme_recip_create(int nfftx, int nffty, int nfftz, int orde
         !radix_plan(h->nz, 1, &h->rp[2]))
       return fail(PME_ERR_UNSUPPORTED, "pme_recip_create: no radix plan");
     pick_tiles(h);
+    h->plane = fft_plane_lds_bytes(h->nx, h->ny, h->rp[0].n) != 0;
     PME_TRY(upload_prefac(h));
     const int hx = h->nx / 2 + 1;
     h->partial_cap = std::max(cols_blocks(hx, h->TXz, h->ny), scalar_sum_blocks((int)Gc));
@@ -619,10 +621,15 @@ extern "C" pme_status pme_recip_r2c_fft(pme_recip_t h) {
   const int hx = h->nx / 2 + 1;
   SolveArgs dummy{};
   mark(h, 2);
-  HIP_TRY(launch_fft_x_r2c(h->grid, h->cgrid, h->nx, h->ny * h->nz, h->NR, axis_args(h, 0), h->stream));
-  mark(h, 3);
-  HIP_TRY(launch_fft_cols(COL_FWD, h->cgrid, h->ny, hx, h->TXy, hx, h->nz, (long long)hx * h->ny, axis_args(h, 1),
-                          dummy, no_reduce(), h->stream));
+  if (h->plane) {
+    HIP_TRY(launch_fft_plane_fwd(h->grid, h->cgrid, h->nx, h->ny, h->nz, axis_args(h, 0), axis_args(h, 1), h->stream));
+    mark(h, 3);
+  } else {
+    HIP_TRY(launch_fft_x_r2c(h->grid, h->cgrid, h->nx, h->ny * h->nz, h->NR, axis_args(h, 0), h->stream));
+    mark(h, 3);
+    HIP_TRY(launch_fft_cols(COL_FWD, h->cgrid, h->ny, hx, h->TXy, hx, h->nz, (long long)hx * h->ny, axis_args(h, 1),
+                            dummy, no_reduce(), h->stream));
+  }
   h->state = ST_XY;
   return PME_OK;
 }
@@ -673,12 +680,20 @@ extern "C" pme_status pme_recip_c2r_fft(pme_recip_t h) {
     return fail(PME_ERR_STATE, "pme_recip_c2r_fft: no spectrum (call r2c_fft first)");
   }
   mark(h, 5);
-  // the y-inverse pass also folds in the pending energy/virial reduction
-  HIP_TRY(launch_fft_cols(COL_INV, h->cgrid, h->ny, hx, h->TXy, hx, h->nz, (long long)hx * h->ny, axis_args(h, 1),
-                          dummy, pending_reduce(h), h->stream));
-  h->pend_solve = h->pend_self = 0;
-  mark(h, 6);
-  HIP_TRY(launch_fft_x_c2r(h->cgrid, h->grid, h->nx, h->ny * h->nz, h->NR, axis_args(h, 0), h->stream));
+  // the y-inverse (or xy plane inverse) pass also folds in the pending
+  // energy/virial reduction
+  if (h->plane) {
+    HIP_TRY(launch_fft_plane_inv(h->cgrid, h->grid, h->nx, h->ny, h->nz, axis_args(h, 0), axis_args(h, 1),
+                                 pending_reduce(h), h->stream));
+    h->pend_solve = h->pend_self = 0;
+    mark(h, 6);
+  } else {
+    HIP_TRY(launch_fft_cols(COL_INV, h->cgrid, h->ny, hx, h->TXy, hx, h->nz, (long long)hx * h->ny, axis_args(h, 1),
+                            dummy, pending_reduce(h), h->stream));
+    h->pend_solve = h->pend_self = 0;
+    mark(h, 6);
+    HIP_TRY(launch_fft_x_c2r(h->cgrid, h->grid, h->nx, h->ny * h->nz, h->NR, axis_args(h, 0), h->stream));
+  }
   h->state = ST_POTENTIAL;
   return PME_OK;
 }

@@ -76,6 +76,7 @@ struct SysDesc {
   int NR, TXy, TXz;        // FFT tiles
   SolveArgs sa;            // influence function; sa.partial = this system's partials
   int nzblk;               // z-pass tiles (= partial count)
+  int plane;               // 1: x and y passes as one xy plane kernel
   void* force;             // caller's force output
   int force_type, stride;
   double* ev_out;          // [10]: energy, virial[9] of the last solve (assigned)
@@ -88,8 +89,16 @@ struct WorkItem {
   int idx;
 };
 
+// Work items are the same for every lane of a workgroup: make that explicit
+// so descriptor fields are read with scalar loads and branch uniformly.
+__device__ __forceinline__ WorkItem uniform_item(WorkItem w) {
+  w.sys = __builtin_amdgcn_readfirstlane(w.sys);
+  w.idx = __builtin_amdgcn_readfirstlane(w.idx);
+  return w;
+}
+
 // Batched stages; each has its own work-item list.
-enum BatchStage { BS_BIN = 0, BS_SPREAD, BS_FFTX, BS_FFTY, BS_FFTZ, BS_GATHER, BS_COUNT };
+enum BatchStage { BS_BIN = 0, BS_SPREAD, BS_FFTX, BS_FFTY, BS_FFTZ, BS_GATHER, BS_PLANE, BS_COUNT };
 
 // Segments of brick b in the chunked record store: s_base[w] = first record
 // of b in chunk w's region, s_pre[w] = exclusive prefix of the counts over w
@@ -156,14 +165,15 @@ hipError_t launch_fill_bspline(int order, const float4* xyzq, int n, float r1, f
 // Sum the pending partials in fixed order (one workgroup) and add them into
 // the energy/virial buffer (CudaPMERecip.cu:871-886 for the scalar sum,
 // :1272-1274 for the self energy).
+template <int NT = kBlock>
 __device__ __forceinline__ void ev_reduce_block(const EvReduce& r) {
-  __shared__ double sh[7 * (kBlock / 64)];
+  __shared__ double sh[7 * (NT / 64)];
   if (r.nsolve > 0) {
     double v[7] = {0, 0, 0, 0, 0, 0, 0};
-    for (int i = threadIdx.x; i < r.nsolve; i += kBlock)
+    for (int i = threadIdx.x; i < r.nsolve; i += NT)
 #pragma unroll
       for (int c = 0; c < 7; c++) v[c] += r.partial[(size_t)i * 7 + c];
-    block_sum<7>(v, sh);
+    block_sum<7, NT>(v, sh);
     if (threadIdx.x == 0) {
       const double hc = 0.5 * kCcelec;
       r.energy[0] += v[0] * hc;
@@ -176,11 +186,26 @@ __device__ __forceinline__ void ev_reduce_block(const EvReduce& r) {
   }
   if (r.nself > 0) {
     double v[1] = {0.0};
-    for (int i = threadIdx.x; i < r.nself; i += kBlock) v[0] += r.self_partial[i];
-    block_sum<1>(v, sh);
+    for (int i = threadIdx.x; i < r.nself; i += NT) v[0] += r.self_partial[i];
+    block_sum<1, NT>(v, sh);
     if (threadIdx.x == 0) r.self_slot[0] += -v[0] * r.kcs;
   }
 }
+
+// xy plane kernels (x and y passes of one z plane in one workgroup):
+// forward reads the real plane and writes its half-spectrum plane, inverse
+// the reverse; the inverse's workgroup 0 also performs a pending
+// energy/virial reduction.
+hipError_t launch_fft_plane_fwd(const float* grid, float2* cg, int nx, int ny, int nz, const AxisArgs& ax,
+                                const AxisArgs& ay, hipStream_t s);
+hipError_t launch_fft_plane_inv(const float2* cg, float* grid, int nx, int ny, int nz, const AxisArgs& ax,
+                                const AxisArgs& ay, const EvReduce& red, hipStream_t s);
+// LDS bytes of a plane kernel (0 when the plane does not fit one workgroup)
+size_t fft_plane_lds_bytes(int nx, int ny, int nf_x);
+// batched plane kernels: items are (system, z plane); with reduce != 0 the
+// plane-0 item of every system assigns that system's energy/virial
+hipError_t launch_batch_plane(int inverse, int reduce, const SysDesc* sys, const WorkItem* items, int nitems, int grid,
+                              size_t lds, hipStream_t s);
 
 // batched launchers: `grid` persistent workgroups loop over the work items
 hipError_t launch_batch_bin(const SysDesc* sys, const WorkItem* items, int nitems, int grid, size_t lds, hipStream_t s);
@@ -204,6 +229,8 @@ size_t fft_cols_lds_bytes(int n, int TX);
 
 constexpr int kSpreadBlock = 1024;  // threads per spread / gather workgroup (16 waves per CU)
 constexpr int kBinBlock = 1024;     // threads per binning workgroup (one atom each at the default chunk)
+constexpr int kPlaneBlock = 1024;   // threads per xy plane FFT workgroup
+constexpr size_t kPlaneLdsBudget = 150 * 1024;  // the plane kernels run one workgroup per CU at most
 constexpr int kGatherBatch = 512;   // candidate records per gather pass (LDS weight cache)
 constexpr int kMaxChunks = 1024;   // binning workgroups per solve (chunk grows with n)
 constexpr int kMinChunk = 1024;    // atoms per binning workgroup

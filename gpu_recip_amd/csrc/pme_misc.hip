@@ -47,7 +47,7 @@ hipError_t launch_self_energy(const float4* xyzq, int n, double* partial, int* n
   return hipGetLastError();
 }
 
-__global__ __launch_bounds__(kBlock) void k_ev_reduce(EvReduce red) { ev_reduce_block(red); }
+__global__ __launch_bounds__(kBlock) void k_ev_reduce(EvReduce red) { ev_reduce_block<kBlock>(red); }
 
 hipError_t launch_ev_reduce(const EvReduce& red, hipStream_t s) {
   hipLaunchKernelGGL(k_ev_reduce, dim3(1), dim3(kBlock), 0, s, red);

@@ -47,6 +47,11 @@ bool smooth235(int n) {
   return n == 1;
 }
 
+static void pack_radices(RadixPlan* rp) {
+  rp->rcode = 0;
+  for (int s = 0; s < rp->nrad; s++) rp->rcode |= (unsigned long long)rp->rad[s] << (5 * s);
+}
+
 bool radix_plan(int n, int tstride, RadixPlan* rp) {
   rp->n = n;
   rp->tstride = tstride;
@@ -59,6 +64,7 @@ bool radix_plan(int n, int tstride, RadixPlan* rp) {
   while (m % 5 == 0) { rp->rad[rp->nrad++] = 5; m /= 5; }
   while (m % 3 == 0) { rp->rad[rp->nrad++] = 3; m /= 3; }
   if (rp->nrad > 12) return false;
+  pack_radices(rp);
   return m == 1;
 }
 
@@ -75,6 +81,7 @@ bool radix_plan_split(int n, int tstride, RadixPlan* rp) {
   rp->nrad = rest.nrad + 1;
   rp->rad[0] = 16;
   for (int s = 0; s < rest.nrad; s++) rp->rad[s + 1] = rest.rad[s];
+  pack_radices(rp);
   return true;
 }
 

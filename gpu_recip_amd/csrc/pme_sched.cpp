@@ -53,6 +53,7 @@ pme_status plan_system(const pme_system_desc& in, int cap_atoms, const Tuning& t
     }
   }
   p->ft = pick_fft_tiles(n[0], n[1], n[2], p->rp[0].n, t.fft_tile);
+  p->plane = fft_plane_lds_bytes(n[0], n[1], p->rp[0].n) != 0 ? 1 : 0;
   const int hx = n[0] / 2 + 1;
   p->nzblk = cols_blocks(hx, p->ft.TXz, n[1]);
   const size_t G = (size_t)n[0] * n[1] * n[2];
@@ -174,6 +175,7 @@ pme_status fill_desc(SysDesc& d, const pme_system_desc& in, const SystemPlan& p,
   d.sa.nf3 = n[2] / 2 + (n[2] % 2);
   d.sa.partial = reinterpret_cast<double*>(b + p.off[5]);
   d.nzblk = p.nzblk;
+  d.plane = p.plane;
   fill_dynamic(d, in, p);
   return PME_OK;
 }
@@ -205,12 +207,19 @@ void build_items(const SysDesc* const* descs, const SystemPlan* const* plans, co
           lds = gather_lds_bytes(orders[k], p.bp.wmax, d.bin.nchunk);
           break;
         case BS_FFTX:
+          if (d.plane) break;
           cnt = (d.bin.ny * d.bin.nz + d.NR - 1) / d.NR;
           lds = fft_x_lds_bytes(d.ax[0].rp.n, d.NR, d.bin.nx);
           break;
         case BS_FFTY:
+          if (d.plane) break;
           cnt = cols_blocks(hx, d.TXy, d.bin.nz);
           lds = fft_cols_lds_bytes(d.bin.ny, d.TXy);
+          break;
+        case BS_PLANE:
+          if (!d.plane) break;
+          cnt = d.bin.nz;
+          lds = fft_plane_lds_bytes(d.bin.nx, d.bin.ny, d.ax[0].rp.n);
           break;
         case BS_FFTZ:
           cnt = cols_blocks(hx, d.TXz, d.bin.ny);
@@ -227,6 +236,7 @@ void build_items(const SysDesc* const* descs, const SystemPlan* const* plans, co
   for (int o : {4, 6, 8}) add_stage(BS_SPREAD, o);
   add_stage(BS_FFTX, 0);
   add_stage(BS_FFTY, 0);
+  add_stage(BS_PLANE, 0);
   add_stage(BS_FFTZ, 0);
   for (int o : {4, 6, 8}) add_stage(BS_GATHER, o);
 }
@@ -245,27 +255,36 @@ static int grid_for(const Tuning& t, int ncu, int count, bool big_block) {
 
 pme_status enqueue_stages(const std::vector<Launch>& launches, const SysDesc* d_sys, const WorkItem* d_items,
                           const Tuning& t, int ncu, hipStream_t stream, hipEvent_t* tev) {
-  const int seq[8][2] = {{BS_BIN, 0}, {BS_SPREAD, 0}, {BS_FFTX, 0}, {BS_FFTY, 0},
-                         {BS_FFTZ, 0}, {BS_FFTY, 1}, {BS_FFTX, 1}, {BS_GATHER, 0}};
+  // timing slot k: the stages run between events k and k+1
+  //   0 bin, 1 spread, 2 x forward and xy-plane forward, 3 y forward, 4 z (+ influence function),
+  //   5 y inverse and xy-plane inverse (with the energy/virial reduction), 6 x inverse, 7 gather
+  const int seq[8][3] = {{BS_BIN, 0, -1},  {BS_SPREAD, 0, -1}, {BS_FFTX, 0, BS_PLANE}, {BS_FFTY, 0, -1},
+                         {BS_FFTZ, 0, -1}, {BS_FFTY, 1, BS_PLANE}, {BS_FFTX, 1, -1},   {BS_GATHER, 0, -1}};
   for (int k = 0; k < 8; k++) {
     if (tev) SCHED_HIP(hipEventRecord(tev[k], stream));
-    const int stage = seq[k][0];
     const bool inv = seq[k][1] != 0;
-    for (const Launch& L : launches) {
-      if (L.stage != stage) continue;
-      const WorkItem* it = d_items + L.offset;
-      const bool big = stage == BS_SPREAD || stage == BS_GATHER;
-      const int grid = grid_for(t, ncu, L.count, big);
-      switch (stage) {
-        case BS_BIN: SCHED_HIP(launch_batch_bin(d_sys, it, L.count, grid, L.lds, stream)); break;
-        case BS_SPREAD: SCHED_HIP(launch_batch_spread(L.order, d_sys, it, L.count, grid, L.lds, stream)); break;
-        case BS_FFTX: SCHED_HIP(launch_batch_fft_x(inv ? 1 : 0, d_sys, it, L.count, grid, L.lds, stream)); break;
-        case BS_FFTY:
-          SCHED_HIP(launch_batch_cols(1, inv ? COL_INV : COL_FWD, inv ? 1 : 0, d_sys, it, L.count, grid, L.lds, stream));
-          break;
-        case BS_FFTZ: SCHED_HIP(launch_batch_cols(2, COL_FWD_SOLVE_INV, 0, d_sys, it, L.count, grid, L.lds, stream)); break;
-        case BS_GATHER: SCHED_HIP(launch_batch_gather(L.order, d_sys, it, L.count, grid, L.lds, stream)); break;
-        default: break;
+    for (int pass = 0; pass < 2; pass++) {
+      const int stage = pass == 0 ? seq[k][0] : seq[k][2];
+      if (stage < 0) continue;
+      for (const Launch& L : launches) {
+        if (L.stage != stage) continue;
+        const WorkItem* it = d_items + L.offset;
+        const bool big = stage == BS_SPREAD || stage == BS_GATHER || stage == BS_PLANE;
+        const int grid = grid_for(t, ncu, L.count, big);
+        switch (stage) {
+          case BS_BIN: SCHED_HIP(launch_batch_bin(d_sys, it, L.count, grid, L.lds, stream)); break;
+          case BS_SPREAD: SCHED_HIP(launch_batch_spread(L.order, d_sys, it, L.count, grid, L.lds, stream)); break;
+          case BS_FFTX: SCHED_HIP(launch_batch_fft_x(inv ? 1 : 0, d_sys, it, L.count, grid, L.lds, stream)); break;
+          case BS_FFTY:
+            SCHED_HIP(launch_batch_cols(1, inv ? COL_INV : COL_FWD, inv ? 1 : 0, d_sys, it, L.count, grid, L.lds, stream));
+            break;
+          case BS_PLANE:
+            SCHED_HIP(launch_batch_plane(inv ? 1 : 0, inv ? 1 : 0, d_sys, it, L.count, grid, L.lds, stream));
+            break;
+          case BS_FFTZ: SCHED_HIP(launch_batch_cols(2, COL_FWD_SOLVE_INV, 0, d_sys, it, L.count, grid, L.lds, stream)); break;
+          case BS_GATHER: SCHED_HIP(launch_batch_gather(L.order, d_sys, it, L.count, grid, L.lds, stream)); break;
+          default: break;
+        }
       }
     }
   }

@@ -25,6 +25,7 @@ struct SystemPlan {
   int prec[3];          // PREC_* per axis (the radix-16 stage on the matrix cores)
   FftTiles ft;
   int nzblk;            // z-pass tiles = energy/virial partials
+  int plane;            // 1: x and y passes as one xy plane kernel
   size_t off[6];        // grid, cgrid, seg_off, rec_pos, rec_idx, partial
   size_t total;         // bytes of the region
 };

@@ -259,7 +259,7 @@ __global__ __launch_bounds__(kBinBlock) void k_batch_bin(const SysDesc* __restri
                                                       const WorkItem* __restrict__ items, int nitems) {
   extern __shared__ int hist[];
   for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
-    const WorkItem w = items[it];
+    const WorkItem w = uniform_item(items[it]);
     const SysDesc& d = sys[w.sys];
     bin_chunk_body(d.xyzq, d.natom, d.bin, w.idx, hist);
     __syncthreads();
@@ -271,7 +271,7 @@ __global__ __launch_bounds__(kSpreadBlock) void k_batch_spread(const SysDesc* __
                                                                const WorkItem* __restrict__ items, int nitems) {
   extern __shared__ __align__(16) unsigned char smem[];
   for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
-    const WorkItem w = items[it];
+    const WorkItem w = uniform_item(items[it]);
     const SysDesc& d = sys[w.sys];
     spread_brick_body<P>(d.bin, w.idx, d.wmax[0], d.wmax[1], d.wmax[2], d.grid, smem);
     __syncthreads();
