@@ -63,6 +63,12 @@ class ServerStats(C.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class RecipHeader(C.Structure):
+    """pme_recip_header: the direct->recip request header (DomdecRecipComm.h:11-15)."""
+    _fields_ = [("inv_boxx", C.c_double), ("inv_boxy", C.c_double), ("inv_boxz", C.c_double),
+                ("ncoord", C.c_int), ("calc_energy", C.c_ubyte), ("calc_virial", C.c_ubyte)]
+
+
 _pi = C.POINTER(C.c_int)
 _pu64 = C.POINTER(C.c_ulonglong)
 _sz = C.c_size_t
@@ -117,6 +123,7 @@ SIGNATURES = {
     "pme_server_create": [_sz, _i, C.POINTER(_vp)],
     "pme_server_destroy": [_vp],
     "pme_server_submit": [_vp, _i, C.POINTER(SystemDesc), _vp, _pu64],
+    "pme_server_submit_header": [_vp, _i, C.POINTER(RecipHeader), _i, _i, _i, _i, _d, _vp, _vp, _pu64],
     "pme_server_wait": [_vp, C.c_ulonglong, _pd, _pd, _pd],
     "pme_server_poll": [_vp, C.c_ulonglong, _pi, _pd, _pd, _pd],
     "pme_server_drain": [_vp],

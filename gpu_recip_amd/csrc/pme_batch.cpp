@@ -595,7 +595,7 @@ extern "C" pme_status pme_batch_autotune(pme_batch_t b, const char* cache_path, 
       return set_error(PME_ERR_STATE, "pme_batch_autotune: resident INT64 (accumulating) outputs");
   float best = 1e30f;
   int bb = b->tune.brick_target, bt = b->tune.fft_tile, bw = b->tune.wg_per_cu;
-  for (int brick : {12, 16, 20, 24})
+  for (int brick : {12, 16, 20, 24, 32})
     for (int tile : {8, 16, 32})
       for (int wpc : {0, 4}) {
         if (pme_batch_set_tuning(b, brick, tile, wpc) != PME_OK) continue;

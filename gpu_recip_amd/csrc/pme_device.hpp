@@ -206,6 +206,20 @@ struct RadixPlan {
 
 __host__ __device__ inline int plan_radix(const RadixPlan& rp, int s) { return (int)((rp.rcode >> (5 * s)) & 31); }
 
+// Quotient of small non-negative ints (n + d < 2^21) from the FP32
+// reciprocal (v_rcp_f32, 1 ulp): the product (n + 1/2) * (1/d) is within
+// 1.5 * 2^-23 relative of n/d + 1/(2d), less than 1/(2d) from it, so the
+// truncation is floor(n/d).  A few full-rate instructions instead of the
+// integer division sequence.
+struct SmallDiv {
+  float inv;
+  int d;
+};
+__device__ __forceinline__ SmallDiv small_div(int d) { return SmallDiv{__builtin_amdgcn_rcpf((float)d), d}; }
+__device__ __forceinline__ int sdiv(int n, const SmallDiv& s) { return (int)(((float)n + 0.5f) * s.inv); }
+// 24-bit multiply (full rate) for index arithmetic on small operands
+__device__ __forceinline__ int mul24(int a, int b) { return __mul24(a, b); }
+
 // One Stockham stage of radix R over nlines interleaved lines.
 // Element j of line l lives at buf[j*ls + l].  Stage with p = product of the
 // previous radices: butterfly i reads i + r T (T = n/R), twiddles input r by
@@ -217,25 +231,31 @@ __device__ __forceinline__ void stockham_stage(const float2* __restrict__ src, f
   const int T = n / R;
   const int nbf = T * nlines;
   const int tscale = (n / (p * R)) * tstride;
+  const SmallDiv dl = small_div(nlines), dp = small_div(p);
+  const int Tls = mul24(T, ls), pls = mul24(p, ls);
   for (int bf = tid; bf < nbf; bf += nthr) {
-    const int i = bf / nlines;
-    const int l = bf - i * nlines;
-    const int k = i % p;
+    const int i = sdiv(bf, dl);
+    const int l = bf - mul24(i, nlines);
+    const int ip = sdiv(i, dp);
+    const int k = i - mul24(ip, p);
     float2 u[R];
+    const float2* s0 = src + mul24(i, ls) + l;
 #pragma unroll
-    for (int r = 0; r < R; r++) u[r] = src[(i + r * T) * ls + l];
+    for (int r = 0; r < R; r++) u[r] = s0[r * Tls];
     if (p > 1) {
+      const int kt = mul24(k, tscale);
 #pragma unroll
       for (int r = 1; r < R; r++) {
-        float2 w = tw[r * k * tscale];
+        float2 w = tw[r * kt];
         if (S > 0) w.y = -w.y;
         u[r] = cmul(u[r], w);
       }
     }
     Dft<R, S>::run(u);
-    const int j = (i / p) * p * R + k;
+    const int j = mul24(mul24(ip, p), R) + k;
+    float2* d0 = dst + mul24(j, ls) + l;
 #pragma unroll
-    for (int r = 0; r < R; r++) dst[(j + r * p) * ls + l] = u[r];
+    for (int r = 0; r < R; r++) d0[r * pls] = u[r];
   }
 }
 
@@ -332,7 +352,7 @@ __device__ __forceinline__ void mfma_dft16_stage(const float2* __restrict__ src,
   for (int blk = wave; blk < nblk; blk += nwave) {
     const int c = blk * 32 + j;
     const bool valid = c < ncols;
-    const int i = valid ? c / nlines : 0;
+    const int i = valid ? sdiv(c, small_div(nlines)) : 0;
     const int l = valid ? c - i * nlines : 0;
     float2 x[8];  // x[4h + t] and x[8 + 4h + t]
 #pragma unroll

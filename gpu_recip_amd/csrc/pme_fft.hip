@@ -48,7 +48,7 @@ __device__ __forceinline__ void fft_x_r2c_body(const float* __restrict__ grid, f
       for (int u = 0; u < kStageU; u++) {
         const int e = e0 + u * kBlock;
         if (e < ntile) {
-          const int row = e / nf;
+          const int row = sdiv(e, small_div(nf));
           b0[(e - row * nf) * LS + row] = v[u];
         }
       }
@@ -67,7 +67,7 @@ __device__ __forceinline__ void fft_x_r2c_body(const float* __restrict__ grid, f
       for (int u = 0; u < kStageU; u++) {
         const int e = e0 + u * kBlock;
         if (e < ntile) {
-          const int row = e / nx;
+          const int row = sdiv(e, small_div(nx));
           b0[(e - row * nx) * LS + row] = make_float2(v[u], 0.f);
         }
       }
@@ -79,7 +79,7 @@ __device__ __forceinline__ void fft_x_r2c_body(const float* __restrict__ grid, f
     // X[k] = E[k] + w^k O[k], E = (Z[k] + Z*[h-k])/2, O = (Z[k] - Z*[h-k])/(2i)
     const int h = nf;
     for (int e = tid; e < nr * hx; e += kBlock) {
-      const int row = e / hx;
+      const int row = sdiv(e, small_div(hx));
       const int k = e - row * hx;
       const float2 zk = res[(k == h ? 0 : k) * LS + row];
       const float2 zc = cconj(res[(k == 0 ? 0 : h - k) * LS + row]);
@@ -90,7 +90,7 @@ __device__ __forceinline__ void fft_x_r2c_body(const float* __restrict__ grid, f
     }
   } else {
     for (int e = tid; e < nr * hx; e += kBlock) {
-      const int row = e / hx;
+      const int row = sdiv(e, small_div(hx));
       const int k = e - row * hx;
       cg[(size_t)(row0 + row) * hx + k] = res[k * LS + row];
     }
@@ -125,7 +125,7 @@ __device__ __forceinline__ void fft_x_c2r_body(const float2* __restrict__ cg, fl
 #pragma unroll
       for (int u = 0; u < kStageU; u++) {
         const int e = e0 + u * kBlock;
-        const int row = e / h;
+        const int row = sdiv(e, small_div(h));
         const int k = e - row * h;
         xk[u] = xc[u] = make_float2(0.f, 0.f);
         if (row < nr) {
@@ -138,7 +138,7 @@ __device__ __forceinline__ void fft_x_c2r_body(const float2* __restrict__ cg, fl
       for (int u = 0; u < kStageU; u++) {
         const int e = e0 + u * kBlock;
         if (e < NR * h) {
-          const int row = e / h;
+          const int row = sdiv(e, small_div(h));
           const int k = e - row * h;
           const float2 A = cadd(xk[u], xc[u]);
           const float2 B = cmul(cconj(tw[k]), csub(xk[u], xc[u]));
@@ -148,7 +148,7 @@ __device__ __forceinline__ void fft_x_c2r_body(const float2* __restrict__ cg, fl
     }
   } else {
     for (int e = tid; e < NR * nx; e += kBlock) {
-      const int row = e / nx;
+      const int row = sdiv(e, small_div(nx));
       const int k = e - row * nx;
       float2 z = make_float2(0.f, 0.f);
       if (row < nr) {
@@ -163,13 +163,13 @@ __device__ __forceinline__ void fft_x_c2r_body(const float2* __restrict__ cg, fl
   if (even) {
     float2* dst = reinterpret_cast<float2*>(grid + (size_t)row0 * nx);
     for (int e = tid; e < nr * nf; e += kBlock) {
-      const int row = e / nf;
+      const int row = sdiv(e, small_div(nf));
       const int m = e - row * nf;
       dst[(size_t)row * nf + m] = res[m * LS + row];
     }
   } else {
     for (int e = tid; e < nr * nx; e += kBlock) {
-      const int row = e / nx;
+      const int row = sdiv(e, small_div(nx));
       const int m = e - row * nx;
       grid[(size_t)(row0 + row) * nx + m] = res[m * LS + row].x;
     }
@@ -233,7 +233,7 @@ __device__ __forceinline__ void fft_plane_fwd_body(const float* __restrict__ gri
       for (int u = 0; u < kStageU; u++) {
         const int e = e0 + u * kPlaneBlock;
         if (e < ne) {
-          const int y = e / nf;
+          const int y = sdiv(e, small_div(nf));
           b0[(e - y * nf) * LSX + y] = v[u];
         }
       }
@@ -252,7 +252,7 @@ __device__ __forceinline__ void fft_plane_fwd_body(const float* __restrict__ gri
       for (int u = 0; u < kStageU; u++) {
         const int e = e0 + u * kPlaneBlock;
         if (e < ne) {
-          const int y = e / nx;
+          const int y = sdiv(e, small_div(nx));
           b0[(e - y * nx) * LSX + y] = make_float2(v[u], 0.f);
         }
       }
@@ -264,7 +264,7 @@ __device__ __forceinline__ void fft_plane_fwd_body(const float* __restrict__ gri
   // R2C post-processing (even nx) and transpose to columns:
   // X[k] = E[k] + w^k O[k], E = (Z[k] + Z*[h-k])/2, O = (Z[k] - Z*[h-k])/(2i)
   for (int e = tid; e < ny * hx; e += kPlaneBlock) {
-    const int y = e / hx;
+    const int y = sdiv(e, small_div(hx));
     const int k = e - y * hx;
     float2 X;
     if (even) {
@@ -284,7 +284,7 @@ __device__ __forceinline__ void fft_plane_fwd_body(const float* __restrict__ gri
   float2* r2 = fft_lines<-1>(q, res, twy, axy.rp, hx, LSY, tid, kPlaneBlock, axy.prec);
   float2* dst = cg + (size_t)z * ny * hx;
   for (int e = tid; e < ny * hx; e += kPlaneBlock) {
-    const int y = e / hx;
+    const int y = sdiv(e, small_div(hx));
     dst[e] = r2[y * LSY + (e - y * hx)];
   }
 }
@@ -316,7 +316,7 @@ __device__ __forceinline__ void fft_plane_inv_body(const float2* __restrict__ cg
     for (int u = 0; u < kStageU; u++) {
       const int e = e0 + u * kPlaneBlock;
       if (e < ne) {
-        const int y = e / hx;
+        const int y = sdiv(e, small_div(hx));
         b0[y * LSY + (e - y * hx)] = v[u];
       }
     }
@@ -329,7 +329,7 @@ __device__ __forceinline__ void fft_plane_inv_body(const float2* __restrict__ cg
     // Z[k] = (X[k] + X*[h-k]) + i w^{-k} (X[k] - X*[h-k]),  k < h
     const int h = nf;
     for (int e = tid; e < ny * h; e += kPlaneBlock) {
-      const int y = e / h;
+      const int y = sdiv(e, small_div(h));
       const int k = e - y * h;
       const float2 xk = r[y * LSY + k];
       const float2 xc = cconj(r[y * LSY + (h - k)]);
@@ -340,7 +340,7 @@ __device__ __forceinline__ void fft_plane_inv_body(const float2* __restrict__ cg
   } else {
     // the full Hermitian spectrum of each row
     for (int e = tid; e < ny * nx; e += kPlaneBlock) {
-      const int y = e / nx;
+      const int y = sdiv(e, small_div(nx));
       const int k = e - y * nx;
       p[k * LSX + y] = k < hx ? r[y * LSY + k] : cconj(r[y * LSY + (nx - k)]);
     }
@@ -351,13 +351,13 @@ __device__ __forceinline__ void fft_plane_inv_body(const float2* __restrict__ cg
   if (even) {
     float2* dst = reinterpret_cast<float2*>(grid + plane);
     for (int e = tid; e < ny * nf; e += kPlaneBlock) {
-      const int y = e / nf;
+      const int y = sdiv(e, small_div(nf));
       dst[e] = s[(e - y * nf) * LSX + y];
     }
   } else {
     float* dst = grid + plane;
     for (int e = tid; e < ny * nx; e += kPlaneBlock) {
-      const int y = e / nx;
+      const int y = sdiv(e, small_div(nx));
       dst[e] = s[(e - y * nx) * LSX + y].x;
     }
   }
@@ -410,37 +410,35 @@ __device__ __forceinline__ float eterm_at(const SolveArgs& s, int k1, int k2, in
   return expf(-s.fac * msq) * s.piv_inv * s.pf1[k1] * s.pf2[k2] * s.pf3[k3] * msq_inv;
 }
 
-__device__ __forceinline__ void ev_accumulate(float eterm, float q2, float fac, float mh1, float mh2, float mh3,
-                                              float msq_inv, double* acc) {
-  const float tmp1 = eterm * q2;
-  const float vterm = 2.0f * (fac + msq_inv);
-  const float tmp2 = tmp1 * vterm;
-  acc[0] += (double)tmp1;
-  acc[1] += (double)(tmp1 * (vterm * mh1 * mh1 - 1.0f));
-  acc[2] += (double)(tmp2 * mh1 * mh2);
-  acc[3] += (double)(tmp2 * mh1 * mh3);
-  acc[4] += (double)(tmp1 * (vterm * mh2 * mh2 - 1.0f));
-  acc[5] += (double)(tmp2 * mh2 * mh3);
-  acc[6] += (double)(tmp1 * (vterm * mh3 * mh3 - 1.0f));
-}
-
 // Multiply one half-spectrum point by eterm; accumulate energy/virial with
-// the reference's weights (mirror term for every k1 >= 1, :708-741).
+// the reference's weights (mirror term for every k1 >= 1, :708-741).  The
+// mirror point (n1-k1, -k2, -k3) has the same |m|^2 and, bsp_mod being
+// symmetric, the same eterm; its m components are -m except on a Nyquist
+// plane of an even axis (m = -n/2 maps to itself), so the diagonal terms
+// double and each off-diagonal term m_a m_b doubles or cancels.
 __device__ __forceinline__ float2 solve_point(const SolveArgs& s, float2 q, int k1, int k2, int k3, double* acc) {
   if ((k1 | k2 | k3) == 0) return make_float2(0.f, 0.f);
   float mh1, mh2, mh3, msq_inv;
   const float eterm = eterm_at(s, k1, k2, k3, mh1, mh2, mh3, msq_inv);
   if (s.calc_ev) {
     const float q2 = q.x * q.x + q.y * q.y;
-    ev_accumulate(eterm, q2, s.fac, mh1, mh2, mh3, msq_inv, acc);
-    if (k1 >= 1) {
-      const int k1s = s.n1 - k1;
-      const int k2s = (s.n2 - k2) % s.n2;
-      const int k3s = (s.n3 - k3) % s.n3;
-      float ms1, ms2, ms3, msqs_inv;
-      const float eterms = eterm_at(s, k1s, k2s, k3s, ms1, ms2, ms3, msqs_inv);
-      ev_accumulate(eterms, q2, s.fac, ms1, ms2, ms3, msqs_inv, acc);
-    }
+    const float tmp1 = eterm * q2;
+    const float vterm = 2.0f * (s.fac + msq_inv);
+    const float tmp2 = tmp1 * vterm;
+    // weight of the point (1) plus its mirror (1 when k1 >= 1)
+    const double w = k1 >= 1 ? 2.0 : 1.0;
+    // mirror sign of each m component: +1 when it maps to itself
+    const bool f1 = !(2 * k1 == s.n1), f2 = !(2 * k2 == s.n2 || k2 == 0), f3 = !(2 * k3 == s.n3 || k3 == 0);
+    const double w12 = k1 >= 1 ? (f1 == f2 ? 2.0 : 0.0) : 1.0;
+    const double w13 = k1 >= 1 ? (f1 == f3 ? 2.0 : 0.0) : 1.0;
+    const double w23 = k1 >= 1 ? (f2 == f3 ? 2.0 : 0.0) : 1.0;
+    acc[0] += w * (double)tmp1;
+    acc[1] += w * (double)(tmp1 * (vterm * mh1 * mh1 - 1.0f));
+    acc[2] += w12 * (double)(tmp2 * mh1 * mh2);
+    acc[3] += w13 * (double)(tmp2 * mh1 * mh3);
+    acc[4] += w * (double)(tmp1 * (vterm * mh2 * mh2 - 1.0f));
+    acc[5] += w23 * (double)(tmp2 * mh2 * mh3);
+    acc[6] += w * (double)(tmp1 * (vterm * mh3 * mh3 - 1.0f));
   }
   return cscale(q, eterm);
 }
@@ -477,7 +475,7 @@ __device__ __forceinline__ void fft_cols_body(float2* __restrict__ cg, int n, in
 #pragma unroll
     for (int u = 0; u < kStageU; u++) {
       const int e = e0 + u * kBlock;
-      const int j = e / TX;
+      const int j = sdiv(e, small_div(TX));
       const int l = e - j * TX;
       v[u] = (e < n * TX && l < nl) ? cg[base + (size_t)j * pt_stride + l] : make_float2(0.f, 0.f);
     }
@@ -498,7 +496,7 @@ __device__ __forceinline__ void fft_cols_body(float2* __restrict__ cg, int n, in
   if (MODE == COL_FWD_SOLVE_INV) {
     const int k2 = ty;
     for (int e = tid; e < n * TX; e += kBlock) {
-      const int j = e / TX;
+      const int j = sdiv(e, small_div(TX));
       const int l = e - j * TX;
       if (l < nl) res[e] = solve_point(sa, res[e], kx0 + l, k2, j, acc);
     }
@@ -508,7 +506,7 @@ __device__ __forceinline__ void fft_cols_body(float2* __restrict__ cg, int n, in
     res = fft_lines<+1>(res, res == b0 ? b1 : b0, tw, ax.rp, TX, TX, tid, kBlock, ax.prec);
   }
   for (int e = tid; e < n * TX; e += kBlock) {
-    const int j = e / TX;
+    const int j = sdiv(e, small_div(TX));
     const int l = e - j * TX;
     if (l < nl) cg[base + (size_t)j * pt_stride + l] = res[e];
   }

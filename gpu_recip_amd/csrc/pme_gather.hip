@@ -5,10 +5,10 @@
 // then takes the atoms whose home copy the binning put in this brick,
 // compacts them with a wave ballot, loads their CURRENT coordinates (so the
 // result is right even if the caller moved atoms after spread_charge) and
-// gathers with G lanes per atom (G = 4 for order 4, 8 for orders 6 and 8):
-// lane g owns stencil column x = ix+g and walks the p*p (y, z) rows out of
-// LDS; the G partial sums are combined with wave shuffles.  An atom whose
-// current base index is no longer in this brick reads the potential from HBM.
+// gathers each atom in one thread: B-spline weights and derivatives in
+// registers, the p x p stencil rows read from LDS in pairs of dwords.  An
+// atom whose current base index is no longer in this brick reads the
+// potential from HBM.
 #include "pme_internal.hpp"
 
 namespace pme {
@@ -35,23 +35,56 @@ __device__ __forceinline__ void store_force(int ft, void* force, int atom, int s
   }
 }
 
+// Gradient sums of one atom from the potential: with w(j) = theta, w'(j) =
+// dtheta per axis,
+//   f1 = sum dthx thy thz phi,  f2 = sum thx dthy thz phi,  f3 = sum thx thy dthz phi,
+// evaluated row by row: a = sum_x thx phi, b = sum_x dthx phi per (y, z) row,
+// then y sums per z plane.  at(jx, jy, jz) returns phi at the stencil point.
+template <int P, typename At>
+__device__ __forceinline__ void gather_sums(const float* tx, const float* dtx, const float* ty, const float* dty,
+                                            const float* tz, const float* dtz, At at, float& f1, float& f2,
+                                            float& f3) {
+  f1 = f2 = f3 = 0.f;
+#pragma unroll
+  for (int jz = 0; jz < P; jz++) {
+    float ay = 0.f, by = 0.f, cy = 0.f;
+#pragma unroll
+    for (int jy = 0; jy < P; jy++) {
+      float ra = 0.f, rb = 0.f;
+#pragma unroll
+      for (int jx = 0; jx < P; jx++) {
+        const float v = at(jx, jy, jz);
+        ra = fmaf(tx[jx], v, ra);
+        rb = fmaf(dtx[jx], v, rb);
+      }
+      ay = fmaf(ty[jy], ra, ay);
+      by = fmaf(ty[jy], rb, by);
+      cy = fmaf(dty[jy], ra, cy);
+    }
+    f1 = fmaf(tz[jz], by, f1);
+    f2 = fmaf(tz[jz], cy, f2);
+    f3 = fmaf(dtz[jz], ay, f3);
+  }
+}
+
 // Brick b of one grid (one kSpreadBlock-thread workgroup); smem: the
 // workgroup's dynamic LDS (gather_lds_bytes).  ft: PME_FORCE_* store mode.
+// The brick's potential plus a (p-1) upper halo is staged in LDS; the home
+// atoms of the brick are compacted with a wave ballot and each is then
+// gathered by ONE thread with its B-spline weights in registers, reading its
+// p x p stencil rows straight from LDS.  An atom whose current base index is
+// no longer in this brick (moved since the binning) reads HBM instead.
 template <int P>
 __device__ __forceinline__ void gather_brick_body(const float4* __restrict__ xyzq, const BinArgs& a, int b, int wxm,
                                                   int wym, int wzm, const float* __restrict__ phi, float ccelec_f,
                                                   int ft, int stride, void* force, unsigned char* smem) {
-  constexpr int G = (P <= 4) ? 4 : 8;
-  constexpr int NT = 6 * P;  // theta x/y/z and dtheta x/y/z per atom
   __shared__ int sh_scan[kSpreadBlock / 64];
   __shared__ int s_cnt;
   const int hxs = wxm + P - 1, hys = wym + P - 1, hzs = wzm + P - 1;
-  float* pl = reinterpret_cast<float*>(smem);                               // halo brick
-  float* s_th = pl + ((hxs * hys * hzs + 3) & ~3);                          // [kGatherBatch][NT]
-  int4* s_meta = reinterpret_cast<int4*>(s_th + kGatherBatch * NT);         // [kGatherBatch] (ix, iy, iz, atom)
-  float* s_q = reinterpret_cast<float*>(s_meta + kGatherBatch);             // [kGatherBatch]
-  int* s_pre = reinterpret_cast<int*>(s_q + kGatherBatch);                  // [nchunk + 1]
-  int* s_base = s_pre + a.nchunk + 1;                                       // [nchunk]
+  float* pl = reinterpret_cast<float*>(smem);                          // halo brick
+  int* s_atom = reinterpret_cast<int*>(pl + ((hxs * hys * hzs + 3) & ~3));  // [kSpreadBlock] home atoms
+  int* s_pre = s_atom + kSpreadBlock;                                   // [nchunk + 1]
+  int* s_base = s_pre + a.nchunk + 1;                                   // [nchunk]
 
   const int tid = threadIdx.x;
   const int bxi = b % a.nbx, byi = (b / a.nbx) % a.nby, bzi = b / (a.nbx * a.nby);
@@ -62,21 +95,22 @@ __device__ __forceinline__ void gather_brick_body(const float4* __restrict__ xyz
   // stage brick + upper halo (periodic) in LDS, kStageU loads in flight
   const int hx = wx + P - 1, hy = wy + P - 1, hz = wz + P - 1;
   const int nh = hx * hy * hz;
+  const SmallDiv dhx = small_div(hx), dhy = small_div(hy);
   for (int i0 = tid; i0 < nh; i0 += kStageU * kSpreadBlock) {
     float v[kStageU];
     int dst[kStageU];
 #pragma unroll
     for (int u = 0; u < kStageU; u++) {
       const int i = i0 + u * kSpreadBlock;
-      const int t = i / hx;
+      const int t = sdiv(i, dhx);
       const int lx = i - t * hx;
-      const int lz = t / hy;
+      const int lz = sdiv(t, dhy);
       const int ly = t - lz * hy;
       int gx = x0 + lx, gy = y0 + ly, gz = z0 + lz;
       if (gx >= a.nx) gx -= a.nx;
       if (gy >= a.ny) gy -= a.ny;
       if (gz >= a.nz) gz -= a.nz;
-      dst[u] = (lz * hys + ly) * hxs + lx;
+      dst[u] = mul24(mul24(lz, hys) + ly, hxs) + lx;
       v[u] = (i < nh) ? phi[(size_t)gx + (size_t)a.nx * ((size_t)gy + (size_t)a.ny * gz)] : 0.f;
     }
 #pragma unroll
@@ -85,15 +119,14 @@ __device__ __forceinline__ void gather_brick_body(const float4* __restrict__ xyz
   }
   const int total = brick_segments<kSpreadBlock>(a, b, s_pre, s_base, sh_scan);  // ends with a barrier
 
-  const int g = tid % G;
-  for (int base = 0; base < total; base += kGatherBatch) {
+  for (int base = 0; base < total; base += kSpreadBlock) {
     if (tid == 0) s_cnt = 0;
     __syncthreads();
     // 1. home atoms of this batch of records, compacted with a wave ballot
     const int v = base + tid;
     int atom = 0;
     bool home = false;
-    if (tid < kGatherBatch && v < total) {
+    if (v < total) {
       const int r = a.rec_idx[brick_record(a, v, s_pre, s_base)];
       home = r < 0;
       atom = r & 0x7fffffff;
@@ -102,119 +135,50 @@ __device__ __forceinline__ void gather_brick_body(const float4* __restrict__ xyz
     int wbase = 0;
     if ((tid & 63) == 0 && m != 0ull) wbase = atomicAdd(&s_cnt, __popcll(m));
     wbase = __shfl(wbase, 0, 64);
-    // 2. current coordinates -> grid index and weights, once per atom
-    if (home) {
-      const int slot = wbase + __popcll(m & __lanemask_lt());
-      const float4 p = xyzq[atom];
-      int ix, iy, iz;
-      float wxf, wyf, wzf;
-      grid_index(p.x, a.r1, a.nx, ix, wxf);
-      grid_index(p.y, a.r2, a.ny, iy, wyf);
-      grid_index(p.z, a.r3, a.nz, iz, wzf);
-      s_meta[slot] = make_int4(safe_index(ix, a.nx), safe_index(iy, a.ny), safe_index(iz, a.nz), atom);
-      s_q[slot] = p.w;
-      float* th = s_th + slot * NT;
-      float t[P], d[P];
-      bspline_theta_dtheta<P>(wxf, t, d);
-#pragma unroll
-      for (int j = 0; j < P; j++) {
-        th[j] = t[j];
-        th[3 * P + j] = d[j];
-      }
-      bspline_theta_dtheta<P>(wyf, t, d);
-#pragma unroll
-      for (int j = 0; j < P; j++) {
-        th[P + j] = t[j];
-        th[4 * P + j] = d[j];
-      }
-      bspline_theta_dtheta<P>(wzf, t, d);
-#pragma unroll
-      for (int j = 0; j < P; j++) {
-        th[2 * P + j] = t[j];
-        th[5 * P + j] = d[j];
-      }
-    }
+    if (home) s_atom[wbase + __popcll(m & __lanemask_lt())] = atom;
     __syncthreads();
-    // 3. G lanes per atom: lane g owns stencil column x = ix + g
-    const int na = s_cnt;
-    for (int a0 = 0; a0 < na; a0 += kSpreadBlock / G) {
-      const int ai = a0 + tid / G;
-      const bool active = ai < na;
-      float f1 = 0.f, f2 = 0.f, f3 = 0.f;
-      if (active && g < P) {
-        const int4 mt = s_meta[ai];
-        const float* th = s_th + ai * NT;
-        const float tx = th[g], dtx = th[3 * P + g];
-        float thy[P], thz[P], dthy[P], dthz[P];
-#pragma unroll
-        for (int j = 0; j < P; j++) {
-          thy[j] = th[P + j];
-          thz[j] = th[2 * P + j];
-          dthy[j] = th[4 * P + j];
-          dthz[j] = th[5 * P + j];
-        }
-        int dx = mt.x - x0, dy = mt.y - y0, dz = mt.z - z0;
-        if (dx >= wx) dx -= a.nx;
-        if (dy >= wy) dy -= a.ny;
-        if (dz >= wz) dz -= a.nz;
-        // f1 = dtx sum_z thz sum_y thy phi, f2 = tx sum_z thz sum_y dthy phi,
-        // f3 = tx sum_z dthz sum_y thy phi: two y sums per z plane
-        float g1 = 0.f, g2 = 0.f, g3 = 0.f;
-        if (dx >= 0 && dy >= 0 && dz >= 0) {
-          const float* col = pl + (dz * hys + dy) * hxs + dx + g;
-#pragma unroll
-          for (int jz = 0; jz < P; jz++) {
-            float sy = 0.f, sdy = 0.f;
-#pragma unroll
-            for (int jy = 0; jy < P; jy++) {
-              const float vv = col[(jz * hys + jy) * hxs];
-              sy = fmaf(thy[jy], vv, sy);
-              sdy = fmaf(dthy[jy], vv, sdy);
-            }
-            g1 = fmaf(thz[jz], sy, g1);
-            g2 = fmaf(thz[jz], sdy, g2);
-            g3 = fmaf(dthz[jz], sy, g3);
-          }
-        } else {
-          // coordinates changed since the binning: potential from HBM
-          int gx = mt.x + g;
-          if (gx >= a.nx) gx -= a.nx;
-#pragma unroll
-          for (int jz = 0; jz < P; jz++) {
-            int gz = mt.z + jz;
-            if (gz >= a.nz) gz -= a.nz;
-            float sy = 0.f, sdy = 0.f;
-#pragma unroll
-            for (int jy = 0; jy < P; jy++) {
-              int gy = mt.y + jy;
-              if (gy >= a.ny) gy -= a.ny;
-              const float vv = phi[(size_t)gx + (size_t)a.nx * ((size_t)gy + (size_t)a.ny * gz)];
-              sy = fmaf(thy[jy], vv, sy);
-              sdy = fmaf(dthy[jy], vv, sdy);
-            }
-            g1 = fmaf(thz[jz], sy, g1);
-            g2 = fmaf(thz[jz], sdy, g2);
-            g3 = fmaf(dthz[jz], sy, g3);
-          }
-        }
-        f1 = dtx * g1;
-        f2 = tx * g2;
-        f3 = tx * g3;
+    // 2. one thread per home atom, weights in registers
+    if (tid < s_cnt) {
+      const int at = s_atom[tid];
+      const float4 p = xyzq[at];
+      int ix, iy, iz;
+      float fx, fy, fz;
+      grid_index(p.x, a.r1, a.nx, ix, fx);
+      grid_index(p.y, a.r2, a.ny, iy, fy);
+      grid_index(p.z, a.r3, a.nz, iz, fz);
+      ix = safe_index(ix, a.nx);
+      iy = safe_index(iy, a.ny);
+      iz = safe_index(iz, a.nz);
+      float tx[P], dtx[P], ty[P], dty[P], tz[P], dtz[P];
+      bspline_theta_dtheta<P>(fx, tx, dtx);
+      bspline_theta_dtheta<P>(fy, ty, dty);
+      bspline_theta_dtheta<P>(fz, tz, dtz);
+      int dx = ix - x0, dy = iy - y0, dz = iz - z0;
+      if (dx >= wx) dx -= a.nx;
+      if (dy >= wy) dy -= a.ny;
+      if (dz >= wz) dz -= a.nz;
+      float f1, f2, f3;
+      if (dx >= 0 && dy >= 0 && dz >= 0) {
+        const float* col = pl + mul24(mul24(dz, hys) + dy, hxs) + dx;
+        const int rs = hxs, ps = mul24(hys, hxs);
+        gather_sums<P>(tx, dtx, ty, dty, tz, dtz,
+                       [&](int jx, int jy, int jz) { return col[jz * ps + jy * rs + jx]; }, f1, f2, f3);
+      } else {
+        // coordinates changed since the binning: potential from HBM
+        gather_sums<P>(tx, dtx, ty, dty, tz, dtz,
+                       [&](int jx, int jy, int jz) {
+                         int gx = ix + jx, gy = iy + jy, gz = iz + jz;
+                         if (gx >= a.nx) gx -= a.nx;
+                         if (gy >= a.ny) gy -= a.ny;
+                         if (gz >= a.nz) gz -= a.nz;
+                         return phi[(size_t)gx + (size_t)a.nx * ((size_t)gy + (size_t)a.ny * gz)];
+                       },
+                       f1, f2, f3);
       }
-#pragma unroll
-      for (int off = 1; off < G; off <<= 1) {
-        f1 += __shfl_xor(f1, off, 64);
-        f2 += __shfl_xor(f2, off, 64);
-        f3 += __shfl_xor(f3, off, 64);
-      }
-      if (active && g == 0) {
-        // CudaPMERecip.cu:1231-1241: q*ccelec, then q*recip*f*nfft in FP32
-        const float q = s_q[ai] * ccelec_f;
-        const float fx = q * a.r1 * f1 * (float)a.nx;
-        const float fy = q * a.r2 * f2 * (float)a.ny;
-        const float fz = q * a.r3 * f3 * (float)a.nz;
-        store_force(ft, force, s_meta[ai].w, stride, fx, fy, fz);
-      }
+      // CudaPMERecip.cu:1231-1241: q*ccelec, then q*recip*f*nfft in FP32
+      const float q = p.w * ccelec_f;
+      store_force(ft, force, at, stride, q * a.r1 * f1 * (float)a.nx, q * a.r2 * f2 * (float)a.ny,
+                  q * a.r3 * f3 * (float)a.nz);
     }
     __syncthreads();
   }
@@ -230,8 +194,8 @@ __global__ __launch_bounds__(kSpreadBlock) void k_gather_brick(const float4* __r
 
 size_t gather_lds_bytes(int order, const int wmax[3], int nchunk) {
   const size_t halo = (size_t)(wmax[0] + order - 1) * (wmax[1] + order - 1) * (wmax[2] + order - 1);
-  return sizeof(float) * ((halo + 3) & ~(size_t)3) + sizeof(float) * kGatherBatch * 6 * order +
-         sizeof(int4) * kGatherBatch + sizeof(float) * kGatherBatch + sizeof(int) * (2 * (size_t)nchunk + 1);
+  return sizeof(float) * ((halo + 3) & ~(size_t)3) + sizeof(int) * kSpreadBlock +
+         sizeof(int) * (2 * (size_t)nchunk + 1);
 }
 
 template <int P>

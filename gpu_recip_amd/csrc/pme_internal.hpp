@@ -231,7 +231,6 @@ constexpr int kSpreadBlock = 1024;  // threads per spread / gather workgroup (16
 constexpr int kBinBlock = 1024;     // threads per binning workgroup (one atom each at the default chunk)
 constexpr int kPlaneBlock = 1024;   // threads per xy plane FFT workgroup
 constexpr size_t kPlaneLdsBudget = 150 * 1024;  // the plane kernels run one workgroup per CU at most
-constexpr int kGatherBatch = 512;   // candidate records per gather pass (LDS weight cache)
 constexpr int kMaxChunks = 1024;   // binning workgroups per solve (chunk grows with n)
 constexpr int kMinChunk = 1024;    // atoms per binning workgroup
 

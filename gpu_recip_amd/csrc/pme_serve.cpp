@@ -541,6 +541,29 @@ extern "C" pme_status pme_server_submit(pme_server_t s, int key, const pme_syste
   return PME_OK;
 }
 
+extern "C" pme_status pme_server_submit_header(pme_server_t s, int key, const pme_recip_header* h, int nfftx,
+                                               int nffty, int nfftz, int order, double kappa, const void* d_xyzq,
+                                               void* d_force3, unsigned long long* ticket) {
+  if (!h) return set_error(PME_ERR_INVALID_ARG, "pme_server_submit_header: null header");
+  pme_system_desc d{};
+  d.nfftx = nfftx;
+  d.nffty = nffty;
+  d.nfftz = nfftz;
+  d.order = order;
+  d.kappa = kappa;
+  d.recip[0] = h->inv_boxx;
+  d.recip[4] = h->inv_boxy;
+  d.recip[8] = h->inv_boxz;
+  d.ncoord = h->ncoord;
+  d.d_xyzq = d_xyzq;
+  d.d_force = d_force3;
+  d.force_type = PME_FORCE_FLOAT3;
+  d.stride = h->ncoord;
+  d.calc_energy_virial = (h->calc_energy || h->calc_virial) ? 1 : 0;
+  d.precision = PME_PREC_FP32;
+  return pme_server_submit(s, key, &d, nullptr, ticket);
+}
+
 static pme_status take_result(pme_server_t s, std::unordered_map<unsigned long long, Result>::iterator it,
                               double* energy, double* virial, double* latency_ms) {
   const Result r = it->second;

@@ -131,7 +131,10 @@ __device__ __forceinline__ int local_offset(int i, int x0, int w, int n) {
 __host__ __device__ constexpr int spread_xstride(int wxm, int P) { return (wxm + 2 * (P - 1)) | 1; }
 
 // Brick b of one grid (one kSpreadBlock-thread workgroup); smem: the
-// workgroup's dynamic LDS (spread_lds_bytes).
+// workgroup's dynamic LDS (spread_lds_bytes).  Each thread takes one record
+// (an atom whose stencil touches the brick), keeps q*theta_x, theta_y,
+// theta_z in registers and adds its p^3 stencil into the brick, the stencil
+// rows outside the brick in y or z masked off (x outside lands in the margin).
 template <int P>
 __device__ __forceinline__ void spread_brick_body(const BinArgs& a, int b, int wxm, int wym, int wzm,
                                                   float* __restrict__ grid, unsigned char* smem) {
@@ -140,10 +143,8 @@ __device__ __forceinline__ void spread_brick_body(const BinArgs& a, int b, int w
   const int bxs = spread_xstride(wxm, P);
   unsigned long long* acc = reinterpret_cast<unsigned long long*>(smem);
   const int acc_n = wzm * wym * bxs;
-  int* s_d = reinterpret_cast<int*>(acc + acc_n);            // [3][kSpreadBlock] local offsets
-  float* s_th = reinterpret_cast<float*>(s_d + 3 * kSpreadBlock);   // [kSpreadBlock][3P]
-  int* s_pre = reinterpret_cast<int*>(s_th + kSpreadBlock * 3 * P);  // [nchunk + 1]
-  int* s_base = s_pre + a.nchunk + 1;                         // [nchunk]
+  int* s_pre = reinterpret_cast<int*>(acc + acc_n);  // [nchunk + 1]
+  int* s_base = s_pre + a.nchunk + 1;                 // [nchunk]
 
   const int tid = threadIdx.x;
   const int bxi = b % a.nbx, byi = (b / a.nbx) % a.nby, bzi = b / (a.nbx * a.nby);
@@ -151,70 +152,58 @@ __device__ __forceinline__ void spread_brick_body(const BinArgs& a, int b, int w
   const int y0 = brick_lo(byi, a.ny, a.nby), wy = brick_lo(byi + 1, a.ny, a.nby) - y0;
   const int z0 = brick_lo(bzi, a.nz, a.nbz), wz = brick_lo(bzi + 1, a.nz, a.nbz) - z0;
   const bool wrapx = (a.nbx == 1), wrapy = (a.nby == 1), wrapz = (a.nbz == 1);
+  const int plane = mul24(wym, bxs);
 
   for (int i = tid; i < acc_n; i += kSpreadBlock) acc[i] = 0ull;
-  const int total = brick_segments<kSpreadBlock>(a, b, s_pre, s_base, sh_scan);
+  const int total = brick_segments<kSpreadBlock>(a, b, s_pre, s_base, sh_scan);  // ends with a barrier
 
-  for (int base = 0; base < total; base += kSpreadBlock) {
-    const int v = base + tid;
-    if (v < total) {
-      const float4 p = a.rec_pos[brick_record(a, v, s_pre, s_base)];
-      int ix, iy, iz;
-      float fx, fy, fz;
-      grid_index(p.x, a.r1, a.nx, ix, fx);
-      grid_index(p.y, a.r2, a.ny, iy, fy);
-      grid_index(p.z, a.r3, a.nz, iz, fz);
-      // x offset into the margin-padded row: in [0, wx + 2(P-1)) always
-      s_d[tid] = local_offset(safe_index(ix, a.nx), x0, wx, a.nx) + (P - 1);
-      s_d[kSpreadBlock + tid] = local_offset(safe_index(iy, a.ny), y0, wy, a.ny);
-      s_d[2 * kSpreadBlock + tid] = local_offset(safe_index(iz, a.nz), z0, wz, a.nz);
-      float th[P];
-      float* o = s_th + tid * 3 * P;
-      bspline_theta<P>(fx, th);
+  for (int v = tid; v < total; v += kSpreadBlock) {
+    const float4 p = a.rec_pos[brick_record(a, v, s_pre, s_base)];
+    int ix, iy, iz;
+    float fx, fy, fz;
+    grid_index(p.x, a.r1, a.nx, ix, fx);
+    grid_index(p.y, a.r2, a.ny, iy, fy);
+    grid_index(p.z, a.r3, a.nz, iz, fz);
+    float tx[P], ty[P], tz[P];
+    bspline_theta<P>(fx, tx);
+    bspline_theta<P>(fy, ty);
+    bspline_theta<P>(fz, tz);
 #pragma unroll
-      for (int j = 0; j < P; j++) o[j] = p.w * th[j];
-      bspline_theta<P>(fy, th);
+    for (int j = 0; j < P; j++) tx[j] *= p.w;
+    // x offset into the margin-padded row: in [0, wx + 2(P-1)) always
+    const int dx = local_offset(safe_index(ix, a.nx), x0, wx, a.nx) + (P - 1);
+    const int dy = local_offset(safe_index(iy, a.ny), y0, wy, a.ny);
+    const int dz = local_offset(safe_index(iz, a.nz), z0, wz, a.nz);
 #pragma unroll
-      for (int j = 0; j < P; j++) o[P + j] = th[j];
-      bspline_theta<P>(fz, th);
-#pragma unroll
-      for (int j = 0; j < P; j++) o[2 * P + j] = th[j];
-    }
-    __syncthreads();
-    const int na = min(kSpreadBlock, total - base);
-    const int nrows = na * P * P;
-    for (int r = tid; r < nrows; r += kSpreadBlock) {
-      const int ai = r / (P * P);
-      const int yz = r - ai * (P * P);
-      const int jz = yz / P;
-      const int jy = yz - jz * P;
-      const float* th = s_th + ai * 3 * P;
-      int ly = s_d[kSpreadBlock + ai] + jy;
-      int lz = s_d[2 * kSpreadBlock + ai] + jz;
-      ly -= (wrapy && ly >= wy) ? a.ny : 0;
+    for (int jz = 0; jz < P; jz++) {
+      int lz = dz + jz;
       lz -= (wrapz && lz >= wz) ? a.nz : 0;
-      if (ly < 0 || ly >= wy || lz < 0 || lz >= wz) continue;
-      const float c = th[P + jy] * th[2 * P + jz];
-      unsigned long long* row = acc + (lz * wym + ly) * bxs + s_d[ai];
-      float tx[P];
+      if (lz < 0 || lz >= wz) continue;
 #pragma unroll
-      for (int jx = 0; jx < P; jx++) tx[jx] = th[jx];
+      for (int jy = 0; jy < P; jy++) {
+        int ly = dy + jy;
+        ly -= (wrapy && ly >= wy) ? a.ny : 0;
+        if (ly < 0 || ly >= wy) continue;
+        const float c = ty[jy] * tz[jz];
+        unsigned long long* row = acc + mul24(lz, plane) + mul24(ly, bxs) + dx;
 #pragma unroll
-      for (int jx = 0; jx < P; jx++) atomicAdd(&row[jx], (unsigned long long)to_fixed40(tx[jx] * c));
+        for (int jx = 0; jx < P; jx++) atomicAdd(&row[jx], (unsigned long long)to_fixed40(tx[jx] * c));
+      }
     }
-    __syncthreads();
   }
+  __syncthreads();
 
   // the brick is complete: one coalesced store per grid point, no memset
   // needed.  With one brick along x the high margin is the periodic image of
   // the row start.
   const int nb = wx * wy * wz;
+  const SmallDiv dwx = small_div(wx), dwy = small_div(wy);
   for (int i = tid; i < nb; i += kSpreadBlock) {
-    const int lx = i % wx;
-    const int t = i / wx;
-    const int ly = t % wy;
-    const int lz = t / wy;
-    const unsigned long long* row = acc + (lz * wym + ly) * bxs + (P - 1);
+    const int t = sdiv(i, dwx);
+    const int lx = i - t * wx;
+    const int lz = sdiv(t, dwy);
+    const int ly = t - lz * wy;
+    const unsigned long long* row = acc + mul24(lz, plane) + mul24(ly, bxs) + (P - 1);
     unsigned long long v = row[lx];
     if (wrapx && lx < P - 1) v += row[lx + a.nx];
     grid[(size_t)(x0 + lx) + (size_t)a.nx * ((size_t)(y0 + ly) + (size_t)a.ny * (z0 + lz))] = from_fixed40(v);
@@ -230,8 +219,7 @@ __global__ __launch_bounds__(kSpreadBlock) void k_spread_brick(BinArgs a, int wx
 
 size_t spread_lds_bytes(int order, const int wmax[3], int nchunk) {
   return sizeof(unsigned long long) * (size_t)wmax[2] * wmax[1] * spread_xstride(wmax[0], order) +
-         sizeof(int) * 3 * kSpreadBlock +
-         sizeof(float) * (size_t)kSpreadBlock * 3 * order + sizeof(int) * (2 * (size_t)nchunk + 1);
+         sizeof(int) * (2 * (size_t)nchunk + 1);
 }
 
 hipError_t launch_spread(int order, const BinArgs& a, const int wmax[3], float* grid, hipStream_t s) {

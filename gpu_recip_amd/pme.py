@@ -286,6 +286,18 @@ class PmeServer:
             self._keep[t.value] = tuple(keep)
         return t.value
 
+    def submit_header(self, key: int, header_bytes: bytes, nfft, order: int, kappa: float, xyzq, force3,
+                      keep=()) -> int:
+        """Serve one direct->recip request: header_bytes is the wire header
+        (three doubles, an int, two bool bytes; DomdecRecipComm.h:11-15),
+        xyzq a device float4 array, force3 a device float3 array."""
+        h = L.RecipHeader.from_buffer_copy(header_bytes.ljust(C.sizeof(L.RecipHeader), b"\0"))
+        t = C.c_ulonglong()
+        L.call("pme_server_submit_header", self.h, int(key), C.byref(h), int(nfft[0]), int(nfft[1]), int(nfft[2]),
+               int(order), float(kappa), _ptr(xyzq), _ptr(force3), C.byref(t))
+        self._keep[t.value] = tuple(keep) + (xyzq, force3)
+        return t.value
+
     def wait(self, ticket: int):
         e, ms = C.c_double(), C.c_double()
         v = (C.c_double * 9)()

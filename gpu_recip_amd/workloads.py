@@ -7,7 +7,8 @@
 
 production: 96,000 atoms, 108x100x96, order 6 (plain five-call path)
 split:      250,002 atoms, 144^3, order 4 (plain path, chosen FFT precision)
-batched:    the 64-system set, one batched solve per step
+batched:    the 64-system set, one batched solve per step (autotuned plan, as
+            in the benchmark)
 small:      8,001 atoms, 48^3, order 4 (plain path)
 Inputs are uploaded once; every solve ends with the energy/virial readback.
 """
@@ -43,6 +44,8 @@ def main(argv=None):
             f = torch.zeros(3 * st, dtype=torch.float32, device=dev)
             ids.append(b.add(PmeBatch.desc(s.nfft, s.order, s.kappa, s.recip, x, s.natom, f, L.FORCE_FLOAT, st,
                                            precision=PRECS[args.precision]), keep=(x, f)))
+        ms = b.autotune(None)  # the plan the benchmark uses
+        print(f"batched: autotuned (brick, tile, wg/cu) = {b.get_tuning()}, {ms:.3f} ms per batched solve", flush=True)
         t0 = time.perf_counter()
         for _ in range(args.solves):
             b.solve()

@@ -254,6 +254,20 @@ pme_status pme_server_destroy(pme_server_t s);
  * with).  PME_ERR_BUDGET if the system could never fit the arena. */
 pme_status pme_server_submit(pme_server_t s, int key, const pme_system_desc* sys, void* ready_event,
                              unsigned long long* ticket);
+/* The direct->recip request header of DomdecRecipComm.h:11-15 (same
+ * layout: three doubles, an int, two one-byte flags).  A recip node serves
+ * each header with a float4 coordinate array in and a float3 force array
+ * out (CudaDomdecRecipLooper.cu:3-37); pme_server_submit_header does that
+ * for a server-side system of fixed grid, order and kappa under `key`.
+ * Energy and virial are computed when either flag is set. */
+typedef struct pme_recip_header {
+  double inv_boxx, inv_boxy, inv_boxz;
+  int ncoord;
+  unsigned char calc_energy, calc_virial;
+} pme_recip_header;
+pme_status pme_server_submit_header(pme_server_t s, int key, const pme_recip_header* h, int nfftx, int nffty,
+                                    int nfftz, int order, double kappa, const void* d_xyzq, void* d_force3,
+                                    unsigned long long* ticket);
 /* Block until the request completes; returns its status.  energy/virial
  * (9)/latency_ms may be NULL.  A ticket's result can be taken once. */
 pme_status pme_server_wait(pme_server_t s, unsigned long long ticket, double* energy, double* virial,

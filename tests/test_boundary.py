@@ -140,3 +140,13 @@ def test_serving_schedule():
     for r in new[:200]:
         assert abs(r.natom / r.box ** 3 - 0.1) < 1e-9 and r.nfft >= r.box and systems.smooth235_at_least(r.box) == r.nfft
     assert len(serving.schedule(1e9, max_requests=7)) == 7
+
+
+def test_recip_header_layout():
+    import struct
+    # DomdecRecipComm.h:11-15: double x3, int, bool, bool (natural alignment: 32 bytes)
+    assert C.sizeof(L.RecipHeader) == 32
+    raw = struct.pack("<dddi??", 1 / 62.23, 1 / 62.0, 1 / 61.0, 23558, True, False)
+    h = L.RecipHeader.from_buffer_copy(raw.ljust(32, b"\0"))
+    assert (h.ncoord, h.calc_energy, h.calc_virial) == (23558, 1, 0)
+    assert abs(h.inv_boxx - 1 / 62.23) < 1e-18

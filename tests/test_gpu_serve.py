@@ -9,7 +9,7 @@ import oracle as O
 from gpu_recip_amd import _lib as L
 from gpu_recip_amd import serving
 from gpu_recip_amd.pme import PmeServer
-from gpu_util import rel_err, rel_rms, run_solve, say, torch_dev
+from gpu_util import rel_err, rel_rms, run_solve, say, to_dev, torch_dev
 
 pytestmark = pytest.mark.gpu
 
@@ -127,4 +127,29 @@ def test_open_loop_short():
     assert st["completed"] == len(reqs) and st["failed"] == 0
     assert 0 < st["p50_ms"] <= st["p99_ms"] <= st["max_ms"]
     assert all(r.result is not None and np.isfinite(r.result[0]) for r in reqs)
+    srv.close()
+
+
+def test_direct_recip_header_requests():
+    """Requests in the direct->recip wire format (header + float4 coordinates
+    in, float3 forces out) give the plain solver's float3 forces."""
+    import struct
+
+    torch, dev = torch_dev()
+    from gpu_recip_amd import systems
+
+    srv = PmeServer(1 << 30, nlanes=2)
+    s = systems.config_plain_small()
+    x = to_dev(s.xyzq)
+    f3 = torch.zeros(3 * s.natom, dtype=torch.float32, device=dev)
+    hdr = struct.pack("<dddi??", 1 / s.box[0], 1 / s.box[1], 1 / s.box[2], s.natom, True, True)
+    t = srv.submit_header(7, hdr, s.nfft, s.order, s.kappa, x, f3)
+    e, v, _ = srv.wait(t)
+    one = run_solve(s.nfft, s.order, s.kappa, s.box, s.xyzq, force_type=L.FORCE_FLOAT3)
+    assert np.array_equal(f3.cpu().numpy().reshape(-1, 3).astype(np.float64), one["force"])
+    assert rel_err(e, one["energy"]) < 1e-12
+    # forces only: energy/virial flags off
+    hdr = struct.pack("<dddi??", 1 / s.box[0], 1 / s.box[1], 1 / s.box[2], s.natom, False, False)
+    e, v, _ = srv.wait(srv.submit_header(7, hdr, s.nfft, s.order, s.kappa, x, f3))
+    assert e == 0.0 and v == [0.0] * 9
     srv.close()
