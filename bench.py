@@ -68,6 +68,29 @@ def stage_bytes(s):
     }
 
 
+# kernel behind each timing slot of the batched solve (order-4 systems whose
+# xy planes fit one workgroup, as in the 64-system set)
+BATCH_KERNELS = {"bin": "pme::k_batch_bin", "spread": "void pme::k_batch_spread<4>",
+                 "fft_fwd_x_or_xy": "void pme::k_batch_plane<0>", "fft_z_solve": "void pme::k_batch_cols<2, 2>",
+                 "fft_inv_y_or_xy": "void pme::k_batch_plane<1>", "gather": "void pme::k_batch_gather<4>"}
+
+
+def measured_traffic(kernel, workload="batched"):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE summary (copy-calibrated), or None."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        cells = [c.strip().strip("`") for c in line.strip().strip("|").split("|")]
+        if len(cells) == 5 and cells[0] == kernel and cells[1] == workload:
+            try:
+                return int(float(cells[4]))
+            except ValueError:
+                return None
+    return None
+
+
 def forces_of(f, n, stride):
     fh = f.cpu().numpy()
     return np.stack([fh[:n], fh[stride:stride + n], fh[2 * stride:2 * stride + n]], 1).astype(np.float64)
@@ -410,7 +433,8 @@ def main():
         "solve_algorithmic_bytes": bytes_b,
         "solve_roofline_frac": bytes_b / (ms_b * 1e-3) / (HBM_PEAK_GBS * 1e9),
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(BATCH_KERNELS.get(kname, "")),
+                     "traffic_source": "profiles/hbm_traffic.md (rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-calibrated)",
                      "algorithmic_bytes_per_launch": sb[kname], "avg_launch_ms": float(stage_ms[k])},
         "served": served,
         "split_precision": split,

@@ -2,7 +2,7 @@
 
     python -m gpu_recip_amd.workloads production [--solves 20] [--precision fp32|bf16x3|fp16x3]
     python -m gpu_recip_amd.workloads split      [--solves 20] [--precision ...]
-    python -m gpu_recip_amd.workloads batched    [--solves 20]
+    python -m gpu_recip_amd.workloads batched    [--solves 20] [--tuning brick,tile,wg_per_cu]
     python -m gpu_recip_amd.workloads small      [--solves 20]
 
 production: 96,000 atoms, 108x100x96, order 6 (plain five-call path)
@@ -25,6 +25,8 @@ def main(argv=None):
     ap.add_argument("workload", choices=["production", "split", "batched", "small"])
     ap.add_argument("--solves", type=int, default=20)
     ap.add_argument("--precision", choices=sorted(PRECS), default="fp32")
+    ap.add_argument("--tuning", default=None,
+                    help="batched: brick,tile,wg_per_cu to use instead of autotuning (e.g. 24,32,0)")
     args = ap.parse_args(argv)
 
     import torch
@@ -44,8 +46,13 @@ def main(argv=None):
             f = torch.zeros(3 * st, dtype=torch.float32, device=dev)
             ids.append(b.add(PmeBatch.desc(s.nfft, s.order, s.kappa, s.recip, x, s.natom, f, L.FORCE_FLOAT, st,
                                            precision=PRECS[args.precision]), keep=(x, f)))
-        ms = b.autotune(None)  # the plan the benchmark uses
-        print(f"batched: autotuned (brick, tile, wg/cu) = {b.get_tuning()}, {ms:.3f} ms per batched solve", flush=True)
+        if args.tuning:
+            b.set_tuning(*(int(v) for v in args.tuning.split(",")))
+            print(f"batched: plan (brick, tile, wg/cu) = {b.get_tuning()}", flush=True)
+        else:
+            ms = b.autotune(None)  # the plan the benchmark uses
+            print(f"batched: autotuned (brick, tile, wg/cu) = {b.get_tuning()}, {ms:.3f} ms per batched solve",
+                  flush=True)
         t0 = time.perf_counter()
         for _ in range(args.solves):
             b.solve()
