@@ -406,8 +406,9 @@ __device__ __forceinline__ float eterm_at(const SolveArgs& s, int k1, int k2, in
   mh2 = s.r2 * (float)m2;
   mh3 = s.r3 * (float)m3;
   const float msq = mh1 * mh1 + mh2 * mh2 + mh3 * mh3;
-  msq_inv = 1.0f / msq;
-  return expf(-s.fac * msq) * s.piv_inv * s.pf1[k1] * s.pf2[k2] * s.pf3[k3] * msq_inv;
+  // hardware reciprocal and exponential (1-2 ulp), as the reference's fast-math build
+  msq_inv = __builtin_amdgcn_rcpf(msq);
+  return __expf(-s.fac * msq) * s.piv_inv * s.pf1[k1] * s.pf2[k2] * s.pf3[k3] * msq_inv;
 }
 
 // Multiply one half-spectrum point by eterm; accumulate energy/virial with

@@ -120,10 +120,15 @@ FftTiles pick_fft_tiles(int nx, int ny, int nz, int nf_x, int tile_target) {
   const int hx = nx / 2 + 1;
   t.NR = nf_x <= 64 ? 16 : (nf_x <= 128 ? 8 : (nf_x <= 256 ? 4 : 2));
   while (t.NR > 1 && fft_x_lds_bytes(nf_x, t.NR, nx) > kLdsBudget) t.NR /= 2;
+  // column tiles of equal width covering hx without padding: the fewest
+  // tiles of at most tile_target columns whose LDS fits the budget
   auto tile = [&](int n) {
-    int tx = tile_target;
-    while (tx > 1 && fft_cols_lds_bytes(n, tx) > kLdsBudget) tx /= 2;
-    if (tx > hx) tx = hx;
+    int ntile = (hx + tile_target - 1) / tile_target;
+    int tx = (hx + ntile - 1) / ntile;
+    while (tx > 1 && fft_cols_lds_bytes(n, tx) > kLdsBudget) {
+      ntile++;
+      tx = (hx + ntile - 1) / ntile;
+    }
     return tx;
   };
   t.TXy = tile(ny);

@@ -157,7 +157,15 @@ __device__ __forceinline__ void spread_brick_body(const BinArgs& a, int b, int w
   for (int i = tid; i < acc_n; i += kSpreadBlock) acc[i] = 0ull;
   const int total = brick_segments<kSpreadBlock>(a, b, s_pre, s_base, sh_scan);  // ends with a barrier
 
-  for (int v = tid; v < total; v += kSpreadBlock) {
+  for (int base = 0; base < total; base += kSpreadBlock) {
+    // records of a batch in a strided order (stride 17, coprime with the
+    // batch size): consecutive records are often atoms of one molecule with
+    // the same stencil rows, and integer sums do not depend on the order
+    const int nbat = min(kSpreadBlock, total - base);
+    if (tid >= nbat) break;
+    const int stride = (nbat % 17) ? 17 : ((nbat % 19) ? 19 : 23);
+    const int lt = tid * stride;
+    const int v = base + (lt - mul24(sdiv(lt, small_div(nbat)), nbat));
     const float4 p = a.rec_pos[brick_record(a, v, s_pre, s_base)];
     int ix, iy, iz;
     float fx, fy, fz;
