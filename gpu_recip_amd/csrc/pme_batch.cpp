@@ -303,7 +303,7 @@ static pme_status build_lists(pme_batch_t b, const std::vector<int>& ids) {
     orders.push_back(b->slots[id].in.order);
   }
   std::vector<WorkItem> items;
-  build_items(descs.data(), plans.data(), orders.data(), (int)order_ids.size(), items, b->launches);
+  build_items(descs.data(), plans.data(), orders.data(), (int)order_ids.size(), b->ncu, items, b->launches);
   for (WorkItem& w : items) w.sys = order_ids[w.sys];
   BHIP(hipStreamSynchronize(b->stream));
   if (items.size() > b->items_cap) {

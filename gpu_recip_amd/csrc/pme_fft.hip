@@ -183,34 +183,57 @@ __global__ __launch_bounds__(kBlock) void k_fft_x_c2r(const float2* __restrict__
 }
 
 // ------------------------------------------------------------- xy planes --
-// One workgroup transforms one z plane through x and y in LDS, so the plane is
-// read once and written once with contiguous accesses:
+// One workgroup transforms np (1 or 2) consecutive z planes through x and y
+// in LDS, so each plane is read once and written once with contiguous
+// accesses:
 //   forward: real rows --x R2C--> half-spectrum rows, transposed in LDS to
-//            columns --y FFT--> the plane of the half spectrum;
+//            columns --y FFT--> the planes of the half spectrum;
 //   inverse: the reverse (y inverse, C2R pre-processing, x inverse).
+// The rows of the np planes are one batch of np*ny x lines (row r of plane p
+// is line p*ny + r, the order they have in memory), and their columns one
+// batch of np*hx y lines (column k of plane p is line p*hx + k), so two
+// planes share every FFT stage and barrier.
 // LDS: twiddles of x and y, two buffers of plane_buf() points.  x lines
-// (rows) have point stride LSX, y lines (columns kx) point stride LSY, both
-// odd so that 8-byte accesses of consecutive lines hit distinct bank pairs.
+// have point stride LSX, y lines point stride LSY, both odd so that 8-byte
+// accesses of consecutive lines hit distinct bank pairs.
 
 __host__ __device__ constexpr int odd_pad(int n) { return (n + 1) | 1; }
 
-__host__ __device__ constexpr int plane_buf(int nx, int ny, int nf) {
-  return (nf * odd_pad(ny) > ny * odd_pad(nx / 2 + 1)) ? nf * odd_pad(ny) : ny * odd_pad(nx / 2 + 1);
+__host__ __device__ constexpr int plane_buf(int nx, int ny, int nf, int np) {
+  return (nf * odd_pad(np * ny) > ny * odd_pad(np * (nx / 2 + 1))) ? nf * odd_pad(np * ny)
+                                                                    : ny * odd_pad(np * (nx / 2 + 1));
 }
 
-size_t fft_plane_lds_bytes(int nx, int ny, int nf_x) {
-  const size_t b = sizeof(float2) * ((size_t)nx + ny + 2 * (size_t)plane_buf(nx, ny, nf_x));
+__host__ __device__ constexpr size_t plane_lds_need(int nx, int ny, int nf, int np) {
+  return sizeof(float2) * ((size_t)nx + ny + 2 * (size_t)plane_buf(nx, ny, nf, np));
+}
+
+size_t fft_plane_lds_bytes(int nx, int ny, int nf_x, int np) {
+  const size_t b = plane_lds_need(nx, ny, nf_x, np);
   return b <= kPlaneLdsBudget ? b : 0;
 }
 
+// planes per workgroup of a batched plane launch allowing up to maxnp: the
+// largest group whose buffers fit (the same on host and device)
+__host__ __device__ inline int plane_group(int nx, int ny, int nf, int maxnp) {
+  int g = maxnp;
+  while (g > 1 && plane_lds_need(nx, ny, nf, g) > kPlaneLdsBudget) g--;
+  return g;
+}
+
+int plane_group_host(int nx, int ny, int nf, int maxnp) { return plane_group(nx, ny, nf, maxnp); }
+
+// planes z .. z+np-1 of the real grid -> the same planes of the half spectrum
 __device__ __forceinline__ void fft_plane_fwd_body(const float* __restrict__ grid, float2* __restrict__ cg, int nx,
-                                                   int ny, const AxisArgs& axx, const AxisArgs& axy, int z,
+                                                   int ny, const AxisArgs& axx, const AxisArgs& axy, int z, int np,
                                                    float2* sm) {
   const int tid = threadIdx.x;
   const int nf = axx.rp.n;  // x line length: nx/2 (even nx) or nx
   const int hx = nx / 2 + 1;
-  const int LSX = odd_pad(ny), LSY = odd_pad(hx);
-  const int bsz = plane_buf(nx, ny, nf);
+  const int nrow = np * ny;   // x lines
+  const int ncol = np * hx;   // y lines
+  const int LSX = odd_pad(nrow), LSY = odd_pad(ncol);
+  const int bsz = plane_buf(nx, ny, nf, np);
   float2* twx = sm;
   float2* twy = twx + nx;
   float2* b0 = twy + ny;
@@ -221,7 +244,7 @@ __device__ __forceinline__ void fft_plane_fwd_body(const float* __restrict__ gri
   const size_t plane = (size_t)z * nx * ny;
   if (even) {
     const float2* src = reinterpret_cast<const float2*>(grid + plane);
-    const int ne = nf * ny;
+    const int ne = nf * nrow;
     for (int e0 = tid; e0 < ne; e0 += kStageU * kPlaneBlock) {
       float2 v[kStageU];
 #pragma unroll
@@ -240,7 +263,7 @@ __device__ __forceinline__ void fft_plane_fwd_body(const float* __restrict__ gri
     }
   } else {
     const float* src = grid + plane;
-    const int ne = nx * ny;
+    const int ne = nx * nrow;
     for (int e0 = tid; e0 < ne; e0 += kStageU * kPlaneBlock) {
       float v[kStageU];
 #pragma unroll
@@ -259,44 +282,50 @@ __device__ __forceinline__ void fft_plane_fwd_body(const float* __restrict__ gri
     }
   }
   __syncthreads();
-  float2* res = fft_lines<-1>(b0, b1, twx, axx.rp, ny, LSX, tid, kPlaneBlock);
+  float2* res = fft_lines<-1>(b0, b1, twx, axx.rp, nrow, LSX, tid, kPlaneBlock);
   float2* q = res == b0 ? b1 : b0;
   // R2C post-processing (even nx) and transpose to columns:
   // X[k] = E[k] + w^k O[k], E = (Z[k] + Z*[h-k])/2, O = (Z[k] - Z*[h-k])/(2i)
-  for (int e = tid; e < ny * hx; e += kPlaneBlock) {
-    const int y = sdiv(e, small_div(hx));
-    const int k = e - y * hx;
+  for (int e = tid; e < nrow * hx; e += kPlaneBlock) {
+    const int r = sdiv(e, small_div(hx));  // x line (row of the plane group)
+    const int k = e - r * hx;
+    const int pl = sdiv(r, small_div(ny));
+    const int y = r - pl * ny;
     float2 X;
     if (even) {
       const int h = nf;
-      const float2 zk = res[(k == h ? 0 : k) * LSX + y];
-      const float2 zc = cconj(res[(k == 0 ? 0 : h - k) * LSX + y]);
+      const float2 zk = res[(k == h ? 0 : k) * LSX + r];
+      const float2 zc = cconj(res[(k == 0 ? 0 : h - k) * LSX + r]);
       const float2 E = cscale(cadd(zk, zc), 0.5f);
       const float2 D = csub(zk, zc);
       const float2 O = make_float2(0.5f * D.y, -0.5f * D.x);
       X = cadd(E, cmul(twx[k], O));
     } else {
-      X = res[k * LSX + y];
+      X = res[k * LSX + r];
     }
-    q[y * LSY + k] = X;
+    q[y * LSY + pl * hx + k] = X;
   }
   __syncthreads();
-  float2* r2 = fft_lines<-1>(q, res, twy, axy.rp, hx, LSY, tid, kPlaneBlock, axy.prec);
+  float2* r2 = fft_lines<-1>(q, res, twy, axy.rp, ncol, LSY, tid, kPlaneBlock, axy.prec);
   float2* dst = cg + (size_t)z * ny * hx;
-  for (int e = tid; e < ny * hx; e += kPlaneBlock) {
-    const int y = sdiv(e, small_div(hx));
-    dst[e] = r2[y * LSY + (e - y * hx)];
+  for (int e = tid; e < nrow * hx; e += kPlaneBlock) {
+    const int r = sdiv(e, small_div(hx));
+    const int pl = sdiv(r, small_div(ny));
+    dst[e] = r2[(r - pl * ny) * LSY + pl * hx + (e - r * hx)];
   }
 }
 
+// planes z .. z+np-1 of the half spectrum -> the same planes of the real grid
 __device__ __forceinline__ void fft_plane_inv_body(const float2* __restrict__ cg, float* __restrict__ grid, int nx,
-                                                   int ny, const AxisArgs& axx, const AxisArgs& axy, int z,
+                                                   int ny, const AxisArgs& axx, const AxisArgs& axy, int z, int np,
                                                    float2* sm) {
   const int tid = threadIdx.x;
   const int nf = axx.rp.n;
   const int hx = nx / 2 + 1;
-  const int LSX = odd_pad(ny), LSY = odd_pad(hx);
-  const int bsz = plane_buf(nx, ny, nf);
+  const int nrow = np * ny;
+  const int ncol = np * hx;
+  const int LSX = odd_pad(nrow), LSY = odd_pad(ncol);
+  const int bsz = plane_buf(nx, ny, nf, np);
   float2* twx = sm;
   float2* twy = twx + nx;
   float2* b0 = twy + ny;
@@ -304,7 +333,7 @@ __device__ __forceinline__ void fft_plane_inv_body(const float2* __restrict__ cg
   for (int i = tid; i < nx; i += kPlaneBlock) twx[i] = axx.tw[i];
   for (int i = tid; i < ny; i += kPlaneBlock) twy[i] = axy.tw[i];
   const float2* src = cg + (size_t)z * ny * hx;
-  const int ne = ny * hx;
+  const int ne = nrow * hx;
   for (int e0 = tid; e0 < ne; e0 += kStageU * kPlaneBlock) {
     float2 v[kStageU];
 #pragma unroll
@@ -316,47 +345,52 @@ __device__ __forceinline__ void fft_plane_inv_body(const float2* __restrict__ cg
     for (int u = 0; u < kStageU; u++) {
       const int e = e0 + u * kPlaneBlock;
       if (e < ne) {
-        const int y = sdiv(e, small_div(hx));
-        b0[y * LSY + (e - y * hx)] = v[u];
+        const int r = sdiv(e, small_div(hx));
+        const int pl = sdiv(r, small_div(ny));
+        b0[(r - pl * ny) * LSY + pl * hx + (e - r * hx)] = v[u];
       }
     }
   }
   __syncthreads();
-  float2* r = fft_lines<+1>(b0, b1, twy, axy.rp, hx, LSY, tid, kPlaneBlock, axy.prec);
+  float2* r = fft_lines<+1>(b0, b1, twy, axy.rp, ncol, LSY, tid, kPlaneBlock, axy.prec);
   float2* p = r == b0 ? b1 : b0;
   const bool even = (nx & 1) == 0;
   if (even) {
     // Z[k] = (X[k] + X*[h-k]) + i w^{-k} (X[k] - X*[h-k]),  k < h
     const int h = nf;
-    for (int e = tid; e < ny * h; e += kPlaneBlock) {
-      const int y = sdiv(e, small_div(h));
-      const int k = e - y * h;
-      const float2 xk = r[y * LSY + k];
-      const float2 xc = cconj(r[y * LSY + (h - k)]);
+    for (int e = tid; e < nrow * h; e += kPlaneBlock) {
+      const int row = sdiv(e, small_div(h));
+      const int k = e - row * h;
+      const int pl = sdiv(row, small_div(ny));
+      const float2* col = r + (row - pl * ny) * LSY + pl * hx;
+      const float2 xk = col[k];
+      const float2 xc = cconj(col[h - k]);
       const float2 A = cadd(xk, xc);
       const float2 B = cmul(cconj(twx[k]), csub(xk, xc));
-      p[k * LSX + y] = make_float2(A.x - B.y, A.y + B.x);
+      p[k * LSX + row] = make_float2(A.x - B.y, A.y + B.x);
     }
   } else {
     // the full Hermitian spectrum of each row
-    for (int e = tid; e < ny * nx; e += kPlaneBlock) {
-      const int y = sdiv(e, small_div(nx));
-      const int k = e - y * nx;
-      p[k * LSX + y] = k < hx ? r[y * LSY + k] : cconj(r[y * LSY + (nx - k)]);
+    for (int e = tid; e < nrow * nx; e += kPlaneBlock) {
+      const int row = sdiv(e, small_div(nx));
+      const int k = e - row * nx;
+      const int pl = sdiv(row, small_div(ny));
+      const float2* col = r + (row - pl * ny) * LSY + pl * hx;
+      p[k * LSX + row] = k < hx ? col[k] : cconj(col[nx - k]);
     }
   }
   __syncthreads();
-  float2* s = fft_lines<+1>(p, r, twx, axx.rp, ny, LSX, tid, kPlaneBlock);
+  float2* s = fft_lines<+1>(p, r, twx, axx.rp, nrow, LSX, tid, kPlaneBlock);
   const size_t plane = (size_t)z * nx * ny;
   if (even) {
     float2* dst = reinterpret_cast<float2*>(grid + plane);
-    for (int e = tid; e < ny * nf; e += kPlaneBlock) {
+    for (int e = tid; e < nrow * nf; e += kPlaneBlock) {
       const int y = sdiv(e, small_div(nf));
       dst[e] = s[(e - y * nf) * LSX + y];
     }
   } else {
     float* dst = grid + plane;
-    for (int e = tid; e < ny * nx; e += kPlaneBlock) {
+    for (int e = tid; e < nrow * nx; e += kPlaneBlock) {
       const int y = sdiv(e, small_div(nx));
       dst[e] = s[(e - y * nx) * LSX + y].x;
     }
@@ -366,7 +400,7 @@ __device__ __forceinline__ void fft_plane_inv_body(const float2* __restrict__ cg
 __global__ __launch_bounds__(kPlaneBlock) void k_fft_plane_fwd(const float* __restrict__ grid, float2* __restrict__ cg,
                                                               int nx, int ny, AxisArgs axx, AxisArgs axy) {
   extern __shared__ __align__(16) float2 sm[];
-  fft_plane_fwd_body(grid, cg, nx, ny, axx, axy, blockIdx.x, sm);
+  fft_plane_fwd_body(grid, cg, nx, ny, axx, axy, blockIdx.x, 1, sm);
 }
 
 __global__ __launch_bounds__(kPlaneBlock) void k_fft_plane_inv(const float2* __restrict__ cg, float* __restrict__ grid,
@@ -374,12 +408,12 @@ __global__ __launch_bounds__(kPlaneBlock) void k_fft_plane_inv(const float2* __r
                                                               EvReduce red) {
   extern __shared__ __align__(16) float2 sm[];
   if ((red.nsolve > 0 || red.nself > 0) && blockIdx.x == 0) ev_reduce_block<kPlaneBlock>(red);
-  fft_plane_inv_body(cg, grid, nx, ny, axx, axy, blockIdx.x, sm);
+  fft_plane_inv_body(cg, grid, nx, ny, axx, axy, blockIdx.x, 1, sm);
 }
 
 hipError_t launch_fft_plane_fwd(const float* grid, float2* cg, int nx, int ny, int nz, const AxisArgs& ax,
                                 const AxisArgs& ay, hipStream_t s) {
-  const size_t lds = fft_plane_lds_bytes(nx, ny, ax.rp.n);
+  const size_t lds = fft_plane_lds_bytes(nx, ny, ax.rp.n, 1);
   if (!lds) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_fft_plane_fwd, dim3(nz), dim3(kPlaneBlock), lds, s, grid, cg, nx, ny, ax, ay);
   return hipGetLastError();
@@ -387,7 +421,7 @@ hipError_t launch_fft_plane_fwd(const float* grid, float2* cg, int nx, int ny, i
 
 hipError_t launch_fft_plane_inv(const float2* cg, float* grid, int nx, int ny, int nz, const AxisArgs& ax,
                                 const AxisArgs& ay, const EvReduce& red, hipStream_t s) {
-  const size_t lds = fft_plane_lds_bytes(nx, ny, ax.rp.n);
+  const size_t lds = fft_plane_lds_bytes(nx, ny, ax.rp.n, 1);
   if (!lds) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_fft_plane_inv, dim3(nz), dim3(kPlaneBlock), lds, s, cg, grid, nx, ny, ax, ay, red);
   return hipGetLastError();
@@ -652,31 +686,35 @@ __global__ __launch_bounds__(kBlock) void k_batch_cols(const SysDesc* __restrict
   }
 }
 
+// items are (system, first plane z); each covers planes z .. z+g-1 with g =
+// plane_group(.., maxnp) of its system (fewer at the top of the grid)
 template <int INV>
 __global__ __launch_bounds__(kPlaneBlock) void k_batch_plane(const SysDesc* __restrict__ sys,
                                                              const WorkItem* __restrict__ items, int nitems,
-                                                             int reduce) {
+                                                             int reduce, int maxnp) {
   extern __shared__ __align__(16) float2 sm[];
   for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
     const WorkItem w = uniform_item(items[it]);
     const SysDesc& d = sys[w.sys];
+    const int npl = min(plane_group(d.bin.nx, d.bin.ny, d.ax[0].rp.n, maxnp), d.bin.nz - w.idx);
     if (INV) {
       if (reduce && w.idx == 0 && d.sa.calc_ev) batch_ev_reduce<kPlaneBlock>(d);
-      fft_plane_inv_body(d.cgrid, d.grid, d.bin.nx, d.bin.ny, d.ax[0], d.ax[1], w.idx, sm);
+      fft_plane_inv_body(d.cgrid, d.grid, d.bin.nx, d.bin.ny, d.ax[0], d.ax[1], w.idx, npl, sm);
     } else {
-      fft_plane_fwd_body(d.grid, d.cgrid, d.bin.nx, d.bin.ny, d.ax[0], d.ax[1], w.idx, sm);
+      fft_plane_fwd_body(d.grid, d.cgrid, d.bin.nx, d.bin.ny, d.ax[0], d.ax[1], w.idx, npl, sm);
     }
     __syncthreads();
   }
 }
 
-hipError_t launch_batch_plane(int inverse, int reduce, const SysDesc* sys, const WorkItem* items, int nitems, int grid,
-                              size_t lds, hipStream_t s) {
+hipError_t launch_batch_plane(int inverse, int reduce, int maxnp, const SysDesc* sys, const WorkItem* items,
+                              int nitems, int grid, size_t lds, hipStream_t s) {
   if (nitems <= 0) return hipSuccess;
+  if (maxnp < 1 || maxnp > kPlaneGroupMax) return hipErrorInvalidValue;
   if (inverse)
-    hipLaunchKernelGGL(k_batch_plane<1>, dim3(grid), dim3(kPlaneBlock), lds, s, sys, items, nitems, reduce);
+    hipLaunchKernelGGL(k_batch_plane<1>, dim3(grid), dim3(kPlaneBlock), lds, s, sys, items, nitems, reduce, maxnp);
   else
-    hipLaunchKernelGGL(k_batch_plane<0>, dim3(grid), dim3(kPlaneBlock), lds, s, sys, items, nitems, reduce);
+    hipLaunchKernelGGL(k_batch_plane<0>, dim3(grid), dim3(kPlaneBlock), lds, s, sys, items, nitems, reduce, maxnp);
   return hipGetLastError();
 }
 

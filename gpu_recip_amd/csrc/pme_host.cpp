@@ -499,7 +499,7 @@ extern "C" pme_status pme_recip_create(int nfftx, int nffty, int nfftz, int orde
         !radix_plan(h->nz, 1, &h->rp[2]))
       return fail(PME_ERR_UNSUPPORTED, "pme_recip_create: no radix plan");
     pick_tiles(h);
-    h->plane = fft_plane_lds_bytes(h->nx, h->ny, h->rp[0].n) != 0;
+    h->plane = fft_plane_lds_bytes(h->nx, h->ny, h->rp[0].n, 1) != 0;
     PME_TRY(upload_prefac(h));
     const int hx = h->nx / 2 + 1;
     h->partial_cap = std::max(cols_blocks(hx, h->TXz, h->ny), scalar_sum_blocks((int)Gc));

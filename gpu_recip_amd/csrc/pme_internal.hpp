@@ -200,12 +200,17 @@ hipError_t launch_fft_plane_fwd(const float* grid, float2* cg, int nx, int ny, i
                                 const AxisArgs& ay, hipStream_t s);
 hipError_t launch_fft_plane_inv(const float2* cg, float* grid, int nx, int ny, int nz, const AxisArgs& ax,
                                 const AxisArgs& ay, const EvReduce& red, hipStream_t s);
-// LDS bytes of a plane kernel (0 when the plane does not fit one workgroup)
-size_t fft_plane_lds_bytes(int nx, int ny, int nf_x);
-// batched plane kernels: items are (system, z plane); with reduce != 0 the
-// plane-0 item of every system assigns that system's energy/virial
-hipError_t launch_batch_plane(int inverse, int reduce, const SysDesc* sys, const WorkItem* items, int nitems, int grid,
-                              size_t lds, hipStream_t s);
+// LDS bytes of a plane kernel over np planes per workgroup (0 when they do
+// not fit one workgroup)
+size_t fft_plane_lds_bytes(int nx, int ny, int nf_x, int np);
+// planes per workgroup of a system in a batched plane launch allowing up to
+// maxnp (the largest group that fits the LDS budget)
+int plane_group_host(int nx, int ny, int nf_x, int maxnp);
+// batched plane kernels: items are (system, first z plane), with
+// plane_group_host(.., maxnp) planes each; with reduce != 0 the plane-0 item
+// of every system assigns that system's energy/virial
+hipError_t launch_batch_plane(int inverse, int reduce, int maxnp, const SysDesc* sys, const WorkItem* items,
+                              int nitems, int grid, size_t lds, hipStream_t s);
 
 // batched launchers: `grid` persistent workgroups loop over the work items
 hipError_t launch_batch_bin(const SysDesc* sys, const WorkItem* items, int nitems, int grid, size_t lds, hipStream_t s);
@@ -230,6 +235,7 @@ size_t fft_cols_lds_bytes(int n, int TX);
 constexpr int kSpreadBlock = 1024;  // threads per spread / gather workgroup (16 waves per CU)
 constexpr int kBinBlock = 1024;     // threads per binning workgroup (one atom each at the default chunk)
 constexpr int kPlaneBlock = 1024;   // threads per xy plane FFT workgroup
+constexpr int kPlaneGroupMax = 2;   // z planes per batched plane workgroup
 constexpr size_t kPlaneLdsBudget = 150 * 1024;  // the plane kernels run one workgroup per CU at most
 constexpr int kMaxChunks = 1024;   // binning workgroups per solve (chunk grows with n)
 constexpr int kMinChunk = 1024;    // atoms per binning workgroup

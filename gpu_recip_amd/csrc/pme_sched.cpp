@@ -53,7 +53,7 @@ pme_status plan_system(const pme_system_desc& in, int cap_atoms, const Tuning& t
     }
   }
   p->ft = pick_fft_tiles(n[0], n[1], n[2], p->rp[0].n, t.fft_tile);
-  p->plane = fft_plane_lds_bytes(n[0], n[1], p->rp[0].n) != 0 ? 1 : 0;
+  p->plane = fft_plane_lds_bytes(n[0], n[1], p->rp[0].n, 1) != 0 ? 1 : 0;
   const int hx = n[0] / 2 + 1;
   p->nzblk = cols_blocks(hx, p->ft.TXz, n[1]);
   const size_t G = (size_t)n[0] * n[1] * n[2];
@@ -180,10 +180,18 @@ pme_status fill_desc(SysDesc& d, const pme_system_desc& in, const SystemPlan& p,
   return PME_OK;
 }
 
-void build_items(const SysDesc* const* descs, const SystemPlan* const* plans, const int* orders, int n,
+void build_items(const SysDesc* const* descs, const SystemPlan* const* plans, const int* orders, int n, int ncu,
                  std::vector<WorkItem>& items, std::vector<Launch>& launches) {
   items.clear();
   launches.clear();
+  // With enough planes to keep every CU busy over several rounds, a plane
+  // workgroup takes two z planes at once (when their buffers fit its LDS):
+  // each FFT stage and barrier then covers twice the butterflies.
+  int total_planes = 0;
+  for (int k = 0; k < n; k++)
+    if (descs[k]->plane) total_planes += descs[k]->bin.nz;
+  const int group = total_planes >= 4 * ncu ? kPlaneGroupMax : 1;
+  // order: the spread/gather order, or the largest plane group of a plane launch
   auto add_stage = [&](int stage, int order) {
     Launch L{stage, order, (int)items.size(), 0, 0};
     for (int k = 0; k < n; k++) {
@@ -216,11 +224,14 @@ void build_items(const SysDesc* const* descs, const SystemPlan* const* plans, co
           cnt = cols_blocks(hx, d.TXy, d.bin.nz);
           lds = fft_cols_lds_bytes(d.bin.ny, d.TXy);
           break;
-        case BS_PLANE:
+        case BS_PLANE: {
           if (!d.plane) break;
-          cnt = d.bin.nz;
-          lds = fft_plane_lds_bytes(d.bin.nx, d.bin.ny, d.ax[0].rp.n);
+          const int g = plane_group_host(d.bin.nx, d.bin.ny, d.ax[0].rp.n, order);
+          lds = fft_plane_lds_bytes(d.bin.nx, d.bin.ny, d.ax[0].rp.n, g);
+          for (int z = 0; z < d.bin.nz; z += g) items.push_back(WorkItem{k, z});
+          L.count += (d.bin.nz + g - 1) / g;
           break;
+        }
         case BS_FFTZ:
           cnt = cols_blocks(hx, d.TXz, d.bin.ny);
           lds = fft_cols_lds_bytes(d.bin.nz, d.TXz);
@@ -236,7 +247,7 @@ void build_items(const SysDesc* const* descs, const SystemPlan* const* plans, co
   for (int o : {4, 6, 8}) add_stage(BS_SPREAD, o);
   add_stage(BS_FFTX, 0);
   add_stage(BS_FFTY, 0);
-  add_stage(BS_PLANE, 0);
+  add_stage(BS_PLANE, group);
   add_stage(BS_FFTZ, 0);
   for (int o : {4, 6, 8}) add_stage(BS_GATHER, o);
 }
@@ -279,7 +290,7 @@ pme_status enqueue_stages(const std::vector<Launch>& launches, const SysDesc* d_
             SCHED_HIP(launch_batch_cols(1, inv ? COL_INV : COL_FWD, inv ? 1 : 0, d_sys, it, L.count, grid, L.lds, stream));
             break;
           case BS_PLANE:
-            SCHED_HIP(launch_batch_plane(inv ? 1 : 0, inv ? 1 : 0, d_sys, it, L.count, grid, L.lds, stream));
+            SCHED_HIP(launch_batch_plane(inv ? 1 : 0, inv ? 1 : 0, L.order, d_sys, it, L.count, grid, L.lds, stream));
             break;
           case BS_FFTZ: SCHED_HIP(launch_batch_cols(2, COL_FWD_SOLVE_INV, 0, d_sys, it, L.count, grid, L.lds, stream)); break;
           case BS_GATHER: SCHED_HIP(launch_batch_gather(L.order, d_sys, it, L.count, grid, L.lds, stream)); break;

@@ -59,7 +59,7 @@ void fill_dynamic(SysDesc& d, const pme_system_desc& in, const SystemPlan& p);
 // One launch of a stage over the items [offset, offset + count).
 struct Launch {
   int stage;   // BatchStage
-  int order;   // spread/gather template order (0 otherwise)
+  int order;   // spread/gather template order; largest plane group of a plane launch (0 otherwise)
   int offset;
   int count;
   size_t lds;  // dynamic LDS bytes: max over the launch's systems
@@ -67,7 +67,7 @@ struct Launch {
 
 // Work lists of systems descs[0..n): items refer to the index in descs.
 // Systems are visited in the given order (callers sort by cost).
-void build_items(const SysDesc* const* descs, const SystemPlan* const* plans, const int* orders, int n,
+void build_items(const SysDesc* const* descs, const SystemPlan* const* plans, const int* orders, int n, int ncu,
                  std::vector<WorkItem>& items, std::vector<Launch>& launches);
 
 // Enqueue the eight stages (bin, spread, x, y, z+solve, y inverse with the

@@ -299,7 +299,7 @@ static pme_status launch_lane(pme_server_t s, Lane& L, std::vector<Request>& sel
   }
   std::vector<WorkItem> items;
   std::vector<Launch> launches;
-  build_items(dp.data(), pp.data(), orders.data(), m, items, launches);
+  build_items(dp.data(), pp.data(), orders.data(), m, s->ncu, items, launches);
   pme_status st = grow_lane(L, m, items.size());
   if (st != PME_OK) return st;
   for (int k = 0; k < m; k++) descs[k].ev_out = L.d_ev + 10 * (size_t)k;

@@ -81,6 +81,35 @@ def test_batch_matches_single_and_oracle(graph):
         assert frms < 1e-5 and rel_err(ek, o["energy"]) < 1e-5 and rel_fro(vk, o["virial"]) < 1e-5
 
 
+def test_plane_pairs_match_single():
+    """A batch with at least 4 z planes per CU (1024 on a 256-CU GPU) runs its
+    xy-plane FFTs two planes per workgroup: odd and even nx, odd nz (a last
+    single plane), orders 4/6/8; forces stay bit-identical to the
+    single-system solver, which transforms one plane per workgroup."""
+    torch, dev = torch_dev()
+    base = _mixed_systems()
+    sl = []
+    for rep in range(4):
+        for i, s in enumerate(base):
+            x = systems.triplets(max(1, s.natom // 3), s.box, systems.SEED_BASE + 700 + 10 * rep + i)
+            sl.append(systems.PmeSystem(f"{s.name}r{rep}", x, s.box, s.nfft, s.order, 0.34))
+    assert sum(s.nfft[2] for s in sl) >= 4 * 256
+    b = PmeBatch(8 << 30)
+    reg = [_add(b, s, torch, dev) for s in sl]
+    ids = [r[0] for r in reg]
+    say(f"batch of {len(ids)} systems, {sum(s.nfft[2] for s in sl)} z planes")
+    for graph in (True, False):
+        b.set_graph(graph)
+        b.solve()
+        e, v = b.results(ids)
+        for s, (sid, x, f, stride), ek, vk in zip(sl, reg, e, v):
+            fb = _forces(f, s.natom, stride)
+            one = run_solve(s.nfft, s.order, s.kappa, s.box, s.xyzq)
+            assert np.array_equal(fb, one["force"]), (s.name, graph)
+            assert rel_err(ek, one["energy"]) < 1e-12, s.name
+            assert rel_fro(vk, one["virial"]) < 1e-12, s.name
+
+
 def test_batched_config_vs_oracle():
     """The 64-system batched workload: every system against the single-system
     solver, eight of them against the FP64 oracle (exact coordinates)."""
