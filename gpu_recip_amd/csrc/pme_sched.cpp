@@ -184,13 +184,15 @@ void build_items(const SysDesc* const* descs, const SystemPlan* const* plans, co
                  std::vector<WorkItem>& items, std::vector<Launch>& launches) {
   items.clear();
   launches.clear();
-  // With enough planes to keep every CU busy over several rounds, a plane
-  // workgroup takes two z planes at once (when their buffers fit its LDS):
-  // each FFT stage and barrier then covers twice the butterflies.
+  // A plane workgroup takes up to `group` z planes at once (as many as fit
+  // its LDS): each FFT stage and barrier then covers that many planes'
+  // butterflies.  The group is the largest that still leaves at least two
+  // items per CU.
   int total_planes = 0;
   for (int k = 0; k < n; k++)
     if (descs[k]->plane) total_planes += descs[k]->bin.nz;
-  const int group = total_planes >= 4 * ncu ? kPlaneGroupMax : 1;
+  int group = kPlaneGroupMax;
+  while (group > 1 && total_planes < 2 * ncu * group) group--;
   // order: the spread/gather order, or the largest plane group of a plane launch
   auto add_stage = [&](int stage, int order) {
     Launch L{stage, order, (int)items.size(), 0, 0};

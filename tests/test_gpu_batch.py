@@ -81,19 +81,23 @@ def test_batch_matches_single_and_oracle(graph):
         assert frms < 1e-5 and rel_err(ek, o["energy"]) < 1e-5 and rel_fro(vk, o["virial"]) < 1e-5
 
 
-def test_plane_pairs_match_single():
-    """A batch with at least 4 z planes per CU (1024 on a 256-CU GPU) runs its
-    xy-plane FFTs two planes per workgroup: odd and even nx, odd nz (a last
-    single plane), orders 4/6/8; forces stay bit-identical to the
-    single-system solver, which transforms one plane per workgroup."""
+@pytest.mark.parametrize("reps", [4, 8])
+def test_plane_groups_match_single(reps):
+    """A batch with g z planes per item for at least 2 items per CU runs its
+    xy-plane FFTs g planes per workgroup (g up to 4, as many as fit LDS; on a
+    256-CU GPU 4 copies of the mixed set give g = 2, 8 copies g = 4): odd and
+    even nx, nz not a multiple of g (a shorter last group), orders 4/6/8.
+    Forces stay bit-identical to the single-system solver, which transforms
+    one plane per workgroup."""
     torch, dev = torch_dev()
     base = _mixed_systems()
     sl = []
-    for rep in range(4):
+    for rep in range(reps):
         for i, s in enumerate(base):
             x = systems.triplets(max(1, s.natom // 3), s.box, systems.SEED_BASE + 700 + 10 * rep + i)
             sl.append(systems.PmeSystem(f"{s.name}r{rep}", x, s.box, s.nfft, s.order, 0.34))
-    assert sum(s.nfft[2] for s in sl) >= 4 * 256
+    g = 2 if reps == 4 else 4
+    assert 2 * 256 * g <= sum(s.nfft[2] for s in sl) < 2 * 256 * (g + 1)
     b = PmeBatch(8 << 30)
     reg = [_add(b, s, torch, dev) for s in sl]
     ids = [r[0] for r in reg]
