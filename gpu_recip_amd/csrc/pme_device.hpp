@@ -408,16 +408,18 @@ __device__ __forceinline__ void mfma_dft16_stage(const float2* __restrict__ src,
 
 // Full transform of nlines lines held in LDS (buf0 -> result).  Returns the
 // buffer holding the result (buf0 or buf1).  Ends with a barrier.  With
-// prec != PREC_FP32 and a plan whose first radix is 16, that stage runs on
-// the matrix cores (mfma_dft16_stage).
-template <int S>
+// SPLIT, prec != PREC_FP32 and a plan whose first radix is 16, that stage runs
+// on the matrix cores (mfma_dft16_stage).  Without SPLIT the matrix-core code
+// is not compiled in, which keeps the register budget (and so the occupancy)
+// of FP32-only kernels at what their Stockham stages need.
+template <int S, bool SPLIT = false>
 __device__ __forceinline__ float2* fft_lines(float2* buf0, float2* buf1, const float2* tw, const RadixPlan& rp,
                              int nlines, int ls, int tid, int nthr, int prec = PREC_FP32) {
   float2* src = buf0;
   float2* dst = buf1;
   int p = 1;
   int s0 = 0;
-  if (prec != PREC_FP32 && rp.nrad > 0 && plan_radix(rp, 0) == 16) {
+  if (SPLIT && prec != PREC_FP32 && rp.nrad > 0 && plan_radix(rp, 0) == 16) {
     if (prec == PREC_BF16X3)
       mfma_dft16_stage<S, PREC_BF16X3>(src, dst, rp.n, nlines, ls, tid, nthr);
     else
@@ -466,13 +468,18 @@ __device__ __forceinline__ void block_sum(double* v, double* sh) {
     for (int c = 0; c < NV; c++) sh[wid * NV + c] = v[c];
   }
   __syncthreads();
+  // thread c sums component c over the waves in wave order (thread c alone
+  // reads and then writes sh[c])
+  if (threadIdx.x < NV) {
+    const int c = threadIdx.x;
+    double s = 0.0;
+    for (int w = 0; w < NT / 64; w++) s += sh[w * NV + c];
+    sh[c] = s;
+  }
+  __syncthreads();
   if (threadIdx.x == 0) {
 #pragma unroll
-    for (int c = 0; c < NV; c++) {
-      double s = 0.0;
-      for (int w = 0; w < NT / 64; w++) s += sh[w * NV + c];
-      v[c] = s;
-    }
+    for (int c = 0; c < NV; c++) v[c] = sh[c];
   }
   __syncthreads();
 }

@@ -224,6 +224,8 @@ __host__ __device__ inline int plane_group(int nx, int ny, int nf, int maxnp) {
 int plane_group_host(int nx, int ny, int nf, int maxnp) { return plane_group(nx, ny, nf, maxnp); }
 
 // planes z .. z+np-1 of the real grid -> the same planes of the half spectrum
+// (SPLIT: the y pass may use the split-precision matrix-core stage)
+template <bool SPLIT>
 __device__ __forceinline__ void fft_plane_fwd_body(const float* __restrict__ grid, float2* __restrict__ cg, int nx,
                                                    int ny, const AxisArgs& axx, const AxisArgs& axy, int z, int np,
                                                    float2* sm) {
@@ -306,7 +308,7 @@ __device__ __forceinline__ void fft_plane_fwd_body(const float* __restrict__ gri
     q[y * LSY + pl * hx + k] = X;
   }
   __syncthreads();
-  float2* r2 = fft_lines<-1>(q, res, twy, axy.rp, ncol, LSY, tid, kPlaneBlock, axy.prec);
+  float2* r2 = fft_lines<-1, SPLIT>(q, res, twy, axy.rp, ncol, LSY, tid, kPlaneBlock, axy.prec);
   float2* dst = cg + (size_t)z * ny * hx;
   for (int e = tid; e < nrow * hx; e += kPlaneBlock) {
     const int r = sdiv(e, small_div(hx));
@@ -316,6 +318,7 @@ __device__ __forceinline__ void fft_plane_fwd_body(const float* __restrict__ gri
 }
 
 // planes z .. z+np-1 of the half spectrum -> the same planes of the real grid
+template <bool SPLIT>
 __device__ __forceinline__ void fft_plane_inv_body(const float2* __restrict__ cg, float* __restrict__ grid, int nx,
                                                    int ny, const AxisArgs& axx, const AxisArgs& axy, int z, int np,
                                                    float2* sm) {
@@ -352,7 +355,7 @@ __device__ __forceinline__ void fft_plane_inv_body(const float2* __restrict__ cg
     }
   }
   __syncthreads();
-  float2* r = fft_lines<+1>(b0, b1, twy, axy.rp, ncol, LSY, tid, kPlaneBlock, axy.prec);
+  float2* r = fft_lines<+1, SPLIT>(b0, b1, twy, axy.rp, ncol, LSY, tid, kPlaneBlock, axy.prec);
   float2* p = r == b0 ? b1 : b0;
   const bool even = (nx & 1) == 0;
   if (even) {
@@ -397,25 +400,30 @@ __device__ __forceinline__ void fft_plane_inv_body(const float2* __restrict__ cg
   }
 }
 
+template <bool SPLIT>
 __global__ __launch_bounds__(kPlaneBlock) void k_fft_plane_fwd(const float* __restrict__ grid, float2* __restrict__ cg,
                                                               int nx, int ny, AxisArgs axx, AxisArgs axy) {
   extern __shared__ __align__(16) float2 sm[];
-  fft_plane_fwd_body(grid, cg, nx, ny, axx, axy, blockIdx.x, 1, sm);
+  fft_plane_fwd_body<SPLIT>(grid, cg, nx, ny, axx, axy, blockIdx.x, 1, sm);
 }
 
+template <bool SPLIT>
 __global__ __launch_bounds__(kPlaneBlock) void k_fft_plane_inv(const float2* __restrict__ cg, float* __restrict__ grid,
                                                               int nx, int ny, AxisArgs axx, AxisArgs axy,
                                                               EvReduce red) {
   extern __shared__ __align__(16) float2 sm[];
   if ((red.nsolve > 0 || red.nself > 0) && blockIdx.x == 0) ev_reduce_block<kPlaneBlock>(red);
-  fft_plane_inv_body(cg, grid, nx, ny, axx, axy, blockIdx.x, 1, sm);
+  fft_plane_inv_body<SPLIT>(cg, grid, nx, ny, axx, axy, blockIdx.x, 1, sm);
 }
 
 hipError_t launch_fft_plane_fwd(const float* grid, float2* cg, int nx, int ny, int nz, const AxisArgs& ax,
                                 const AxisArgs& ay, hipStream_t s) {
   const size_t lds = fft_plane_lds_bytes(nx, ny, ax.rp.n, 1);
   if (!lds) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_fft_plane_fwd, dim3(nz), dim3(kPlaneBlock), lds, s, grid, cg, nx, ny, ax, ay);
+  if (ay.prec != PREC_FP32)
+    hipLaunchKernelGGL(k_fft_plane_fwd<true>, dim3(nz), dim3(kPlaneBlock), lds, s, grid, cg, nx, ny, ax, ay);
+  else
+    hipLaunchKernelGGL(k_fft_plane_fwd<false>, dim3(nz), dim3(kPlaneBlock), lds, s, grid, cg, nx, ny, ax, ay);
   return hipGetLastError();
 }
 
@@ -423,7 +431,10 @@ hipError_t launch_fft_plane_inv(const float2* cg, float* grid, int nx, int ny, i
                                 const AxisArgs& ay, const EvReduce& red, hipStream_t s) {
   const size_t lds = fft_plane_lds_bytes(nx, ny, ax.rp.n, 1);
   if (!lds) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_fft_plane_inv, dim3(nz), dim3(kPlaneBlock), lds, s, cg, grid, nx, ny, ax, ay, red);
+  if (ay.prec != PREC_FP32)
+    hipLaunchKernelGGL(k_fft_plane_inv<true>, dim3(nz), dim3(kPlaneBlock), lds, s, cg, grid, nx, ny, ax, ay, red);
+  else
+    hipLaunchKernelGGL(k_fft_plane_inv<false>, dim3(nz), dim3(kPlaneBlock), lds, s, cg, grid, nx, ny, ax, ay, red);
   return hipGetLastError();
 }
 
@@ -439,10 +450,15 @@ __device__ __forceinline__ float eterm_at(const SolveArgs& s, int k1, int k2, in
   mh1 = s.r1 * (float)m1;
   mh2 = s.r2 * (float)m2;
   mh3 = s.r3 * (float)m3;
-  const float msq = mh1 * mh1 + mh2 * mh2 + mh3 * mh3;
+  // grouped as in solve_lines, which takes the (k1, k2) factors out of its
+  // loop over k3, so both give the same value; the fused multiply-adds are
+  // explicit so that every kernel rounds them alike
+  const float s12 = fmaf(mh2, mh2, mh1 * mh1);
+  const float msq = fmaf(mh3, mh3, s12);
+  const float p12 = s.piv_inv * s.pf1[k1] * s.pf2[k2];
   // hardware reciprocal and exponential (1-2 ulp), as the reference's fast-math build
   msq_inv = __builtin_amdgcn_rcpf(msq);
-  return __expf(-s.fac * msq) * s.piv_inv * s.pf1[k1] * s.pf2[k2] * s.pf3[k3] * msq_inv;
+  return __expf(-s.fac * msq) * p12 * s.pf3[k3] * msq_inv;
 }
 
 // Multiply one half-spectrum point by eterm; accumulate energy/virial with
@@ -456,7 +472,7 @@ __device__ __forceinline__ float2 solve_point(const SolveArgs& s, float2 q, int 
   float mh1, mh2, mh3, msq_inv;
   const float eterm = eterm_at(s, k1, k2, k3, mh1, mh2, mh3, msq_inv);
   if (s.calc_ev) {
-    const float q2 = q.x * q.x + q.y * q.y;
+    const float q2 = fmaf(q.y, q.y, q.x * q.x);
     const float tmp1 = eterm * q2;
     const float vterm = 2.0f * (s.fac + msq_inv);
     const float tmp2 = tmp1 * vterm;
@@ -478,6 +494,82 @@ __device__ __forceinline__ float2 solve_point(const SolveArgs& s, float2 q, int 
   return cscale(q, eterm);
 }
 
+// The influence function over the z lines of a column tile (in LDS, point j
+// of line l at res[j * TX + l]; lines l < nl are kx = kx0 + l at ky = k2).
+// Thread t owns line t mod TX and visits k3 = t / TX, t / TX + kBlock / TX,
+// ..., so the (k1, k2) factors of eterm and of the energy/virial terms are
+// formed once per thread: scalar_sum_ortho_kernel CudaPMERecip.cu:671-747
+// per point, with the sums over k3 of one line taken before the line's
+// constant factors are applied.  Per line, with the point weight w (2 when
+// k1 >= 1: the point and its mirror), tmp1 = eterm |Q|^2, tmp2 = tmp1 vterm:
+//   E   += w sum tmp1                  V11 += w (mh1^2 sum tmp2 - sum tmp1)
+//   V22 += w (mh2^2 sum tmp2 - sum tmp1)  V33 += w (sum tmp2 mh3^2 - sum tmp1)
+//   V12 += w12 mh1 mh2 sum tmp2      V13 += mh1 sum w13 tmp2 mh3
+//   V23 += mh2 sum w23 tmp2 mh3
+// (the off-diagonal mirror weights are those of solve_point; w13 and w23
+// depend on k3 through the zero/Nyquist test of m3).
+__device__ __forceinline__ void solve_lines(const SolveArgs& s, float2* __restrict__ res, int n, int TX, int nl,
+                                            int kx0, int k2, double* acc) {
+  const int tid = threadIdx.x;
+  const SmallDiv dtx = small_div(TX);
+  const int j0 = sdiv(tid, dtx);
+  const int l = tid - j0 * TX;
+  const int jstep = sdiv(kBlock, dtx);  // >= 4: column tiles are at most 64 wide
+  static_assert(kBlock >= 4 * 64, "solve_lines: every line needs a thread");
+  if (j0 >= jstep || l >= nl) return;
+  const int k1 = kx0 + l;
+  const int m1 = k1 >= s.nf1 ? k1 - s.n1 : k1;
+  const int m2 = k2 >= s.nf2 ? k2 - s.n2 : k2;
+  const float mh1 = s.r1 * (float)m1;
+  const float mh2 = s.r2 * (float)m2;
+  const float s12 = fmaf(mh2, mh2, mh1 * mh1);
+  const float p12 = s.piv_inv * s.pf1[k1] * s.pf2[k2];
+  const bool f1 = !(2 * k1 == s.n1), f2 = !(2 * k2 == s.n2 || k2 == 0);
+  const bool k1pos = k1 >= 1;
+  double sA = 0.0, sB = 0.0, sC = 0.0, sD13 = 0.0, sD23 = 0.0;
+  for (int j = j0; j < n; j += jstep) {
+    float2* pt = res + mul24(j, TX) + l;
+    if ((k1 | k2 | j) == 0) {
+      *pt = make_float2(0.f, 0.f);
+      continue;
+    }
+    const int m3 = j >= s.nf3 ? j - s.n3 : j;
+    const float mh3 = s.r3 * (float)m3;
+    const float msq = fmaf(mh3, mh3, s12);
+    const float msq_inv = __builtin_amdgcn_rcpf(msq);
+    const float eterm = __expf(-s.fac * msq) * p12 * s.pf3[j] * msq_inv;
+    const float2 q = *pt;
+    if (s.calc_ev) {
+      const float q2 = fmaf(q.y, q.y, q.x * q.x);
+      const float tmp1 = eterm * q2;
+      const float vterm = 2.0f * (s.fac + msq_inv);
+      const float tmp2 = tmp1 * vterm;
+      const float t3 = tmp2 * mh3;
+      const bool f3 = !(2 * j == s.n3 || j == 0);
+      const float w13 = k1pos ? (f1 == f3 ? 2.f : 0.f) : 1.f;
+      const float w23 = k1pos ? (f2 == f3 ? 2.f : 0.f) : 1.f;
+      sA += (double)tmp1;
+      sB += (double)tmp2;
+      sC += (double)(t3 * mh3);
+      sD13 += (double)(w13 * t3);
+      sD23 += (double)(w23 * t3);
+    }
+    *pt = cscale(q, eterm);
+  }
+  if (s.calc_ev) {
+    const double w = k1pos ? 2.0 : 1.0;
+    const double w12 = k1pos ? (f1 == f2 ? 2.0 : 0.0) : 1.0;
+    const double a1 = (double)mh1, a2 = (double)mh2;
+    acc[0] += w * sA;
+    acc[1] += w * (a1 * a1 * sB - sA);
+    acc[2] += w12 * a1 * a2 * sB;
+    acc[3] += a1 * sD13;
+    acc[4] += w * (a2 * a2 * sB - sA);
+    acc[5] += a2 * sD23;
+    acc[6] += w * (sC - sA);
+  }
+}
+
 // Per-workgroup energy/virial partial (fixed summation order inside the
 // workgroup).  The total is formed later by ev_reduce_block in one workgroup.
 __device__ __forceinline__ void ev_partial(const SolveArgs& s, double* acc, double* sh, int blk) {
@@ -492,7 +584,7 @@ __device__ __forceinline__ void ev_partial(const SolveArgs& s, double* acc, doub
 
 // Column tile (tx, ty) of one grid: lines kx0 = tx*TX ..., outer index ty
 // (ky for the z pass); blk indexes the energy/virial partial of the tile.
-template <int MODE>
+template <int MODE, bool SPLIT>
 __device__ __forceinline__ void fft_cols_body(float2* __restrict__ cg, int n, int hx, int TX, long long pt_stride,
                                               long long outer_stride, const AxisArgs& ax, const SolveArgs& sa,
                                               int tx, int ty, int blk, float2* sm) {
@@ -523,22 +615,17 @@ __device__ __forceinline__ void fft_cols_body(float2* __restrict__ cg, int n, in
   __syncthreads();
   float2* res;
   if (MODE == COL_INV) {
-    res = fft_lines<+1>(b0, b1, tw, ax.rp, TX, TX, tid, kBlock, ax.prec);
+    res = fft_lines<+1, SPLIT>(b0, b1, tw, ax.rp, TX, TX, tid, kBlock, ax.prec);
   } else {
-    res = fft_lines<-1>(b0, b1, tw, ax.rp, TX, TX, tid, kBlock, ax.prec);
+    res = fft_lines<-1, SPLIT>(b0, b1, tw, ax.rp, TX, TX, tid, kBlock, ax.prec);
   }
   double acc[7] = {0, 0, 0, 0, 0, 0, 0};
   if (MODE == COL_FWD_SOLVE_INV) {
-    const int k2 = ty;
-    for (int e = tid; e < n * TX; e += kBlock) {
-      const int j = sdiv(e, small_div(TX));
-      const int l = e - j * TX;
-      if (l < nl) res[e] = solve_point(sa, res[e], kx0 + l, k2, j, acc);
-    }
+    solve_lines(sa, res, n, TX, nl, kx0, ty, acc);
     __syncthreads();
   }
   if (MODE == COL_FWD_SOLVE_INV || MODE == COL_FWD_INV) {
-    res = fft_lines<+1>(res, res == b0 ? b1 : b0, tw, ax.rp, TX, TX, tid, kBlock, ax.prec);
+    res = fft_lines<+1, SPLIT>(res, res == b0 ? b1 : b0, tw, ax.rp, TX, TX, tid, kBlock, ax.prec);
   }
   for (int e = tid; e < n * TX; e += kBlock) {
     const int j = sdiv(e, small_div(TX));
@@ -548,13 +635,13 @@ __device__ __forceinline__ void fft_cols_body(float2* __restrict__ cg, int n, in
   if (MODE == COL_FWD_SOLVE_INV && sa.calc_ev) ev_partial(sa, acc, sh_red, blk);
 }
 
-template <int MODE>
+template <int MODE, bool SPLIT>
 __global__ __launch_bounds__(kBlock) void k_fft_cols(float2* __restrict__ cg, int n, int hx, int TX,
                                                      long long pt_stride, long long outer_stride, AxisArgs ax,
                                                      SolveArgs sa, EvReduce red) {
   extern __shared__ __align__(16) float2 sm[];
   if ((red.nsolve > 0 || red.nself > 0) && blockIdx.x == 0 && blockIdx.y == 0) ev_reduce_block<kBlock>(red);
-  fft_cols_body<MODE>(cg, n, hx, TX, pt_stride, outer_stride, ax, sa, blockIdx.x, blockIdx.y,
+  fft_cols_body<MODE, SPLIT>(cg, n, hx, TX, pt_stride, outer_stride, ax, sa, blockIdx.x, blockIdx.y,
                       blockIdx.x + gridDim.x * blockIdx.y, sm);
 }
 
@@ -600,22 +687,21 @@ hipError_t launch_fft_cols(int mode, float2* cg, int n, int hx, int TX, long lon
                            hipStream_t s) {
   const size_t lds = fft_cols_lds_bytes(n, TX);
   const dim3 grid((hx + TX - 1) / TX, nouter);
+  const bool split = ax.prec != PREC_FP32;
+#define PME_COLS(M)                                                                                         \
+  if (split)                                                                                              \
+    hipLaunchKernelGGL((k_fft_cols<M, true>), grid, dim3(kBlock), lds, s, cg, n, hx, TX, pt_stride, outer_stride, ax, sa, red); \
+  else                                                                                                    \
+    hipLaunchKernelGGL((k_fft_cols<M, false>), grid, dim3(kBlock), lds, s, cg, n, hx, TX, pt_stride, outer_stride, ax, sa, red);
   switch (mode) {
-    case COL_FWD:
-      hipLaunchKernelGGL(k_fft_cols<COL_FWD>, grid, dim3(kBlock), lds, s, cg, n, hx, TX, pt_stride, outer_stride, ax, sa, red);
-      break;
-    case COL_INV:
-      hipLaunchKernelGGL(k_fft_cols<COL_INV>, grid, dim3(kBlock), lds, s, cg, n, hx, TX, pt_stride, outer_stride, ax, sa, red);
-      break;
-    case COL_FWD_SOLVE_INV:
-      hipLaunchKernelGGL(k_fft_cols<COL_FWD_SOLVE_INV>, grid, dim3(kBlock), lds, s, cg, n, hx, TX, pt_stride, outer_stride, ax, sa, red);
-      break;
-    case COL_FWD_INV:
-      hipLaunchKernelGGL(k_fft_cols<COL_FWD_INV>, grid, dim3(kBlock), lds, s, cg, n, hx, TX, pt_stride, outer_stride, ax, sa, red);
-      break;
+    case COL_FWD: PME_COLS(COL_FWD); break;
+    case COL_INV: PME_COLS(COL_INV); break;
+    case COL_FWD_SOLVE_INV: PME_COLS(COL_FWD_SOLVE_INV); break;
+    case COL_FWD_INV: PME_COLS(COL_FWD_INV); break;
     default:
       return hipErrorInvalidValue;
   }
+#undef PME_COLS
   return hipGetLastError();
 }
 
@@ -667,7 +753,7 @@ __device__ __forceinline__ void batch_ev_reduce(const SysDesc& d) {
   }
 }
 
-template <int AXIS, int MODE>
+template <int AXIS, int MODE, bool SPLIT>
 __global__ __launch_bounds__(kBlock) void k_batch_cols(const SysDesc* __restrict__ sys,
                                                        const WorkItem* __restrict__ items, int nitems, int reduce) {
   extern __shared__ __align__(16) float2 sm[];
@@ -681,14 +767,15 @@ __global__ __launch_bounds__(kBlock) void k_batch_cols(const SysDesc* __restrict
     const long long pt = AXIS == 1 ? (long long)hx : (long long)hx * d.bin.ny;
     const long long outer = AXIS == 1 ? (long long)hx * d.bin.ny : (long long)hx;
     const int ntx = (hx + TX - 1) / TX;
-    fft_cols_body<MODE>(d.cgrid, n, hx, TX, pt, outer, d.ax[AXIS], d.sa, w.idx % ntx, w.idx / ntx, w.idx, sm);
+    fft_cols_body<MODE, SPLIT>(d.cgrid, n, hx, TX, pt, outer, d.ax[AXIS], d.sa, w.idx % ntx, w.idx / ntx, w.idx, sm);
     __syncthreads();
   }
 }
 
 // items are (system, first plane z); each covers planes z .. z+g-1 with g =
-// plane_group(.., maxnp) of its system (fewer at the top of the grid)
-template <int INV>
+// plane_group(.., maxnp) of its system (fewer at the top of the grid).
+// SPLIT: every system of the launch may use the matrix-core stage in y.
+template <int INV, bool SPLIT>
 __global__ __launch_bounds__(kPlaneBlock) void k_batch_plane(const SysDesc* __restrict__ sys,
                                                              const WorkItem* __restrict__ items, int nitems,
                                                              int reduce, int maxnp) {
@@ -699,22 +786,23 @@ __global__ __launch_bounds__(kPlaneBlock) void k_batch_plane(const SysDesc* __re
     const int npl = min(plane_group(d.bin.nx, d.bin.ny, d.ax[0].rp.n, maxnp), d.bin.nz - w.idx);
     if (INV) {
       if (reduce && w.idx == 0 && d.sa.calc_ev) batch_ev_reduce<kPlaneBlock>(d);
-      fft_plane_inv_body(d.cgrid, d.grid, d.bin.nx, d.bin.ny, d.ax[0], d.ax[1], w.idx, npl, sm);
+      fft_plane_inv_body<SPLIT>(d.cgrid, d.grid, d.bin.nx, d.bin.ny, d.ax[0], d.ax[1], w.idx, npl, sm);
     } else {
-      fft_plane_fwd_body(d.grid, d.cgrid, d.bin.nx, d.bin.ny, d.ax[0], d.ax[1], w.idx, npl, sm);
+      fft_plane_fwd_body<SPLIT>(d.grid, d.cgrid, d.bin.nx, d.bin.ny, d.ax[0], d.ax[1], w.idx, npl, sm);
     }
     __syncthreads();
   }
 }
 
-hipError_t launch_batch_plane(int inverse, int reduce, int maxnp, const SysDesc* sys, const WorkItem* items,
-                              int nitems, int grid, size_t lds, hipStream_t s) {
+hipError_t launch_batch_plane(int inverse, int reduce, int maxnp, int split, const SysDesc* sys,
+                              const WorkItem* items, int nitems, int grid, size_t lds, hipStream_t s) {
   if (nitems <= 0) return hipSuccess;
   if (maxnp < 1 || maxnp > kPlaneGroupMax) return hipErrorInvalidValue;
-  if (inverse)
-    hipLaunchKernelGGL(k_batch_plane<1>, dim3(grid), dim3(kPlaneBlock), lds, s, sys, items, nitems, reduce, maxnp);
-  else
-    hipLaunchKernelGGL(k_batch_plane<0>, dim3(grid), dim3(kPlaneBlock), lds, s, sys, items, nitems, reduce, maxnp);
+  const dim3 g(grid), b(kPlaneBlock);
+  if (inverse && split) hipLaunchKernelGGL((k_batch_plane<1, true>), g, b, lds, s, sys, items, nitems, reduce, maxnp);
+  else if (inverse) hipLaunchKernelGGL((k_batch_plane<1, false>), g, b, lds, s, sys, items, nitems, reduce, maxnp);
+  else if (split) hipLaunchKernelGGL((k_batch_plane<0, true>), g, b, lds, s, sys, items, nitems, reduce, maxnp);
+  else hipLaunchKernelGGL((k_batch_plane<0, false>), g, b, lds, s, sys, items, nitems, reduce, maxnp);
   return hipGetLastError();
 }
 
@@ -728,14 +816,20 @@ hipError_t launch_batch_fft_x(int inverse, const SysDesc* sys, const WorkItem* i
   return hipGetLastError();
 }
 
-hipError_t launch_batch_cols(int axis, int mode, int reduce, const SysDesc* sys, const WorkItem* items, int nitems,
-                             int grid, size_t lds, hipStream_t s) {
+hipError_t launch_batch_cols(int axis, int mode, int reduce, int split, const SysDesc* sys, const WorkItem* items,
+                             int nitems, int grid, size_t lds, hipStream_t s) {
   if (nitems <= 0) return hipSuccess;
   const dim3 g(grid), b(kBlock);
-  if (axis == 1 && mode == COL_FWD) hipLaunchKernelGGL((k_batch_cols<1, COL_FWD>), g, b, lds, s, sys, items, nitems, reduce);
-  else if (axis == 1 && mode == COL_INV) hipLaunchKernelGGL((k_batch_cols<1, COL_INV>), g, b, lds, s, sys, items, nitems, reduce);
-  else if (axis == 2 && mode == COL_FWD_SOLVE_INV) hipLaunchKernelGGL((k_batch_cols<2, COL_FWD_SOLVE_INV>), g, b, lds, s, sys, items, nitems, reduce);
+#define PME_BCOLS(A, M)                                                                                 \
+  if (split)                                                                                          \
+    hipLaunchKernelGGL((k_batch_cols<A, M, true>), g, b, lds, s, sys, items, nitems, reduce);         \
+  else                                                                                                \
+    hipLaunchKernelGGL((k_batch_cols<A, M, false>), g, b, lds, s, sys, items, nitems, reduce);
+  if (axis == 1 && mode == COL_FWD) { PME_BCOLS(1, COL_FWD); }
+  else if (axis == 1 && mode == COL_INV) { PME_BCOLS(1, COL_INV); }
+  else if (axis == 2 && mode == COL_FWD_SOLVE_INV) { PME_BCOLS(2, COL_FWD_SOLVE_INV); }
   else return hipErrorInvalidValue;
+#undef PME_BCOLS
   return hipGetLastError();
 }
 

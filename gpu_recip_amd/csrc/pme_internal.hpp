@@ -208,9 +208,11 @@ size_t fft_plane_lds_bytes(int nx, int ny, int nf_x, int np);
 int plane_group_host(int nx, int ny, int nf_x, int maxnp);
 // batched plane kernels: items are (system, first z plane), with
 // plane_group_host(.., maxnp) planes each; with reduce != 0 the plane-0 item
-// of every system assigns that system's energy/virial
-hipError_t launch_batch_plane(int inverse, int reduce, int maxnp, const SysDesc* sys, const WorkItem* items,
-                              int nitems, int grid, size_t lds, hipStream_t s);
+// of every system assigns that system's energy/virial.  split != 0: the
+// kernel with the matrix-core y stage (systems whose y axis is not FP32);
+// split == 0 requires FP32 y axes.
+hipError_t launch_batch_plane(int inverse, int reduce, int maxnp, int split, const SysDesc* sys,
+                              const WorkItem* items, int nitems, int grid, size_t lds, hipStream_t s);
 
 // batched launchers: `grid` persistent workgroups loop over the work items
 hipError_t launch_batch_bin(const SysDesc* sys, const WorkItem* items, int nitems, int grid, size_t lds, hipStream_t s);
@@ -221,9 +223,10 @@ hipError_t launch_batch_gather(int order, const SysDesc* sys, const WorkItem* it
 hipError_t launch_batch_fft_x(int inverse, const SysDesc* sys, const WorkItem* items, int nitems, int grid, size_t lds,
                               hipStream_t s);
 // axis 1 (y) or 2 (z); mode COL_*; with reduce != 0 the first tile of every
-// system sums that system's energy/virial partials into ev_out
-hipError_t launch_batch_cols(int axis, int mode, int reduce, const SysDesc* sys, const WorkItem* items, int nitems,
-                             int grid, size_t lds, hipStream_t s);
+// system sums that system's energy/virial partials into ev_out; split as
+// for launch_batch_plane, for the transformed axis
+hipError_t launch_batch_cols(int axis, int mode, int reduce, int split, const SysDesc* sys, const WorkItem* items,
+                             int nitems, int grid, size_t lds, hipStream_t s);
 
 // LDS sizes (bytes) the launchers request, exposed for plan validation
 size_t bin_lds_bytes(int nbins);

@@ -194,12 +194,19 @@ void build_items(const SysDesc* const* descs, const SystemPlan* const* plans, co
   int group = kPlaneGroupMax;
   while (group > 1 && total_planes < 2 * ncu * group) group--;
   // order: the spread/gather order, or the largest plane group of a plane launch
-  auto add_stage = [&](int stage, int order) {
-    Launch L{stage, order, (int)items.size(), 0, 0};
+  // split: the FFT launches of systems whose transformed axis uses the
+  // split-precision matrix-core stage go to kernels compiled with it, the
+  // others to FP32-only kernels
+  auto add_stage = [&](int stage, int order, int split) {
+    Launch L{stage, order, split, (int)items.size(), 0, 0};
     for (int k = 0; k < n; k++) {
       const SysDesc& d = *descs[k];
       const SystemPlan& p = *plans[k];
       if ((stage == BS_SPREAD || stage == BS_GATHER) && orders[k] != order) continue;
+      if (stage == BS_FFTY || stage == BS_PLANE || stage == BS_FFTZ) {
+        const int a = stage == BS_FFTZ ? 2 : 1;
+        if ((d.ax[a].prec != PME_PREC_FP32) != (split != 0)) continue;
+      }
       int cnt = 0;
       size_t lds = 0;
       const int hx = d.bin.nx / 2 + 1;
@@ -245,13 +252,13 @@ void build_items(const SysDesc* const* descs, const SystemPlan* const* plans, co
     }
     if (L.count > 0) launches.push_back(L);
   };
-  add_stage(BS_BIN, 0);
-  for (int o : {4, 6, 8}) add_stage(BS_SPREAD, o);
-  add_stage(BS_FFTX, 0);
-  add_stage(BS_FFTY, 0);
-  add_stage(BS_PLANE, group);
-  add_stage(BS_FFTZ, 0);
-  for (int o : {4, 6, 8}) add_stage(BS_GATHER, o);
+  add_stage(BS_BIN, 0, 0);
+  for (int o : {4, 6, 8}) add_stage(BS_SPREAD, o, 0);
+  add_stage(BS_FFTX, 0, 0);
+  for (int sp : {0, 1}) add_stage(BS_FFTY, 0, sp);
+  for (int sp : {0, 1}) add_stage(BS_PLANE, group, sp);
+  for (int sp : {0, 1}) add_stage(BS_FFTZ, 0, sp);
+  for (int o : {4, 6, 8}) add_stage(BS_GATHER, o, 0);
 }
 
 static int grid_for(const Tuning& t, int ncu, int count, bool big_block) {
@@ -289,12 +296,16 @@ pme_status enqueue_stages(const std::vector<Launch>& launches, const SysDesc* d_
           case BS_SPREAD: SCHED_HIP(launch_batch_spread(L.order, d_sys, it, L.count, grid, L.lds, stream)); break;
           case BS_FFTX: SCHED_HIP(launch_batch_fft_x(inv ? 1 : 0, d_sys, it, L.count, grid, L.lds, stream)); break;
           case BS_FFTY:
-            SCHED_HIP(launch_batch_cols(1, inv ? COL_INV : COL_FWD, inv ? 1 : 0, d_sys, it, L.count, grid, L.lds, stream));
+            SCHED_HIP(launch_batch_cols(1, inv ? COL_INV : COL_FWD, inv ? 1 : 0, L.split, d_sys, it, L.count, grid, L.lds,
+                                        stream));
             break;
           case BS_PLANE:
-            SCHED_HIP(launch_batch_plane(inv ? 1 : 0, inv ? 1 : 0, L.order, d_sys, it, L.count, grid, L.lds, stream));
+            SCHED_HIP(launch_batch_plane(inv ? 1 : 0, inv ? 1 : 0, L.order, L.split, d_sys, it, L.count, grid, L.lds,
+                                         stream));
             break;
-          case BS_FFTZ: SCHED_HIP(launch_batch_cols(2, COL_FWD_SOLVE_INV, 0, d_sys, it, L.count, grid, L.lds, stream)); break;
+          case BS_FFTZ:
+            SCHED_HIP(launch_batch_cols(2, COL_FWD_SOLVE_INV, 0, L.split, d_sys, it, L.count, grid, L.lds, stream));
+            break;
           case BS_GATHER: SCHED_HIP(launch_batch_gather(L.order, d_sys, it, L.count, grid, L.lds, stream)); break;
           default: break;
         }

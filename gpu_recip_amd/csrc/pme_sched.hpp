@@ -60,6 +60,7 @@ void fill_dynamic(SysDesc& d, const pme_system_desc& in, const SystemPlan& p);
 struct Launch {
   int stage;   // BatchStage
   int order;   // spread/gather template order; largest plane group of a plane launch (0 otherwise)
+  int split;   // FFT launches: 1 for the kernels with the split-precision matrix-core stage
   int offset;
   int count;
   size_t lds;  // dynamic LDS bytes: max over the launch's systems
