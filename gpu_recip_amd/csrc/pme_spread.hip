@@ -20,11 +20,25 @@
 
 namespace pme {
 
+// Divisors of the brick arithmetic of one grid, as reciprocals: grid edges
+// are at most 512 and a brick count at most the edge, so every numerator
+// below is under 2^19 and sdiv gives the exact integer quotient.
+struct BrickDivs {
+  SmallDiv n[3], nb[3];
+};
+__device__ __forceinline__ BrickDivs brick_divs(const BinArgs& a) {
+  return BrickDivs{{small_div(a.nx), small_div(a.ny), small_div(a.nz)},
+                   {small_div(a.nbx), small_div(a.nby), small_div(a.nbz)}};
+}
+// brick_lo(b, n, nb) and brick_of(i, n, nb) without integer division
+__device__ __forceinline__ int brick_lo_d(int b, int n, const SmallDiv& dnb) { return sdiv(b * n, dnb); }
+__device__ __forceinline__ int brick_of_d(int i, int nb, const SmallDiv& dn) { return sdiv((i + 1) * nb - 1, dn); }
+
 // Does the stencil i..i+P-1 of an atom in brick b run past the brick's end
 // (into brick b+1, periodic)?  An axis with one brick wraps onto itself.
-__device__ __forceinline__ int spills(int i, int b, int n, int nb, int P) {
+__device__ __forceinline__ int spills(int i, int b, int n, int nb, int P, const SmallDiv& dnb) {
   if (nb == 1) return 0;
-  return (i + P - 1 >= brick_lo(b + 1, n, nb)) ? 1 : 0;
+  return (i + P - 1 >= brick_lo_d(b + 1, n, dnb)) ? 1 : 0;
 }
 
 struct AtomBricks {
@@ -32,7 +46,7 @@ struct AtomBricks {
   int sx, sy, sz;  // 1 if the stencil spills into the next brick
 };
 
-__device__ __forceinline__ AtomBricks atom_bricks(const float4& p, const BinArgs& a) {
+__device__ __forceinline__ AtomBricks atom_bricks(const float4& p, const BinArgs& a, const BrickDivs& dv) {
   int ix, iy, iz;
   float w;
   grid_index(p.x, a.r1, a.nx, ix, w);
@@ -42,12 +56,12 @@ __device__ __forceinline__ AtomBricks atom_bricks(const float4& p, const BinArgs
   iy = safe_index(iy, a.ny);
   iz = safe_index(iz, a.nz);
   AtomBricks r;
-  r.bx = brick_of(ix, a.nx, a.nbx);
-  r.by = brick_of(iy, a.ny, a.nby);
-  r.bz = brick_of(iz, a.nz, a.nbz);
-  r.sx = spills(ix, r.bx, a.nx, a.nbx, a.order);
-  r.sy = spills(iy, r.by, a.ny, a.nby, a.order);
-  r.sz = spills(iz, r.bz, a.nz, a.nbz, a.order);
+  r.bx = brick_of_d(ix, a.nbx, dv.n[0]);
+  r.by = brick_of_d(iy, a.nby, dv.n[1]);
+  r.bz = brick_of_d(iz, a.nbz, dv.n[2]);
+  r.sx = spills(ix, r.bx, a.nx, a.nbx, a.order, dv.nb[0]);
+  r.sy = spills(iy, r.by, a.ny, a.nby, a.order, dv.nb[1]);
+  r.sz = spills(iz, r.bz, a.nz, a.nbz, a.order, dv.nb[2]);
   return r;
 }
 
@@ -58,7 +72,8 @@ __device__ __forceinline__ int next_brick(int b, int d, int nb) {
 
 // Chunk w of the counting sort (one kBinBlock-thread workgroup).  hist: nbins
 // ints of LDS.  With the default chunk of kBinBlock atoms each thread bins one
-// atom and keeps it in registers between the counting and the placing pass.
+// atom and keeps it and its bricks in registers between the counting and the
+// placing pass.
 __device__ __forceinline__ void bin_chunk_body(const float4* __restrict__ xyzq, int n, const BinArgs& a, int w,
                                                int* hist) {
   __shared__ int sh_scan[kBinBlock / 64];
@@ -66,14 +81,16 @@ __device__ __forceinline__ void bin_chunk_body(const float4* __restrict__ xyzq, 
   const int tid = threadIdx.x;
   const int a0 = w * a.chunk;
   const int a1 = min(a0 + a.chunk, n);
+  const BrickDivs dv = brick_divs(a);
   for (int i = tid; i < nbins; i += kBinBlock) hist[i] = 0;
   // first atom of this thread, loaded while the histogram is cleared
   const int first = a0 + tid;
   float4 p0 = make_float4(0.f, 0.f, 0.f, 0.f);
   if (first < a1) p0 = xyzq[first];
+  const AtomBricks k0 = atom_bricks(p0, a, dv);
   __syncthreads();
   for (int atom = first; atom < a1; atom += kBinBlock) {
-    const AtomBricks k = atom_bricks(atom == first ? p0 : xyzq[atom], a);
+    const AtomBricks k = atom == first ? k0 : atom_bricks(xyzq[atom], a, dv);
     for (int dz = 0; dz <= k.sz; dz++)
       for (int dy = 0; dy <= k.sy; dy++)
         for (int dx = 0; dx <= k.sx; dx++) {
@@ -91,7 +108,7 @@ __device__ __forceinline__ void bin_chunk_body(const float4* __restrict__ xyzq, 
   int* ri = a.rec_idx + (size_t)w * a.rec_stride;
   for (int atom = first; atom < a1; atom += kBinBlock) {
     const float4 p = atom == first ? p0 : xyzq[atom];
-    const AtomBricks k = atom_bricks(p, a);
+    const AtomBricks k = atom == first ? k0 : atom_bricks(p, a, dv);
     for (int dz = 0; dz <= k.sz; dz++)
       for (int dy = 0; dy <= k.sy; dy++)
         for (int dx = 0; dx <= k.sx; dx++) {
