@@ -73,6 +73,7 @@ struct pme_batch_s {
   int d_sys_cap = 0;
   double* d_ev = nullptr;    // [d_sys_cap * 10]
   double* h_ev = nullptr;    // pinned mirror
+  int ev_copied = 0;         // slots the solve's own d_ev -> h_ev copy covers
   TableCache tables;
   // work lists of the current subset
   std::vector<int> cur_ids;
@@ -100,6 +101,7 @@ static void drop_graph(pme_batch_t b) {
 
 static void invalidate(pme_batch_t b) {
   b->lists_valid = false;
+  b->ev_copied = 0;
   drop_graph(b);
 }
 
@@ -319,8 +321,16 @@ static pme_status build_lists(pme_batch_t b, const std::vector<int>& ids) {
   return PME_OK;
 }
 
+// The stages, then the copy of every slot's energy/virial into the pinned
+// mirror (part of the captured graph), so reading results needs no further
+// device work.
 static pme_status enqueue(pme_batch_t b) {
-  return enqueue_stages(b->launches, b->d_sys, b->d_items, b->tune, b->ncu, b->stream, b->timing ? b->tev : nullptr);
+  BTRY(enqueue_stages(b->launches, b->d_sys, b->d_items, b->tune, b->ncu, b->stream, b->timing ? b->tev : nullptr));
+  const int n = (int)b->slots.size();
+  if (n > 0)
+    BHIP(hipMemcpyAsync(b->h_ev, b->d_ev, sizeof(double) * 10 * (size_t)n, hipMemcpyDeviceToHost, b->stream));
+  b->ev_copied = n;
+  return PME_OK;
 }
 
 static pme_status run_solve(pme_batch_t b);
@@ -392,7 +402,9 @@ extern "C" pme_status pme_batch_results(pme_batch_t b, int n, const int* ids, do
       return set_error(PME_ERR_INVALID_ARG, "pme_batch_results: unknown id %d", id);
     hi = std::max(hi, id + 1);
   }
-  if (hi > 0) BHIP(hipMemcpy(b->h_ev, b->d_ev, sizeof(double) * 10 * hi, hipMemcpyDeviceToHost));
+  // the solves copy d_ev into h_ev themselves; slots added since then are
+  // read here
+  if (hi > b->ev_copied) BHIP(hipMemcpy(b->h_ev, b->d_ev, sizeof(double) * 10 * hi, hipMemcpyDeviceToHost));
   if (b->timing && b->lists_valid)
     for (int k = 0; k < 8; k++) BHIP(hipEventElapsedTime(&b->stage_ms[k], b->tev[k], b->tev[k + 1]));
   for (size_t k = 0; k < v.size(); k++) {

@@ -9,6 +9,8 @@ from __future__ import annotations
 
 import ctypes as C
 
+import numpy as np
+
 from . import _lib as L
 
 
@@ -220,11 +222,16 @@ class PmeBatch:
         L.call("pme_batch_solve", self.h, n, arr)
 
     def results(self, ids):
+        """(energies (n,), virials (n, 9)) of the listed systems after the
+        last solve, as float64 arrays (waits for the batch's stream)."""
+        if ids is None:
+            raise ValueError("PmeBatch.results: list the system ids")
         n, arr = self._ids(ids)
-        e = (C.c_double * n)()
-        v = (C.c_double * (9 * n))()
-        L.call("pme_batch_results", self.h, n, arr, e, v)
-        return list(e), [list(v[9 * k:9 * k + 9]) for k in range(n)]
+        e = np.empty(n, np.float64)
+        v = np.empty((n, 9), np.float64)
+        pd = C.POINTER(C.c_double)
+        L.call("pme_batch_results", self.h, n, arr, e.ctypes.data_as(pd), v.ctypes.data_as(pd))
+        return e, v
 
     def set_graph(self, enable: bool) -> None:
         L.call("pme_batch_set_graph", self.h, int(bool(enable)))

@@ -290,7 +290,7 @@ def test_stage_timing():
     ms = b.get_timing()
     say(f"  batch stage ms: {[round(v, 4) for v in ms]}")
     assert len(ms) == 8 and all(v >= 0 for v in ms) and ms[1] > 0 and ms[4] > 0 and ms[7] > 0
-    assert e0 == e1
+    assert np.array_equal(e0, e1)
     b.set_timing(False)
     with pytest.raises(L.PmeError):
         b.get_timing()
