@@ -71,8 +71,9 @@ def stage_bytes(s):
 # kernel behind each timing slot of the batched solve (order-4 systems whose
 # xy planes fit one workgroup, as in the 64-system set)
 BATCH_KERNELS = {"bin": "pme::k_batch_bin", "spread": "void pme::k_batch_spread<4>",
-                 "fft_fwd_x_or_xy": "void pme::k_batch_plane<0>", "fft_z_solve": "void pme::k_batch_cols<2, 2>",
-                 "fft_inv_y_or_xy": "void pme::k_batch_plane<1>", "gather": "void pme::k_batch_gather<4>"}
+                 "fft_fwd_x_or_xy": "void pme::k_batch_plane<0, false>",
+                 "fft_z_solve": "void pme::k_batch_cols<2, 2, false>",
+                 "fft_inv_y_or_xy": "void pme::k_batch_plane<1, false>", "gather": "void pme::k_batch_gather<4>"}
 
 
 def measured_traffic(kernel, workload="batched"):
