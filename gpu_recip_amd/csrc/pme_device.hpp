@@ -93,6 +93,15 @@ __device__ __forceinline__ long long to_fixed40(float v) {
   const double t = __fma_rn((double)v, 1099511627776.0, 6755399441055744.0);
   return (long long)((unsigned long long)__double_as_longlong(t) - 0x4338000000000000ull);
 }
+// The same for a value already multiplied by 2^40 (|v| < 2^51): one FP64 add
+// of 1.5*2^52 rounds it to an integer in the low mantissa bits.  With the
+// scale folded into an operand beforehand (a power-of-two factor, so every
+// FP32 product is the 2^40 multiple of the unscaled one), this gives
+// to_fixed40's value with one constant operand instead of two.
+__device__ __forceinline__ long long to_fixed_scaled(float v) {
+  const double t = __dadd_rn((double)v, 6755399441055744.0);
+  return (long long)((unsigned long long)__double_as_longlong(t) - 0x4338000000000000ull);
+}
 // back to FP32: ((float)val)*2^-40 as reduce_force<long long, float> (reduce.cu:66-78)
 __device__ __forceinline__ float from_fixed40(unsigned long long u) {
   return (float)((double)(long long)u * (1.0 / 1099511627776.0));

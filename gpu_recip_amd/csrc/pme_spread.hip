@@ -193,8 +193,10 @@ __device__ __forceinline__ void spread_brick_body(const BinArgs& a, int b, int w
     bspline_theta<P>(fx, tx);
     bspline_theta<P>(fy, ty);
     bspline_theta<P>(fz, tz);
+    // q * 2^40 (exact): the fixed-point scale rides on theta_x
+    const float qs = p.w * 1099511627776.0f;
 #pragma unroll
-    for (int j = 0; j < P; j++) tx[j] *= p.w;
+    for (int j = 0; j < P; j++) tx[j] *= qs;
     // x offset into the margin-padded row: in [0, wx + 2(P-1)) always
     const int dx = local_offset(safe_index(ix, a.nx), x0, wx, a.nx) + (P - 1);
     const int dy = local_offset(safe_index(iy, a.ny), y0, wy, a.ny);
@@ -212,7 +214,7 @@ __device__ __forceinline__ void spread_brick_body(const BinArgs& a, int b, int w
         const float c = ty[jy] * tz[jz];
         unsigned long long* row = acc + mul24(lz, plane) + mul24(ly, bxs) + dx;
 #pragma unroll
-        for (int jx = 0; jx < P; jx++) atomicAdd(&row[jx], (unsigned long long)to_fixed40(tx[jx] * c));
+        for (int jx = 0; jx < P; jx++) atomicAdd(&row[jx], (unsigned long long)to_fixed_scaled(tx[jx] * c));
       }
     }
   }

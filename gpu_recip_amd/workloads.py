@@ -4,12 +4,16 @@
     python -m gpu_recip_amd.workloads split      [--solves 20] [--precision ...]
     python -m gpu_recip_amd.workloads batched    [--solves 20] [--tuning brick,tile,wg_per_cu]
     python -m gpu_recip_amd.workloads small      [--solves 20]
+    python -m gpu_recip_amd.workloads served     [--solves 1000]
 
 production: 96,000 atoms, 108x100x96, order 6 (plain five-call path)
 split:      250,002 atoms, 144^3, order 4 (plain path, chosen FFT precision)
 batched:    the 64-system set, one batched solve per step (autotuned plan, as
             in the benchmark)
 small:      8,001 atoms, 48^3, order 4 (plain path)
+served:     the first --solves requests of the serving schedule (5k-400k
+            atoms, grids 40-160, order 4 or 6), submitted at once to a
+            2-lane server (saturation), after a short warm-up burst
 Inputs are uploaded once; every solve ends with the energy/virial readback.
 """
 from __future__ import annotations
@@ -22,7 +26,7 @@ PRECS = {"fp32": 0, "bf16x3": 1, "fp16x3": 2}
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("workload", choices=["production", "split", "batched", "small"])
+    ap.add_argument("workload", choices=["production", "split", "batched", "small", "served"])
     ap.add_argument("--solves", type=int, default=20)
     ap.add_argument("--precision", choices=sorted(PRECS), default="fp32")
     ap.add_argument("--tuning", default=None,
@@ -37,6 +41,24 @@ def main(argv=None):
 
     dev = torch.device("cuda:0")
     t0 = time.perf_counter()
+    if args.workload == "served":
+        from . import serving
+        from .pme import PmeServer
+
+        reqs = serving.schedule(1e9, max_requests=args.solves)
+        serving.materialise(reqs, dev)
+        srv = PmeServer(64 << 30, nlanes=2)
+        warm = serving.schedule(0.2, seed=serving.SERVE_SEED + 99)
+        serving.materialise(warm, dev)
+        serving.run_burst(srv, warm)
+        for r in warm:
+            srv.release(r.key)
+        wall = serving.run_burst(srv, reqs)
+        st = srv.stats()
+        print(f"served: {len(reqs)} requests in {wall:.3f} s ({len(reqs) / wall:.0f} solves/s), "
+              f"mean batch {st['mean_batch']:.1f}", flush=True)
+        srv.close()
+        return
     if args.workload == "batched":
         b = PmeBatch(32 << 30)
         ids = []
