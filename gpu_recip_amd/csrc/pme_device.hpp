@@ -204,6 +204,74 @@ template <int S> struct Dft<8, S> {
   }
 };
 
+// ---- register DFTs of composite length (four-step FFT codelets) -----------
+// cos and sin of 2 pi m / N in double at compile time: Taylor series after
+// reduction to [-pi, pi]; the quarter-turn points are exact.
+constexpr double ct_trig(int m, int N, bool sine) {
+  m %= N;
+  if (m < 0) m += N;
+  if ((4 * m) % N == 0) {
+    const int q = (4 * m) / N;  // quarter turns
+    const double c[4] = {1.0, 0.0, -1.0, 0.0}, s[4] = {0.0, 1.0, 0.0, -1.0};
+    return sine ? s[q] : c[q];
+  }
+  double x = 2.0 * 3.14159265358979323846 * (double)(2 * m > N ? m - N : m) / (double)N;
+  double term = sine ? x : 1.0, sum = term;
+  for (int i = 1; i < 24; i++) {
+    term *= -x * x / (sine ? (double)(2 * i) * (2 * i + 1) : (double)(2 * i - 1) * (2 * i));
+    sum += term;
+  }
+  return sum;
+}
+template <int N> struct CtTwiddles {
+  float c[N], s[N];
+  constexpr CtTwiddles() : c(), s() {
+    for (int m = 0; m < N; m++) {
+      c[m] = (float)ct_trig(m, N, false);
+      s[m] = (float)ct_trig(m, N, true);
+    }
+  }
+};
+
+// y_k = sum_j a_j e^{S 2 pi i jk/N} in registers for N = A * B (A in {2,3,4,5}):
+// DFT_A over j = b + B a for each b, twiddle e^{S 2 pi i b ka / N}, DFT_B over
+// b for each ka, output k = ka + A kb.  Lengths 1, 2, 3, 4, 5, 8 use Dft.
+template <int N, int S> struct DftN {
+  static constexpr int A = N % 4 == 0 ? 4 : (N % 2 == 0 ? 2 : (N % 3 == 0 ? 3 : 5));
+  static constexpr int B = N / A;
+  static __device__ __forceinline__ void run(float2* x) {
+    constexpr CtTwiddles<N> W{};
+    float2 t[N];
+#pragma unroll
+    for (int b = 0; b < B; b++) {
+      float2 u[A];
+#pragma unroll
+      for (int a = 0; a < A; a++) u[a] = x[b + B * a];
+      Dft<A, S>::run(u);
+#pragma unroll
+      for (int ka = 0; ka < A; ka++)
+        t[b * A + ka] = (b * ka) % N == 0 ? u[ka] : cmul(u[ka], make_float2(W.c[(b * ka) % N], (float)S * W.s[(b * ka) % N]));
+    }
+#pragma unroll
+    for (int ka = 0; ka < A; ka++) {
+      float2 v[B];
+#pragma unroll
+      for (int b = 0; b < B; b++) v[b] = t[b * A + ka];
+      DftN<B, S>::run(v);
+#pragma unroll
+      for (int kb = 0; kb < B; kb++) x[ka + A * kb] = v[kb];
+    }
+  }
+};
+template <int S> struct DftN<1, S> {
+  static __device__ __forceinline__ void run(float2*) {}
+};
+template <int S> struct DftN<2, S> : Dft<2, S> {};
+template <int S> struct DftN<3, S> : Dft<3, S> {};
+template <int S> struct DftN<4, S> : Dft<4, S> {};
+template <int S> struct DftN<5, S> : Dft<5, S> {};
+template <int S> struct DftN<8, S> : Dft<8, S> {};
+
 // Radix plan of one transform length (Stockham autosort, mixed radix).
 struct RadixPlan {
   int n;         // transform length
@@ -233,13 +301,14 @@ __device__ __forceinline__ int mul24(int a, int b) { return __mul24(a, b); }
 // Element j of line l lives at buf[j*ls + l].  Stage with p = product of the
 // previous radices: butterfly i reads i + r T (T = n/R), twiddles input r by
 // w^{r k n/(pR)} (k = i mod p) and writes (i/p) p R + k + r p.
+// tscale = (n / (p R)) * (twiddle table stride), passed in so that no stage
+// needs a run-time integer division.
 template <int R, int S>
 __device__ __forceinline__ void stockham_stage(const float2* __restrict__ src, float2* __restrict__ dst,
-                                               const float2* __restrict__ tw, int n, int p, int tstride,
+                                               const float2* __restrict__ tw, int n, int p, int tscale,
                                                int nlines, int ls, int tid, int nthr) {
   const int T = n / R;
   const int nbf = T * nlines;
-  const int tscale = (n / (p * R)) * tstride;
   const SmallDiv dl = small_div(nlines), dp = small_div(p);
   const int Tls = mul24(T, ls), pls = mul24(p, ls);
   for (int bf = tid; bf < nbf; bf += nthr) {
@@ -440,14 +509,15 @@ __device__ __forceinline__ float2* fft_lines(float2* buf0, float2* buf1, const f
     p = 16;
     s0 = 1;
   }
+  int q = p == 1 ? rp.n : rp.n / 16;  // n / p, then divided by each stage's (compile-time) radix
   for (int s = s0; s < rp.nrad; s++) {
     const int R = plan_radix(rp, s);
     switch (R) {
-      case 2: stockham_stage<2, S>(src, dst, tw, rp.n, p, rp.tstride, nlines, ls, tid, nthr); break;
-      case 3: stockham_stage<3, S>(src, dst, tw, rp.n, p, rp.tstride, nlines, ls, tid, nthr); break;
-      case 4: stockham_stage<4, S>(src, dst, tw, rp.n, p, rp.tstride, nlines, ls, tid, nthr); break;
-      case 5: stockham_stage<5, S>(src, dst, tw, rp.n, p, rp.tstride, nlines, ls, tid, nthr); break;
-      default: stockham_stage<8, S>(src, dst, tw, rp.n, p, rp.tstride, nlines, ls, tid, nthr); break;
+      case 2: q /= 2; stockham_stage<2, S>(src, dst, tw, rp.n, p, q * rp.tstride, nlines, ls, tid, nthr); break;
+      case 3: q /= 3; stockham_stage<3, S>(src, dst, tw, rp.n, p, q * rp.tstride, nlines, ls, tid, nthr); break;
+      case 4: q /= 4; stockham_stage<4, S>(src, dst, tw, rp.n, p, q * rp.tstride, nlines, ls, tid, nthr); break;
+      case 5: q /= 5; stockham_stage<5, S>(src, dst, tw, rp.n, p, q * rp.tstride, nlines, ls, tid, nthr); break;
+      default: q /= 8; stockham_stage<8, S>(src, dst, tw, rp.n, p, q * rp.tstride, nlines, ls, tid, nthr); break;
     }
     __syncthreads();
     float2* t = src;

@@ -580,6 +580,198 @@ __device__ __forceinline__ void ev_partial(const SolveArgs& s, double* acc, doub
   }
 }
 
+// ---- z pass as a register four-step FFT -------------------------------------
+// n = N1 N2, j = j2 + N2 j1, k = k1 + N1 k2:
+//   X[k1 + N1 k2] = sum_j2 w_N2^{j2 k2} w_n^{j2 k1} sum_j1 w_N1^{j1 k1} x[j2 + N2 j1].
+// Work unit (line l, q): in the forward half q = j2 (an N1-point DFT of the
+// column j2 + N2 j1 in registers, the twiddle w_n^{j2 k1}, one LDS
+// exchange), then q = k1 (an N2-point DFT in registers).  That thread then
+// holds the whole line's spectrum at k3 = k1 + N1 k2, applies the influence
+// function there and runs the inverse the same way back (the conjugate
+// steps in reverse order).  Two LDS exchanges per direction instead of one
+// per radix stage, and no index arithmetic inside the DFTs.
+#define PME_F4_PAIRS(X)                                                                             \
+  X(0, 4, 4) X(1, 4, 5) X(2, 4, 6) X(3, 5, 5) X(4, 3, 9) X(5, 5, 6) X(6, 4, 8) X(7, 6, 6) X(8, 5, 8) \
+  X(9, 5, 9) X(10, 6, 8) X(11, 5, 10) X(12, 6, 9) X(13, 6, 10) X(14, 8, 8) X(15, 8, 9) X(16, 5, 15)  \
+  X(17, 8, 10) X(18, 9, 9) X(19, 9, 10) X(20, 8, 12) X(21, 10, 10) X(22, 9, 12) X(23, 10, 12)        \
+  X(24, 8, 16) X(25, 9, 15) X(26, 12, 12) X(27, 10, 15) X(28, 10, 16) X(29, 12, 15) X(30, 12, 16)
+
+int four_step_code(int n) {
+#define PME_F4_CODE(c, a, b) \
+  if (n == (a) * (b)) return c;
+  PME_F4_PAIRS(PME_F4_CODE)
+#undef PME_F4_CODE
+  return -1;
+}
+
+// eterm of one point of a z line; (k1, k2) factors precomputed (as in
+// solve_lines, so the value is the same)
+struct LineFactors {
+  float mh1, mh2, s12, p12;
+  bool f1, f2, k1pos;
+  int k1, k2;
+};
+__device__ __forceinline__ LineFactors line_factors(const SolveArgs& s, int k1, int k2) {
+  LineFactors f;
+  const int m1 = k1 >= s.nf1 ? k1 - s.n1 : k1;
+  const int m2 = k2 >= s.nf2 ? k2 - s.n2 : k2;
+  f.mh1 = s.r1 * (float)m1;
+  f.mh2 = s.r2 * (float)m2;
+  f.s12 = fmaf(f.mh2, f.mh2, f.mh1 * f.mh1);
+  f.p12 = s.piv_inv * s.pf1[k1] * s.pf2[k2];
+  f.f1 = !(2 * k1 == s.n1);
+  f.f2 = !(2 * k2 == s.n2 || k2 == 0);
+  f.k1pos = k1 >= 1;
+  f.k1 = k1;
+  f.k2 = k2;
+  return f;
+}
+
+// Q <- eterm Q at k3 of the line; energy/virial sums as in solve_lines
+__device__ __forceinline__ float2 solve_line_point(const SolveArgs& s, const LineFactors& f, int k3, float2 q,
+                                                   double* sums) {
+  if ((f.k1 | f.k2 | k3) == 0) return make_float2(0.f, 0.f);
+  const int m3 = k3 >= s.nf3 ? k3 - s.n3 : k3;
+  const float mh3 = s.r3 * (float)m3;
+  const float msq = fmaf(mh3, mh3, f.s12);
+  const float msq_inv = __builtin_amdgcn_rcpf(msq);
+  const float eterm = __expf(-s.fac * msq) * f.p12 * s.pf3[k3] * msq_inv;
+  if (s.calc_ev) {
+    const float q2 = fmaf(q.y, q.y, q.x * q.x);
+    const float tmp1 = eterm * q2;
+    const float vterm = 2.0f * (s.fac + msq_inv);
+    const float tmp2 = tmp1 * vterm;
+    const float t3 = tmp2 * mh3;
+    const bool f3 = !(2 * k3 == s.n3 || k3 == 0);
+    const float w13 = f.k1pos ? (f.f1 == f3 ? 2.f : 0.f) : 1.f;
+    const float w23 = f.k1pos ? (f.f2 == f3 ? 2.f : 0.f) : 1.f;
+    sums[0] += (double)tmp1;
+    sums[1] += (double)tmp2;
+    sums[2] += (double)(t3 * mh3);
+    sums[3] += (double)(w13 * t3);
+    sums[4] += (double)(w23 * t3);
+  }
+  return cscale(q, eterm);
+}
+
+__device__ __forceinline__ void fold_line_sums(const LineFactors& f, const double* sums, double* acc) {
+  const double w = f.k1pos ? 2.0 : 1.0;
+  const double w12 = f.k1pos ? (f.f1 == f.f2 ? 2.0 : 0.0) : 1.0;
+  const double a1 = (double)f.mh1, a2 = (double)f.mh2;
+  acc[0] += w * sums[0];
+  acc[1] += w * (a1 * a1 * sums[1] - sums[0]);
+  acc[2] += w12 * a1 * a2 * sums[1];
+  acc[3] += a1 * sums[3];
+  acc[4] += w * (a2 * a2 * sums[1] - sums[0]);
+  acc[5] += a2 * sums[4];
+  acc[6] += w * (sums[2] - sums[0]);
+}
+
+// One column tile of the z pass (as fft_cols_body<COL_FWD_SOLVE_INV>) with
+// n = N1 N2: TX lines kx0 .. kx0+TX-1 at ky = ty, point j of line l at
+// base + j pt_stride + l.  sm: twiddles (ntw) and two n*TX buffers.
+template <int N1, int N2>
+__device__ __forceinline__ void zsolve_four_step(float2* __restrict__ cg, int hx, int TX, long long pt_stride,
+                                                 long long outer_stride, const AxisArgs& ax, const SolveArgs& sa,
+                                                 int tx, int ty, int blk, float2* sm) {
+  constexpr int n = N1 * N2;
+  __shared__ double sh_red[7 * (kBlock / 64)];
+  const int tid = threadIdx.x;
+  float2* tw = sm;
+  float2* b0 = sm + ax.ntw;
+  float2* b1 = b0 + n * TX;
+  for (int i = tid; i < ax.ntw; i += kBlock) tw[i] = ax.tw[i];
+  const int ts = ax.rp.tstride;
+  const int kx0 = tx * TX;
+  const int nl = min(TX, hx - kx0);
+  const size_t base = (size_t)kx0 + (size_t)ty * (size_t)outer_stride;
+  const SmallDiv dtx = small_div(TX);
+  for (int e0 = tid; e0 < n * TX; e0 += kStageU * kBlock) {
+    float2 v[kStageU];
+#pragma unroll
+    for (int u = 0; u < kStageU; u++) {
+      const int e = e0 + u * kBlock;
+      const int j = sdiv(e, dtx);
+      const int l = e - j * TX;
+      v[u] = (e < n * TX && l < nl) ? cg[base + (size_t)j * pt_stride + l] : make_float2(0.f, 0.f);
+    }
+#pragma unroll
+    for (int u = 0; u < kStageU; u++) {
+      const int e = e0 + u * kBlock;
+      if (e < n * TX) b0[e] = v[u];
+    }
+  }
+  __syncthreads();
+  // forward, first half: unit (l, j2)
+  for (int u = tid; u < N2 * TX; u += kBlock) {
+    const int j2 = sdiv(u, dtx);
+    const int l = u - j2 * TX;
+    float2 x[N1];
+#pragma unroll
+    for (int j1 = 0; j1 < N1; j1++) x[j1] = b0[(j2 + N2 * j1) * TX + l];
+    DftN<N1, -1>::run(x);
+#pragma unroll
+    for (int k1 = 0; k1 < N1; k1++) {
+      const float2 y = k1 == 0 ? x[0] : cmul(x[k1], tw[mul24(j2 * k1, ts)]);
+      b1[(k1 * N2 + j2) * TX + l] = y;
+    }
+  }
+  __syncthreads();
+  // forward second half, influence function, inverse first half: unit (l, k1)
+  double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+  for (int u = tid; u < N1 * TX; u += kBlock) {
+    const int k1 = sdiv(u, dtx);
+    const int l = u - k1 * TX;
+    float2 X[N2];
+#pragma unroll
+    for (int j2 = 0; j2 < N2; j2++) X[j2] = b1[(k1 * N2 + j2) * TX + l];
+    DftN<N2, -1>::run(X);  // X[k2]: point k3 = k1 + N1 k2
+    if (l < nl) {
+      const LineFactors f = line_factors(sa, kx0 + l, ty);
+      double sums[5] = {0, 0, 0, 0, 0};
+#pragma unroll
+      for (int k2 = 0; k2 < N2; k2++) X[k2] = solve_line_point(sa, f, k1 + N1 * k2, X[k2], sums);
+      if (sa.calc_ev) fold_line_sums(f, sums, acc);
+    }
+    DftN<N2, +1>::run(X);  // X[j2]
+#pragma unroll
+    for (int j2 = 0; j2 < N2; j2++) {
+      const float2 t = tw[mul24(j2 * k1, ts)];
+      const float2 z = j2 == 0 ? X[0] : cmul(X[j2], make_float2(t.x, -t.y));
+      b0[(k1 * N2 + j2) * TX + l] = z;
+    }
+  }
+  __syncthreads();
+  // inverse second half: unit (l, j2), then the store
+  for (int u = tid; u < N2 * TX; u += kBlock) {
+    const int j2 = sdiv(u, dtx);
+    const int l = u - j2 * TX;
+    float2 z[N1];
+#pragma unroll
+    for (int k1 = 0; k1 < N1; k1++) z[k1] = b0[(k1 * N2 + j2) * TX + l];
+    DftN<N1, +1>::run(z);  // z[j1]: point j = j2 + N2 j1
+    if (l < nl) {
+#pragma unroll
+      for (int j1 = 0; j1 < N1; j1++) cg[base + (size_t)(j2 + N2 * j1) * pt_stride + l] = z[j1];
+    }
+  }
+  if (sa.calc_ev) ev_partial(sa, acc, sh_red, blk);
+}
+
+__device__ __forceinline__ void zsolve_four_step_dispatch(int code, float2* __restrict__ cg, int hx, int TX,
+                                                          long long pt_stride, long long outer_stride,
+                                                          const AxisArgs& ax, const SolveArgs& sa, int tx, int ty,
+                                                          int blk, float2* sm) {
+  switch (code) {
+#define PME_F4_CASE(c, a, b) \
+  case c: zsolve_four_step<a, b>(cg, hx, TX, pt_stride, outer_stride, ax, sa, tx, ty, blk, sm); break;
+    PME_F4_PAIRS(PME_F4_CASE)
+#undef PME_F4_CASE
+    default: break;
+  }
+}
+
+
 // ------------------------------------------------------------ y / z passes --
 
 // Column tile (tx, ty) of one grid: lines kx0 = tx*TX ..., outer index ty
@@ -588,6 +780,10 @@ template <int MODE, bool SPLIT>
 __device__ __forceinline__ void fft_cols_body(float2* __restrict__ cg, int n, int hx, int TX, long long pt_stride,
                                               long long outer_stride, const AxisArgs& ax, const SolveArgs& sa,
                                               int tx, int ty, int blk, float2* sm) {
+  if (MODE == COL_FWD_SOLVE_INV && !SPLIT && ax.f4 >= 0) {
+    zsolve_four_step_dispatch(ax.f4, cg, hx, TX, pt_stride, outer_stride, ax, sa, tx, ty, blk, sm);
+    return;
+  }
   __shared__ double sh_red[7 * (kBlock / 64)];
   const int tid = threadIdx.x;
   float2* tw = sm;

@@ -45,6 +45,7 @@ struct AxisArgs {
   const float2* tw;        // e^{-2 pi i k/ntw}, k < ntw (FP64-computed, stored FP32)
   int ntw;                 // table length (the grid edge)
   int prec;                // PREC_*: a radix-16 first stage runs on the matrix cores
+  int f4;                  // z pass: four-step factor pair (four_step_code), -1: Stockham stages
 };
 
 // Influence function + energy/virial (scalar_sum), CudaPMERecip.cu:594-901
@@ -200,6 +201,10 @@ hipError_t launch_fft_plane_fwd(const float* grid, float2* cg, int nx, int ny, i
                                 const AxisArgs& ay, hipStream_t s);
 hipError_t launch_fft_plane_inv(const float2* cg, float* grid, int nx, int ny, int nz, const AxisArgs& ax,
                                 const AxisArgs& ay, const EvReduce& red, hipStream_t s);
+// Factor pair code of the register four-step z pass for a z length n
+// (-1: none, the Stockham stages are used)
+int four_step_code(int n);
+
 // LDS bytes of a plane kernel over np planes per workgroup (0 when they do
 // not fit one workgroup)
 size_t fft_plane_lds_bytes(int nx, int ny, int nf_x, int np);
