@@ -279,6 +279,7 @@ struct RadixPlan {
   int rad[12];   // radix of each stage (2, 3, 4, 5, 8; 16 for a matrix-core first stage)
   int tstride;   // twiddle table stride: table length / n
   unsigned long long rcode;  // rad[] packed 5 bits per stage (read by the kernels: no indexed arrays)
+  int f4a, f4b;  // register four-step factors n = f4a * f4b (codelet lengths), 0: Stockham stages
 };
 
 __host__ __device__ inline int plan_radix(const RadixPlan& rp, int s) { return (int)((rp.rcode >> (5 * s)) & 31); }
@@ -484,17 +485,97 @@ __device__ __forceinline__ void mfma_dft16_stage(const float2* __restrict__ src,
   }
 }
 
+// ---- four-step transform of lines in LDS ----------------------------------
+// n = A B, j = j2 + B j1, k = k1 + A k2 (A, B codelet lengths).  Pass A: unit
+// (line l, j2) loads the A points j2 + B j1 into registers, runs DFT_A and
+// multiplies by w_n^{S j2 k1}, writing (k1, j2) to the other buffer; pass B:
+// unit (l, k1) runs DFT_B over j2 and writes X[k1 + A k2] back in natural
+// order.  Two LDS passes (two barriers) for the whole transform; consecutive
+// units take consecutive lines, so the accesses of a wave are contiguous.
+template <int A, int S>
+__device__ __forceinline__ void four_step_pass_a(const float2* __restrict__ src, float2* __restrict__ dst,
+                                                 const float2* __restrict__ tw, int B, int tstride, int nlines,
+                                                 int ls, int tid, int nthr) {
+  const SmallDiv dl = small_div(nlines);
+  const int Bls = mul24(B, ls);
+  for (int u = tid; u < B * nlines; u += nthr) {
+    const int j2 = sdiv(u, dl);
+    const int l = u - mul24(j2, nlines);
+    float2 x[A];
+    const float2* s0 = src + mul24(j2, ls) + l;
+#pragma unroll
+    for (int j1 = 0; j1 < A; j1++) x[j1] = s0[j1 * Bls];
+    DftN<A, S>::run(x);
+    float2* d0 = dst + mul24(j2, ls) + l;
+    const int ts = mul24(j2, tstride);
+    d0[0] = x[0];
+#pragma unroll
+    for (int k1 = 1; k1 < A; k1++) {
+      float2 w = tw[k1 * ts];
+      if (S > 0) w.y = -w.y;
+      d0[k1 * Bls] = cmul(x[k1], w);
+    }
+  }
+}
+template <int B, int S>
+__device__ __forceinline__ void four_step_pass_b(const float2* __restrict__ src, float2* __restrict__ dst, int A,
+                                                 int nlines, int ls, int tid, int nthr) {
+  const SmallDiv dl = small_div(nlines);
+  const int Als = mul24(A, ls);
+  for (int u = tid; u < A * nlines; u += nthr) {
+    const int k1 = sdiv(u, dl);
+    const int l = u - mul24(k1, nlines);
+    float2 x[B];
+    const float2* s0 = src + mul24(mul24(k1, B), ls) + l;
+#pragma unroll
+    for (int j2 = 0; j2 < B; j2++) x[j2] = s0[j2 * ls];
+    DftN<B, S>::run(x);
+    float2* d0 = dst + mul24(k1, ls) + l;
+#pragma unroll
+    for (int k2 = 0; k2 < B; k2++) d0[k2 * Als] = x[k2];
+  }
+}
+// codelet lengths of the four-step passes
+#define PME_F4_LENGTHS(X) X(2) X(3) X(4) X(5) X(6) X(8) X(9) X(10) X(12) X(16)
+template <int S>
+__device__ __forceinline__ float2* fft_lines_four_step(float2* buf0, float2* buf1, const float2* tw,
+                                                       const RadixPlan& rp, int nlines, int ls, int tid, int nthr) {
+  switch (rp.f4a) {
+#define PME_F4A(L) \
+  case L: four_step_pass_a<L, S>(buf0, buf1, tw, rp.f4b, rp.tstride, nlines, ls, tid, nthr); break;
+    PME_F4_LENGTHS(PME_F4A)
+#undef PME_F4A
+    default: break;
+  }
+  __syncthreads();
+  switch (rp.f4b) {
+#define PME_F4B(L) \
+  case L: four_step_pass_b<L, S>(buf1, buf0, rp.f4a, nlines, ls, tid, nthr); break;
+    PME_F4_LENGTHS(PME_F4B)
+#undef PME_F4B
+    default: break;
+  }
+  __syncthreads();
+  return buf0;
+}
+
 // Full transform of nlines lines held in LDS (buf0 -> result).  Returns the
 // buffer holding the result (buf0 or buf1).  Ends with a barrier.  With
 // SPLIT, prec != PREC_FP32 and a plan whose first radix is 16, that stage runs
 // on the matrix cores (mfma_dft16_stage).  Without SPLIT the matrix-core code
 // is not compiled in, which keeps the register budget (and so the occupancy)
-// of FP32-only kernels at what their Stockham stages need.
-template <int S, bool SPLIT = false>
+// of FP32-only kernels at what their Stockham stages need.  Lengths with a
+// four-step factor pair (rp.f4a > 0) run fft_lines_four_step instead of
+// the Stockham stages (except the matrix-core variant); FOUR_STEP = false
+// keeps the Stockham stages (the x pass over a few long rows per workgroup,
+// where the four-step units leave most threads idle).
+template <int S, bool SPLIT = false, bool FOUR_STEP = true>
 __device__ __forceinline__ float2* fft_lines(float2* buf0, float2* buf1, const float2* tw, const RadixPlan& rp,
                              int nlines, int ls, int tid, int nthr, int prec = PREC_FP32) {
   float2* src = buf0;
   float2* dst = buf1;
+  if (FOUR_STEP && rp.f4a > 0 && !(SPLIT && prec != PREC_FP32))
+    return fft_lines_four_step<S>(buf0, buf1, tw, rp, nlines, ls, tid, nthr);
   int p = 1;
   int s0 = 0;
   if (SPLIT && prec != PREC_FP32 && rp.nrad > 0 && plan_radix(rp, 0) == 16) {

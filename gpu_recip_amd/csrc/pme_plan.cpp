@@ -52,9 +52,26 @@ static void pack_radices(RadixPlan* rp) {
   for (int s = 0; s < rp->nrad; s++) rp->rcode |= (unsigned long long)rp->rad[s] << (5 * s);
 }
 
+// Factor pair of the register four-step transform: n = a * b with both
+// codelet lengths (2, 3, 4, 5, 6, 8, 9, 10, 12, 16), the most balanced such
+// pair; (0, 0) when there is none.
+static void four_step_factors(int n, int* fa, int* fb) {
+  static const int L[] = {2, 3, 4, 5, 6, 8, 9, 10, 12, 16};
+  *fa = *fb = 0;
+  int best = 1 << 30;
+  for (int a : L)
+    for (int b : L)
+      if (a * b == n && a <= b && b < best) {
+        best = b;
+        *fa = a;
+        *fb = b;
+      }
+}
+
 bool radix_plan(int n, int tstride, RadixPlan* rp) {
   rp->n = n;
   rp->tstride = tstride;
+  four_step_factors(n, &rp->f4a, &rp->f4b);
   rp->nrad = 0;
   int m = n;
   while (m % 8 == 0 && m > 8) { rp->rad[rp->nrad++] = 8; m /= 8; }
@@ -78,6 +95,7 @@ bool radix_plan_split(int n, int tstride, RadixPlan* rp) {
   }
   rp->n = n;
   rp->tstride = tstride;
+  rp->f4a = rp->f4b = 0;  // the matrix-core stage runs through the Stockham path
   rp->nrad = rest.nrad + 1;
   rp->rad[0] = 16;
   for (int s = 0; s < rest.nrad; s++) rp->rad[s + 1] = rest.rad[s];
