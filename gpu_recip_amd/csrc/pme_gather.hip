@@ -74,7 +74,11 @@ __device__ __forceinline__ void gather_sums(const float* tx, const float* dtx, c
 // gathered by ONE thread with its B-spline weights in registers, reading its
 // p x p stencil rows straight from LDS.  An atom whose current base index is
 // no longer in this brick (moved since the binning) reads HBM instead.
-template <int P>
+// FROM_REC: the coordinates are the binning's copy in the brick's record
+// store (contiguous per brick) instead of xyzq[atom] (scattered); only for
+// solves that bin and gather the same coordinates (the batched path: one
+// solve reads the caller's array once).
+template <int P, bool FROM_REC>
 __device__ __forceinline__ void gather_brick_body(const float4* __restrict__ xyzq, const BinArgs& a, int b, int wxm,
                                                   int wym, int wzm, const float* __restrict__ phi, float ccelec_f,
                                                   int ft, int stride, void* force, unsigned char* smem) {
@@ -83,7 +87,8 @@ __device__ __forceinline__ void gather_brick_body(const float4* __restrict__ xyz
   const int hxs = wxm + P - 1, hys = wym + P - 1, hzs = wzm + P - 1;
   float* pl = reinterpret_cast<float*>(smem);                          // halo brick
   int* s_atom = reinterpret_cast<int*>(pl + ((hxs * hys * hzs + 3) & ~3));  // [kSpreadBlock] home atoms
-  int* s_pre = s_atom + kSpreadBlock;                                   // [nchunk + 1]
+  int* s_slot = s_atom + kSpreadBlock;                                  // [kSpreadBlock] their records
+  int* s_pre = s_slot + kSpreadBlock;                                   // [nchunk + 1]
   int* s_base = s_pre + a.nchunk + 1;                                   // [nchunk]
 
   const int tid = threadIdx.x;
@@ -124,10 +129,11 @@ __device__ __forceinline__ void gather_brick_body(const float4* __restrict__ xyz
     __syncthreads();
     // 1. home atoms of this batch of records, compacted with a wave ballot
     const int v = base + tid;
-    int atom = 0;
+    int atom = 0, slot = 0;
     bool home = false;
     if (v < total) {
-      const int r = a.rec_idx[brick_record(a, v, s_pre, s_base)];
+      slot = (int)brick_record(a, v, s_pre, s_base);
+      const int r = a.rec_idx[slot];
       home = r < 0;
       atom = r & 0x7fffffff;
     }
@@ -135,12 +141,16 @@ __device__ __forceinline__ void gather_brick_body(const float4* __restrict__ xyz
     int wbase = 0;
     if ((tid & 63) == 0 && m != 0ull) wbase = atomicAdd(&s_cnt, __popcll(m));
     wbase = __shfl(wbase, 0, 64);
-    if (home) s_atom[wbase + __popcll(m & __lanemask_lt())] = atom;
+    if (home) {
+      const int k = wbase + __popcll(m & __lanemask_lt());
+      s_atom[k] = atom;
+      if (FROM_REC) s_slot[k] = slot;
+    }
     __syncthreads();
     // 2. one thread per home atom, weights in registers
     if (tid < s_cnt) {
       const int at = s_atom[tid];
-      const float4 p = xyzq[at];
+      const float4 p = FROM_REC ? a.rec_pos[s_slot[tid]] : xyzq[at];
       int ix, iy, iz;
       float fx, fy, fz;
       grid_index(p.x, a.r1, a.nx, ix, fx);
@@ -189,12 +199,12 @@ __global__ __launch_bounds__(kSpreadBlock) void k_gather_brick(const float4* __r
                                                                int wym, int wzm, const float* __restrict__ phi,
                                                                float ccelec_f, int ft, int stride, void* force) {
   extern __shared__ __align__(16) unsigned char smem[];
-  gather_brick_body<P>(xyzq, a, blockIdx.x, wxm, wym, wzm, phi, ccelec_f, ft, stride, force, smem);
+  gather_brick_body<P, false>(xyzq, a, blockIdx.x, wxm, wym, wzm, phi, ccelec_f, ft, stride, force, smem);
 }
 
 size_t gather_lds_bytes(int order, const int wmax[3], int nchunk) {
   const size_t halo = (size_t)(wmax[0] + order - 1) * (wmax[1] + order - 1) * (wmax[2] + order - 1);
-  return sizeof(float) * ((halo + 3) & ~(size_t)3) + sizeof(int) * kSpreadBlock +
+  return sizeof(float) * ((halo + 3) & ~(size_t)3) + 2 * sizeof(int) * kSpreadBlock +
          sizeof(int) * (2 * (size_t)nchunk + 1);
 }
 
@@ -227,7 +237,7 @@ __global__ __launch_bounds__(kSpreadBlock) void k_batch_gather(const SysDesc* __
   for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
     const WorkItem w = uniform_item(items[it]);
     const SysDesc& d = sys[w.sys];
-    gather_brick_body<P>(d.xyzq, d.bin, w.idx, d.wmax[0], d.wmax[1], d.wmax[2], d.grid, (float)kCcelec, d.force_type,
+    gather_brick_body<P, true>(d.xyzq, d.bin, w.idx, d.wmax[0], d.wmax[1], d.wmax[2], d.grid, (float)kCcelec, d.force_type,
                          d.stride, d.force, smem);
     __syncthreads();
   }
