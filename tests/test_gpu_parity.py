@@ -145,7 +145,15 @@ def test_spread_edge_inputs():
 
 # --------------------------------------------------------------------- FFT --
 
-@pytest.mark.parametrize("nfft", [(48, 48, 48), (108, 100, 96), (45, 27, 50), (64, 64, 64), (25, 36, 75), (144, 40, 30)])
+# Lengths whose transform runs as a register four-step (n = a*b, codelets
+# 2, 3, 4, 5, 6, 8, 9, 10, 12, 16) on every axis, in plane and separate-pass
+# layouts, next to Stockham-only lengths (75, 125) and the mixed shapes above.
+FOUR_STEP_SHAPES = [(32, 10, 12), (36, 18, 27), (40, 50, 54), (48, 60, 64), (81, 81, 45), (96, 100, 128),
+                    (256, 144, 160), (20, 16, 192), (150, 125, 75)]
+
+
+@pytest.mark.parametrize("nfft", [(48, 48, 48), (108, 100, 96), (45, 27, 50), (64, 64, 64), (25, 36, 75), (144, 40, 30)]
+                         + FOUR_STEP_SHAPES)
 def test_r2c_spectrum_and_roundtrip(nfft):
     torch, dev = torch_dev()
     box = tuple(0.9 * n for n in nfft)
@@ -201,6 +209,12 @@ def test_full_solve_vs_oracle(name, make):
     ((45, 27, 25), (40.0, 24.0, 22.0), 6),
     ((8, 8, 8), (12.0, 12.0, 12.0), 4),
     ((60, 16, 30), (55.0, 15.0, 27.0), 4),
+    # fused z pass as a register four-step: 75 = 5*15, 81 = 9*9, 128 = 8*16, 160 = 10*16, 12 = 3*4
+    ((40, 36, 75), (36.0, 33.0, 68.0), 4),
+    ((30, 32, 81), (27.0, 29.0, 73.0), 6),
+    ((24, 30, 128), (22.0, 27.0, 115.0), 4),
+    ((20, 24, 160), (18.0, 22.0, 144.0), 8),
+    ((40, 45, 12), (36.0, 41.0, 11.0), 4),
 ])
 def test_full_solve_shapes(nfft, box, order):
     s = small_system(nfft, box, order, 1500, 19)
