@@ -100,6 +100,7 @@ __device__ __forceinline__ void gather_brick_body(const float4* __restrict__ xyz
   // stage brick + upper halo (periodic) in LDS, kStageU loads in flight
   const int hx = wx + P - 1, hy = wy + P - 1, hz = wz + P - 1;
   const int nh = hx * hy * hz;
+  const int nxny = mul24(a.nx, a.ny);  // grid offsets fit 32 bits (edges <= 512)
   const SmallDiv dhx = small_div(hx), dhy = small_div(hy);
   for (int i0 = tid; i0 < nh; i0 += kStageU * kSpreadBlock) {
     float v[kStageU];
@@ -116,7 +117,7 @@ __device__ __forceinline__ void gather_brick_body(const float4* __restrict__ xyz
       if (gy >= a.ny) gy -= a.ny;
       if (gz >= a.nz) gz -= a.nz;
       dst[u] = mul24(mul24(lz, hys) + ly, hxs) + lx;
-      v[u] = (i < nh) ? phi[(size_t)gx + (size_t)a.nx * ((size_t)gy + (size_t)a.ny * gz)] : 0.f;
+      v[u] = (i < nh) ? phi[gx + mul24(a.nx, gy) + mul24(nxny, gz)] : 0.f;
     }
 #pragma unroll
     for (int u = 0; u < kStageU; u++)

@@ -223,7 +223,9 @@ __device__ __forceinline__ void spread_brick_body(const BinArgs& a, int b, int w
   // the brick is complete: one coalesced store per grid point, no memset
   // needed.  With one brick along x the high margin is the periodic image of
   // the row start.
+  // Grid offsets in 32 bits (edges <= 512: nx*ny < 2^19, offsets < 2^27).
   const int nb = wx * wy * wz;
+  const int nxny = mul24(a.nx, a.ny);
   const SmallDiv dwx = small_div(wx), dwy = small_div(wy);
   for (int i = tid; i < nb; i += kSpreadBlock) {
     const int t = sdiv(i, dwx);
@@ -233,7 +235,7 @@ __device__ __forceinline__ void spread_brick_body(const BinArgs& a, int b, int w
     const unsigned long long* row = acc + mul24(lz, plane) + mul24(ly, bxs) + (P - 1);
     unsigned long long v = row[lx];
     if (wrapx && lx < P - 1) v += row[lx + a.nx];
-    grid[(size_t)(x0 + lx) + (size_t)a.nx * ((size_t)(y0 + ly) + (size_t)a.ny * (z0 + lz))] = from_fixed40(v);
+    grid[(x0 + lx) + mul24(a.nx, y0 + ly) + mul24(nxny, z0 + lz)] = from_fixed40(v);
   }
 }
 
