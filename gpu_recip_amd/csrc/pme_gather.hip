@@ -123,7 +123,7 @@ __device__ __forceinline__ void gather_brick_body(const float4* __restrict__ xyz
     for (int u = 0; u < kStageU; u++)
       if (i0 + u * kSpreadBlock < nh) pl[dst[u]] = v[u];
   }
-  const int total = brick_segments<kSpreadBlock>(a, b, s_pre, s_base, sh_scan);  // ends with a barrier
+  const int total = brick_segments<kSpreadBlock>(a, brick_seg_load(a, b), s_pre, s_base, sh_scan);  // ends with a barrier
 
   for (int base = 0; base < total; base += kSpreadBlock) {
     if (tid == 0) s_cnt = 0;

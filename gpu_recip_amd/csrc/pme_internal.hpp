@@ -104,14 +104,28 @@ enum BatchStage { BS_BIN = 0, BS_SPREAD, BS_FFTX, BS_FFTY, BS_FFTZ, BS_GATHER, B
 // Segments of brick b in the chunked record store: s_base[w] = first record
 // of b in chunk w's region, s_pre[w] = exclusive prefix of the counts over w
 // (s_pre[nchunk] = total).  Returns the total.  NT == blockDim.x.
+// The segment bounds of chunk w = threadIdx.x (every chunk has a thread:
+// nchunk <= kMaxChunks <= NT), loaded into registers so that a caller can
+// issue the loads first and overlap their latency with other work.
+struct SegBounds {
+  int lo, hi;
+};
+__device__ __forceinline__ SegBounds brick_seg_load(const BinArgs& a, int b) {
+  SegBounds s{0, 0};
+  const int w = threadIdx.x;
+  if (w < a.nchunk) {
+    const int* seg = a.seg_off + (size_t)w * (a.nbx * a.nby * a.nbz + 1);
+    s.lo = seg[b];
+    s.hi = seg[b + 1];
+  }
+  return s;
+}
 template <int NT>
-__device__ __forceinline__ int brick_segments(const BinArgs& a, int b, int* s_pre, int* s_base, int* sh_scan) {
-  const int nbins = a.nbx * a.nby * a.nbz;
-  for (int w = threadIdx.x; w < a.nchunk; w += NT) {
-    const int* seg = a.seg_off + (size_t)w * (nbins + 1);
-    const int lo = seg[b];
-    s_base[w] = lo;
-    s_pre[w] = seg[b + 1] - lo;
+__device__ __forceinline__ int brick_segments(const BinArgs& a, const SegBounds& sb, int* s_pre, int* s_base,
+                                              int* sh_scan) {
+  if ((int)threadIdx.x < a.nchunk) {
+    s_base[threadIdx.x] = sb.lo;
+    s_pre[threadIdx.x] = sb.hi - sb.lo;
   }
   __syncthreads();
   const int total = block_exclusive_scan<NT>(s_pre, a.nchunk, sh_scan);
@@ -246,6 +260,7 @@ constexpr int kPlaneBlock = 1024;   // threads per xy plane FFT workgroup
 constexpr int kPlaneGroupMax = 4;   // z planes per batched plane workgroup (at most)
 constexpr size_t kPlaneLdsBudget = 150 * 1024;  // the plane kernels run one workgroup per CU at most
 constexpr int kMaxChunks = 1024;   // binning workgroups per solve (chunk grows with n)
+static_assert(kMaxChunks <= kSpreadBlock, "brick_segments: one thread per chunk");
 constexpr int kMinChunk = 1024;    // atoms per binning workgroup
 
 }  // namespace pme

@@ -171,8 +171,9 @@ __device__ __forceinline__ void spread_brick_body(const BinArgs& a, int b, int w
   const bool wrapx = (a.nbx == 1), wrapy = (a.nby == 1), wrapz = (a.nbz == 1);
   const int plane = mul24(wym, bxs);
 
+  const SegBounds sb = brick_seg_load(a, b);  // in flight while the brick is cleared
   for (int i = tid; i < acc_n; i += kSpreadBlock) acc[i] = 0ull;
-  const int total = brick_segments<kSpreadBlock>(a, b, s_pre, s_base, sh_scan);  // ends with a barrier
+  const int total = brick_segments<kSpreadBlock>(a, sb, s_pre, s_base, sh_scan);  // ends with a barrier
 
   for (int base = 0; base < total; base += kSpreadBlock) {
     // records of a batch in a strided order (stride 17, coprime with the
@@ -223,19 +224,25 @@ __device__ __forceinline__ void spread_brick_body(const BinArgs& a, int b, int w
   // the brick is complete: one coalesced store per grid point, no memset
   // needed.  With one brick along x the high margin is the periodic image of
   // the row start.
-  // Grid offsets in 32 bits (edges <= 512: nx*ny < 2^19, offsets < 2^27).
-  const int nb = wx * wy * wz;
+  // Two x-neighbours per thread (the second can fall on the row's margin,
+  // inside the row's LDS stride, and is then not stored).  Grid offsets in
+  // 32 bits (edges <= 512: nx*ny < 2^19, offsets < 2^27).
+  const int hw = (wx + 1) >> 1;
+  const int npair = hw * wy * wz;
   const int nxny = mul24(a.nx, a.ny);
-  const SmallDiv dwx = small_div(wx), dwy = small_div(wy);
-  for (int i = tid; i < nb; i += kSpreadBlock) {
-    const int t = sdiv(i, dwx);
-    const int lx = i - t * wx;
+  const SmallDiv dhw = small_div(hw), dwy = small_div(wy);
+  for (int i = tid; i < npair; i += kSpreadBlock) {
+    const int t = sdiv(i, dhw);
+    const int lx = 2 * (i - t * hw);
     const int lz = sdiv(t, dwy);
     const int ly = t - lz * wy;
     const unsigned long long* row = acc + mul24(lz, plane) + mul24(ly, bxs) + (P - 1);
-    unsigned long long v = row[lx];
-    if (wrapx && lx < P - 1) v += row[lx + a.nx];
-    grid[(x0 + lx) + mul24(a.nx, y0 + ly) + mul24(nxny, z0 + lz)] = from_fixed40(v);
+    unsigned long long v0 = row[lx], v1 = row[lx + 1];
+    if (wrapx && lx < P - 1) v0 += row[lx + a.nx];
+    if (wrapx && lx + 1 < P - 1) v1 += row[lx + 1 + a.nx];
+    float* g = grid + ((x0 + lx) + mul24(a.nx, y0 + ly) + mul24(nxny, z0 + lz));
+    g[0] = from_fixed40(v0);
+    if (lx + 1 < wx) g[1] = from_fixed40(v1);
   }
 }
 
