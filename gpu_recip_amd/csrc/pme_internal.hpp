@@ -255,12 +255,12 @@ size_t fft_x_lds_bytes(int nf, int NR, int ntw);
 size_t fft_cols_lds_bytes(int n, int TX);
 
 constexpr int kSpreadBlock = 1024;  // threads per spread / gather workgroup (16 waves per CU)
-constexpr int kBinBlock = 1024;     // threads per binning workgroup (one atom each at the default chunk)
+constexpr int kBinBlock = 512;     // threads per binning workgroup (one atom each at the default chunk)
 constexpr int kPlaneBlock = 1024;   // threads per xy plane FFT workgroup
 constexpr int kPlaneGroupMax = 4;   // z planes per batched plane workgroup (at most)
 constexpr size_t kPlaneLdsBudget = 150 * 1024;  // the plane kernels run one workgroup per CU at most
 constexpr int kMaxChunks = 1024;   // binning workgroups per solve (chunk grows with n)
 static_assert(kMaxChunks <= kSpreadBlock, "brick_segments: one thread per chunk");
-constexpr int kMinChunk = 1024;    // atoms per binning workgroup
+constexpr int kMinChunk = 512;    // atoms per binning workgroup
 
 }  // namespace pme
