@@ -612,7 +612,7 @@ extern "C" pme_status pme_batch_autotune(pme_batch_t b, const char* cache_path, 
       for (int wpc : {0, 4}) {
         if (pme_batch_set_tuning(b, brick, tile, wpc) != PME_OK) continue;
         float ms = 0.f;
-        if (time_solves(b, 2, 5, &ms) != PME_OK) continue;
+        if (time_solves(b, 3, 10, &ms) != PME_OK) continue;
         if (ms < best) {
           best = ms;
           bb = brick;
