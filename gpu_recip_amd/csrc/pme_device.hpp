@@ -82,22 +82,14 @@ __device__ __forceinline__ void bspline_theta_dtheta(float w, float* th, float* 
   th[0] = div * (1.0f - w) * th[0];
 }
 
-// 64-bit fixed point with scale 2^40 (gpu_utils.h:9): the product by 2^40 is
-// exact in FP32, then rounded to the nearest integer (ties to even), as
+// 64-bit fixed point with scale 2^40 (gpu_utils.h:9): a contribution v is
+// stored as round(v * 2^40) to the nearest integer (ties to even), as
 // lliroundf(FORCE_SCALE*val) in write_grid<long long> (CudaPMERecip.cu:51-58).
-// round(v * 2^40) to the nearest integer (ties to even), as 64-bit two's
-// complement: v*2^40 + 1.5*2^52 in FP64 puts the rounded integer in the low
-// mantissa bits (exact for |v| < 2^11); subtracting the constant's bit pattern
-// leaves it.  Same value as llrintf(v * 2^40), in 3 instructions.
-__device__ __forceinline__ long long to_fixed40(float v) {
-  const double t = __fma_rn((double)v, 1099511627776.0, 6755399441055744.0);
-  return (long long)((unsigned long long)__double_as_longlong(t) - 0x4338000000000000ull);
-}
-// The same for a value already multiplied by 2^40 (|v| < 2^51): one FP64 add
-// of 1.5*2^52 rounds it to an integer in the low mantissa bits.  With the
-// scale folded into an operand beforehand (a power-of-two factor, so every
-// FP32 product is the 2^40 multiple of the unscaled one), this gives
-// to_fixed40's value with one constant operand instead of two.
+// The caller folds 2^40 into an operand beforehand (a power-of-two factor,
+// so the FP32 product is exactly the 2^40 multiple of the unscaled one) and
+// passes v * 2^40 (|v * 2^40| < 2^51): one FP64 add of 1.5*2^52 puts the
+// rounded integer in the low mantissa bits as 64-bit two's complement, and
+// subtracting the constant's bit pattern leaves it.
 __device__ __forceinline__ long long to_fixed_scaled(float v) {
   const double t = __dadd_rn((double)v, 6755399441055744.0);
   return (long long)((unsigned long long)__double_as_longlong(t) - 0x4338000000000000ull);
