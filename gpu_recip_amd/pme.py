@@ -164,7 +164,7 @@ class PmeBatch:
 
     Systems are admitted with add() against the HBM budget and stay resident;
     solve() enqueues one solve of a subset (all by default) on the batch's
-    stream, results() waits and returns (energy[n], virial[n, 9]) as lists.
+    stream, results() waits and returns (energy[n], virial[n, 9]) as float64 arrays.
     """
 
     def __init__(self, hbm_budget_bytes: int, stream=None):
