@@ -114,6 +114,51 @@ def test_plane_groups_match_single(reps):
             assert rel_fro(vk, one["virial"]) < 1e-12, s.name
 
 
+def test_batch_edge_systems():
+    """A batch holding an empty system (no atoms, null pointers), atoms far
+    outside the box (any periodic image) and a clustered system whose atoms
+    all fall in one brick: the empty system gives zero energy and virial, the
+    others the single-system solver's forces bit for bit and the oracle's
+    within FP32 rounding; graph and eager solves alike."""
+    torch, dev = torch_dev()
+    rng = np.random.default_rng(11)
+    nfft, box = (48, 40, 36), (40.0, 33.0, 30.0)
+    far = np.zeros((300, 4), np.float32)
+    far[:, :3] = rng.uniform(-500, 500, (300, 3))
+    far[:, 3] = rng.uniform(-1, 1, 300)
+    cluster = np.zeros((6000, 4), np.float32)
+    cluster[:, :3] = rng.uniform(0, 1.5, (6000, 3))
+    cluster[:, 3] = rng.uniform(-1, 1, 6000)
+    sl = [systems.PmeSystem("empty", np.zeros((0, 4), np.float32), box, nfft, 4, 0.34),
+          systems.PmeSystem("far", far, box, nfft, 6, 0.34),
+          systems.PmeSystem("cluster", cluster, box, nfft, 4, 0.34)]
+    b = PmeBatch(4 << 30)
+    reg = []
+    for s in sl:
+        if s.natom == 0:
+            d = PmeBatch.desc(s.nfft, s.order, s.kappa, s.recip, None, 0, None, L.FORCE_FLOAT, 0)
+            reg.append((b.add(d), None, None, 0))
+        else:
+            reg.append(_add(b, s, torch, dev))
+    ids = [r[0] for r in reg]
+    for graph in (True, False):
+        b.set_graph(graph)
+        b.solve()
+        e, v = b.results(ids)
+        assert e[0] == 0.0 and np.all(v[0] == 0.0)
+        for k in (1, 2):
+            s = sl[k]
+            _, x, f, stride = reg[k]
+            fb = _forces(f, s.natom, stride)
+            one = run_solve(s.nfft, s.order, s.kappa, s.box, s.xyzq)
+            assert np.array_equal(fb, one["force"]), (s.name, graph)
+            assert rel_err(e[k], one["energy"]) < 1e-12, s.name
+            o = O.solve(O.OracleParams(*s.nfft, s.order, s.kappa, s.box, index_mode=O.INDEX_F32), s.xyzq)
+            frms = rel_rms(fb, o["force"])
+            say(f"  {s.name} graph={graph}: force rel-RMS vs oracle {frms:.2e}")
+            assert frms < 1e-4 and rel_err(e[k], o["energy"]) < 1e-4
+
+
 def test_batched_config_vs_oracle():
     """The 64-system batched workload: every system against the single-system
     solver, eight of them against the FP64 oracle (exact coordinates)."""

@@ -50,6 +50,40 @@ def test_alone_vs_mix():
     srv.close()
 
 
+def test_empty_request_in_a_burst():
+    """A request with no atoms (null pointers) served in a burst with others:
+    zero energy and virial for it, the others unchanged against serving them
+    without it."""
+    torch, dev = torch_dev()
+    reqs = _small_schedule(n=6, nmax=20000, seed=9)
+    serving.materialise(reqs, dev)
+    srv = PmeServer(4 << 30, nlanes=2)
+    serving.run_burst(srv, reqs)
+    ref = [(_forces(r), r.result[0], r.result[1]) for r in reqs]
+    srv.close()
+    srv = PmeServer(4 << 30, nlanes=2)
+    r0 = reqs[0]
+    empty = L.SystemDesc()
+    empty.nfftx = empty.nffty = empty.nfftz = r0.nfft
+    empty.order = r0.order
+    empty.kappa = r0.kappa
+    for i in range(9):
+        empty.recip[i] = (1.0 / r0.box) if i in (0, 4, 8) else 0.0
+    empty.ncoord = 0
+    empty.force_type = L.FORCE_FLOAT
+    empty.calc_energy_virial = 1
+    for r in reqs:
+        r.force.zero_()
+    tickets = [srv.submit(10 ** 6, empty)] + [srv.submit(r.key, serving.desc_of(r)) for r in reqs]
+    srv.drain()
+    e, v, _ = srv.wait(tickets[0])
+    assert e == 0.0 and all(x == 0.0 for x in v)
+    for r, t, (f, e0, v0) in zip(reqs, tickets[1:], ref):
+        e1, v1, _ = srv.wait(t)
+        assert np.array_equal(_forces(r), f) and e1 == e0 and v1 == v0
+    srv.close()
+
+
 def test_served_equals_plain_and_oracle():
     torch, dev = torch_dev()
     reqs = _small_schedule(n=10, nmax=30000, seed=5)
