@@ -86,13 +86,16 @@ __device__ __forceinline__ void bspline_theta_dtheta(float w, float* th, float* 
 // stored as round(v * 2^40) to the nearest integer (ties to even), as
 // lliroundf(FORCE_SCALE*val) in write_grid<long long> (CudaPMERecip.cu:51-58).
 // The caller folds 2^40 into an operand beforehand (a power-of-two factor,
-// so the FP32 product is exactly the 2^40 multiple of the unscaled one) and
-// passes v * 2^40 (|v * 2^40| < 2^51): one FP64 add of 1.5*2^52 puts the
-// rounded integer in the low mantissa bits as 64-bit two's complement, and
-// subtracting the constant's bit pattern leaves it.
-__device__ __forceinline__ long long to_fixed_scaled(float v) {
-  const double t = __dadd_rn((double)v, 6755399441055744.0);
-  return (long long)((unsigned long long)__double_as_longlong(t) - 0x4338000000000000ull);
+// so the product is exactly the 2^40 multiple of the unscaled one) and
+// passes the two FP32 factors a, b widened to FP64 (|a b| < 2^51).  The
+// product of two FP32 values has at most 48 significant bits, so it is exact
+// in FP64, and one FMA with 1.5*2^52 rounds it once, to the nearest integer
+// (ties to even), into the low mantissa bits as 64-bit two's complement;
+// subtracting the constant's bit pattern leaves it.  The constant's low word
+// is zero, so the subtraction is one 32-bit add on the high word.
+__device__ __forceinline__ unsigned long long to_fixed_product(double a, double b) {
+  const double t = __fma_rn(a, b, 6755399441055744.0);
+  return (unsigned long long)__double_as_longlong(t) - 0x4338000000000000ull;
 }
 // back to FP32: ((float)val)*2^-40 as reduce_force<long long, float> (reduce.cu:66-78)
 __device__ __forceinline__ float from_fixed40(unsigned long long u) {

@@ -84,12 +84,12 @@ __device__ __forceinline__ void gather_brick_body(const float4* __restrict__ xyz
                                                   int ft, int stride, void* force, unsigned char* smem) {
   __shared__ int sh_scan[kSpreadBlock / 64];
   __shared__ int s_cnt;
-  const int hxs = wxm + P - 1, hys = wym + P - 1, hzs = wzm + P - 1;
-  float* pl = reinterpret_cast<float*>(smem);                          // halo brick
-  int* s_atom = reinterpret_cast<int*>(pl + ((hxs * hys * hzs + 3) & ~3));  // [kSpreadBlock] home atoms
-  int* s_slot = s_atom + kSpreadBlock;                                  // [kSpreadBlock] their records
-  int* s_pre = s_slot + kSpreadBlock;                                   // [nchunk + 1]
-  int* s_base = s_pre + a.nchunk + 1;                                   // [nchunk]
+  const int hmax = (wxm + P - 1) * (wym + P - 1) * (wzm + P - 1);
+  float* pl = reinterpret_cast<float*>(smem);                             // halo brick, [hz][hy][hx]
+  float4* s_pos = reinterpret_cast<float4*>(pl + ((hmax + 3) & ~3));      // [kGatherCap] home atoms' coordinates
+  int* s_atom = reinterpret_cast<int*>(s_pos + kGatherCap);               // [kGatherCap] their indices
+  int* s_pre = s_atom + kGatherCap;                                        // [nchunk + 1]
+  int* s_base = s_pre + a.nchunk + 1;                                      // [nchunk]
 
   const int tid = threadIdx.x;
   const int bxi = b % a.nbx, byi = (b / a.nbx) % a.nby, bzi = b / (a.nbx * a.nby);
@@ -97,16 +97,20 @@ __device__ __forceinline__ void gather_brick_body(const float4* __restrict__ xyz
   const int y0 = brick_lo(byi, a.ny, a.nby), wy = brick_lo(byi + 1, a.ny, a.nby) - y0;
   const int z0 = brick_lo(bzi, a.nz, a.nbz), wz = brick_lo(bzi + 1, a.nz, a.nbz) - z0;
 
-  // stage brick + upper halo (periodic) in LDS, kStageU loads in flight
+  // the record segment bounds are in flight while the halo is staged
+  const SegBounds sb = brick_seg_load(a, b);
+
+  // stage brick + upper halo (periodic) in LDS in this brick's own [hz][hy][hx]
+  // layout, so LDS word i is halo point i; up to kGatherStageU loads in
+  // flight per thread (one round for halos up to kGatherStageU * kSpreadBlock)
   const int hx = wx + P - 1, hy = wy + P - 1, hz = wz + P - 1;
   const int nh = hx * hy * hz;
   const int nxny = mul24(a.nx, a.ny);  // grid offsets fit 32 bits (edges <= 512)
   const SmallDiv dhx = small_div(hx), dhy = small_div(hy);
-  for (int i0 = tid; i0 < nh; i0 += kStageU * kSpreadBlock) {
-    float v[kStageU];
-    int dst[kStageU];
+  for (int i0 = tid; i0 < nh; i0 += kGatherStageU * kSpreadBlock) {
+    float v[kGatherStageU];
 #pragma unroll
-    for (int u = 0; u < kStageU; u++) {
+    for (int u = 0; u < kGatherStageU; u++) {
       const int i = i0 + u * kSpreadBlock;
       const int t = sdiv(i, dhx);
       const int lx = i - t * hx;
@@ -116,42 +120,69 @@ __device__ __forceinline__ void gather_brick_body(const float4* __restrict__ xyz
       if (gx >= a.nx) gx -= a.nx;
       if (gy >= a.ny) gy -= a.ny;
       if (gz >= a.nz) gz -= a.nz;
-      dst[u] = mul24(mul24(lz, hys) + ly, hxs) + lx;
       v[u] = (i < nh) ? phi[gx + mul24(a.nx, gy) + mul24(nxny, gz)] : 0.f;
     }
 #pragma unroll
-    for (int u = 0; u < kStageU; u++)
-      if (i0 + u * kSpreadBlock < nh) pl[dst[u]] = v[u];
+    for (int u = 0; u < kGatherStageU; u++) {
+      const int i = i0 + u * kSpreadBlock;
+      if (i < nh) pl[i] = v[u];
+    }
   }
-  const int total = brick_segments<kSpreadBlock>(a, brick_seg_load(a, b), s_pre, s_base, sh_scan);  // ends with a barrier
+  const int total = brick_segments<kSpreadBlock>(a, sb, s_pre, s_base, sh_scan);  // ends with a barrier
 
-  for (int base = 0; base < total; base += kSpreadBlock) {
+  for (int base = 0; base < total; base += kGatherCap) {
     if (tid == 0) s_cnt = 0;
     __syncthreads();
-    // 1. home atoms of this batch of records, compacted with a wave ballot
-    const int v = base + tid;
-    int atom = 0, slot = 0;
-    bool home = false;
-    if (v < total) {
-      slot = (int)brick_record(a, v, s_pre, s_base);
-      const int r = a.rec_idx[slot];
-      home = r < 0;
-      atom = r & 0x7fffffff;
+    // 1. the home atoms of up to kGatherCap records (kRecPerThread per
+    //    thread), compacted with wave ballots into s_atom / s_pos.  Every
+    //    load of the round is issued before the first is used; the batched
+    //    path reads the record's coordinate copy beside its index.
+    int atom[kRecPerThread];
+    bool home[kRecPerThread];
+    float4 pos[kRecPerThread];
+#pragma unroll
+    for (int r = 0; r < kRecPerThread; r++) {
+      const int v = base + r * kSpreadBlock + tid;
+      atom[r] = 0;  // not a home copy
+      pos[r] = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (v < total) {
+        const size_t slot = brick_record(a, v, s_pre, s_base);
+        atom[r] = a.rec_idx[slot];
+        if (FROM_REC) pos[r] = a.rec_pos[slot];
+      }
     }
-    const unsigned long long m = __ballot(home);
+#pragma unroll
+    for (int r = 0; r < kRecPerThread; r++) {
+      home[r] = atom[r] < 0;  // bit 31: the atom's home copy
+      atom[r] &= 0x7fffffff;
+      if (!FROM_REC && home[r]) pos[r] = xyzq[atom[r]];
+    }
+    unsigned long long m[kRecPerThread];
+    int nwave = 0;
+#pragma unroll
+    for (int r = 0; r < kRecPerThread; r++) {
+      m[r] = __ballot(home[r]);
+      nwave += __popcll(m[r]);
+    }
     int wbase = 0;
-    if ((tid & 63) == 0 && m != 0ull) wbase = atomicAdd(&s_cnt, __popcll(m));
+    if ((tid & 63) == 0 && nwave != 0) wbase = atomicAdd(&s_cnt, nwave);
     wbase = __shfl(wbase, 0, 64);
-    if (home) {
-      const int k = wbase + __popcll(m & __lanemask_lt());
-      s_atom[k] = atom;
-      if (FROM_REC) s_slot[k] = slot;
+    const unsigned long long lt = __lanemask_lt();
+#pragma unroll
+    for (int r = 0; r < kRecPerThread; r++) {
+      if (home[r]) {
+        const int k = wbase + __popcll(m[r] & lt);
+        s_atom[k] = atom[r];
+        s_pos[k] = pos[r];
+      }
+      wbase += __popcll(m[r]);
     }
     __syncthreads();
     // 2. one thread per home atom, weights in registers
-    if (tid < s_cnt) {
-      const int at = s_atom[tid];
-      const float4 p = FROM_REC ? a.rec_pos[s_slot[tid]] : xyzq[at];
+    const int cnt = s_cnt;
+    for (int k = tid; k < cnt; k += kSpreadBlock) {
+      const int at = s_atom[k];
+      const float4 p = s_pos[k];
       int ix, iy, iz;
       float fx, fy, fz;
       grid_index(p.x, a.r1, a.nx, ix, fx);
@@ -170,8 +201,8 @@ __device__ __forceinline__ void gather_brick_body(const float4* __restrict__ xyz
       if (dz >= wz) dz -= a.nz;
       float f1, f2, f3;
       if (dx >= 0 && dy >= 0 && dz >= 0) {
-        const float* col = pl + mul24(mul24(dz, hys) + dy, hxs) + dx;
-        const int rs = hxs, ps = mul24(hys, hxs);
+        const float* col = pl + mul24(mul24(dz, hy) + dy, hx) + dx;
+        const int rs = hx, ps = mul24(hy, hx);
         gather_sums<P>(tx, dtx, ty, dty, tz, dtz,
                        [&](int jx, int jy, int jz) { return col[jz * ps + jy * rs + jx]; }, f1, f2, f3);
       } else {
@@ -205,7 +236,7 @@ __global__ __launch_bounds__(kSpreadBlock) void k_gather_brick(const float4* __r
 
 size_t gather_lds_bytes(int order, const int wmax[3], int nchunk) {
   const size_t halo = (size_t)(wmax[0] + order - 1) * (wmax[1] + order - 1) * (wmax[2] + order - 1);
-  return sizeof(float) * ((halo + 3) & ~(size_t)3) + 2 * sizeof(int) * kSpreadBlock +
+  return sizeof(float) * ((halo + 3) & ~(size_t)3) + (sizeof(float4) + sizeof(int)) * kGatherCap +
          sizeof(int) * (2 * (size_t)nchunk + 1);
 }
 

@@ -255,6 +255,9 @@ size_t fft_x_lds_bytes(int nf, int NR, int ntw);
 size_t fft_cols_lds_bytes(int n, int TX);
 
 constexpr int kSpreadBlock = 1024;  // threads per spread / gather workgroup (16 waves per CU)
+constexpr int kRecPerThread = 2;    // gather: brick records read per thread per round
+constexpr int kGatherCap = kRecPerThread * kSpreadBlock;  // gather: home atoms per round (LDS lists)
+constexpr int kGatherStageU = 24;   // gather: halo loads in flight per thread (one round up to 24 K points)
 constexpr int kBinBlock = 512;     // threads per binning workgroup (one atom each at the default chunk)
 constexpr int kPlaneBlock = 1024;   // threads per xy plane FFT workgroup
 constexpr int kPlaneGroupMax = 4;   // z planes per batched plane workgroup (at most)

@@ -147,11 +147,66 @@ __device__ __forceinline__ int local_offset(int i, int x0, int w, int n) {
 // rows of consecutive lanes fall in different banks.
 __host__ __device__ constexpr int spread_xstride(int wxm, int P) { return (wxm + 2 * (P - 1)) | 1; }
 
+// One record's p^3 stencil added into the brick accumulator acc (the x
+// margin absorbs rows that leave the brick in x; rows outside it in y or z
+// are skipped).
+template <int P>
+__device__ __forceinline__ void spread_record(const BinArgs& a, float4 p, int x0, int wx, int y0, int wy, int z0,
+                                              int wz, bool wrapy, bool wrapz, int bxs, int plane,
+                                              unsigned long long* __restrict__ acc) {
+  int ix, iy, iz;
+  float fx, fy, fz;
+  grid_index(p.x, a.r1, a.nx, ix, fx);
+  grid_index(p.y, a.r2, a.ny, iy, fy);
+  grid_index(p.z, a.r3, a.nz, iz, fz);
+  float tx[P], ty[P], tz[P];
+  bspline_theta<P>(fx, tx);
+  bspline_theta<P>(fy, ty);
+  bspline_theta<P>(fz, tz);
+  // q * 2^40 (exact): the fixed-point scale rides on theta_x, widened to
+  // FP64 once per record so each stencil point costs one FMA
+  const float qs = p.w * 1099511627776.0f;
+  double txd[P];
+#pragma unroll
+  for (int j = 0; j < P; j++) txd[j] = (double)(tx[j] * qs);
+  // x offset into the margin-padded row: in [0, wx + 2(P-1)) always
+  const int dx = local_offset(safe_index(ix, a.nx), x0, wx, a.nx) + (P - 1);
+  const int dy = local_offset(safe_index(iy, a.ny), y0, wy, a.ny);
+  const int dz = local_offset(safe_index(iz, a.nz), z0, wz, a.nz);
+  // stencil rows inside the brick (bit j of ym / zm) and their LDS offsets,
+  // once per record instead of per row
+  int yo[P], zo[P];
+  unsigned ym = 0u, zm = 0u;
+#pragma unroll
+  for (int j = 0; j < P; j++) {
+    int ly = dy + j;
+    ly -= (wrapy && ly >= wy) ? a.ny : 0;
+    ym |= (unsigned)(ly >= 0 && ly < wy) << j;
+    yo[j] = mul24(ly, bxs);
+    int lz = dz + j;
+    lz -= (wrapz && lz >= wz) ? a.nz : 0;
+    zm |= (unsigned)(lz >= 0 && lz < wz) << j;
+    zo[j] = mul24(lz, plane) + dx;
+  }
+#pragma unroll
+  for (int jz = 0; jz < P; jz++) {
+    if (!((zm >> jz) & 1u)) continue;
+#pragma unroll
+    for (int jy = 0; jy < P; jy++) {
+      if (!((ym >> jy) & 1u)) continue;
+      const double c = (double)(ty[jy] * tz[jz]);
+      unsigned long long* row = acc + (zo[jz] + yo[jy]);
+#pragma unroll
+      for (int jx = 0; jx < P; jx++) atomicAdd(&row[jx], to_fixed_product(txd[jx], c));
+    }
+  }
+}
+
 // Brick b of one grid (one kSpreadBlock-thread workgroup); smem: the
 // workgroup's dynamic LDS (spread_lds_bytes).  Each thread takes one record
-// (an atom whose stencil touches the brick), keeps q*theta_x, theta_y,
-// theta_z in registers and adds its p^3 stencil into the brick, the stencil
-// rows outside the brick in y or z masked off (x outside lands in the margin).
+// per batch (an atom whose stencil touches the brick), keeps q*theta_x,
+// theta_y, theta_z in registers and adds its p^3 stencil into the brick
+// (spread_record).
 template <int P>
 __device__ __forceinline__ void spread_brick_body(const BinArgs& a, int b, int wxm, int wym, int wzm,
                                                   float* __restrict__ grid, unsigned char* smem) {
@@ -184,40 +239,8 @@ __device__ __forceinline__ void spread_brick_body(const BinArgs& a, int b, int w
     const int stride = (nbat % 17) ? 17 : ((nbat % 19) ? 19 : 23);
     const int lt = tid * stride;
     const int v = base + (lt - mul24(sdiv(lt, small_div(nbat)), nbat));
-    const float4 p = a.rec_pos[brick_record(a, v, s_pre, s_base)];
-    int ix, iy, iz;
-    float fx, fy, fz;
-    grid_index(p.x, a.r1, a.nx, ix, fx);
-    grid_index(p.y, a.r2, a.ny, iy, fy);
-    grid_index(p.z, a.r3, a.nz, iz, fz);
-    float tx[P], ty[P], tz[P];
-    bspline_theta<P>(fx, tx);
-    bspline_theta<P>(fy, ty);
-    bspline_theta<P>(fz, tz);
-    // q * 2^40 (exact): the fixed-point scale rides on theta_x
-    const float qs = p.w * 1099511627776.0f;
-#pragma unroll
-    for (int j = 0; j < P; j++) tx[j] *= qs;
-    // x offset into the margin-padded row: in [0, wx + 2(P-1)) always
-    const int dx = local_offset(safe_index(ix, a.nx), x0, wx, a.nx) + (P - 1);
-    const int dy = local_offset(safe_index(iy, a.ny), y0, wy, a.ny);
-    const int dz = local_offset(safe_index(iz, a.nz), z0, wz, a.nz);
-#pragma unroll
-    for (int jz = 0; jz < P; jz++) {
-      int lz = dz + jz;
-      lz -= (wrapz && lz >= wz) ? a.nz : 0;
-      if (lz < 0 || lz >= wz) continue;
-#pragma unroll
-      for (int jy = 0; jy < P; jy++) {
-        int ly = dy + jy;
-        ly -= (wrapy && ly >= wy) ? a.ny : 0;
-        if (ly < 0 || ly >= wy) continue;
-        const float c = ty[jy] * tz[jz];
-        unsigned long long* row = acc + mul24(lz, plane) + mul24(ly, bxs) + dx;
-#pragma unroll
-        for (int jx = 0; jx < P; jx++) atomicAdd(&row[jx], (unsigned long long)to_fixed_scaled(tx[jx] * c));
-      }
-    }
+    spread_record<P>(a, a.rec_pos[brick_record(a, v, s_pre, s_base)], x0, wx, y0, wy, z0, wz, wrapy, wrapz, bxs,
+                     plane, acc);
   }
   __syncthreads();
 
