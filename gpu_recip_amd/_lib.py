@@ -17,12 +17,15 @@ PME_OK = 0
 STATUS = {
     0: "PME_OK", 1: "PME_ERR_INVALID_ARG", 2: "PME_ERR_UNSUPPORTED", 3: "PME_ERR_OOM",
     4: "PME_ERR_BUDGET", 5: "PME_ERR_DEVICE", 6: "PME_ERR_STATE", 7: "PME_ERR_NOT_FOUND",
+    8: "PME_ERR_STOPPED",
 }
+PME_ERR_STOPPED = 8
 FFT_COLUMN, FFT_SLAB, FFT_BOX = 0, 1, 2
 ACCUM_FLOAT, ACCUM_INT32, ACCUM_INT64 = 0, 1, 2
 FORCE_FLOAT, FORCE_FLOAT3, FORCE_INT64 = 0, 1, 2
 GRID_REAL, GRID_SPECTRUM = 0, 1
 PREC_FP32, PREC_BF16X3, PREC_FP16X3 = 0, 1, 2
+CALC_EV, CALC_SELF = 1, 2
 
 
 class PmeError(RuntimeError):
@@ -67,6 +70,12 @@ class RecipHeader(C.Structure):
     """pme_recip_header: the direct->recip request header (DomdecRecipComm.h:11-15)."""
     _fields_ = [("inv_boxx", C.c_double), ("inv_boxy", C.c_double), ("inv_boxz", C.c_double),
                 ("ncoord", C.c_int), ("calc_energy", C.c_ubyte), ("calc_virial", C.c_ubyte)]
+
+
+class RequestResult(C.Structure):
+    """pme_request_result: what one served request returns."""
+    _fields_ = [("energy", C.c_double), ("energy_self", C.c_double), ("virial", C.c_double * 9),
+                ("latency_ms", C.c_double)]
 
 
 _pi = C.POINTER(C.c_int)
@@ -114,6 +123,7 @@ SIGNATURES = {
     "pme_batch_remove": [_vp, _i],
     "pme_batch_solve": [_vp, _i, _pi],
     "pme_batch_results": [_vp, _i, _pi, _pd, _pd],
+    "pme_batch_results_self": [_vp, _i, _pi, _pd],
     "pme_batch_set_graph": [_vp, _i],
     "pme_batch_set_tuning": [_vp, _i, _i, _i],
     "pme_batch_get_tuning": [_vp, _pi, _pi, _pi],
@@ -125,6 +135,7 @@ SIGNATURES = {
     "pme_server_submit": [_vp, _i, C.POINTER(SystemDesc), _vp, _pu64],
     "pme_server_submit_header": [_vp, _i, C.POINTER(RecipHeader), _i, _i, _i, _i, _d, _vp, _vp, _pu64],
     "pme_server_wait": [_vp, C.c_ulonglong, _pd, _pd, _pd],
+    "pme_server_wait_result": [_vp, C.c_ulonglong, C.POINTER(RequestResult)],
     "pme_server_poll": [_vp, C.c_ulonglong, _pi, _pd, _pd, _pd],
     "pme_server_drain": [_vp],
     "pme_server_release": [_vp, _i],

@@ -71,7 +71,7 @@ struct pme_batch_s {
   std::vector<Slot> slots;
   SysDesc* d_sys = nullptr;  // [slots.capacity]
   int d_sys_cap = 0;
-  double* d_ev = nullptr;    // [d_sys_cap * 10]
+  double* d_ev = nullptr;    // [d_sys_cap * kEvOut]
   double* h_ev = nullptr;    // pinned mirror
   int ev_copied = 0;         // slots the solve's own d_ev -> h_ev copy covers
   TableCache tables;
@@ -118,7 +118,7 @@ static pme_status alloc_slot(pme_batch_t b, Slot& s, int slot_index) {
   s.bytes = s.plan.total;
   b->used += s.bytes;
   BTRY(fill_desc(s.d, s.in, s.plan, s.region, b->tables));
-  s.d.ev_out = b->d_ev + 10 * (size_t)slot_index;
+  s.d.ev_out = b->d_ev + kEvOut * (size_t)slot_index;
   return PME_OK;
 }
 
@@ -145,10 +145,10 @@ static pme_status ensure_desc_capacity(pme_batch_t b, int need) {
   double* nev = nullptr;
   double* nh = nullptr;
   BHIP(hipMalloc(&nd, sizeof(SysDesc) * cap));
-  BHIP(hipMalloc(&nev, sizeof(double) * 10 * cap));
-  BHIP(hipHostMalloc(&nh, sizeof(double) * 10 * cap, hipHostMallocDefault));
-  BHIP(hipMemset(nev, 0, sizeof(double) * 10 * cap));
-  std::memset(nh, 0, sizeof(double) * 10 * cap);
+  BHIP(hipMalloc(&nev, sizeof(double) * kEvOut * cap));
+  BHIP(hipHostMalloc(&nh, sizeof(double) * kEvOut * cap, hipHostMallocDefault));
+  BHIP(hipMemset(nev, 0, sizeof(double) * kEvOut * cap));
+  std::memset(nh, 0, sizeof(double) * kEvOut * cap);
   (void)hipFree(b->d_sys);
   (void)hipFree(b->d_ev);
   (void)hipHostFree(b->h_ev);
@@ -159,7 +159,7 @@ static pme_status ensure_desc_capacity(pme_batch_t b, int need) {
   // every resident descriptor points into d_ev: refresh and re-upload
   for (size_t i = 0; i < b->slots.size(); i++) {
     if (!b->slots[i].used) continue;
-    b->slots[i].d.ev_out = b->d_ev + 10 * i;
+    b->slots[i].d.ev_out = b->d_ev + kEvOut * i;
     BHIP(hipMemcpy(b->d_sys + i, &b->slots[i].d, sizeof(SysDesc), hipMemcpyHostToDevice));
   }
   invalidate(b);
@@ -328,7 +328,7 @@ static pme_status enqueue(pme_batch_t b) {
   BTRY(enqueue_stages(b->launches, b->d_sys, b->d_items, b->tune, b->ncu, b->stream, b->timing ? b->tev : nullptr));
   const int n = (int)b->slots.size();
   if (n > 0)
-    BHIP(hipMemcpyAsync(b->h_ev, b->d_ev, sizeof(double) * 10 * (size_t)n, hipMemcpyDeviceToHost, b->stream));
+    BHIP(hipMemcpyAsync(b->h_ev, b->d_ev, sizeof(double) * kEvOut * (size_t)n, hipMemcpyDeviceToHost, b->stream));
   b->ev_copied = n;
   return PME_OK;
 }
@@ -392,27 +392,47 @@ static pme_status run_solve(pme_batch_t b) {
   return PME_OK;
 }
 
-extern "C" pme_status pme_batch_results(pme_batch_t b, int n, const int* ids, double* energy, double* virial) {
-  if (!b) return set_error(PME_ERR_INVALID_ARG, "pme_batch_results: null batch");
-  const std::vector<int> v = resolve_ids(b, n, ids);
+// Wait for the stream and make h_ev current for the listed ids (v: resolved).
+static pme_status fetch_ev(pme_batch_t b, int n, const int* ids, const char* who, std::vector<int>& v) {
+  v = resolve_ids(b, n, ids);
   BHIP(hipStreamSynchronize(b->stream));
   int hi = 0;
   for (int id : v) {
     if (id < 0 || id >= (int)b->slots.size() || !b->slots[id].used)
-      return set_error(PME_ERR_INVALID_ARG, "pme_batch_results: unknown id %d", id);
+      return set_error(PME_ERR_INVALID_ARG, "%s: unknown id %d", who, id);
     hi = std::max(hi, id + 1);
   }
   // the solves copy d_ev into h_ev themselves; slots added since then are
   // read here
-  if (hi > b->ev_copied) BHIP(hipMemcpy(b->h_ev, b->d_ev, sizeof(double) * 10 * hi, hipMemcpyDeviceToHost));
+  if (hi > b->ev_copied) BHIP(hipMemcpy(b->h_ev, b->d_ev, sizeof(double) * kEvOut * hi, hipMemcpyDeviceToHost));
   if (b->timing && b->lists_valid)
     for (int k = 0; k < 8; k++) BHIP(hipEventElapsedTime(&b->stage_ms[k], b->tev[k], b->tev[k + 1]));
+  return PME_OK;
+}
+
+extern "C" pme_status pme_batch_results(pme_batch_t b, int n, const int* ids, double* energy, double* virial) {
+  if (!b) return set_error(PME_ERR_INVALID_ARG, "pme_batch_results: null batch");
+  std::vector<int> v;
+  const pme_status st = fetch_ev(b, n, ids, "pme_batch_results", v);
+  if (st != PME_OK) return st;
   for (size_t k = 0; k < v.size(); k++) {
-    const double* e = b->h_ev + 10 * (size_t)v[k];
-    const bool on = b->slots[v[k]].in.calc_energy_virial != 0;
+    const double* e = b->h_ev + kEvOut * (size_t)v[k];
+    const bool on = (b->slots[v[k]].in.calc_energy_virial & PME_CALC_EV) != 0;
     if (energy) energy[k] = on ? e[0] : 0.0;
     if (virial)
       for (int c = 0; c < 9; c++) virial[9 * k + c] = on ? e[1 + c] : 0.0;
+  }
+  return PME_OK;
+}
+
+extern "C" pme_status pme_batch_results_self(pme_batch_t b, int n, const int* ids, double* energy_self) {
+  if (!b) return set_error(PME_ERR_INVALID_ARG, "pme_batch_results_self: null batch");
+  std::vector<int> v;
+  const pme_status st = fetch_ev(b, n, ids, "pme_batch_results_self", v);
+  if (st != PME_OK) return st;
+  for (size_t k = 0; k < v.size(); k++) {
+    const bool on = (b->slots[v[k]].in.calc_energy_virial & PME_CALC_SELF) != 0;
+    if (energy_self) energy_self[k] = on ? b->h_ev[kEvOut * (size_t)v[k] + 10] : 0.0;
   }
   return PME_OK;
 }

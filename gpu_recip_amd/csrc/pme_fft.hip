@@ -927,25 +927,34 @@ __global__ __launch_bounds__(kBlock) void k_batch_fft_x(const SysDesc* __restric
   }
 }
 
-// energy/virial of one system from its z-pass partials (fixed order, one
-// workgroup), assigned to ev_out
+// energy/virial of one system from its z-pass partials and its self energy
+// from the binning's per-chunk sums of q*q (fixed order, one workgroup),
+// assigned to ev_out
 template <int NT = kBlock>
 __device__ __forceinline__ void batch_ev_reduce(const SysDesc& d) {
   __shared__ double sh[7 * (NT / 64)];
-  double v[7] = {0, 0, 0, 0, 0, 0, 0};
-  for (int i = threadIdx.x; i < d.nzblk; i += NT)
+  if (d.sa.calc_ev) {
+    double v[7] = {0, 0, 0, 0, 0, 0, 0};
+    for (int i = threadIdx.x; i < d.nzblk; i += NT)
 #pragma unroll
-    for (int c = 0; c < 7; c++) v[c] += d.sa.partial[(size_t)i * 7 + c];
-  block_sum<7, NT>(v, sh);
-  if (threadIdx.x == 0) {
-    const double hc = 0.5 * kCcelec;
-    double* o = d.ev_out;
-    o[0] = v[0] * hc;
-    const double v0 = -hc * v[1], v1 = -hc * v[2], v2 = -hc * v[3];
-    const double v3 = -hc * v[4], v4 = -hc * v[5], v5 = -hc * v[6];
-    o[1] = v0; o[2] = v1; o[3] = v2;
-    o[4] = v1; o[5] = v3; o[6] = v4;
-    o[7] = v2; o[8] = v4; o[9] = v5;
+      for (int c = 0; c < 7; c++) v[c] += d.sa.partial[(size_t)i * 7 + c];
+    block_sum<7, NT>(v, sh);
+    if (threadIdx.x == 0) {
+      const double hc = 0.5 * kCcelec;
+      double* o = d.ev_out;
+      o[0] = v[0] * hc;
+      const double v0 = -hc * v[1], v1 = -hc * v[2], v2 = -hc * v[3];
+      const double v3 = -hc * v[4], v4 = -hc * v[5], v5 = -hc * v[6];
+      o[1] = v0; o[2] = v1; o[3] = v2;
+      o[4] = v1; o[5] = v3; o[6] = v4;
+      o[7] = v2; o[8] = v4; o[9] = v5;
+    }
+  }
+  if (d.calc_self) {
+    double v[1] = {0.0};
+    for (int i = threadIdx.x; i < d.bin.nchunk; i += NT) v[0] += d.self_partial[i];
+    block_sum<1, NT>(v, sh);
+    if (threadIdx.x == 0) d.ev_out[10] = -v[0] * d.kcs;
   }
 }
 
@@ -956,7 +965,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_cols(const SysDesc* __restrict
   for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
     const WorkItem w = uniform_item(items[it]);
     const SysDesc& d = sys[w.sys];
-    if (reduce && w.idx == 0 && d.sa.calc_ev) batch_ev_reduce<kBlock>(d);
+    if (reduce && w.idx == 0 && (d.sa.calc_ev || d.calc_self)) batch_ev_reduce<kBlock>(d);
     const int hx = d.bin.nx / 2 + 1;
     const int n = AXIS == 1 ? d.bin.ny : d.bin.nz;
     const int TX = AXIS == 1 ? d.TXy : d.TXz;
@@ -981,7 +990,7 @@ __global__ __launch_bounds__(kPlaneBlock) void k_batch_plane(const SysDesc* __re
     const SysDesc& d = sys[w.sys];
     const int npl = min(plane_group(d.bin.nx, d.bin.ny, d.ax[0].rp.n, maxnp), d.bin.nz - w.idx);
     if (INV) {
-      if (reduce && w.idx == 0 && d.sa.calc_ev) batch_ev_reduce<kPlaneBlock>(d);
+      if (reduce && w.idx == 0 && (d.sa.calc_ev || d.calc_self)) batch_ev_reduce<kPlaneBlock>(d);
       fft_plane_inv_body<SPLIT>(d.cgrid, d.grid, d.bin.nx, d.bin.ny, d.ax[0], d.ax[1], w.idx, npl, sm);
     } else {
       fft_plane_fwd_body<SPLIT>(d.grid, d.cgrid, d.bin.nx, d.bin.ny, d.ax[0], d.ax[1], w.idx, npl, sm);

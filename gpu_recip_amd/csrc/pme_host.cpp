@@ -58,6 +58,7 @@ extern "C" const char* pme_status_string(pme_status s) {
     case PME_ERR_DEVICE: return "device error";
     case PME_ERR_STATE: return "stage called out of order";
     case PME_ERR_NOT_FOUND: return "not found";
+    case PME_ERR_STOPPED: return "stop request";
   }
   return "unknown";
 }

@@ -80,8 +80,12 @@ struct SysDesc {
   int plane;               // 1: x and y passes as one xy plane kernel
   void* force;             // caller's force output
   int force_type, stride;
-  double* ev_out;          // [10]: energy, virial[9] of the last solve (assigned)
+  double* ev_out;          // [kEvOut]: energy, virial[9], self energy of the last solve (assigned)
+  double* self_partial;    // [bin.nchunk]: sum of q*q per binning chunk (calc_self)
+  int calc_self;           // 1: self energy (CudaPMERecip.cu:2053-2061)
+  double kcs;              // kappa*ccelec/sqrt(pi)
 };
+constexpr int kEvOut = 11;  // doubles per system in the batched energy/virial output
 
 // One workgroup-sized task of a batched stage: system index and the tile of
 // that system (chunk, brick, row tile or column tile).

@@ -29,6 +29,8 @@ pme_status validate_system(const pme_system_desc* s) {
     return set_error(PME_ERR_INVALID_ARG, "system: force_type %d", s->force_type);
   if (s->precision < PME_PREC_FP32 || s->precision > PME_PREC_FP16X3)
     return set_error(PME_ERR_INVALID_ARG, "system: precision %d", s->precision);
+  if (s->calc_energy_virial < 0 || s->calc_energy_virial > (PME_CALC_EV | PME_CALC_SELF))
+    return set_error(PME_ERR_INVALID_ARG, "system: calc_energy_virial %d", s->calc_energy_virial);
   if (s->force_type != PME_FORCE_FLOAT3 && s->stride < s->ncoord)
     return set_error(PME_ERR_INVALID_ARG, "system: stride %d < ncoord %d", s->stride, s->ncoord);
   return PME_OK;
@@ -59,14 +61,15 @@ pme_status plan_system(const pme_system_desc& in, int cap_atoms, const Tuning& t
   const size_t G = (size_t)n[0] * n[1] * n[2];
   const size_t Gc = (size_t)hx * n[1] * n[2];
   const size_t rs = 8 * (size_t)p->cp.chunk * p->cp.nchunk;
-  const size_t sz[6] = {sizeof(float) * G,
+  const size_t sz[7] = {sizeof(float) * G,
                         sizeof(float2) * Gc,
                         sizeof(int) * (size_t)p->cp.nchunk * (p->bp.nbins + 1),
                         sizeof(float4) * rs,
                         sizeof(int) * rs,
-                        sizeof(double) * 7 * (size_t)p->nzblk};
+                        sizeof(double) * 7 * (size_t)p->nzblk,
+                        sizeof(double) * (size_t)p->cp.nchunk};
   size_t o = 0;
-  for (int i = 0; i < 6; i++) {
+  for (int i = 0; i < 7; i++) {
     p->off[i] = o;
     o += align256(sz[i]);
   }
@@ -129,7 +132,9 @@ void fill_dynamic(SysDesc& d, const pme_system_desc& in, const SystemPlan& p) {
   const double inv_vol = in.recip[0] * in.recip[4] * in.recip[8];
   d.sa.piv_inv = (float)(inv_vol / kPiS);
   d.sa.fac = (float)(kPiS * kPiS / (in.kappa * in.kappa));
-  d.sa.calc_ev = in.calc_energy_virial ? 1 : 0;
+  d.sa.calc_ev = (in.calc_energy_virial & PME_CALC_EV) ? 1 : 0;
+  d.calc_self = (in.calc_energy_virial & PME_CALC_SELF) ? 1 : 0;
+  d.kcs = in.kappa * kCcelec / std::sqrt(kPiS);
 }
 
 pme_status fill_desc(SysDesc& d, const pme_system_desc& in, const SystemPlan& p, void* base, TableCache& tables) {
@@ -175,6 +180,7 @@ pme_status fill_desc(SysDesc& d, const pme_system_desc& in, const SystemPlan& p,
   d.sa.nf2 = n[1] / 2 + (n[1] % 2);
   d.sa.nf3 = n[2] / 2 + (n[2] % 2);
   d.sa.partial = reinterpret_cast<double*>(b + p.off[5]);
+  d.self_partial = reinterpret_cast<double*>(b + p.off[6]);
   d.nzblk = p.nzblk;
   d.plane = p.plane;
   fill_dynamic(d, in, p);

@@ -26,7 +26,7 @@ struct SystemPlan {
   FftTiles ft;
   int nzblk;            // z-pass tiles = energy/virial partials
   int plane;            // 1: x and y passes as one xy plane kernel
-  size_t off[6];        // grid, cgrid, seg_off, rec_pos, rec_idx, partial
+  size_t off[7];        // grid, cgrid, seg_off, rec_pos, rec_idx, partial, self_partial
   size_t total;         // bytes of the region
 };
 

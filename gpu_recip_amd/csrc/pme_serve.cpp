@@ -108,6 +108,7 @@ struct Result {
   pme_status status = PME_OK;
   std::string msg;
   double energy = 0.0;
+  double energy_self = 0.0;
   double virial[9] = {};
   double latency_ms = 0.0;
 };
@@ -123,7 +124,7 @@ struct Lane {
   size_t cap_sys = 0, cap_items = 0;
   unsigned char* h_stage = nullptr;  // pinned: descriptors then items
   size_t stage_cap = 0;
-  double* h_ev = nullptr;            // pinned, cap_sys * 10
+  double* h_ev = nullptr;            // pinned, cap_sys * kEvOut
   std::vector<void*> retired;        // buffers replaced while growing, freed at the end
 };
 
@@ -163,9 +164,10 @@ static void publish(pme_server_t s, const Request& r, pme_status status, const s
   res.done = true;
   res.status = status;
   res.msg = msg;
-  if (ev) {
+  if (ev) {  // kEvOut doubles: energy, virial[9], self energy
     res.energy = ev[0];
     for (int c = 0; c < 9; c++) res.virial[c] = ev[1 + c];
+    res.energy_self = ev[10];
   }
   res.latency_ms = (t_done - r.t_submit) * 1e3;
   if (status == PME_OK) {
@@ -257,8 +259,8 @@ static pme_status grow_lane(Lane& L, size_t nsys, size_t nitems) {
     L.d_ev = nullptr;
     L.h_ev = nullptr;
     if (hipMalloc(&L.d_sys, sizeof(SysDesc) * cap) != hipSuccess ||
-        hipMalloc(&L.d_ev, sizeof(double) * 10 * cap) != hipSuccess ||
-        hipHostMalloc(&L.h_ev, sizeof(double) * 10 * cap, hipHostMallocDefault) != hipSuccess)
+        hipMalloc(&L.d_ev, sizeof(double) * kEvOut * cap) != hipSuccess ||
+        hipHostMalloc(&L.h_ev, sizeof(double) * kEvOut * cap, hipHostMallocDefault) != hipSuccess)
       return set_error(PME_ERR_OOM, "serving lane buffers");
     L.cap_sys = cap;
   }
@@ -302,7 +304,7 @@ static pme_status launch_lane(pme_server_t s, Lane& L, std::vector<Request>& sel
   build_items(dp.data(), pp.data(), orders.data(), m, s->ncu, items, launches);
   pme_status st = grow_lane(L, m, items.size());
   if (st != PME_OK) return st;
-  for (int k = 0; k < m; k++) descs[k].ev_out = L.d_ev + 10 * (size_t)k;
+  for (int k = 0; k < m; k++) descs[k].ev_out = L.d_ev + kEvOut * (size_t)k;
   std::memcpy(L.h_stage, descs.data(), sizeof(SysDesc) * m);
   std::memcpy(L.h_stage + sizeof(SysDesc) * L.cap_sys, items.data(), sizeof(WorkItem) * items.size());
   for (const Request& r : sel)
@@ -314,7 +316,7 @@ static pme_status launch_lane(pme_server_t s, Lane& L, std::vector<Request>& sel
     return set_error(PME_ERR_DEVICE, "serving lane upload");
   st = enqueue_stages(launches, L.d_sys, L.d_items, tune, s->ncu, L.stream, nullptr);
   if (st != PME_OK) return st;
-  if (hipMemcpyAsync(L.h_ev, L.d_ev, sizeof(double) * 10 * m, hipMemcpyDeviceToHost, L.stream) != hipSuccess ||
+  if (hipMemcpyAsync(L.h_ev, L.d_ev, sizeof(double) * kEvOut * m, hipMemcpyDeviceToHost, L.stream) != hipSuccess ||
       hipEventRecord(L.done, L.stream) != hipSuccess)
     return set_error(PME_ERR_DEVICE, "serving lane readback");
   for (const Request& r : sel) s->resident[r.key].busy = true;
@@ -342,8 +344,10 @@ static bool retire(pme_server_t s) {
         it->second.busy = false;
         it->second.last_use = ++s->use_clock;
       }
-      double ev[10] = {};
-      if (r.in.calc_energy_virial) std::memcpy(ev, L.h_ev + 10 * k, sizeof(ev));
+      double ev[kEvOut] = {};
+      const double* h = L.h_ev + kEvOut * k;
+      if (r.in.calc_energy_virial & PME_CALC_EV) std::memcpy(ev, h, sizeof(double) * 10);
+      if (r.in.calc_energy_virial & PME_CALC_SELF) ev[10] = h[10];
       publish(s, r, q == hipSuccess ? PME_OK : PME_ERR_DEVICE, msg, ev, t);
     }
     L.reqs.clear();
@@ -545,6 +549,8 @@ extern "C" pme_status pme_server_submit_header(pme_server_t s, int key, const pm
                                                int nffty, int nfftz, int order, double kappa, const void* d_xyzq,
                                                void* d_force3, unsigned long long* ticket) {
   if (!h) return set_error(PME_ERR_INVALID_ARG, "pme_server_submit_header: null header");
+  // negative ncoord: the STOP message of DomdecRecipComm::send_stop
+  if (h->ncoord < 0) return set_error(PME_ERR_STOPPED, "pme_server_submit_header: STOP header (ncoord %d)", h->ncoord);
   pme_system_desc d{};
   d.nfftx = nfftx;
   d.nffty = nffty;
@@ -559,7 +565,9 @@ extern "C" pme_status pme_server_submit_header(pme_server_t s, int key, const pm
   d.d_force = d_force3;
   d.force_type = PME_FORCE_FLOAT3;
   d.stride = h->ncoord;
-  d.calc_energy_virial = (h->calc_energy || h->calc_virial) ? 1 : 0;
+  // CudaDomdecRecip::calc (CudaDomdecRecip.h:70-78): scalar_sum with the
+  // two flags, then calc_self_energy when calc_energy is set
+  d.calc_energy_virial = ((h->calc_energy || h->calc_virial) ? PME_CALC_EV : 0) | (h->calc_energy ? PME_CALC_SELF : 0);
   d.precision = PME_PREC_FP32;
   return pme_server_submit(s, key, &d, nullptr, ticket);
 }
@@ -584,6 +592,18 @@ extern "C" pme_status pme_server_wait(pme_server_t s, unsigned long long ticket,
   if (it == s->results.end()) return set_error(PME_ERR_NOT_FOUND, "pme_server_wait: unknown ticket %llu", ticket);
   s->cv_done.wait(g, [&] { return s->results[ticket].done; });
   return take_result(s, s->results.find(ticket), energy, virial, latency_ms);
+}
+
+extern "C" pme_status pme_server_wait_result(pme_server_t s, unsigned long long ticket, pme_request_result* out) {
+  if (!s || !out) return set_error(PME_ERR_INVALID_ARG, "pme_server_wait_result: null argument");
+  std::unique_lock<std::mutex> g(s->mu);
+  auto it = s->results.find(ticket);
+  if (it == s->results.end())
+    return set_error(PME_ERR_NOT_FOUND, "pme_server_wait_result: unknown ticket %llu", ticket);
+  s->cv_done.wait(g, [&] { return s->results[ticket].done; });
+  it = s->results.find(ticket);
+  out->energy_self = it->second.energy_self;
+  return take_result(s, it, &out->energy, out->virial, &out->latency_ms);
 }
 
 extern "C" pme_status pme_server_poll(pme_server_t s, unsigned long long ticket, int* done, double* energy,

@@ -305,10 +305,23 @@ hipError_t launch_spread(int order, const BinArgs& a, const int wmax[3], float* 
 __global__ __launch_bounds__(kBinBlock) void k_batch_bin(const SysDesc* __restrict__ sys,
                                                       const WorkItem* __restrict__ items, int nitems) {
   extern __shared__ int hist[];
+  __shared__ double sh_self[kBinBlock / 64];
   for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
     const WorkItem w = uniform_item(items[it]);
     const SysDesc& d = sys[w.sys];
     bin_chunk_body(d.xyzq, d.natom, d.bin, w.idx, hist);
+    if (d.calc_self) {
+      // this chunk's share of the self energy (calc_self_energy_kernel,
+      // CudaPMERecip.cu:1252-1275): q*q in FP32, summed in FP64
+      const int a0 = w.idx * d.bin.chunk, a1 = min(a0 + d.bin.chunk, d.natom);
+      double v[1] = {0.0};
+      for (int i = a0 + (int)threadIdx.x; i < a1; i += kBinBlock) {
+        const float q = d.xyzq[i].w;
+        v[0] += (double)(q * q);
+      }
+      block_sum<1, kBinBlock>(v, sh_self);
+      if (threadIdx.x == 0) d.self_partial[w.idx] = v[0];
+    }
     __syncthreads();
   }
 }

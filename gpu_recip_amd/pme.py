@@ -192,7 +192,8 @@ class PmeBatch:
         d.d_force = _ptr(force)
         d.force_type = int(force_type)
         d.stride = int(ncoord if stride is None else stride)
-        d.calc_energy_virial = int(bool(calc_energy_virial))
+        # True/False, or the L.CALC_EV / L.CALC_SELF bits
+        d.calc_energy_virial = int(calc_energy_virial)
         d.precision = int(precision)
         return d
 
@@ -232,6 +233,16 @@ class PmeBatch:
         pd = C.POINTER(C.c_double)
         L.call("pme_batch_results", self.h, n, arr, e.ctypes.data_as(pd), v.ctypes.data_as(pd))
         return e, v
+
+    def results_self(self, ids):
+        """Self energies (n,) of the last solve for systems described with
+        L.CALC_SELF (0 for the others)."""
+        if ids is None:
+            raise ValueError("PmeBatch.results_self: list the system ids")
+        n, arr = self._ids(ids)
+        es = np.empty(n, np.float64)
+        L.call("pme_batch_results_self", self.h, n, arr, es.ctypes.data_as(C.POINTER(C.c_double)))
+        return es
 
     def set_graph(self, enable: bool) -> None:
         L.call("pme_batch_set_graph", self.h, int(bool(enable)))
@@ -297,7 +308,11 @@ class PmeServer:
                       keep=()) -> int:
         """Serve one direct->recip request: header_bytes is the wire header
         (three doubles, an int, two bool bytes; DomdecRecipComm.h:11-15),
-        xyzq a device float4 array, force3 a device float3 array."""
+        xyzq a device float4 array, force3 a device float3 array.  Computes
+        what CudaDomdecRecip::calc does: the energy and virial when either
+        flag is set and the self energy when calc_energy is (wait_result).
+        A STOP header (ncoord < 0) raises PmeError with status
+        L.PME_ERR_STOPPED."""
         h = L.RecipHeader.from_buffer_copy(header_bytes.ljust(C.sizeof(L.RecipHeader), b"\0"))
         t = C.c_ulonglong()
         L.call("pme_server_submit_header", self.h, int(key), C.byref(h), int(nfft[0]), int(nfft[1]), int(nfft[2]),
@@ -313,6 +328,16 @@ class PmeServer:
         finally:
             self._keep.pop(ticket, None)
         return e.value, list(v), ms.value
+
+    def wait_result(self, ticket: int) -> dict:
+        """Block for the request: energy, energy_self, virial (9), latency_ms."""
+        r = L.RequestResult()
+        try:
+            L.call("pme_server_wait_result", self.h, C.c_ulonglong(ticket), C.byref(r))
+        finally:
+            self._keep.pop(ticket, None)
+        return {"energy": r.energy, "energy_self": r.energy_self, "virial": list(r.virial),
+                "latency_ms": r.latency_ms}
 
     def poll(self, ticket: int):
         """None while in progress, else (energy, virial, latency_ms)."""

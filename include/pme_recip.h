@@ -30,7 +30,8 @@ typedef enum pme_status {
   PME_ERR_BUDGET = 4,        /* request exceeds the configured HBM budget     */
   PME_ERR_DEVICE = 5,        /* HIP runtime error                             */
   PME_ERR_STATE = 6,         /* stage called out of order (e.g. gather before c2r) */
-  PME_ERR_NOT_FOUND = 7      /* unknown energy term name                      */
+  PME_ERR_NOT_FOUND = 7,     /* unknown energy term name                      */
+  PME_ERR_STOPPED = 8        /* a STOP request header (ncoord < 0)            */
 } pme_status;
 
 typedef void* pme_stream_t;                       /* hipStream_t */
@@ -169,6 +170,9 @@ pme_status pme_fill_bspline(int order, int nfftx, int nffty, int nfftz,
  * (same kernels, same results), energy and virial assigned per system. */
 typedef struct pme_batch_s* pme_batch_t;
 
+/* pme_system_desc.calc_energy_virial bits (1 keeps its meaning: energy and virial) */
+enum { PME_CALC_EV = 1, PME_CALC_SELF = 2 };
+
 typedef struct pme_system_desc {
   int nfftx, nffty, nfftz;   /* grid (2/3/5-smooth, >= order, <= 512) */
   int order;                 /* 4, 6 or 8 */
@@ -179,7 +183,8 @@ typedef struct pme_system_desc {
   void* d_force;             /* device output, laid out per force_type */
   int force_type;            /* PME_FORCE_* (INT64 adds, the others store) */
   int stride;                /* SoA stride for FLOAT / INT64 (>= ncoord) */
-  int calc_energy_virial;    /* energy and virial of this system */
+  int calc_energy_virial;    /* PME_CALC_* bits: PME_CALC_EV the reciprocal energy and
+                                virial, PME_CALC_SELF the self energy (calc_self_energy) */
   int precision;             /* PME_PREC_* of the y/z FFT passes */
 } pme_system_desc;
 
@@ -199,6 +204,10 @@ pme_status pme_batch_solve(pme_batch_t b, int n, const int* ids);
 /* Wait for the last solve and copy energy[n] and virial[n*9] of the listed
  * systems (either pointer may be NULL). */
 pme_status pme_batch_results(pme_batch_t b, int n, const int* ids, double* energy, double* virial);
+/* Self energies -kappa*ccelec/sqrt(pi)*sum(q*q) (CudaPMERecip.cu:2053-2061)
+ * of the listed systems from the last solve, for systems described with
+ * PME_CALC_SELF (0 for the others); waits like pme_batch_results. */
+pme_status pme_batch_results_self(pme_batch_t b, int n, const int* ids, double* energy_self);
 /* hipGraph capture/replay of the solve sequence (default on). */
 pme_status pme_batch_set_graph(pme_batch_t b, int enable);
 /* Tuning knobs: brick edge target for spread/gather (>= 8), y/z FFT tile
@@ -258,8 +267,12 @@ pme_status pme_server_submit(pme_server_t s, int key, const pme_system_desc* sys
  * layout: three doubles, an int, two one-byte flags).  A recip node serves
  * each header with a float4 coordinate array in and a float3 force array
  * out (CudaDomdecRecipLooper.cu:3-37); pme_server_submit_header does that
- * for a server-side system of fixed grid, order and kappa under `key`.
- * Energy and virial are computed when either flag is set. */
+ * for a server-side system of fixed grid, order and kappa under `key`, as
+ * CudaDomdecRecip::calc (CudaDomdecRecip.h:70-78): the reciprocal energy
+ * and virial when either flag is set, and the self energy when calc_energy
+ * is set (read with pme_server_wait_result).  A header with ncoord < 0 is
+ * the reference's STOP message (DomdecRecipComm.cpp:140-144): it is
+ * answered with PME_ERR_STOPPED and queues nothing. */
 typedef struct pme_recip_header {
   double inv_boxx, inv_boxy, inv_boxz;
   int ncoord;
@@ -272,6 +285,14 @@ pme_status pme_server_submit_header(pme_server_t s, int key, const pme_recip_hea
  * (9)/latency_ms may be NULL.  A ticket's result can be taken once. */
 pme_status pme_server_wait(pme_server_t s, unsigned long long ticket, double* energy, double* virial,
                            double* latency_ms);
+/* Everything a request returns: reciprocal energy, self energy (requests
+ * with PME_CALC_SELF), virial and submit-to-visible latency. */
+typedef struct pme_request_result {
+  double energy, energy_self;
+  double virial[9];
+  double latency_ms;
+} pme_request_result;
+pme_status pme_server_wait_result(pme_server_t s, unsigned long long ticket, pme_request_result* out);
 /* Non-blocking: *done = 0 while in progress (result left in place). */
 pme_status pme_server_poll(pme_server_t s, unsigned long long ticket, int* done, double* energy, double* virial,
                            double* latency_ms);

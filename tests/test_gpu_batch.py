@@ -159,6 +159,39 @@ def test_batch_edge_systems():
             assert frms < 1e-4 and rel_err(e[k], o["energy"]) < 1e-4
 
 
+def test_batch_self_energy():
+    """Systems described with CALC_SELF get the self energy of
+    calc_self_energy (CudaPMERecip.cu:2053-2061) from the binning pass:
+    equal to the single-system solver's within FP64 summation order and to
+    the oracle; CALC_SELF alone leaves energy and virial zero and the forces
+    unchanged; systems without it report 0.  Graph and eager solves alike."""
+    torch, dev = torch_dev()
+    sl = systems.config_batched(6)
+    flags = [L.CALC_EV | L.CALC_SELF, L.CALC_SELF, L.CALC_EV, L.CALC_EV | L.CALC_SELF, 0, L.CALC_SELF]
+    b = PmeBatch(4 << 30)
+    reg = [_add(b, s, torch, dev, calc_ev=fl) for s, fl in zip(sl, flags)]
+    ids = [r[0] for r in reg]
+    for graph in (True, False):
+        b.set_graph(graph)
+        b.solve()
+        e, v = b.results(ids)
+        es = b.results_self(ids)
+        for k, (s, fl) in enumerate(zip(sl, flags)):
+            one = run_solve(s.nfft, s.order, s.kappa, s.box, s.xyzq, self_energy=True)
+            if fl & L.CALC_SELF:
+                assert rel_err(es[k], one["self"]) < 1e-12, (s.name, graph)
+                assert rel_err(es[k], O.self_energy(s.xyzq, s.kappa)) < 1e-6, s.name
+            else:
+                assert es[k] == 0.0
+            if fl & L.CALC_EV:
+                assert rel_err(e[k], one["energy"]) < 1e-12
+            else:
+                assert e[k] == 0.0 and np.all(v[k] == 0.0)
+            _, x, f, stride = reg[k]
+            assert np.array_equal(_forces(f, s.natom, stride), one["force"]), (s.name, graph)
+    say(f"  self energies {es[0]:.6f} .. {es[-1]:.6f}")
+
+
 def test_batched_config_vs_oracle():
     """The 64-system batched workload: every system against the single-system
     solver, eight of them against the FP64 oracle (exact coordinates)."""

@@ -186,4 +186,20 @@ def test_direct_recip_header_requests():
     hdr = struct.pack("<dddi??", 1 / s.box[0], 1 / s.box[1], 1 / s.box[2], s.natom, False, False)
     e, v, _ = srv.wait(srv.submit_header(7, hdr, s.nfft, s.order, s.kappa, x, f3))
     assert e == 0.0 and v == [0.0] * 9
+    # CudaDomdecRecip::calc adds the self energy when calc_energy is set
+    one = run_solve(s.nfft, s.order, s.kappa, s.box, s.xyzq, force_type=L.FORCE_FLOAT3, self_energy=True)
+    hdr = struct.pack("<dddi??", 1 / s.box[0], 1 / s.box[1], 1 / s.box[2], s.natom, True, False)
+    r = srv.wait_result(srv.submit_header(7, hdr, s.nfft, s.order, s.kappa, x, f3))
+    assert rel_err(r["energy"], one["energy"]) < 1e-12
+    assert rel_err(r["energy_self"], one["self"]) < 1e-12
+    assert rel_err(r["energy_self"], O.self_energy(s.xyzq, s.kappa)) < 1e-6
+    hdr = struct.pack("<dddi??", 1 / s.box[0], 1 / s.box[1], 1 / s.box[2], s.natom, False, True)
+    r = srv.wait_result(srv.submit_header(7, hdr, s.nfft, s.order, s.kappa, x, f3))
+    assert r["energy_self"] == 0.0 and rel_err(r["virial"][0], one["virial"][0]) < 1e-12
+    # the STOP message (DomdecRecipComm::send_stop: ncoord = -1) is refused
+    hdr = struct.pack("<dddi??", 1.0, 1.0, 1.0, -1, False, False)
+    with pytest.raises(L.PmeError) as ei:
+        srv.submit_header(7, hdr, s.nfft, s.order, s.kappa, x, f3)
+    assert ei.value.status == L.PME_ERR_STOPPED
+    assert srv.stats()["submitted"] == 4
     srv.close()
