@@ -168,6 +168,7 @@ def main():
     ids = [r[0] for r in reg]
     tune_ms = None if args.no_autotune else batch.autotune(None)
     tuning = batch.get_tuning()
+    chains = batch.get_chains()
 
     def bstep():
         batch.solve()
@@ -184,7 +185,10 @@ def main():
     nsolve = len(sl)
     atoms_b = sum(s.natom for s in sl)
 
-    # per-stage kernel times of the batched solve (HIP events on its stream)
+    # per-stage kernel times of the batched solve (HIP events on its stream):
+    # every stage as one launch over all 64 systems (one chain, one workgroup
+    # per work item), the tuned brick and FFT tile
+    batch.set_tuning(tuning[0], tuning[1], 0)
     batch.set_timing(True)
     stage_ms = np.zeros(len(STAGES))
     nt = max(10, args.steps // 4)
@@ -192,6 +196,7 @@ def main():
         bstep()
         stage_ms += np.array(batch.get_timing())
     batch.set_timing(False)
+    batch.set_tuning(*tuning)
     stage_ms /= nt
     sb = {k: sum(stage_bytes(s)[k] for s in sl) for k in STAGES}
     k = int(np.argmax(stage_ms))
@@ -420,7 +425,8 @@ def main():
                    "grid_edges": [min(min(s.nfft) for s in sl), max(max(s.nfft) for s in sl)], "order": 4,
                    "kappa": 0.34, "step": "one batched solve of all 64 systems (hipGraph replay) + energy/virial "
                                           "readback", "tuning": {"brick_target": tuning[0], "fft_tile": tuning[1],
-                                                                 "wg_per_cu": tuning[2], "autotune_ms": tune_ms}},
+                                                                 "wg_per_cu": tuning[2], "chains": chains,
+                                                                 "autotune_ms": tune_ms}},
         "atom_steps_per_s": atoms_b * args.steps / dt_b,
         "plain_same_systems": {"solves_per_s": plain_c3, "atom_steps_per_s": atoms_b * psteps / dt_p,
                                "path": "the 64 systems one at a time: 5 stage calls + readback + sync each"},

@@ -75,10 +75,17 @@ struct pme_batch_s {
   double* h_ev = nullptr;    // pinned mirror
   int ev_copied = 0;         // slots the solve's own d_ev -> h_ev copy covers
   TableCache tables;
-  // work lists of the current subset
+  // work lists of the current subset: with two chains the systems are split
+  // in two halves of about equal cost whose stage sequences run on `stream`
+  // and `aux` side by side (one chain's kernel tails and launch gaps are
+  // filled by the other's work)
   std::vector<int> cur_ids;
   bool lists_valid = false;
-  std::vector<Launch> launches;
+  int nchains = kBatchChains;
+  std::vector<Launch> launches;   // chain on `stream`
+  std::vector<Launch> launches2;  // chain on `aux` (empty: one chain)
+  hipStream_t aux = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   WorkItem* d_items = nullptr;
   size_t items_cap = 0;
   // graph of the current subset
@@ -181,6 +188,12 @@ extern "C" pme_status pme_batch_create(size_t hbm_budget_bytes, pme_stream_t str
     }
     b->own_stream = true;
   }
+  if (hipStreamCreateWithFlags(&b->aux, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&b->ev_join, hipEventDisableTiming) != hipSuccess) {
+    pme_batch_destroy(b);
+    return set_error(PME_ERR_DEVICE, "pme_batch_create: stream/event creation failed");
+  }
   int dev = 0;
   hipDeviceProp_t prop;
   if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess) {
@@ -199,6 +212,7 @@ extern "C" pme_status pme_batch_create(size_t hbm_budget_bytes, pme_stream_t str
 extern "C" pme_status pme_batch_destroy(pme_batch_t b) {
   if (!b) return PME_OK;
   (void)hipStreamSynchronize(b->stream);
+  if (b->aux) (void)hipStreamSynchronize(b->aux);
   drop_graph(b);
   for (auto& s : b->slots)
     if (s.used) free_slot_state(b, s);
@@ -211,6 +225,9 @@ extern "C" pme_status pme_batch_destroy(pme_batch_t b) {
     if (e) (void)hipEventDestroy(e);
   if (b->ev_in) (void)hipEventDestroy(b->ev_in);
   if (b->ev_done) (void)hipEventDestroy(b->ev_done);
+  if (b->ev_fork) (void)hipEventDestroy(b->ev_fork);
+  if (b->ev_join) (void)hipEventDestroy(b->ev_join);
+  if (b->aux) (void)hipStreamDestroy(b->aux);
   if (b->own_stream) (void)hipStreamDestroy(b->stream);
   delete b;
   return PME_OK;
@@ -296,17 +313,30 @@ static pme_status build_lists(pme_batch_t b, const std::vector<int>& ids) {
     const double ca = system_cost(b->slots[x].in), cc = system_cost(b->slots[y].in);
     return ca > cc || (ca == cc && x < y);
   });
-  std::vector<const SysDesc*> descs;
-  std::vector<const SystemPlan*> plans;
-  std::vector<int> orders;
-  for (int id : order_ids) {
-    descs.push_back(&b->slots[id].d);
-    plans.push_back(&b->slots[id].plan);
-    orders.push_back(b->slots[id].in.order);
-  }
+  // two chains (not while timing stages): systems dealt alternately in
+  // decreasing cost, each chain planned for half of the CUs
+  const bool two = b->nchains > 1 && !b->timing && order_ids.size() >= 2;
+  std::vector<int> part[2];
+  for (size_t k = 0; k < order_ids.size(); k++) part[two ? (k & 1) : 0].push_back(order_ids[k]);
   std::vector<WorkItem> items;
-  build_items(descs.data(), plans.data(), orders.data(), (int)order_ids.size(), b->ncu, items, b->launches);
-  for (WorkItem& w : items) w.sys = order_ids[w.sys];
+  b->launches2.clear();
+  for (int c = 0; c < (two ? 2 : 1); c++) {
+    std::vector<const SysDesc*> descs;
+    std::vector<const SystemPlan*> plans;
+    std::vector<int> orders;
+    for (int id : part[c]) {
+      descs.push_back(&b->slots[id].d);
+      plans.push_back(&b->slots[id].plan);
+      orders.push_back(b->slots[id].in.order);
+    }
+    std::vector<WorkItem> it;
+    std::vector<Launch>& L = c == 0 ? b->launches : b->launches2;
+    build_items(descs.data(), plans.data(), orders.data(), (int)part[c].size(), two ? std::max(1, b->ncu / 2) : b->ncu,
+                it, L);
+    for (Launch& l : L) l.offset += (int)items.size();
+    for (WorkItem& w : it) w.sys = part[c][w.sys];
+    items.insert(items.end(), it.begin(), it.end());
+  }
   BHIP(hipStreamSynchronize(b->stream));
   if (items.size() > b->items_cap) {
     (void)hipFree(b->d_items);
@@ -325,7 +355,16 @@ static pme_status build_lists(pme_batch_t b, const std::vector<int>& ids) {
 // mirror (part of the captured graph), so reading results needs no further
 // device work.
 static pme_status enqueue(pme_batch_t b) {
-  BTRY(enqueue_stages(b->launches, b->d_sys, b->d_items, b->tune, b->ncu, b->stream, b->timing ? b->tev : nullptr));
+  if (!b->launches2.empty()) {  // fork the second chain onto aux, join before the copy
+    BHIP(hipEventRecord(b->ev_fork, b->stream));
+    BHIP(hipStreamWaitEvent(b->aux, b->ev_fork, 0));
+    BTRY(enqueue_stages(b->launches, b->d_sys, b->d_items, b->tune, b->ncu, b->stream, nullptr));
+    BTRY(enqueue_stages(b->launches2, b->d_sys, b->d_items, b->tune, b->ncu, b->aux, nullptr));
+    BHIP(hipEventRecord(b->ev_join, b->aux));
+    BHIP(hipStreamWaitEvent(b->stream, b->ev_join, 0));
+  } else {
+    BTRY(enqueue_stages(b->launches, b->d_sys, b->d_items, b->tune, b->ncu, b->stream, b->timing ? b->tev : nullptr));
+  }
   const int n = (int)b->slots.size();
   if (n > 0)
     BHIP(hipMemcpyAsync(b->h_ev, b->d_ev, sizeof(double) * kEvOut * (size_t)n, hipMemcpyDeviceToHost, b->stream));
@@ -479,6 +518,7 @@ extern "C" pme_status pme_batch_set_timing(pme_batch_t b, int enable) {
   BHIP(hipStreamSynchronize(b->stream));
   if (enable && !b->tev[0])
     for (int k = 0; k < 9; k++) BHIP(hipEventCreate(&b->tev[k]));
+  if (b->timing != (enable != 0)) b->lists_valid = false;  // timed solves run as one chain
   b->timing = enable != 0;
   for (float& v : b->stage_ms) v = 0.f;
   return PME_OK;
@@ -511,6 +551,24 @@ extern "C" pme_status pme_batch_set_tuning(pme_batch_t b, int brick_target, int 
     if (st == PME_ERR_BUDGET || st == PME_ERR_UNSUPPORTED) b->tune = old;  // nothing was freed
     return st;
   }
+  return PME_OK;
+}
+
+extern "C" pme_status pme_batch_set_chains(pme_batch_t b, int nchains) {
+  if (!b) return set_error(PME_ERR_INVALID_ARG, "pme_batch_set_chains: null batch");
+  if (nchains != 1 && nchains != 2) return set_error(PME_ERR_INVALID_ARG, "pme_batch_set_chains: %d", nchains);
+  if (nchains != b->nchains) {
+    BHIP(hipStreamSynchronize(b->stream));
+    b->nchains = nchains;
+    b->lists_valid = false;
+    drop_graph(b);
+  }
+  return PME_OK;
+}
+
+extern "C" pme_status pme_batch_get_chains(pme_batch_t b, int* nchains) {
+  if (!b || !nchains) return set_error(PME_ERR_INVALID_ARG, "pme_batch_get_chains: null argument");
+  *nchains = b->nchains;
   return PME_OK;
 }
 
@@ -575,7 +633,7 @@ static std::map<std::string, std::vector<double>> read_cache(const char* path) {
     std::stringstream vs(t.substr(lb + 1, rb - lb - 1));
     std::string tok;
     while (std::getline(vs, tok, ',')) vals.push_back(std::atof(tok.c_str()));
-    if (vals.size() == 4) m[key] = vals;
+    if (vals.size() == 5) m[key] = vals;  // brick, tile, wg/cu, chains, ms
     i = rb + 1;
   }
   return m;
@@ -589,7 +647,7 @@ static void write_cache(const char* path, const std::map<std::string, std::vecto
   size_t k = 0;
   for (auto& kv : m) {
     f << "  \"" << kv.first << "\": [" << (int)kv.second[0] << ", " << (int)kv.second[1] << ", " << (int)kv.second[2]
-      << ", " << kv.second[3] << "]" << (++k < m.size() ? "," : "") << "\n";
+      << ", " << (int)kv.second[3] << ", " << kv.second[4] << "]" << (++k < m.size() ? "," : "") << "\n";
   }
   f << "}\n";
 }
@@ -618,7 +676,8 @@ extern "C" pme_status pme_batch_autotune(pme_batch_t b, const char* cache_path, 
   auto hit = cache.find(key);
   if (hit != cache.end()) {
     BTRY(pme_batch_set_tuning(b, (int)hit->second[0], (int)hit->second[1], (int)hit->second[2]));
-    if (best_ms) *best_ms = (float)hit->second[3];
+    BTRY(pme_batch_set_chains(b, (int)hit->second[3]));
+    if (best_ms) *best_ms = (float)hit->second[4];
     return PME_OK;
   }
   // the outputs of INT64 systems accumulate: tuning solves would add into them
@@ -626,22 +685,25 @@ extern "C" pme_status pme_batch_autotune(pme_batch_t b, const char* cache_path, 
     if (s.used && s.in.force_type == PME_FORCE_INT64)
       return set_error(PME_ERR_STATE, "pme_batch_autotune: resident INT64 (accumulating) outputs");
   float best = 1e30f;
-  int bb = b->tune.brick_target, bt = b->tune.fft_tile, bw = b->tune.wg_per_cu;
+  int bb = b->tune.brick_target, bt = b->tune.fft_tile, bw = b->tune.wg_per_cu, bc = b->nchains;
   for (int brick : {12, 16, 20, 24, 32})
     for (int tile : {8, 16, 32})
-      for (int wpc : {0, 4}) {
-        if (pme_batch_set_tuning(b, brick, tile, wpc) != PME_OK) continue;
-        float ms = 0.f;
-        if (time_solves(b, 3, 10, &ms) != PME_OK) continue;
-        if (ms < best) {
-          best = ms;
-          bb = brick;
-          bt = tile;
-          bw = wpc;
+      for (int wpc : {0, 4})
+        for (int ch : {1, 2}) {
+          if (pme_batch_set_tuning(b, brick, tile, wpc) != PME_OK || pme_batch_set_chains(b, ch) != PME_OK) continue;
+          float ms = 0.f;
+          if (time_solves(b, 3, 10, &ms) != PME_OK) continue;
+          if (ms < best) {
+            best = ms;
+            bb = brick;
+            bt = tile;
+            bw = wpc;
+            bc = ch;
+          }
         }
-      }
   BTRY(pme_batch_set_tuning(b, bb, bt, bw));
-  cache[key] = {(double)bb, (double)bt, (double)bw, (double)best};
+  BTRY(pme_batch_set_chains(b, bc));
+  cache[key] = {(double)bb, (double)bt, (double)bw, (double)bc, (double)best};
   write_cache(cache_path, cache);
   if (best_ms) *best_ms = best;
   return PME_OK;

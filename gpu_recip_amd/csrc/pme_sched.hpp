@@ -30,6 +30,8 @@ struct SystemPlan {
   size_t total;         // bytes of the region
 };
 
+constexpr int kBatchChains = 2;  // default stage chains of a batched solve (1 or 2)
+
 struct Tuning {
   int brick_target = kDefaultBrick;
   int fft_tile = kDefaultFftTile;

@@ -255,6 +255,15 @@ class PmeBatch:
         L.call("pme_batch_get_tuning", self.h, *[C.byref(x) for x in v])
         return tuple(x.value for x in v)
 
+    def set_chains(self, nchains: int) -> None:
+        """1 or 2 stage chains per solve (pme_batch_set_chains)."""
+        L.call("pme_batch_set_chains", self.h, int(nchains))
+
+    def get_chains(self) -> int:
+        v = C.c_int()
+        L.call("pme_batch_get_chains", self.h, C.byref(v))
+        return v.value
+
     def autotune(self, cache_path=None) -> float:
         ms = C.c_float()
         L.call("pme_batch_autotune", self.h, None if cache_path is None else str(cache_path).encode(), C.byref(ms))

@@ -2,7 +2,7 @@
 
     python -m gpu_recip_amd.workloads production [--solves 20] [--precision fp32|bf16x3|fp16x3]
     python -m gpu_recip_amd.workloads split      [--solves 20] [--precision ...]
-    python -m gpu_recip_amd.workloads batched    [--solves 20] [--tuning brick,tile,wg_per_cu]
+    python -m gpu_recip_amd.workloads batched    [--solves 20] [--tuning brick,tile,wg_per_cu[,chains]]
     python -m gpu_recip_amd.workloads small      [--solves 20]
     python -m gpu_recip_amd.workloads served     [--solves 1000]
 
@@ -30,7 +30,7 @@ def main(argv=None):
     ap.add_argument("--solves", type=int, default=20)
     ap.add_argument("--precision", choices=sorted(PRECS), default="fp32")
     ap.add_argument("--tuning", default=None,
-                    help="batched: brick,tile,wg_per_cu to use instead of autotuning (e.g. 24,32,0)")
+                    help="batched: brick,tile,wg_per_cu[,chains] to use instead of autotuning (e.g. 24,32,4,2)")
     args = ap.parse_args(argv)
 
     import torch
@@ -69,12 +69,15 @@ def main(argv=None):
             ids.append(b.add(PmeBatch.desc(s.nfft, s.order, s.kappa, s.recip, x, s.natom, f, L.FORCE_FLOAT, st,
                                            precision=PRECS[args.precision]), keep=(x, f)))
         if args.tuning:
-            b.set_tuning(*(int(v) for v in args.tuning.split(",")))
-            print(f"batched: plan (brick, tile, wg/cu) = {b.get_tuning()}", flush=True)
+            v = [int(t) for t in args.tuning.split(",")]
+            b.set_tuning(*v[:3])
+            if len(v) > 3:
+                b.set_chains(v[3])
+            print(f"batched: plan (brick, tile, wg/cu) = {b.get_tuning()}, {b.get_chains()} chain(s)", flush=True)
         else:
             ms = b.autotune(None)  # the plan the benchmark uses
-            print(f"batched: autotuned (brick, tile, wg/cu) = {b.get_tuning()}, {ms:.3f} ms per batched solve",
-                  flush=True)
+            print(f"batched: autotuned (brick, tile, wg/cu) = {b.get_tuning()}, {b.get_chains()} chain(s), "
+                  f"{ms:.3f} ms per batched solve", flush=True)
         t0 = time.perf_counter()
         for _ in range(args.solves):
             b.solve()

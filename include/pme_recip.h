@@ -215,10 +215,18 @@ pme_status pme_batch_set_graph(pme_batch_t b, int enable);
  * workgroup per work item).  Re-plans every resident system. */
 pme_status pme_batch_set_tuning(pme_batch_t b, int brick_target, int fft_tile, int wg_per_cu);
 pme_status pme_batch_get_tuning(pme_batch_t b, int* brick_target, int* fft_tile, int* wg_per_cu);
-/* Autotuner: time the current resident set over a grid of tuning choices,
- * keep the fastest, and persist it in a JSON cache keyed by the set's
- * signature and the GPU architecture (cache_path may be NULL).  Returns the
- * best time per solve (ms). */
+/* Stage chains of a solve: 2 (default) splits the systems into two halves of
+ * about equal cost whose stage sequences run on two streams side by side
+ * (joined before the energy/virial copy), so one half's kernel tails and
+ * launch gaps are filled by the other's work; 1 runs one sequence.  Results
+ * do not depend on it.  Solves with stage timing on run one chain. */
+pme_status pme_batch_set_chains(pme_batch_t b, int nchains);
+pme_status pme_batch_get_chains(pme_batch_t b, int* nchains);
+/* Autotuner: time the current resident set over a grid of tuning choices
+ * (brick target, FFT tile, workgroups per CU, stage chains), keep the
+ * fastest, and persist it in a JSON cache keyed by the set's signature and
+ * the GPU architecture (cache_path may be NULL).  Returns the best time per
+ * solve (ms). */
 pme_status pme_batch_autotune(pme_batch_t b, const char* cache_path, float* best_ms);
 pme_status pme_batch_memory(pme_batch_t b, size_t* used_bytes, size_t* budget_bytes);
 /* Per-stage kernel times of the last solve (HIP events on the batch's

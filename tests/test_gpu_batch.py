@@ -192,6 +192,38 @@ def test_batch_self_energy():
     say(f"  self energies {es[0]:.6f} .. {es[-1]:.6f}")
 
 
+@pytest.mark.parametrize("wg_per_cu", [0, 4])
+def test_two_chains_match_one(wg_per_cu):
+    """A solve split into two stage chains on two streams gives the forces,
+    energies, virials and self energies of the one-chain solve bit for bit,
+    graph replays and eager solves alike (the same kernels per system)."""
+    torch, dev = torch_dev()
+    sl = systems.config_batched(9)
+    b = PmeBatch(8 << 30)
+    reg = [_add(b, s, torch, dev, calc_ev=L.CALC_EV | L.CALC_SELF) for s in sl]
+    ids = [r[0] for r in reg]
+    b.set_tuning(24, 32, wg_per_cu)
+    out = {}
+    for chains in (1, 2):
+        b.set_chains(chains)
+        assert b.get_chains() == chains
+        for graph in (True, False):
+            b.set_graph(graph)
+            for f in (r[2] for r in reg):
+                f.zero_()
+            for _ in range(2):
+                b.solve()
+            e, v = b.results(ids)
+            es = b.results_self(ids)
+            out[(chains, graph)] = ([_forces(r[2], s.natom, r[3]) for s, r in zip(sl, reg)], e, v, es)
+    ref = out[(1, False)]
+    for key, (fs, e, v, es) in out.items():
+        assert all(np.array_equal(a, c) for a, c in zip(fs, ref[0])), key
+        assert np.array_equal(e, ref[1]) and np.array_equal(v, ref[2]) and np.array_equal(es, ref[3]), key
+    with pytest.raises(L.PmeError):
+        b.set_chains(3)
+
+
 def test_batched_config_vs_oracle():
     """The 64-system batched workload: every system against the single-system
     solver, eight of them against the FP64 oracle (exact coordinates)."""
