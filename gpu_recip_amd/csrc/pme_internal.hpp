@@ -127,6 +127,27 @@ __device__ __forceinline__ SegBounds brick_seg_load(const BinArgs& a, int b) {
 template <int NT>
 __device__ __forceinline__ int brick_segments(const BinArgs& a, const SegBounds& sb, int* s_pre, int* s_base,
                                               int* sh_scan) {
+  if (a.nchunk <= 64) {
+    // every chunk's bounds are in wave 0's registers: one wave-level scan and
+    // one barrier
+    if (threadIdx.x < 64) {
+      const int lane = threadIdx.x;
+      const int cnt = lane < a.nchunk ? sb.hi - sb.lo : 0;
+      int v = cnt;
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1) {
+        const int t = __shfl_up(v, off, 64);
+        if (lane >= off) v += t;
+      }
+      if (lane < a.nchunk) {
+        s_base[lane] = sb.lo;
+        s_pre[lane] = v - cnt;
+      }
+      if (lane == 63) s_pre[a.nchunk] = v;  // lanes past nchunk add nothing
+    }
+    __syncthreads();
+    return s_pre[a.nchunk];
+  }
   if ((int)threadIdx.x < a.nchunk) {
     s_base[threadIdx.x] = sb.lo;
     s_pre[threadIdx.x] = sb.hi - sb.lo;

@@ -202,6 +202,35 @@ __device__ __forceinline__ void spread_record(const BinArgs& a, float4 p, int x0
   }
 }
 
+// Store phase of a brick: pair i of x-neighbours (lx, lx + 1) of row (ly,
+// lz) converted from fixed point and stored; the second can fall on the
+// row's margin, inside the row's LDS stride, and is then not stored.  WRAP
+// (one brick along x): the high margin is the periodic image of the row
+// start and is added in.  Products of small non-negative integers in 24 bits
+// (edges <= 512), grid offsets in 32 bits (nx*ny < 2^19, offsets < 2^27).
+template <int P, bool WRAP>
+__device__ __forceinline__ void store_brick_pairs(const unsigned long long* __restrict__ acc, float* __restrict__ grid,
+                                                  int nx, int ny, int x0, int y0, int z0, int wx, int wy, int hw,
+                                                  int npair, int bxs, int plane) {
+  const int nxny = mul24(nx, ny);
+  const SmallDiv dhw = small_div(hw), dwy = small_div(wy);
+  for (int i = threadIdx.x; i < npair; i += kSpreadBlock) {
+    const int t = sdiv(i, dhw);
+    const int lx = 2 * (i - mul24(t, hw));
+    const int lz = sdiv(t, dwy);
+    const int ly = t - mul24(lz, wy);
+    const unsigned long long* row = acc + (mul24(lz, plane) + mul24(ly, bxs) + (P - 1));
+    unsigned long long v0 = row[lx], v1 = row[lx + 1];
+    if (WRAP) {
+      if (lx < P - 1) v0 += row[lx + nx];
+      if (lx + 1 < P - 1) v1 += row[lx + 1 + nx];
+    }
+    float* g = grid + ((x0 + lx) + mul24(nx, y0 + ly) + mul24(nxny, z0 + lz));
+    g[0] = from_fixed40(v0);
+    if (lx + 1 < wx) g[1] = from_fixed40(v1);
+  }
+}
+
 // Brick b of one grid (one kSpreadBlock-thread workgroup); smem: the
 // workgroup's dynamic LDS (spread_lds_bytes).  Each thread takes one record
 // per batch (an atom whose stencil touches the brick), keeps q*theta_x,
@@ -245,28 +274,13 @@ __device__ __forceinline__ void spread_brick_body(const BinArgs& a, int b, int w
   __syncthreads();
 
   // the brick is complete: one coalesced store per grid point, no memset
-  // needed.  With one brick along x the high margin is the periodic image of
-  // the row start.
-  // Two x-neighbours per thread (the second can fall on the row's margin,
-  // inside the row's LDS stride, and is then not stored).  Grid offsets in
-  // 32 bits (edges <= 512: nx*ny < 2^19, offsets < 2^27).
+  // needed; two x-neighbours per thread
   const int hw = (wx + 1) >> 1;
   const int npair = hw * wy * wz;
-  const int nxny = mul24(a.nx, a.ny);
-  const SmallDiv dhw = small_div(hw), dwy = small_div(wy);
-  for (int i = tid; i < npair; i += kSpreadBlock) {
-    const int t = sdiv(i, dhw);
-    const int lx = 2 * (i - t * hw);
-    const int lz = sdiv(t, dwy);
-    const int ly = t - lz * wy;
-    const unsigned long long* row = acc + mul24(lz, plane) + mul24(ly, bxs) + (P - 1);
-    unsigned long long v0 = row[lx], v1 = row[lx + 1];
-    if (wrapx && lx < P - 1) v0 += row[lx + a.nx];
-    if (wrapx && lx + 1 < P - 1) v1 += row[lx + 1 + a.nx];
-    float* g = grid + ((x0 + lx) + mul24(a.nx, y0 + ly) + mul24(nxny, z0 + lz));
-    g[0] = from_fixed40(v0);
-    if (lx + 1 < wx) g[1] = from_fixed40(v1);
-  }
+  if (wrapx)
+    store_brick_pairs<P, true>(acc, grid, a.nx, a.ny, x0, y0, z0, wx, wy, hw, npair, bxs, plane);
+  else
+    store_brick_pairs<P, false>(acc, grid, a.nx, a.ny, x0, y0, z0, wx, wy, hw, npair, bxs, plane);
 }
 
 template <int P>
