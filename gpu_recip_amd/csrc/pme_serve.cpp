@@ -26,9 +26,11 @@
 #include <deque>
 #include <map>
 #include <mutex>
+#include <set>
 #include <string>
 #include <thread>
 #include <unordered_map>
+#include <unordered_set>
 #include <vector>
 
 #include "../../include/pme_recip.h"
@@ -66,7 +68,9 @@ struct Arena {
     }
     return false;
   }
-  void release(size_t off, size_t bytes) {
+  // Returns the free block the released range ends up in (after merging with
+  // its neighbours) as (offset, bytes).
+  std::pair<size_t, size_t> release(size_t off, size_t bytes) {
     bytes = (bytes + 255) & ~(size_t)255;
     used -= bytes;
     auto it = free_.emplace(off, bytes).first;
@@ -80,8 +84,20 @@ struct Arena {
       if (pv->first + pv->second == it->first) {
         pv->second += it->second;
         free_.erase(it);
+        it = pv;
       }
     }
+    return {it->first, it->second};
+  }
+  // Allocate `bytes` from the start of the free block at `block`.
+  void take(size_t block, size_t bytes, size_t* off) {
+    bytes = (bytes + 255) & ~(size_t)255;
+    auto it = free_.find(block);
+    const size_t rest = it->second - bytes;
+    free_.erase(it);
+    if (rest) free_[block + bytes] = rest;
+    used += bytes;
+    *off = block;
   }
 };
 
@@ -152,10 +168,27 @@ struct pme_server_s {
   Arena arena;
   TableCache tables;
   std::unordered_map<int, Resident> resident;
+  std::set<std::pair<unsigned long long, int>> lru;  // (last_use, key) of every resident, oldest first
   std::deque<Request> pending;
   std::vector<Lane> lanes;
   unsigned long long use_clock = 0;
 };
+
+// Mark a resident as just used (its place in the LRU order).
+static void lru_touch(pme_server_t s, int key, Resident& R) {
+  if (R.last_use) s->lru.erase({R.last_use, key});
+  R.last_use = ++s->use_clock;
+  s->lru.insert({R.last_use, key});
+}
+
+// Drop a resident: its arena block, LRU entry and map entry.  Returns the
+// free block its memory joined.
+static std::pair<size_t, size_t> drop_resident(pme_server_t s, std::unordered_map<int, Resident>::iterator it) {
+  const std::pair<size_t, size_t> blk = s->arena.release(it->second.off, it->second.bytes);
+  s->lru.erase({it->second.last_use, it->first});
+  s->resident.erase(it);
+  return blk;
+}
 
 static void publish(pme_server_t s, const Request& r, pme_status status, const std::string& msg, const double* ev,
                     double t_done) {
@@ -181,32 +214,39 @@ static void publish(pme_server_t s, const Request& r, pme_status status, const s
 
 // ---- admission ----------------------------------------------------------------
 
-// Evict idle residents (LRU first) until `bytes` fit; never touches `keep`.
-static bool make_room(pme_server_t s, size_t bytes, const std::vector<int>& keep, size_t* off) {
-  while (!s->arena.alloc(bytes, off)) {
-    int victim = -1;
-    unsigned long long oldest = ~0ull;
-    for (auto& kv : s->resident) {
-      if (kv.second.busy) continue;
-      if (std::find(keep.begin(), keep.end(), kv.first) != keep.end()) continue;
-      if (kv.second.last_use < oldest) {
-        oldest = kv.second.last_use;
-        victim = kv.first;
-      }
+// Evict idle residents (LRU first) until `bytes` fit; never touches `keep`
+// (the keys already selected for the batch being formed).  The walk from the
+// oldest entry skips only residents in flight or selected, which were used
+// last.  When the first-fit search fails, only the block an eviction frees
+// (merged with its free neighbours) can newly fit, so that block alone is
+// checked after each eviction.
+static bool make_room(pme_server_t s, size_t bytes, const std::unordered_set<int>& keep, size_t* off) {
+  if (s->arena.alloc(bytes, off)) return true;
+  const size_t need = (bytes + 255) & ~(size_t)255;
+  for (;;) {
+    auto victim = s->resident.end();
+    for (const auto& e : s->lru) {
+      auto it = s->resident.find(e.second);
+      if (it->second.busy || keep.count(e.second)) continue;
+      victim = it;
+      break;
     }
-    if (victim < 0) return false;
-    Resident& v = s->resident[victim];
-    s->arena.release(v.off, v.bytes);
-    s->resident.erase(victim);
-    std::lock_guard<std::mutex> g(s->mu);
-    s->st.evictions++;
+    if (victim == s->resident.end()) return false;
+    const std::pair<size_t, size_t> blk = drop_resident(s, victim);
+    {
+      std::lock_guard<std::mutex> g(s->mu);
+      s->st.evictions++;
+    }
+    if (blk.second >= need) {
+      s->arena.take(blk.first, need, off);
+      return true;
+    }
   }
-  return true;
 }
 
 enum AdmitResult { ADMIT_OK, ADMIT_WAIT, ADMIT_FAIL };
 
-static AdmitResult admit(pme_server_t s, const Request& r, const Tuning& tune, const std::vector<int>& keep,
+static AdmitResult admit(pme_server_t s, const Request& r, const Tuning& tune, const std::unordered_set<int>& keep,
                          std::string* err) {
   auto it = s->resident.find(r.key);
   if (it != s->resident.end()) {
@@ -216,11 +256,10 @@ static AdmitResult admit(pme_server_t s, const Request& r, const Tuning& tune, c
     if (same) {
       R.in = r.in;
       fill_dynamic(R.d, R.in, R.plan);
-      R.last_use = ++s->use_clock;
+      lru_touch(s, r.key, R);
       return ADMIT_OK;
     }
-    s->arena.release(R.off, R.bytes);  // idle (keys in flight are not admitted)
-    s->resident.erase(it);
+    drop_resident(s, it);  // idle (keys in flight are not admitted)
   }
   Resident R;
   R.in = r.in;
@@ -240,8 +279,8 @@ static AdmitResult admit(pme_server_t s, const Request& r, const Tuning& tune, c
     *err = pme_last_error();
     return ADMIT_FAIL;
   }
-  R.last_use = ++s->use_clock;
-  s->resident[r.key] = R;
+  Resident& slot = s->resident[r.key] = R;
+  lru_touch(s, r.key, slot);
   std::lock_guard<std::mutex> g(s->mu);
   s->st.admitted_new++;
   return ADMIT_OK;
@@ -342,7 +381,7 @@ static bool retire(pme_server_t s) {
       auto it = s->resident.find(r.key);
       if (it != s->resident.end()) {
         it->second.busy = false;
-        it->second.last_use = ++s->use_clock;
+        lru_touch(s, r.key, it->second);
       }
       double ev[kEvOut] = {};
       const double* h = L.h_ev + kEvOut * k;
@@ -375,15 +414,12 @@ static void worker_main(pme_server_t s) {
       for (const Lane& L : s->lanes) lanes_busy = lanes_busy || L.busy;
       if (s->stop && s->pending.empty() && !lanes_busy) break;
     }
-    for (int key : rel) {
-      auto it = s->resident.find(key);
-      if (it != s->resident.end() && !it->second.busy) {
-        bool queued = false;
-        for (const Request& r : s->pending) queued = queued || r.key == key;
-        if (!queued) {
-          s->arena.release(it->second.off, it->second.bytes);
-          s->resident.erase(it);
-        }
+    if (!rel.empty()) {
+      std::unordered_set<int> queued;
+      for (const Request& r : s->pending) queued.insert(r.key);
+      for (int key : rel) {
+        auto it = s->resident.find(key);
+        if (it != s->resident.end() && !it->second.busy && !queued.count(key)) drop_resident(s, it);
       }
     }
     bool progress = retire(s);
@@ -395,7 +431,7 @@ static void worker_main(pme_server_t s) {
       }
     if (idle && !s->pending.empty()) {
       std::vector<Request> sel;
-      std::vector<int> keys;
+      std::unordered_set<int> keys;
       std::deque<Request> rest;
       bool blocked = false;
       bool any_busy = false;
@@ -405,7 +441,7 @@ static void worker_main(pme_server_t s) {
         s->pending.pop_front();
         auto it = s->resident.find(r.key);
         const bool in_flight = it != s->resident.end() && it->second.busy;
-        if (blocked || in_flight || std::find(keys.begin(), keys.end(), r.key) != keys.end()) {
+        if (blocked || in_flight || keys.count(r.key)) {
           rest.push_back(r);  // keeps its place for the next batch
           continue;
         }
@@ -413,7 +449,7 @@ static void worker_main(pme_server_t s) {
         const AdmitResult a = admit(s, r, tune, keys, &err);
         if (a == ADMIT_OK) {
           sel.push_back(r);
-          keys.push_back(r.key);
+          keys.insert(r.key);
         } else if (a == ADMIT_WAIT && (any_busy || !sel.empty())) {
           blocked = true;  // oldest first: wait for space instead of overtaking
           rest.push_back(r);
