@@ -121,6 +121,33 @@ def test_eviction_under_small_arena():
     srv.close()
 
 
+def test_burst_into_a_full_arena():
+    """A second burst of new systems into an arena full of idle residents
+    evicts the least recently used ones (never a system of the batch being
+    formed): every request completes with the plain solver's forces, and the
+    arena stays within its budget."""
+    torch, dev = torch_dev()
+    first = _small_schedule(n=24, nmax=20000, seed=21)
+    second = _small_schedule(n=24, nmax=20000, seed=22)
+    for r in second:
+        r.key += 1000  # new systems only
+    serving.materialise(first, dev)
+    serving.materialise(second, dev)
+    srv = PmeServer(24 << 20, nlanes=2)
+    serving.run_burst(srv, first)
+    st1 = srv.stats()
+    srv.reset_stats()
+    serving.run_burst(srv, second)
+    st2 = srv.stats()
+    say(f"  full arena: first burst resident {st1['resident']}, second burst evictions {st2['evictions']}")
+    assert st2["completed"] == len(second) and st2["failed"] == 0 and st2["evictions"] > 0
+    assert st2["arena_used"] <= st2["arena_budget"]
+    for r in second[::3]:
+        one = run_solve((r.nfft,) * 3, r.order, r.kappa, (r.box,) * 3, r.xyzq.cpu().numpy())
+        assert np.array_equal(_forces(r), one["force"])
+    srv.close()
+
+
 def test_rejects_and_release():
     torch, dev = torch_dev()
     reqs = _small_schedule(n=2, nmax=6000, seed=9)
