@@ -86,6 +86,7 @@ struct pme_batch_s {
   std::vector<Launch> launches2;  // chain on `aux` (empty: one chain)
   hipStream_t aux = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  int* d_ctr = nullptr;  // work-queue counters: 2 per launch, chain 2 after chain 1's kMaxLaunches
   WorkItem* d_items = nullptr;
   size_t items_cap = 0;
   // graph of the current subset
@@ -190,7 +191,9 @@ extern "C" pme_status pme_batch_create(size_t hbm_budget_bytes, pme_stream_t str
   }
   if (hipStreamCreateWithFlags(&b->aux, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&b->ev_join, hipEventDisableTiming) != hipSuccess) {
+      hipEventCreateWithFlags(&b->ev_join, hipEventDisableTiming) != hipSuccess ||
+      hipMalloc(&b->d_ctr, sizeof(int) * 4 * kMaxLaunches) != hipSuccess ||
+      hipMemset(b->d_ctr, 0, sizeof(int) * 4 * kMaxLaunches) != hipSuccess) {
     pme_batch_destroy(b);
     return set_error(PME_ERR_DEVICE, "pme_batch_create: stream/event creation failed");
   }
@@ -221,6 +224,7 @@ extern "C" pme_status pme_batch_destroy(pme_batch_t b) {
   (void)hipFree(b->d_ev);
   (void)hipHostFree(b->h_ev);
   (void)hipFree(b->d_items);
+  (void)hipFree(b->d_ctr);
   for (hipEvent_t e : b->tev)
     if (e) (void)hipEventDestroy(e);
   if (b->ev_in) (void)hipEventDestroy(b->ev_in);
@@ -358,12 +362,14 @@ static pme_status enqueue(pme_batch_t b) {
   if (!b->launches2.empty()) {  // fork the second chain onto aux, join before the copy
     BHIP(hipEventRecord(b->ev_fork, b->stream));
     BHIP(hipStreamWaitEvent(b->aux, b->ev_fork, 0));
-    BTRY(enqueue_stages(b->launches, b->d_sys, b->d_items, b->tune, b->ncu, b->stream, nullptr));
-    BTRY(enqueue_stages(b->launches2, b->d_sys, b->d_items, b->tune, b->ncu, b->aux, nullptr));
+    BTRY(enqueue_stages(b->launches, b->d_sys, b->d_items, b->d_ctr, b->tune, b->ncu, b->stream, nullptr));
+    BTRY(enqueue_stages(b->launches2, b->d_sys, b->d_items, b->d_ctr + 2 * kMaxLaunches, b->tune, b->ncu, b->aux,
+                        nullptr));
     BHIP(hipEventRecord(b->ev_join, b->aux));
     BHIP(hipStreamWaitEvent(b->stream, b->ev_join, 0));
   } else {
-    BTRY(enqueue_stages(b->launches, b->d_sys, b->d_items, b->tune, b->ncu, b->stream, b->timing ? b->tev : nullptr));
+    BTRY(enqueue_stages(b->launches, b->d_sys, b->d_items, b->d_ctr, b->tune, b->ncu, b->stream,
+                        b->timing ? b->tev : nullptr));
   }
   const int n = (int)b->slots.size();
   if (n > 0)

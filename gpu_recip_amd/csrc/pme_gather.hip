@@ -264,24 +264,26 @@ hipError_t launch_gather(int order, int force_type, const float4* xyzq, const Bi
 
 template <int P>
 __global__ __launch_bounds__(kSpreadBlock) void k_batch_gather(const SysDesc* __restrict__ sys,
-                                                               const WorkItem* __restrict__ items, int nitems) {
+                                                               const WorkItem* __restrict__ items, int nitems,
+                                                               int* ctr) {
   extern __shared__ __align__(16) unsigned char smem[];
-  for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+  for (int it = blockIdx.x; it < nitems; it = next_item(nitems, ctr)) {
     const WorkItem w = uniform_item(items[it]);
     const SysDesc& d = sys[w.sys];
     gather_brick_body<P, true>(d.xyzq, d.bin, w.idx, d.wmax[0], d.wmax[1], d.wmax[2], d.grid, (float)kCcelec, d.force_type,
                          d.stride, d.force, smem);
     __syncthreads();
   }
+  items_done(nitems, ctr);
 }
 
-hipError_t launch_batch_gather(int order, const SysDesc* sys, const WorkItem* items, int nitems, int grid, size_t lds,
-                               hipStream_t s) {
+hipError_t launch_batch_gather(int order, const SysDesc* sys, const WorkItem* items, int nitems, int* ctr, int grid,
+                               size_t lds, hipStream_t s) {
   if (nitems <= 0) return hipSuccess;
   switch (order) {
-    case 4: hipLaunchKernelGGL(k_batch_gather<4>, dim3(grid), dim3(kSpreadBlock), lds, s, sys, items, nitems); break;
-    case 6: hipLaunchKernelGGL(k_batch_gather<6>, dim3(grid), dim3(kSpreadBlock), lds, s, sys, items, nitems); break;
-    case 8: hipLaunchKernelGGL(k_batch_gather<8>, dim3(grid), dim3(kSpreadBlock), lds, s, sys, items, nitems); break;
+    case 4: hipLaunchKernelGGL(k_batch_gather<4>, dim3(grid), dim3(kSpreadBlock), lds, s, sys, items, nitems, ctr); break;
+    case 6: hipLaunchKernelGGL(k_batch_gather<6>, dim3(grid), dim3(kSpreadBlock), lds, s, sys, items, nitems, ctr); break;
+    case 8: hipLaunchKernelGGL(k_batch_gather<8>, dim3(grid), dim3(kSpreadBlock), lds, s, sys, items, nitems, ctr); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

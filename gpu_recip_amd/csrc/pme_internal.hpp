@@ -94,6 +94,34 @@ struct WorkItem {
   int idx;
 };
 
+// Work distribution of the batched spread and gather kernels (one brick per
+// item, one workgroup per CU at a time).  Workgroup b starts with item b.
+// With fewer workgroups than items (persistent launches) a workgroup that
+// finishes a brick takes the next one from the launch's counter ctr[0], so
+// the bricks balance as under the hardware dispatcher.  The fetch comes after
+// the brick's stores are issued and before the next brick's loads, since a
+// wait for those loads would also wait for an earlier fetch.  The last
+// workgroup to finish (counted in ctr[1]) puts both counters back to zero for
+// the next launch that uses them (stream order makes that launch see the
+// zeros).  One lane per workgroup touches the counters, with vector atomics.
+// The other batched kernels stride statically: their items are many and
+// short, and one counter shared by every XCD would serialize them.
+__device__ __forceinline__ int next_item(int nitems, int* ctr) {
+  if ((int)gridDim.x >= nitems) return nitems;
+  __shared__ int s_next;
+  __syncthreads();
+  if (threadIdx.x == 0) s_next = (int)gridDim.x + atomicAdd(&ctr[0], 1);
+  __syncthreads();
+  return s_next;
+}
+__device__ __forceinline__ void items_done(int nitems, int* ctr) {
+  if ((int)gridDim.x >= nitems) return;
+  if (threadIdx.x == 0 && atomicAdd(&ctr[1], 1) == (int)gridDim.x - 1) {
+    atomicExch(&ctr[0], 0);
+    atomicExch(&ctr[1], 0);
+  }
+}
+
 // Work items are the same for every lane of a workgroup: make that explicit
 // so descriptor fields are read with scalar loads and branch uniformly.
 __device__ __forceinline__ WorkItem uniform_item(WorkItem w) {
@@ -260,9 +288,9 @@ hipError_t launch_batch_plane(int inverse, int reduce, int maxnp, int split, con
 
 // batched launchers: `grid` persistent workgroups loop over the work items
 hipError_t launch_batch_bin(const SysDesc* sys, const WorkItem* items, int nitems, int grid, size_t lds, hipStream_t s);
-hipError_t launch_batch_spread(int order, const SysDesc* sys, const WorkItem* items, int nitems, int grid, size_t lds,
+hipError_t launch_batch_spread(int order, const SysDesc* sys, const WorkItem* items, int nitems, int* ctr, int grid, size_t lds,
                                hipStream_t s);
-hipError_t launch_batch_gather(int order, const SysDesc* sys, const WorkItem* items, int nitems, int grid, size_t lds,
+hipError_t launch_batch_gather(int order, const SysDesc* sys, const WorkItem* items, int nitems, int* ctr, int grid, size_t lds,
                                hipStream_t s);
 hipError_t launch_batch_fft_x(int inverse, const SysDesc* sys, const WorkItem* items, int nitems, int grid, size_t lds,
                               hipStream_t s);

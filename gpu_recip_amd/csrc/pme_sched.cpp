@@ -268,7 +268,15 @@ void build_items(const SysDesc* const* descs, const SystemPlan* const* plans, co
   for (int o : {4, 6, 8}) add_stage(BS_GATHER, o, 0);
 }
 
-static int grid_for(const Tuning& t, int ncu, int count, bool big_block) {
+// Workgroups of a batched launch.  Spread and gather (queue != 0) always run
+// one persistent workgroup per CU taking bricks from the launch's work queue
+// (their LDS brick leaves room for one workgroup per CU anyway, and a
+// persistent workgroup saves the start of a new one per brick).  The others
+// run one workgroup per item with wg_per_cu == 0, else wg_per_cu persistent
+// workgroups per CU (one for the LDS-filling plane kernels) striding over the
+// items.
+static int grid_for(const Tuning& t, int ncu, int count, bool big_block, bool queue) {
+  if (queue) return std::max(1, std::min(count, ncu));
   if (t.wg_per_cu <= 0) return count;
   const int per_cu = big_block ? 1 : t.wg_per_cu;
   return std::max(1, std::min(count, ncu * per_cu));
@@ -281,7 +289,8 @@ static int grid_for(const Tuning& t, int ncu, int count, bool big_block) {
   } while (0)
 
 pme_status enqueue_stages(const std::vector<Launch>& launches, const SysDesc* d_sys, const WorkItem* d_items,
-                          const Tuning& t, int ncu, hipStream_t stream, hipEvent_t* tev) {
+                          int* d_ctr, const Tuning& t, int ncu, hipStream_t stream, hipEvent_t* tev) {
+  if (launches.size() > (size_t)kMaxLaunches) return set_error(PME_ERR_UNSUPPORTED, "batched solve: too many launches");
   // timing slot k: the stages run between events k and k+1
   //   0 bin, 1 spread, 2 x forward and xy-plane forward, 3 y forward, 4 z (+ influence function),
   //   5 y inverse and xy-plane inverse (with the energy/virial reduction), 6 x inverse, 7 gather
@@ -293,14 +302,16 @@ pme_status enqueue_stages(const std::vector<Launch>& launches, const SysDesc* d_
     for (int pass = 0; pass < 2; pass++) {
       const int stage = pass == 0 ? seq[k][0] : seq[k][2];
       if (stage < 0) continue;
-      for (const Launch& L : launches) {
+      for (size_t li = 0; li < launches.size(); li++) {
+        const Launch& L = launches[li];
         if (L.stage != stage) continue;
         const WorkItem* it = d_items + L.offset;
-        const bool big = stage == BS_SPREAD || stage == BS_GATHER || stage == BS_PLANE;
-        const int grid = grid_for(t, ncu, L.count, big);
+        int* ctr = d_ctr + 2 * li;  // this launch's work-queue counters (spread, gather)
+        const bool queue = stage == BS_SPREAD || stage == BS_GATHER;
+        const int grid = grid_for(t, ncu, L.count, stage == BS_PLANE, queue);
         switch (stage) {
           case BS_BIN: SCHED_HIP(launch_batch_bin(d_sys, it, L.count, grid, L.lds, stream)); break;
-          case BS_SPREAD: SCHED_HIP(launch_batch_spread(L.order, d_sys, it, L.count, grid, L.lds, stream)); break;
+          case BS_SPREAD: SCHED_HIP(launch_batch_spread(L.order, d_sys, it, L.count, ctr, grid, L.lds, stream)); break;
           case BS_FFTX: SCHED_HIP(launch_batch_fft_x(inv ? 1 : 0, d_sys, it, L.count, grid, L.lds, stream)); break;
           case BS_FFTY:
             SCHED_HIP(launch_batch_cols(1, inv ? COL_INV : COL_FWD, inv ? 1 : 0, L.split, d_sys, it, L.count, grid, L.lds,
@@ -313,7 +324,7 @@ pme_status enqueue_stages(const std::vector<Launch>& launches, const SysDesc* d_
           case BS_FFTZ:
             SCHED_HIP(launch_batch_cols(2, COL_FWD_SOLVE_INV, 0, L.split, d_sys, it, L.count, grid, L.lds, stream));
             break;
-          case BS_GATHER: SCHED_HIP(launch_batch_gather(L.order, d_sys, it, L.count, grid, L.lds, stream)); break;
+          case BS_GATHER: SCHED_HIP(launch_batch_gather(L.order, d_sys, it, L.count, ctr, grid, L.lds, stream)); break;
           default: break;
         }
       }

@@ -75,9 +75,13 @@ void build_items(const SysDesc* const* descs, const SystemPlan* const* plans, co
 
 // Enqueue the eight stages (bin, spread, x, y, z+solve, y inverse with the
 // energy/virial reduction, x inverse, gather) on `stream`.  With tev != NULL,
-// tev[k] is recorded before stage k and tev[8] after the last.
+// tev[k] is recorded before stage k and tev[8] after the last.  d_ctr: two
+// zeroed ints per launch (the persistent launches' work-queue counters, left
+// zeroed by each launch); launches that may run at the same time need
+// distinct counters.
+constexpr int kMaxLaunches = 64;  // launches of one stage sequence (d_ctr holds 2 ints each)
 pme_status enqueue_stages(const std::vector<Launch>& launches, const SysDesc* d_sys, const WorkItem* d_items,
-                          const Tuning& t, int ncu, hipStream_t stream, hipEvent_t* tev);
+                          int* d_ctr, const Tuning& t, int ncu, hipStream_t stream, hipEvent_t* tev);
 
 // Cost used to order systems (largest first).
 double system_cost(const pme_system_desc& in);

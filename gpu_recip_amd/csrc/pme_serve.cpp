@@ -136,6 +136,7 @@ struct Lane {
   std::vector<Request> reqs;  // in flight, in lane order
   SysDesc* d_sys = nullptr;
   WorkItem* d_items = nullptr;
+  int* d_ctr = nullptr;              // work-queue counters, 2 per launch (zeroed at creation)
   double* d_ev = nullptr;
   size_t cap_sys = 0, cap_items = 0;
   unsigned char* h_stage = nullptr;  // pinned: descriptors then items
@@ -353,7 +354,7 @@ static pme_status launch_lane(pme_server_t s, Lane& L, std::vector<Request>& sel
       hipMemcpyAsync(L.d_items, L.h_stage + sizeof(SysDesc) * L.cap_sys, sizeof(WorkItem) * items.size(),
                      hipMemcpyHostToDevice, L.stream) != hipSuccess)
     return set_error(PME_ERR_DEVICE, "serving lane upload");
-  st = enqueue_stages(launches, L.d_sys, L.d_items, tune, s->ncu, L.stream, nullptr);
+  st = enqueue_stages(launches, L.d_sys, L.d_items, L.d_ctr, tune, s->ncu, L.stream, nullptr);
   if (st != PME_OK) return st;
   if (hipMemcpyAsync(L.h_ev, L.d_ev, sizeof(double) * kEvOut * m, hipMemcpyDeviceToHost, L.stream) != hipSuccess ||
       hipEventRecord(L.done, L.stream) != hipSuccess)
@@ -515,7 +516,9 @@ extern "C" pme_status pme_server_create(size_t hbm_budget_bytes, int nlanes, pme
   s->lanes.resize(nlanes);
   for (Lane& L : s->lanes) {
     if (hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreateWithFlags(&L.done, hipEventDisableTiming) != hipSuccess) {
+        hipEventCreateWithFlags(&L.done, hipEventDisableTiming) != hipSuccess ||
+        hipMalloc(&L.d_ctr, sizeof(int) * 2 * kMaxLaunches) != hipSuccess ||
+        hipMemset(L.d_ctr, 0, sizeof(int) * 2 * kMaxLaunches) != hipSuccess) {
       pme_server_destroy(s);
       return set_error(PME_ERR_DEVICE, "pme_server_create: stream/event creation failed");
     }
@@ -538,6 +541,7 @@ extern "C" pme_status pme_server_destroy(pme_server_t s) {
     if (L.stream) (void)hipStreamSynchronize(L.stream);
     (void)hipFree(L.d_sys);
     (void)hipFree(L.d_items);
+    (void)hipFree(L.d_ctr);
     (void)hipFree(L.d_ev);
     for (void* p : L.retired) (void)hipFree(p);
     if (L.h_stage) (void)hipHostFree(L.h_stage);

@@ -342,14 +342,16 @@ __global__ __launch_bounds__(kBinBlock) void k_batch_bin(const SysDesc* __restri
 
 template <int P>
 __global__ __launch_bounds__(kSpreadBlock) void k_batch_spread(const SysDesc* __restrict__ sys,
-                                                               const WorkItem* __restrict__ items, int nitems) {
+                                                               const WorkItem* __restrict__ items, int nitems,
+                                                               int* ctr) {
   extern __shared__ __align__(16) unsigned char smem[];
-  for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+  for (int it = blockIdx.x; it < nitems; it = next_item(nitems, ctr)) {
     const WorkItem w = uniform_item(items[it]);
     const SysDesc& d = sys[w.sys];
     spread_brick_body<P>(d.bin, w.idx, d.wmax[0], d.wmax[1], d.wmax[2], d.grid, smem);
     __syncthreads();
   }
+  items_done(nitems, ctr);
 }
 
 hipError_t launch_batch_bin(const SysDesc* sys, const WorkItem* items, int nitems, int grid, size_t lds,
@@ -359,13 +361,13 @@ hipError_t launch_batch_bin(const SysDesc* sys, const WorkItem* items, int nitem
   return hipGetLastError();
 }
 
-hipError_t launch_batch_spread(int order, const SysDesc* sys, const WorkItem* items, int nitems, int grid, size_t lds,
-                               hipStream_t s) {
+hipError_t launch_batch_spread(int order, const SysDesc* sys, const WorkItem* items, int nitems, int* ctr, int grid,
+                               size_t lds, hipStream_t s) {
   if (nitems <= 0) return hipSuccess;
   switch (order) {
-    case 4: hipLaunchKernelGGL(k_batch_spread<4>, dim3(grid), dim3(kSpreadBlock), lds, s, sys, items, nitems); break;
-    case 6: hipLaunchKernelGGL(k_batch_spread<6>, dim3(grid), dim3(kSpreadBlock), lds, s, sys, items, nitems); break;
-    case 8: hipLaunchKernelGGL(k_batch_spread<8>, dim3(grid), dim3(kSpreadBlock), lds, s, sys, items, nitems); break;
+    case 4: hipLaunchKernelGGL(k_batch_spread<4>, dim3(grid), dim3(kSpreadBlock), lds, s, sys, items, nitems, ctr); break;
+    case 6: hipLaunchKernelGGL(k_batch_spread<6>, dim3(grid), dim3(kSpreadBlock), lds, s, sys, items, nitems, ctr); break;
+    case 8: hipLaunchKernelGGL(k_batch_spread<8>, dim3(grid), dim3(kSpreadBlock), lds, s, sys, items, nitems, ctr); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
