@@ -186,8 +186,9 @@ def main():
     atoms_b = sum(s.natom for s in sl)
 
     # per-stage kernel times of the batched solve (HIP events on its stream):
-    # every stage as one launch over all 64 systems (one chain, one workgroup
-    # per work item), the tuned brick and FFT tile
+    # every stage as one launch over all 64 systems (one chain; one workgroup
+    # per work item, spread and gather one per CU from a work queue), the tuned
+    # brick and FFT tile
     batch.set_tuning(tuning[0], tuning[1], 0)
     batch.set_timing(True)
     stage_ms = np.zeros(len(STAGES))
