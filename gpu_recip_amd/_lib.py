@@ -122,8 +122,8 @@ SIGNATURES = {
     "pme_batch_update": [_vp, _i, C.POINTER(SystemDesc)],
     "pme_batch_remove": [_vp, _i],
     "pme_batch_solve": [_vp, _i, _pi],
-    "pme_batch_results": [_vp, _i, _pi, _pd, _pd],
-    "pme_batch_results_self": [_vp, _i, _pi, _pd],
+    "pme_batch_results": [_vp, _i, _pi, _vp, _vp],
+    "pme_batch_results_self": [_vp, _i, _pi, _vp],
     "pme_batch_set_graph": [_vp, _i],
     "pme_batch_set_tuning": [_vp, _i, _i, _i],
     "pme_batch_get_tuning": [_vp, _pi, _pi, _pi],

@@ -172,6 +172,7 @@ class PmeBatch:
         L.call("pme_batch_create", C.c_size_t(int(hbm_budget_bytes)), _stream_ptr(stream), C.byref(h))
         self.h = h
         self._keep = {}  # id -> tensors the descriptor points at
+        self._id_arrays = {}  # tuple of ids -> (n, ctypes int array), reused by repeated solves
 
     def __del__(self):
         if getattr(self, "h", None) is not None and L._lib is not None:
@@ -211,12 +212,17 @@ class PmeBatch:
         L.call("pme_batch_remove", self.h, int(sid))
         self._keep.pop(sid, None)
 
-    @staticmethod
-    def _ids(ids):
+    def _ids(self, ids):
+        # a solve loop passes the same ids every step: keep their C array
         if ids is None:
             return 0, None
-        arr = (C.c_int * len(ids))(*[int(i) for i in ids])
-        return len(ids), arr
+        key = tuple(ids)
+        hit = self._id_arrays.get(key)
+        if hit is None:
+            if len(self._id_arrays) >= 16:
+                self._id_arrays.clear()
+            hit = self._id_arrays[key] = (len(key), (C.c_int * len(key))(*[int(i) for i in key]))
+        return hit
 
     def solve(self, ids=None) -> None:
         n, arr = self._ids(ids)
@@ -230,8 +236,7 @@ class PmeBatch:
         n, arr = self._ids(ids)
         e = np.empty(n, np.float64)
         v = np.empty((n, 9), np.float64)
-        pd = C.POINTER(C.c_double)
-        L.call("pme_batch_results", self.h, n, arr, e.ctypes.data_as(pd), v.ctypes.data_as(pd))
+        L.call("pme_batch_results", self.h, n, arr, e.ctypes.data, v.ctypes.data)
         return e, v
 
     def results_self(self, ids):
@@ -241,7 +246,7 @@ class PmeBatch:
             raise ValueError("PmeBatch.results_self: list the system ids")
         n, arr = self._ids(ids)
         es = np.empty(n, np.float64)
-        L.call("pme_batch_results_self", self.h, n, arr, es.ctypes.data_as(C.POINTER(C.c_double)))
+        L.call("pme_batch_results_self", self.h, n, arr, es.ctypes.data)
         return es
 
     def set_graph(self, enable: bool) -> None:
