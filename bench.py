@@ -156,7 +156,11 @@ def main():
 
     # ---------------- batched: 64 systems, one graph replay per step ----------
     sl = systems.config_batched()
-    batch = PmeBatch(32 << 30)
+    # the batch runs on a stream of the caller's (as an MD engine would pass
+    # its own), so no ordering against the null stream is needed per solve;
+    # the inputs below are complete before the first solve (synchronize)
+    bstream = torch.cuda.Stream()
+    batch = PmeBatch(32 << 30, stream=bstream)
     reg = []
     for s in sl:
         x = torch.from_numpy(s.xyzq).to(dev)
@@ -166,6 +170,7 @@ def main():
                         keep=(x, f))
         reg.append((sid, x, f, stride))
     ids = [r[0] for r in reg]
+    torch.cuda.synchronize()
     tune_ms = None if args.no_autotune else batch.autotune(None)
     tuning = batch.get_tuning()
     chains = batch.get_chains()
