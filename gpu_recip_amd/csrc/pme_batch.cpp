@@ -405,7 +405,10 @@ extern "C" pme_status pme_batch_solve(pme_batch_t b, int n, const int* ids) {
   }
   if (v.empty()) return PME_OK;
   if (!b->lists_valid || v != b->cur_ids) BTRY(build_lists(b, v));
-  if (b->own_stream) {  // after the null stream's pending work (inputs written there)
+  // after the null stream's pending work (inputs written there); an idle
+  // null stream has nothing to order against, and skipping the cross-stream
+  // wait then saves a dependency hop on the device per solve
+  if (b->own_stream && hipStreamQuery(nullptr) != hipSuccess) {
     BHIP(hipEventRecord(b->ev_in, nullptr));
     BHIP(hipStreamWaitEvent(b->stream, b->ev_in, 0));
   }
