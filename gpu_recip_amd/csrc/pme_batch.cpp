@@ -195,7 +195,7 @@ extern "C" pme_status pme_batch_create(size_t hbm_budget_bytes, pme_stream_t str
       hipMalloc(&b->d_ctr, sizeof(int) * 4 * kMaxLaunches) != hipSuccess ||
       hipMemset(b->d_ctr, 0, sizeof(int) * 4 * kMaxLaunches) != hipSuccess) {
     pme_batch_destroy(b);
-    return set_error(PME_ERR_DEVICE, "pme_batch_create: stream/event creation failed");
+    return set_error(PME_ERR_DEVICE, "pme_batch_create: second stream, events or work-queue counters failed");
   }
   int dev = 0;
   hipDeviceProp_t prop;

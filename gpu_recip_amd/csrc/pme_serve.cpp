@@ -464,7 +464,7 @@ extern "C" pme_status pme_server_create(size_t hbm_budget_bytes, int nlanes, pme
         hipMalloc(&L.d_ctr, sizeof(int) * 2 * kMaxLaunches) != hipSuccess ||
         hipMemset(L.d_ctr, 0, sizeof(int) * 2 * kMaxLaunches) != hipSuccess) {
       pme_server_destroy(s);
-      return set_error(PME_ERR_DEVICE, "pme_server_create: stream/event creation failed");
+      return set_error(PME_ERR_DEVICE, "pme_server_create: lane stream, event or work-queue counters failed");
     }
   }
   s->st.arena_budget = s->arena.size;
