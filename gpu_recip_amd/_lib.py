@@ -186,8 +186,17 @@ def call(name: str, *args) -> None:
     check(getattr(load(), name)(*args), name)
 
 
+_dbl9_cache: dict = {}
+
+
 def dbl9(recip) -> C.Array:
-    arr = (C.c_double * 9)()
-    for i, v in enumerate(recip):
-        arr[i] = float(v)
+    """A C double[9] of the reciprocal box.  The C ABI copies the values during
+    the call, so the arrays of recently used boxes are kept and reused (a
+    solve passes the same box to three calls)."""
+    key = tuple(recip)
+    arr = _dbl9_cache.get(key)
+    if arr is None:
+        if len(_dbl9_cache) >= 64:
+            _dbl9_cache.clear()
+        arr = _dbl9_cache[key] = (C.c_double * 9)(*[float(v) for v in key])
     return arr
