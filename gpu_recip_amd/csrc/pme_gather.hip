@@ -2,13 +2,12 @@
 //
 // One workgroup per grid brick.  The workgroup stages the potential of its
 // brick plus a (p-1) halo on the upper side in LDS with coalesced loads.  It
-// then takes the atoms whose home copy the binning put in this brick,
-// compacts them with a wave ballot, loads their CURRENT coordinates (so the
-// result is right even if the caller moved atoms after spread_charge) and
-// gathers each atom in one thread: B-spline weights and derivatives in
-// registers, the p x p stencil rows read from LDS in pairs of dwords.  An
-// atom whose current base index is no longer in this brick reads the
-// potential from HBM.
+// then takes the atoms the binning put first in the brick's record segments
+// (its home atoms), loads their CURRENT coordinates (so the result is right
+// even if the caller moved atoms after spread_charge) and gathers each atom
+// in one thread: B-spline weights and derivatives in registers, the p x p
+// stencil rows read from LDS.  An atom whose current base index is no longer
+// in this brick reads the potential from HBM.
 #include "pme_internal.hpp"
 
 namespace pme {
@@ -67,29 +66,73 @@ __device__ __forceinline__ void gather_sums(const float* tx, const float* dtx, c
   }
 }
 
+// One atom's gather: B-spline weights and derivatives in registers, its
+// p x p stencil rows read from the brick's LDS potential (pl, [hz][hy][hx]),
+// or from HBM when the atom's current base index is no longer in the brick
+// (coordinates changed since the binning).
+template <int P>
+__device__ __forceinline__ void gather_atom(const BinArgs& a, const float4& p, int at, int x0, int y0, int z0, int wx,
+                                            int wy, int wz, int hx, int hy, const float* __restrict__ pl,
+                                            const float* __restrict__ phi, float ccelec_f, int ft, int stride,
+                                            void* force) {
+  int ix, iy, iz;
+  float fx, fy, fz;
+  grid_index(p.x, a.r1, a.nx, ix, fx);
+  grid_index(p.y, a.r2, a.ny, iy, fy);
+  grid_index(p.z, a.r3, a.nz, iz, fz);
+  ix = safe_index(ix, a.nx);
+  iy = safe_index(iy, a.ny);
+  iz = safe_index(iz, a.nz);
+  float tx[P], dtx[P], ty[P], dty[P], tz[P], dtz[P];
+  bspline_theta_dtheta<P>(fx, tx, dtx);
+  bspline_theta_dtheta<P>(fy, ty, dty);
+  bspline_theta_dtheta<P>(fz, tz, dtz);
+  int dx = ix - x0, dy = iy - y0, dz = iz - z0;
+  if (dx >= wx) dx -= a.nx;
+  if (dy >= wy) dy -= a.ny;
+  if (dz >= wz) dz -= a.nz;
+  float f1, f2, f3;
+  if (dx >= 0 && dy >= 0 && dz >= 0) {
+    const float* col = pl + mul24(mul24(dz, hy) + dy, hx) + dx;
+    const int rs = hx, ps = mul24(hy, hx);
+    gather_sums<P>(tx, dtx, ty, dty, tz, dtz, [&](int jx, int jy, int jz) { return col[jz * ps + jy * rs + jx]; }, f1,
+                   f2, f3);
+  } else {
+    gather_sums<P>(tx, dtx, ty, dty, tz, dtz,
+                   [&](int jx, int jy, int jz) {
+                     int gx = ix + jx, gy = iy + jy, gz = iz + jz;
+                     if (gx >= a.nx) gx -= a.nx;
+                     if (gy >= a.ny) gy -= a.ny;
+                     if (gz >= a.nz) gz -= a.nz;
+                     return phi[(size_t)gx + (size_t)a.nx * ((size_t)gy + (size_t)a.ny * gz)];
+                   },
+                   f1, f2, f3);
+  }
+  // CudaPMERecip.cu:1231-1241: q*ccelec, then q*recip*f*nfft in FP32
+  const float q = p.w * ccelec_f;
+  store_force(ft, force, at, stride, q * a.r1 * f1 * (float)a.nx, q * a.r2 * f2 * (float)a.ny,
+              q * a.r3 * f3 * (float)a.nz);
+}
+
 // Brick b of one grid (one kSpreadBlock-thread workgroup); smem: the
 // workgroup's dynamic LDS (gather_lds_bytes).  ft: PME_FORCE_* store mode.
 // The brick's potential plus a (p-1) upper halo is staged in LDS; the home
-// atoms of the brick are compacted with a wave ballot and each is then
-// gathered by ONE thread with its B-spline weights in registers, reading its
-// p x p stencil rows straight from LDS.  An atom whose current base index is
-// no longer in this brick (moved since the binning) reads HBM instead.
-// FROM_REC: the coordinates are the binning's copy in the brick's record
-// store (contiguous per brick) instead of xyzq[atom] (scattered); only for
-// solves that bin and gather the same coordinates (the batched path: one
-// solve reads the caller's array once).
+// records of the brick (the atoms whose base grid index it holds, first in
+// each of its record segments) are then gathered one atom per thread.
+// FROM_REC: the coordinates are the binning's copy in the record store
+// (contiguous per brick) instead of xyzq[atom] (scattered); only for solves
+// that bin and gather the same coordinates (the batched path: one solve reads
+// the caller's array once).  Otherwise the current coordinates are read, so
+// an atom moved since the binning is still right (gather_atom).
 template <int P, bool FROM_REC>
 __device__ __forceinline__ void gather_brick_body(const float4* __restrict__ xyzq, const BinArgs& a, int b, int wxm,
                                                   int wym, int wzm, const float* __restrict__ phi, float ccelec_f,
                                                   int ft, int stride, void* force, unsigned char* smem) {
   __shared__ int sh_scan[kSpreadBlock / 64];
-  __shared__ int s_cnt;
   const int hmax = (wxm + P - 1) * (wym + P - 1) * (wzm + P - 1);
-  float* pl = reinterpret_cast<float*>(smem);                             // halo brick, [hz][hy][hx]
-  float4* s_pos = reinterpret_cast<float4*>(pl + ((hmax + 3) & ~3));      // [kGatherCap] home atoms' coordinates
-  int* s_atom = reinterpret_cast<int*>(s_pos + kGatherCap);               // [kGatherCap] their indices
-  int* s_pre = s_atom + kGatherCap;                                        // [nchunk + 1]
-  int* s_base = s_pre + a.nchunk + 1;                                      // [nchunk]
+  float* pl = reinterpret_cast<float*>(smem);                 // halo brick, [hz][hy][hx]
+  int* s_pre = reinterpret_cast<int*>(pl + ((hmax + 3) & ~3));  // [nchunk + 1]
+  int* s_base = s_pre + a.nchunk + 1;                           // [nchunk]
 
   const int tid = threadIdx.x;
   const int bxi = b % a.nbx, byi = (b / a.nbx) % a.nby, bzi = b / (a.nbx * a.nby);
@@ -97,8 +140,8 @@ __device__ __forceinline__ void gather_brick_body(const float4* __restrict__ xyz
   const int y0 = brick_lo(byi, a.ny, a.nby), wy = brick_lo(byi + 1, a.ny, a.nby) - y0;
   const int z0 = brick_lo(bzi, a.nz, a.nbz), wz = brick_lo(bzi + 1, a.nz, a.nbz) - z0;
 
-  // the record segment bounds are in flight while the halo is staged
-  const SegBounds sb = brick_seg_load(a, b);
+  // the home-record segment bounds are in flight while the halo is staged
+  const SegBounds sb = brick_seg_load<true>(a, b);
 
   // stage brick + upper halo (periodic) in LDS in this brick's own [hz][hy][hx]
   // layout, so LDS word i is halo point i; up to kGatherStageU loads in
@@ -130,20 +173,15 @@ __device__ __forceinline__ void gather_brick_body(const float4* __restrict__ xyz
   }
   const int total = brick_segments<kSpreadBlock>(a, sb, s_pre, s_base, sh_scan);  // ends with a barrier
 
+  // up to kRecPerThread home records per thread per round, every load of the
+  // round issued before the first is used
   for (int base = 0; base < total; base += kGatherCap) {
-    if (tid == 0) s_cnt = 0;
-    __syncthreads();
-    // 1. the home atoms of up to kGatherCap records (kRecPerThread per
-    //    thread), compacted with wave ballots into s_atom / s_pos.  Every
-    //    load of the round is issued before the first is used; the batched
-    //    path reads the record's coordinate copy beside its index.
     int atom[kRecPerThread];
-    bool home[kRecPerThread];
     float4 pos[kRecPerThread];
 #pragma unroll
     for (int r = 0; r < kRecPerThread; r++) {
       const int v = base + r * kSpreadBlock + tid;
-      atom[r] = 0;  // not a home copy
+      atom[r] = -1;
       pos[r] = make_float4(0.f, 0.f, 0.f, 0.f);
       if (v < total) {
         const size_t slot = brick_record(a, v, s_pre, s_base);
@@ -151,78 +189,15 @@ __device__ __forceinline__ void gather_brick_body(const float4* __restrict__ xyz
         if (FROM_REC) pos[r] = a.rec_pos[slot];
       }
     }
+    if (!FROM_REC) {
 #pragma unroll
-    for (int r = 0; r < kRecPerThread; r++) {
-      home[r] = atom[r] < 0;  // bit 31: the atom's home copy
-      atom[r] &= 0x7fffffff;
-      if (!FROM_REC && home[r]) pos[r] = xyzq[atom[r]];
+      for (int r = 0; r < kRecPerThread; r++)
+        if (atom[r] >= 0) pos[r] = xyzq[atom[r]];
     }
-    unsigned long long m[kRecPerThread];
-    int nwave = 0;
 #pragma unroll
-    for (int r = 0; r < kRecPerThread; r++) {
-      m[r] = __ballot(home[r]);
-      nwave += __popcll(m[r]);
-    }
-    int wbase = 0;
-    if ((tid & 63) == 0 && nwave != 0) wbase = atomicAdd(&s_cnt, nwave);
-    wbase = __shfl(wbase, 0, 64);
-    const unsigned long long lt = __lanemask_lt();
-#pragma unroll
-    for (int r = 0; r < kRecPerThread; r++) {
-      if (home[r]) {
-        const int k = wbase + __popcll(m[r] & lt);
-        s_atom[k] = atom[r];
-        s_pos[k] = pos[r];
-      }
-      wbase += __popcll(m[r]);
-    }
-    __syncthreads();
-    // 2. one thread per home atom, weights in registers
-    const int cnt = s_cnt;
-    for (int k = tid; k < cnt; k += kSpreadBlock) {
-      const int at = s_atom[k];
-      const float4 p = s_pos[k];
-      int ix, iy, iz;
-      float fx, fy, fz;
-      grid_index(p.x, a.r1, a.nx, ix, fx);
-      grid_index(p.y, a.r2, a.ny, iy, fy);
-      grid_index(p.z, a.r3, a.nz, iz, fz);
-      ix = safe_index(ix, a.nx);
-      iy = safe_index(iy, a.ny);
-      iz = safe_index(iz, a.nz);
-      float tx[P], dtx[P], ty[P], dty[P], tz[P], dtz[P];
-      bspline_theta_dtheta<P>(fx, tx, dtx);
-      bspline_theta_dtheta<P>(fy, ty, dty);
-      bspline_theta_dtheta<P>(fz, tz, dtz);
-      int dx = ix - x0, dy = iy - y0, dz = iz - z0;
-      if (dx >= wx) dx -= a.nx;
-      if (dy >= wy) dy -= a.ny;
-      if (dz >= wz) dz -= a.nz;
-      float f1, f2, f3;
-      if (dx >= 0 && dy >= 0 && dz >= 0) {
-        const float* col = pl + mul24(mul24(dz, hy) + dy, hx) + dx;
-        const int rs = hx, ps = mul24(hy, hx);
-        gather_sums<P>(tx, dtx, ty, dty, tz, dtz,
-                       [&](int jx, int jy, int jz) { return col[jz * ps + jy * rs + jx]; }, f1, f2, f3);
-      } else {
-        // coordinates changed since the binning: potential from HBM
-        gather_sums<P>(tx, dtx, ty, dty, tz, dtz,
-                       [&](int jx, int jy, int jz) {
-                         int gx = ix + jx, gy = iy + jy, gz = iz + jz;
-                         if (gx >= a.nx) gx -= a.nx;
-                         if (gy >= a.ny) gy -= a.ny;
-                         if (gz >= a.nz) gz -= a.nz;
-                         return phi[(size_t)gx + (size_t)a.nx * ((size_t)gy + (size_t)a.ny * gz)];
-                       },
-                       f1, f2, f3);
-      }
-      // CudaPMERecip.cu:1231-1241: q*ccelec, then q*recip*f*nfft in FP32
-      const float q = p.w * ccelec_f;
-      store_force(ft, force, at, stride, q * a.r1 * f1 * (float)a.nx, q * a.r2 * f2 * (float)a.ny,
-                  q * a.r3 * f3 * (float)a.nz);
-    }
-    __syncthreads();
+    for (int r = 0; r < kRecPerThread; r++)
+      if (atom[r] >= 0)
+        gather_atom<P>(a, pos[r], atom[r], x0, y0, z0, wx, wy, wz, hx, hy, pl, phi, ccelec_f, ft, stride, force);
   }
 }
 
@@ -236,8 +211,7 @@ __global__ __launch_bounds__(kSpreadBlock) void k_gather_brick(const float4* __r
 
 size_t gather_lds_bytes(int order, const int wmax[3], int nchunk) {
   const size_t halo = (size_t)(wmax[0] + order - 1) * (wmax[1] + order - 1) * (wmax[2] + order - 1);
-  return sizeof(float) * ((halo + 3) & ~(size_t)3) + (sizeof(float4) + sizeof(int)) * kGatherCap +
-         sizeof(int) * (2 * (size_t)nchunk + 1);
+  return sizeof(float) * ((halo + 3) & ~(size_t)3) + sizeof(int) * (2 * (size_t)nchunk + 1);
 }
 
 template <int P>

@@ -201,7 +201,9 @@ extern "C" pme_status pme_ev_energy_pointer(pme_ev_t ev, const char* name, doubl
 extern "C" pme_status pme_ev_virial_pointer(pme_ev_t ev, double** dptr) {
   if (!ev || !dptr) return fail(PME_ERR_INVALID_ARG, "pme_ev_virial_pointer: null argument");
   PME_TRY(ev_flush(ev, nullptr, false));
-  *dptr = reinterpret_cast<double*>(ev->d_buf + kOffVirmat);
+  // the start of the Virial_t blob (sforce_dp first, virmat at double 162),
+  // as getVirialPointer returns it (CudaEnergyVirial.cu:239-242)
+  *dptr = reinterpret_cast<double*>(ev->d_buf);
   return PME_OK;
 }
 
@@ -246,7 +248,7 @@ struct pme_recip_s {
   int nb[3] = {1, 1, 1}, wmax[3] = {0, 0, 0}, nbins = 0;
   int bins_for = -1;          // atom capacity of the record store
   int chunk = 0, nchunk = 0, nchunk_cap = 0;
-  int* seg_off = nullptr;     // [nchunk_cap][nbins + 1]
+  int* seg_off = nullptr;     // [nchunk_cap][seg_stride(nbins)]
   float4* rec_pos = nullptr;  // [nchunk_cap * 8 * chunk]
   int* rec_idx = nullptr;
 
@@ -368,7 +370,7 @@ static pme_status ensure_bins(pme_recip_t h, int n) {
   h->rec_idx = nullptr;
   h->bins_for = -1;
   const size_t rs = 8 * (size_t)chunk;
-  HIP_TRY(hipMalloc(&h->seg_off, sizeof(int) * (size_t)nchunk * (h->nbins + 1)));
+  HIP_TRY(hipMalloc(&h->seg_off, sizeof(int) * (size_t)nchunk * seg_stride(h->nbins)));
   HIP_TRY(hipMalloc(&h->rec_pos, sizeof(float4) * rs * nchunk));
   HIP_TRY(hipMalloc(&h->rec_idx, sizeof(int) * rs * nchunk));
   h->chunk = chunk;

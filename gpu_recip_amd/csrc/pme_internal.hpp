@@ -10,9 +10,14 @@ namespace pme {
 // Atoms sorted into grid bricks by a chunked counting sort.  Binning
 // workgroup w takes atoms [w*chunk, (w+1)*chunk) and writes one record per
 // brick an atom's stencil touches (1 to 8) into its own region of rec_stride
-// slots, grouped by brick; seg_off[w][b] is the start of brick b's records in
-// that region (exclusive prefix, seg_off[w][nbins] = total).  No global
-// atomics: counts and cursors live in LDS.
+// slots, grouped by brick, the brick of the atom's base grid index (its home)
+// first.  Per chunk the segment table holds seg_stride(nbins) ints: seg[b] =
+// start of brick b's records in the region (exclusive prefix, seg[nbins] =
+// total), then hend[b] = end of its home records (seg[b] <= hend[b] <=
+// seg[b+1]).  rec_pos holds every record's (x, y, z, q); rec_idx the atom
+// index of the home records only.  No global atomics: counts and cursors live
+// in LDS.
+__host__ __device__ constexpr int seg_stride(int nbins) { return 2 * nbins + 1; }
 struct BinArgs {
   int nx, ny, nz;          // grid
   float r1, r2, r3;        // (float) recip diagonal, as the reference casts it
@@ -21,9 +26,9 @@ struct BinArgs {
   int chunk;               // atoms per binning workgroup
   int nchunk;              // binning workgroups
   int rec_stride;          // record slots per chunk region (8 * chunk)
-  int* seg_off;            // [nchunk][nbins + 1]
+  int* seg_off;            // [nchunk][seg_stride(nbins)]
   float4* rec_pos;         // [nchunk * rec_stride] (x, y, z, q) copies
-  int* rec_idx;            // [nchunk * rec_stride] atom index, bit 31 set on the home copy
+  int* rec_idx;            // [nchunk * rec_stride] atom index (home records)
 };
 
 // Deferred energy/virial reduction: per-workgroup partials written by the
@@ -138,17 +143,20 @@ enum BatchStage { BS_BIN = 0, BS_SPREAD, BS_FFTX, BS_FFTY, BS_FFTZ, BS_GATHER, B
 // (s_pre[nchunk] = total).  Returns the total.  NT == blockDim.x.
 // The segment bounds of chunk w = threadIdx.x (every chunk has a thread:
 // nchunk <= kMaxChunks <= NT), loaded into registers so that a caller can
-// issue the loads first and overlap their latency with other work.
+// issue the loads first and overlap their latency with other work.  HOME:
+// the home records of the brick only (its gather), else all its records.
 struct SegBounds {
   int lo, hi;
 };
+template <bool HOME>
 __device__ __forceinline__ SegBounds brick_seg_load(const BinArgs& a, int b) {
   SegBounds s{0, 0};
   const int w = threadIdx.x;
   if (w < a.nchunk) {
-    const int* seg = a.seg_off + (size_t)w * (a.nbx * a.nby * a.nbz + 1);
+    const int nbins = a.nbx * a.nby * a.nbz;
+    const int* seg = a.seg_off + (size_t)w * seg_stride(nbins);
     s.lo = seg[b];
-    s.hi = seg[b + 1];
+    s.hi = HOME ? seg[nbins + 1 + b] : seg[b + 1];
   }
   return s;
 }
@@ -309,13 +317,13 @@ size_t fft_cols_lds_bytes(int n, int TX);
 
 constexpr int kSpreadBlock = 1024;  // threads per spread / gather workgroup (16 waves per CU)
 constexpr int kRecPerThread = 2;    // gather: brick records read per thread per round
-constexpr int kGatherCap = kRecPerThread * kSpreadBlock;  // gather: home atoms per round (LDS lists)
+constexpr int kGatherCap = kRecPerThread * kSpreadBlock;  // gather: home atoms per round
 constexpr int kGatherStageU = 24;   // gather: halo loads in flight per thread (one round up to 24 K points)
 constexpr int kBinBlock = 512;     // threads per binning workgroup (one atom each at the default chunk)
 constexpr int kPlaneBlock = 1024;   // threads per xy plane FFT workgroup
 constexpr int kPlaneGroupMax = 4;   // z planes per batched plane workgroup (at most)
 constexpr size_t kPlaneLdsBudget = 150 * 1024;  // the plane kernels run one workgroup per CU at most
-constexpr int kMaxChunks = 1024;   // binning workgroups per solve (chunk grows with n)
+constexpr int kMaxChunks = 512;    // binning workgroups per solve (chunk grows with n)
 static_assert(kMaxChunks <= kSpreadBlock, "brick_segments: one thread per chunk");
 constexpr int kMinChunk = 512;    // atoms per binning workgroup
 

@@ -18,7 +18,7 @@ pme_status set_error(pme_status s, const char* fmt, ...) __attribute__((format(p
 
 constexpr size_t kLdsBudget = 64 * 1024;        // FFT kernels (several workgroups per CU)
 constexpr size_t kBrickLdsBudget = 150 * 1024;  // spread/gather: one 1024-thread workgroup per CU (160 KB LDS)
-constexpr int kMaxBins = 16384;                 // LDS histogram of the binning kernel
+constexpr int kMaxBins = 8192;                  // LDS histogram of the binning kernel (two counters per brick)
 constexpr int kDefaultBrick = 16;               // grid points per brick edge (target)
 constexpr int kDefaultFftTile = 16;             // lines per y/z FFT tile (target)
 

@@ -63,7 +63,7 @@ pme_status plan_system(const pme_system_desc& in, int cap_atoms, const Tuning& t
   const size_t rs = 8 * (size_t)p->cp.chunk * p->cp.nchunk;
   const size_t sz[7] = {sizeof(float) * G,
                         sizeof(float2) * Gc,
-                        sizeof(int) * (size_t)p->cp.nchunk * (p->bp.nbins + 1),
+                        sizeof(int) * (size_t)p->cp.nchunk * seg_stride(p->bp.nbins),
                         sizeof(float4) * rs,
                         sizeof(int) * rs,
                         sizeof(double) * 7 * (size_t)p->nzblk,

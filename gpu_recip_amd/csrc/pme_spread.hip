@@ -70,19 +70,24 @@ __device__ __forceinline__ int next_brick(int b, int d, int nb) {
   return b >= nb ? b - nb : b;
 }
 
-// Chunk w of the counting sort (one kBinBlock-thread workgroup).  hist: nbins
-// ints of LDS.  With the default chunk of kBinBlock atoms each thread bins one
-// atom and keeps it and its bricks in registers between the counting and the
-// placing pass.
+// Chunk w of the counting sort (one kBinBlock-thread workgroup).  hist:
+// 2*nbins ints of LDS: per brick the count of home records, then the count of
+// all records; after the scan, the cursors of the home records and of the
+// other (spill) records.  Each brick's segment holds its home records first,
+// so that the gather reads exactly the atoms it owns.  With the default chunk
+// of kBinBlock atoms each thread bins one atom and keeps it and its bricks in
+// registers between the counting and the placing pass.
 __device__ __forceinline__ void bin_chunk_body(const float4* __restrict__ xyzq, int n, const BinArgs& a, int w,
                                                int* hist) {
   __shared__ int sh_scan[kBinBlock / 64];
   const int nbins = a.nbx * a.nby * a.nbz;
+  int* hcnt = hist;          // [nbins] home counts, then home cursors
+  int* tcnt = hist + nbins;  // [nbins] record counts, then spill cursors
   const int tid = threadIdx.x;
   const int a0 = w * a.chunk;
   const int a1 = min(a0 + a.chunk, n);
   const BrickDivs dv = brick_divs(a);
-  for (int i = tid; i < nbins; i += kBinBlock) hist[i] = 0;
+  for (int i = tid; i < 2 * nbins; i += kBinBlock) hist[i] = 0;
   // first atom of this thread, loaded while the histogram is cleared
   const int first = a0 + tid;
   float4 p0 = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -91,17 +96,25 @@ __device__ __forceinline__ void bin_chunk_body(const float4* __restrict__ xyzq, 
   __syncthreads();
   for (int atom = first; atom < a1; atom += kBinBlock) {
     const AtomBricks k = atom == first ? k0 : atom_bricks(xyzq[atom], a, dv);
+    atomicAdd(&hcnt[k.bx + a.nbx * (k.by + a.nby * k.bz)], 1);
     for (int dz = 0; dz <= k.sz; dz++)
       for (int dy = 0; dy <= k.sy; dy++)
         for (int dx = 0; dx <= k.sx; dx++) {
           const int b = next_brick(k.bx, dx, a.nbx) + a.nbx * (next_brick(k.by, dy, a.nby) + a.nby * next_brick(k.bz, dz, a.nbz));
-          atomicAdd(&hist[b], 1);
+          atomicAdd(&tcnt[b], 1);
         }
   }
   __syncthreads();
-  const int total = block_exclusive_scan<kBinBlock>(hist, nbins, sh_scan);
-  int* seg = a.seg_off + (size_t)w * (nbins + 1);
-  for (int i = tid; i < nbins; i += kBinBlock) seg[i] = hist[i];
+  const int total = block_exclusive_scan<kBinBlock>(tcnt, nbins, sh_scan);
+  int* seg = a.seg_off + (size_t)w * seg_stride(nbins);
+  int* hend = seg + nbins + 1;
+  for (int i = tid; i < nbins; i += kBinBlock) {
+    const int start = tcnt[i], nh = hcnt[i];
+    seg[i] = start;
+    hend[i] = start + nh;
+    hcnt[i] = start;       // home cursor
+    tcnt[i] = start + nh;  // spill cursor
+  }
   if (tid == 0) seg[nbins] = total;
   __syncthreads();
   float4* rp = a.rec_pos + (size_t)w * a.rec_stride;
@@ -113,9 +126,13 @@ __device__ __forceinline__ void bin_chunk_body(const float4* __restrict__ xyzq, 
       for (int dy = 0; dy <= k.sy; dy++)
         for (int dx = 0; dx <= k.sx; dx++) {
           const int b = next_brick(k.bx, dx, a.nbx) + a.nbx * (next_brick(k.by, dy, a.nby) + a.nby * next_brick(k.bz, dz, a.nbz));
-          const int pos = atomicAdd(&hist[b], 1);
-          rp[pos] = p;
-          ri[pos] = ((dx | dy | dz) == 0) ? (int)((unsigned)atom | 0x80000000u) : atom;
+          if ((dx | dy | dz) == 0) {
+            const int pos = atomicAdd(&hcnt[b], 1);
+            rp[pos] = p;
+            ri[pos] = atom;
+          } else {
+            rp[atomicAdd(&tcnt[b], 1)] = p;
+          }
         }
   }
 }
@@ -125,7 +142,7 @@ __global__ __launch_bounds__(kBinBlock) void k_bin_chunks(const float4* __restri
   bin_chunk_body(xyzq, n, a, blockIdx.x, hist);
 }
 
-size_t bin_lds_bytes(int nbins) { return sizeof(int) * (size_t)nbins; }
+size_t bin_lds_bytes(int nbins) { return 2 * sizeof(int) * (size_t)nbins; }
 
 hipError_t launch_bin_atoms(const float4* xyzq, int n, const BinArgs& a, hipStream_t s) {
   const int nbins = a.nbx * a.nby * a.nbz;
@@ -255,7 +272,7 @@ __device__ __forceinline__ void spread_brick_body(const BinArgs& a, int b, int w
   const bool wrapx = (a.nbx == 1), wrapy = (a.nby == 1), wrapz = (a.nbz == 1);
   const int plane = mul24(wym, bxs);
 
-  const SegBounds sb = brick_seg_load(a, b);  // in flight while the brick is cleared
+  const SegBounds sb = brick_seg_load<false>(a, b);  // in flight while the brick is cleared
   for (int i = tid; i < acc_n; i += kSpreadBlock) acc[i] = 0ull;
   const int total = brick_segments<kSpreadBlock>(a, sb, s_pre, s_base, sh_scan);  // ends with a barrier
 
