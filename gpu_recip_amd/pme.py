@@ -64,6 +64,25 @@ class CudaEnergyVirial:
         L.call("pme_ev_get_virial", self.h, v)
         return list(v)
 
+    def getSforce(self) -> list[float]:
+        """sforce[27*3] = sforce_dp + sforce_fp / 2^30 of the host copy."""
+        v = (C.c_double * 81)()
+        L.call("pme_ev_get_sforce", self.h, v)
+        return list(v)
+
+    def getEnergyPointer(self, name: str) -> int:
+        """Device address of energy term `name` (a double)."""
+        p = C.c_void_p()
+        L.call("pme_ev_energy_pointer", self.h, name.encode(), C.byref(p))
+        return int(p.value or 0)
+
+    def getVirialPointer(self) -> int:
+        """Device address of the Virial_t blob: sforce_dp[27][3] doubles,
+        sforce_fp[27][3] int64, virmat[9] doubles, then the energy terms."""
+        p = C.c_void_p()
+        L.call("pme_ev_virial_pointer", self.h, C.byref(p))
+        return int(p.value or 0)
+
 
 class CudaPMERecip:
     """The reciprocal-space solver (CudaPMERecip<AT,CT,CT2>).

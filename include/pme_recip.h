@@ -78,7 +78,9 @@ pme_status pme_ev_get_energy(pme_ev_t ev, const char* name, double* out); /* :24
 pme_status pme_ev_get_virial(pme_ev_t ev, double virmat[9]);              /* :261-265 (host copy) */
 pme_status pme_ev_get_sforce(pme_ev_t ev, double sforce[81]);             /* :270-278 */
 pme_status pme_ev_energy_pointer(pme_ev_t ev, const char* name, double** dptr); /* :231-234 */
-pme_status pme_ev_virial_pointer(pme_ev_t ev, double** dptr);             /* :239-242 */
+/* *dptr = start of the device Virial_t blob (sforce_dp[27][3] first; virmat at
+ * double 162), as getVirialPointer returns (Virial_t*)d_buffer, :239-242 */
+pme_status pme_ev_virial_pointer(pme_ev_t ev, double** dptr);
 
 /* ---------------- the reciprocal solver: CudaPMERecip ---------------- */
 

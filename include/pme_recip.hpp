@@ -44,6 +44,14 @@ inline void check(pme_status s, const char* where) {
 // FFTtype, CudaPMERecip.h:30
 enum FFTtype { COLUMN = PME_FFT_COLUMN, SLAB = PME_FFT_SLAB, BOX = PME_FFT_BOX };
 
+// Layout of the device energy/virial blob, CudaEnergyVirial.h:13-18 (only
+// used to compute addresses); the energy terms follow virmat.
+struct Virial_t {
+  double sforce_dp[27][3];
+  long long int sforce_fp[27][3];
+  double virmat[9];
+};
+
 // CudaEnergyVirial: named FP64 energy terms + 3x3 virial in one device blob
 class CudaEnergyVirial {
  public:
@@ -63,10 +71,10 @@ class CudaEnergyVirial {
     pme::check(pme_ev_energy_pointer(h_, name.c_str(), &p), "getEnergyPointer");
     return p;
   }
-  double* getVirialPointer() {
+  Virial_t* getVirialPointer() {
     double* p = nullptr;
     pme::check(pme_ev_virial_pointer(h_, &p), "getVirialPointer");
-    return p;
+    return reinterpret_cast<Virial_t*>(p);
   }
   double getEnergy(const char* name) {
     double e = 0.0;

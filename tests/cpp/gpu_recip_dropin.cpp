@@ -41,8 +41,14 @@ static std::vector<T> load(const char* path) {
 // Force<T>::realloc stride (Force.h:63-69): aligned to 256-byte boundaries
 static int force_stride(int n) { return ((((n - 1 + 32) * (int)sizeof(float)) - 1) / 256 + 1) * 256 / (int)sizeof(float); }
 
-static bool run(int order, const std::vector<float>& h_xyzq, const std::vector<double>& fref, double energy_comp,
-                const double* virial_comp, double tol_e, double tol_v, double tol_f) {
+// AT: the charge-grid accumulation type of the instantiation (float as in
+// gpu_recip.cu; int as CudaDomdecRecip.h:12, the production one; long long);
+// fft_type: BOX as every reference caller, or COLUMN / SLAB.
+template <typename AT>
+static bool run(int order, FFTtype fft_type, const char* what, const std::vector<float>& h_xyzq,
+                const std::vector<double>& fref, double energy_comp, const double* virial_comp, double tol_e,
+                double tol_v, double tol_f) {
+  std::printf("CudaPMERecip<%s, float, float2>, fft type %d, order %d\n", what, (int)fft_type, order);
   // Settings for the data, gpu_recip.cu:71-80
   const double boxx = 62.23, boxy = 62.23, boxz = 62.23, kappa = 0.320;
   const int nfftx = 64, nffty = 64, nfftz = 64;
@@ -62,8 +68,8 @@ static bool run(int order, const std::vector<float>& h_xyzq, const std::vector<d
   bool ok = true;
   {
     CudaEnergyVirial energyVirial;
-    CudaPMERecip<float, float, float2> PMErecip(nfftx, nffty, nfftz, order, BOX, 1, 0, energyVirial, "recip",
-                                                "self");
+    CudaPMERecip<AT, float, float2> PMErecip(nfftx, nffty, nfftz, order, fft_type, 1, 0, energyVirial, "recip",
+                                             "self");
     // warm up with energy and virial (gpu_recip.cu:118-138)
     energyVirial.clear();
     PMErecip.spread_charge(xyzq, ncoord, recip);
@@ -149,8 +155,12 @@ int main(int argc, char** argv) {
                         -0.1026251692E+00, -0.6175456643E+01, -0.1026251692E+00, -0.5606459148E+03};
   bool ok = true;
   try {
-    ok = run(4, xyzq, f4, 0.5939730270E+03, v4, 2.0e-3, 4.0e-3, 1.0e-3) && ok;
-    ok = run(6, xyzq, f6, 0.5951144996E+03, v6, 1.3e-3, 2.0e-3, 3.6e-4) && ok;
+    ok = run<float>(4, BOX, "float", xyzq, f4, 0.5939730270E+03, v4, 2.0e-3, 4.0e-3, 1.0e-3) && ok;
+    ok = run<float>(6, BOX, "float", xyzq, f6, 0.5951144996E+03, v6, 1.3e-3, 2.0e-3, 3.6e-4) && ok;
+    ok = run<int>(4, BOX, "int", xyzq, f4, 0.5939730270E+03, v4, 2.0e-3, 4.0e-3, 1.0e-3) && ok;
+    ok = run<int>(6, COLUMN, "int", xyzq, f6, 0.5951144996E+03, v6, 1.3e-3, 2.0e-3, 3.6e-4) && ok;
+    ok = run<long long>(4, SLAB, "long long", xyzq, f4, 0.5939730270E+03, v4, 2.0e-3, 4.0e-3, 1.0e-3) && ok;
+    ok = run<long long>(6, BOX, "long long", xyzq, f6, 0.5951144996E+03, v6, 1.3e-3, 2.0e-3, 3.6e-4) && ok;
   } catch (const pme::Error& e) {
     std::printf("pme error: %s\n", e.what());
     return 1;

@@ -1,5 +1,8 @@
 """The C++ drop-in (include/pme_recip.hpp) running the reference's gpu_recip
-test4/test6 sequence with its tolerances (gpu_recip.cu:68-522).  GPU only."""
+test4/test6 sequence with its tolerances (gpu_recip.cu:68-522) through
+CudaPMERecip<float|int|long long, float, float2> (CudaPMERecip.cu:2584-2600;
+<int, float, float2> is the production instantiation, CudaDomdecRecip.h:12).
+GPU only."""
 import os
 import subprocess
 
@@ -26,4 +29,7 @@ def test_cpp_dropin_reference_sequence(tmp_path):
     print(r.stdout, r.stderr, flush=True)
     assert r.returncode == 0
     assert "ALL OK" in r.stdout
-    assert r.stdout.count(" OK (tolerance") == 6
+    # three checks (energy, virial, forces) for each of the six instantiations
+    # (AT = float, int, long long; BOX, COLUMN and SLAB handles; orders 4 and 6)
+    assert r.stdout.count(" OK (tolerance") == 18
+    assert "FAILED" not in r.stdout

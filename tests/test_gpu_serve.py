@@ -230,3 +230,42 @@ def test_direct_recip_header_requests():
     assert ei.value.status == L.PME_ERR_STOPPED
     assert srv.stats()["submitted"] == 4
     srv.close()
+
+
+def test_full_size_burst_with_largest_requests():
+    """The serving workload's largest requests at full size: a 360,000-atom
+    order-6 system (cubic 153 A box, 160^3 grid) and a follow-up step of it,
+    served in one burst with smaller requests.  Each result equals the plain
+    single-system path bit for bit, and the large one is checked against the
+    FP64 oracle (relative-RMS force error reported)."""
+    torch, dev = torch_dev()
+    natom = 360000
+    box = (natom / 0.1) ** (1.0 / 3.0)
+    nfft = serving.systems.smooth235_at_least(box / 1.0)
+    assert nfft >= 150
+    big = serving.ServeRequest(0.0, 900, natom, box, nfft, 6, 0.34, False)
+    follow = serving.ServeRequest(0.0, 900, natom, box, nfft, 6, 0.34, True, base=0)
+    small = _small_schedule(n=6, nmax=60000, seed=21)
+    for i, r in enumerate(small):
+        r.key += 1000
+        if r.followup:
+            r.base += 2
+    reqs = [big, follow] + small
+    serving.materialise(reqs, dev)
+    say(f"  burst: {len(reqs)} requests, largest {natom} atoms, grid {nfft}^3, order 6")
+    srv = PmeServer(16 << 30, nlanes=2)
+    serving.run_burst(srv, reqs)
+    st = srv.stats()
+    assert st["completed"] == len(reqs)
+    srv.close()
+    for r in reqs:
+        xyzq = r.xyzq.cpu().numpy()
+        one = run_solve((r.nfft,) * 3, r.order, r.kappa, (r.box,) * 3, xyzq)
+        assert np.array_equal(_forces(r), one["force"]), r.key
+        assert rel_err(r.result[0], one["energy"]) < 1e-12
+    xyzq = big.xyzq.cpu().numpy()
+    o = O.solve(O.OracleParams(nfft, nfft, nfft, 6, 0.34, (box,) * 3), xyzq)
+    frms = rel_rms(_forces(big), o["force"])
+    erel = rel_err(big.result[0], o["energy"])
+    say(f"  {natom} atoms, {nfft}^3, order 6: force rel-RMS vs FP64 {frms:.2e}, energy rel {erel:.2e}")
+    assert frms < 1e-4 and erel < 1e-4
