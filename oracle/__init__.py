@@ -1,4 +1,5 @@
-"""FP64 host oracle for the PME reciprocal solve (test infrastructure only).
+"""FP64 host oracle for the PME reciprocal solve and the direct-space Ewald
+sum (test infrastructure only).
 
 Only tests/, ``__graft_entry__.smoke()`` and bench.py's ``cpu_baseline`` leg may
 import this package.  It is the checker; the product path in ``gpu_recip_amd``
@@ -26,4 +27,6 @@ from .oracle import (  # noqa: F401
     self_energy,
     solve,
     ewald_recip,
+    direct_ewald,
+    pair14,
 )

@@ -93,6 +93,10 @@ def lib():
     L.orc_solve.argtypes = [P, C.c_int, _fp, _dp, C.POINTER(C.c_double), _dp, C.c_void_p, C.c_void_p]
     L.orc_solve.restype = C.c_int
     L.orc_ewald_recip.argtypes = [C.c_int, _fp, _dp, C.c_double, _ip, C.POINTER(C.c_double), C.c_void_p, _dp, C.c_int]
+    L.orc_direct_ewald.argtypes = [C.c_int, _fp, _dp, C.c_double, C.c_double, C.c_int, C.c_void_p, _dp, _dp, C.c_int]
+    L.orc_direct_ewald.restype = C.c_double
+    L.orc_pair14.argtypes = [C.c_int, _fp, _dp, C.c_double, C.c_double, C.c_int, C.c_void_p, _dp, _dp]
+    L.orc_pair14.restype = C.c_double
     _lib = L
     return L
 
@@ -215,3 +219,28 @@ def ewald_recip(xyzq, box, kappa, kmax, want_force=True, nthreads=0):
                           np.asarray(kmax, np.int32), C.byref(e),
                           f.ctypes.data if want_force else None, v, nthreads)
     return {"energy": e.value, "force": f, "virial": v}
+
+
+def direct_ewald(xyzq, box, kappa, roff, excl=None, nthreads=0):
+    """Direct-space Ewald by brute force over all pairs and periodic images
+    within roff, excluded pairs skipped (orc_direct_ewald).  Returns
+    dict(energy, force (N,3) = +dE/dr, virial (9,) = sum_pairs r_a dE/dr_b)."""
+    x = _xyzq(xyzq)
+    f = np.zeros((len(x), 3))
+    v = np.zeros(9)
+    ex = None if excl is None or len(excl) == 0 else np.ascontiguousarray(excl, np.int32).reshape(-1, 2)
+    e = lib().orc_direct_ewald(len(x), x, np.asarray(box, np.float64), float(kappa), float(roff),
+                               0 if ex is None else len(ex), None if ex is None else ex.ctypes.data, f, v, nthreads)
+    return {"energy": e, "force": f, "virial": v}
+
+
+def pair14(xyzq, box, kappa, fac, pairs):
+    """1-4 list energy (orc_pair14): pairs (M,3) of (i, j, ishift); fac = e14fac
+    for interactions, 0 for exclusions.  Returns dict(energy, force, virial)."""
+    x = _xyzq(xyzq)
+    f = np.zeros((len(x), 3))
+    v = np.zeros(9)
+    pl = np.ascontiguousarray(pairs, np.int32).reshape(-1, 3)
+    e = lib().orc_pair14(len(x), x, np.asarray(box, np.float64), float(kappa), float(fac), len(pl),
+                         pl.ctypes.data if len(pl) else None, f, v)
+    return {"energy": e, "force": f, "virial": v}

@@ -593,3 +593,109 @@ void orc_ewald_recip(int natom, const float* xyzq, const double box[3], double k
     virial9[6] = -vir[2]; virial9[7] = -vir[4]; virial9[8] = -vir[5];
   }
 }
+
+/* ------------------------------------------------------------------------
+ * Direct-space Ewald (the real-space half; CudaPMEDirectForce).
+ * Pair term for the EWALD model, CudaDirectForceKernels.cu:352-390:
+ *   U(r) = ccelec qi qj (erfc(kappa r) + fac - 1) / r,
+ * fac = 1 for the non-bonded pairs (CudaDirectForce_util.h:184-251: only
+ * pairs within roff that the exclusion masks leave), e14fac for the 1-4
+ * interactions and 0 for the 1-4 exclusions (:506-660).  Gradient +dU/dr.
+ * ------------------------------------------------------------------------ */
+static void pair_ewald(double dx, double dy, double dz, double qq, double kappa, double fac, double* u,
+                       double g[3]) {
+  const double r2 = dx * dx + dy * dy + dz * dz;
+  const double r = sqrt(r2);
+  const double e = (erfc(kappa * r) + fac - 1.0) / r;
+  /* r dU/dr = -qq (2 kappa / sqrt(pi)) exp(-kappa^2 r^2) - qq e */
+  const double rdu = -qq * (2.0 / sqrt(PI)) * kappa * exp(-kappa * kappa * r2) - qq * e;
+  *u = qq * e;
+  const double s = rdu / r2;
+  g[0] = s * dx;
+  g[1] = s * dy;
+  g[2] = s * dz;
+}
+
+static int excluded(int i, int j, int nexcl, const int* excl) {
+  for (int k = 0; k < nexcl; k++) {
+    const int a = excl[2 * k], b = excl[2 * k + 1];
+    if ((a == i && b == j) || (a == j && b == i)) return 1;
+  }
+  return 0;
+}
+
+double orc_direct_ewald(int natom, const float* xyzq, const double box[3], double kappa, double roff, int nexcl,
+                        const int* excl, double* force, double* virial9, int nthreads) {
+  const int nt = pick_threads(nthreads);
+  const double roff2 = roff * roff;
+  double e_tot = 0.0, vir[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+#pragma omp parallel num_threads(nt)
+  {
+    double e_loc = 0.0, v_loc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    double* f_loc = force ? (double*)calloc((size_t)3 * natom, sizeof(double)) : NULL;
+#pragma omp for schedule(dynamic, 8)
+    for (int i = 0; i < natom; i++) {
+      for (int j = i + 1; j < natom; j++) {
+        if (nexcl && excluded(i, j, nexcl, excl)) continue;
+        const double qq = ORC_CCELEC * (double)xyzq[4 * i + 3] * (double)xyzq[4 * j + 3];
+        for (int s = 0; s < 27; s++) {
+          const double sx = (s % 3 - 1) * box[0], sy = ((s / 3) % 3 - 1) * box[1], sz = (s / 9 - 1) * box[2];
+          const double dx = (double)xyzq[4 * i] - (double)xyzq[4 * j] + sx;
+          const double dy = (double)xyzq[4 * i + 1] - (double)xyzq[4 * j + 1] + sy;
+          const double dz = (double)xyzq[4 * i + 2] - (double)xyzq[4 * j + 2] + sz;
+          if (dx * dx + dy * dy + dz * dz >= roff2) continue;
+          double u, g[3];
+          pair_ewald(dx, dy, dz, qq, kappa, 1.0, &u, g);
+          e_loc += u;
+          const double d[3] = {dx, dy, dz};
+          for (int a = 0; a < 3; a++)
+            for (int b = 0; b < 3; b++) v_loc[3 * a + b] += d[a] * g[b];
+          if (f_loc)
+            for (int a = 0; a < 3; a++) {
+              f_loc[3 * i + a] += g[a];
+              f_loc[3 * j + a] -= g[a];
+            }
+        }
+      }
+    }
+#pragma omp critical
+    {
+      e_tot += e_loc;
+      for (int k = 0; k < 9; k++) vir[k] += v_loc[k];
+      if (f_loc)
+        for (int k = 0; k < 3 * natom; k++) force[k] += f_loc[k];
+    }
+    free(f_loc);
+  }
+  if (virial9)
+    for (int k = 0; k < 9; k++) virial9[k] += vir[k];
+  return e_tot;
+}
+
+double orc_pair14(int natom, const float* xyzq, const double box[3], double kappa, double fac, int npair,
+                  const int* list, double* force, double* virial9) {
+  double e = 0.0;
+  for (int k = 0; k < npair; k++) {
+    const int i = list[3 * k], j = list[3 * k + 1], s = list[3 * k + 2];
+    if (i < 0 || j < 0 || i >= natom || j >= natom) continue;
+    /* calc_box_shift, gpu_utils.h:312-320 */
+    const double sx = (s % 3 - 1) * box[0], sy = ((s / 3) % 3 - 1) * box[1], sz = (s / 9 - 1) * box[2];
+    const double dx = (double)xyzq[4 * i] - (double)xyzq[4 * j] + sx;
+    const double dy = (double)xyzq[4 * i + 1] - (double)xyzq[4 * j + 1] + sy;
+    const double dz = (double)xyzq[4 * i + 2] - (double)xyzq[4 * j + 2] + sz;
+    const double qq = ORC_CCELEC * (double)xyzq[4 * i + 3] * (double)xyzq[4 * j + 3];
+    double u, g[3];
+    pair_ewald(dx, dy, dz, qq, kappa, fac, &u, g);
+    e += u;
+    const double d[3] = {dx, dy, dz};
+    if (virial9)
+      for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) virial9[3 * a + b] += d[a] * g[b];
+    if (force)
+      for (int a = 0; a < 3; a++) {
+        force[3 * i + a] += g[a];
+        force[3 * j + a] -= g[a];
+      }
+  }
+  return e;
+}

@@ -84,6 +84,23 @@ void orc_ewald_recip(int natom, const float* xyzq, const double box[3], double k
                      const int kmax[3], double* energy, double* force, double* virial9,
                      int nthreads);
 
+/* f3: direct-space Ewald, FP64 brute force over every pair i < j and every
+ * periodic image (27 shifts of an orthorhombic box) closer than roff, except
+ * the nexcl pairs excl[2k], excl[2k+1] (skipped in every image):
+ * sum ccelec qi qj erfc(kappa r)/r (CudaDirectForceKernels.cu:352-390,
+ * CudaDirectForce_util.h:184-251).  Gradient +dE/dr ADDED to force (AoS
+ * natom*3, may be NULL); pair virial sum_pairs r_a dE/dr_b ADDED to virial9
+ * (may be NULL).  Returns the energy. */
+double orc_direct_ewald(int natom, const float* xyzq, const double box[3], double kappa, double roff,
+                        int nexcl, const int* excl, double* force, double* virial9, int nthreads);
+/* 1-4 lists: entries list[3k..3k+2] = (i, j, ishift); the shift of ishift
+ * (gpu_utils.h:312-320) is added to xi - xj; pair energy
+ * ccelec qi qj (erfc(kappa r) + fac - 1)/r with fac = e14fac for the 1-4
+ * interactions and 0 for the exclusions (CudaDirectForceKernels.cu:506-660).
+ * Gradient and pair virial ADDED as above.  Returns the energy. */
+double orc_pair14(int natom, const float* xyzq, const double box[3], double kappa, double fac, int npair,
+                  const int* list, double* force, double* virial9);
+
 #ifdef __cplusplus
 }
 #endif
