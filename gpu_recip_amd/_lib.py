@@ -111,6 +111,7 @@ SIGNATURES = {
     "pme_recip_gather_force": [_vp, _vp, _i, _pd, _i, _i, _vp],
     "pme_recip_calc_self_energy": [_vp, _vp, _i, _d],
     "pme_recip_solve": [_vp, _vp, _i, _pd, _d, _i, _i, _i, _i, _vp],
+    "pme_recip_solve_noalloc": [_vp, _vp, _i, _pd, _d, _i, _i, _i, _i, _vp],
     "pme_recip_copy_grid": [_vp, _i, _vp, C.c_size_t],
     "pme_recip_set_timing": [_vp, _i],
     "pme_recip_get_timing": [_vp, C.POINTER(C.c_float)],

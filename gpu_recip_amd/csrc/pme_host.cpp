@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -754,6 +755,127 @@ extern "C" pme_status pme_recip_solve(pme_recip_t h, const void* d_xyzq, int nco
   PME_TRY(pme_recip_c2r_fft(h));
   PME_TRY(pme_recip_gather_force(h, d_xyzq, ncoord, recip, stride, force_type, d_force));
   return PME_OK;
+}
+
+// ------------------------------------------------- allocation-free solve --
+// pme_recip_solve with every device work buffer of the solve (real grid, half
+// spectrum, binning record store and segment table, energy/virial partials)
+// carved out of one file-scope device array instead of the handle's buffers,
+// for callers that cannot allocate (graph capture, serving loops).  Each GPU
+// holds its own copy of the array until the process ends; its device address
+// comes from hipGetSymbolAddress once per device (the array's name is only a
+// host-side stand-in).  Every call on a GPU shares that GPU's copy: calls that
+// overlap on one GPU (other streams, other threads) race, calls ordered on one
+// stream do not.  Nothing in it needs clearing between calls: every kernel of
+// the solve writes what it later reads.
+constexpr size_t kScratchBytes = 64ull << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+constexpr int kScratchDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[kScratchDevices] = {};
+
+static pme_status scratch_address(void** out) {
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kScratchDevices)
+    return fail(PME_ERR_UNSUPPORTED, "pme_recip_solve_noalloc: device %d (at most %d devices)", dev, kScratchDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(PME_ERR_DEVICE, "pme_recip_solve_noalloc: hipGetSymbolAddress: %s", hipGetErrorString(e));
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return PME_OK;
+}
+
+// Offsets of the work buffers of an n-atom solve in the scratch array
+// (256-byte aligned); returns the bytes used.
+struct ScratchLayout {
+  size_t grid, cgrid, seg_off, rec_pos, rec_idx, partial, total;
+  int chunk, nchunk;
+};
+static ScratchLayout scratch_layout(pme_recip_t h, int n) {
+  ScratchLayout L{};
+  const ChunkPlan cp = plan_chunks(n);
+  L.chunk = cp.chunk;
+  L.nchunk = cp.nchunk;
+  const size_t G = (size_t)h->nx * h->ny * h->nz;
+  const size_t Gc = (size_t)(h->nx / 2 + 1) * h->ny * h->nz;
+  const size_t rs = 8 * (size_t)cp.chunk * cp.nchunk;
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    const size_t at = off;
+    off += (bytes + 255) & ~(size_t)255;
+    return at;
+  };
+  L.grid = take(sizeof(float) * G);
+  L.cgrid = take(sizeof(float2) * Gc);
+  L.seg_off = take(sizeof(int) * (size_t)cp.nchunk * seg_stride(h->nbins));
+  L.rec_pos = take(sizeof(float4) * rs);
+  L.rec_idx = take(sizeof(int) * rs);
+  L.partial = take(sizeof(double) * 7 * (size_t)h->partial_cap);
+  L.total = off;
+  return L;
+}
+
+extern "C" pme_status pme_recip_solve_noalloc(pme_recip_t h, const void* d_xyzq, int ncoord, const double recip[9],
+                                              double kappa, int calc_energy, int calc_virial, int stride,
+                                              int force_type, void* d_force) {
+  if (!h || !recip) return fail(PME_ERR_INVALID_ARG, "pme_recip_solve_noalloc: null argument");
+  if (ncoord < 0) return fail(PME_ERR_INVALID_ARG, "pme_recip_solve_noalloc: ncoord %d", ncoord);
+  if (!is_ortho(recip))
+    return fail(PME_ERR_UNSUPPORTED, "pme_recip_solve_noalloc: only orthorhombic boxes are supported");
+  const ScratchLayout L = scratch_layout(h, ncoord);
+  if (L.total > kScratchBytes)
+    return fail(PME_ERR_BUDGET,
+                "pme_recip_solve_noalloc: %d atoms on a %dx%dx%d grid need %zu bytes of work buffers, "
+                "more than the %zu-byte static scratch",
+                ncoord, h->nx, h->ny, h->nz, L.total, kScratchBytes);
+  void* vbase = nullptr;
+  PME_TRY(scratch_address(&vbase));
+  char* base = static_cast<char*>(vbase);
+  // reductions still pending read the handle's own partials: fold them in first
+  PME_TRY(flush_pending(h, h->stream, false));
+  float* grid = h->grid;
+  float2* cgrid = h->cgrid;
+  int* seg_off = h->seg_off;
+  float4* rec_pos = h->rec_pos;
+  int* rec_idx = h->rec_idx;
+  double* partial = h->partial;
+  const int bins_for = h->bins_for, chunk = h->chunk, nchunk = h->nchunk, nchunk_cap = h->nchunk_cap;
+  h->grid = reinterpret_cast<float*>(base + L.grid);
+  h->cgrid = reinterpret_cast<float2*>(base + L.cgrid);
+  h->seg_off = reinterpret_cast<int*>(base + L.seg_off);
+  h->rec_pos = reinterpret_cast<float4*>(base + L.rec_pos);
+  h->rec_idx = reinterpret_cast<int*>(base + L.rec_idx);
+  h->partial = reinterpret_cast<double*>(base + L.partial);
+  // the record store is sized for exactly this call, so the binning never reallocates
+  h->chunk = L.chunk;
+  h->nchunk = h->nchunk_cap = L.nchunk;
+  h->bins_for = std::max(ncoord, 1);
+  h->bin_key_n = -1;
+  pme_status st = pme_recip_solve(h, d_xyzq, ncoord, recip, kappa, calc_energy, calc_virial, stride, force_type,
+                                  d_force);
+  // a reduction still pending reads the scratch partials: launch it now
+  if (st == PME_OK) st = flush_pending(h, h->stream, false);
+  h->grid = grid;
+  h->cgrid = cgrid;
+  h->seg_off = seg_off;
+  h->rec_pos = rec_pos;
+  h->rec_idx = rec_idx;
+  h->partial = partial;
+  h->bins_for = bins_for;
+  h->chunk = chunk;
+  h->nchunk = nchunk;
+  h->nchunk_cap = nchunk_cap;
+  // the potential and the records are in the scratch, not in the handle
+  h->state = ST_EMPTY;
+  h->bin_key_n = -1;
+  h->pend_solve = 0;
+  return st;
 }
 
 extern "C" pme_status pme_recip_copy_grid(pme_recip_t h, int which, void* d_dst, size_t bytes) {

@@ -146,6 +146,13 @@ class CudaPMERecip:
         L.call("pme_recip_solve", self.h, _ptr(xyzq), int(ncoord), L.dbl9(recip), float(kappa),
                int(bool(calc_energy)), int(bool(calc_virial)), int(stride), int(force_type), _ptr(force))
 
+    def solve_noalloc(self, xyzq, ncoord, recip, kappa, calc_energy, calc_virial, stride, force,
+                      force_type=L.FORCE_FLOAT) -> None:
+        """solve() with the work buffers in the library's static device
+        scratch (no allocation; PME_ERR_BUDGET beyond its size)."""
+        L.call("pme_recip_solve_noalloc", self.h, _ptr(xyzq), int(ncoord), L.dbl9(recip), float(kappa),
+               int(bool(calc_energy)), int(bool(calc_virial)), int(stride), int(force_type), _ptr(force))
+
     def copy_grid(self, which, dst) -> None:
         L.call("pme_recip_copy_grid", self.h, int(which), _ptr(dst), int(dst.numel() * dst.element_size()))
 

@@ -136,6 +136,22 @@ pme_status pme_recip_solve(pme_recip_t h, const void* d_xyzq, int ncoord,
                            const double recip[9], double kappa, int calc_energy,
                            int calc_virial, int stride, int force_type, void* d_force);
 
+/* pme_recip_solve that allocates nothing.  No counterpart in the reference:
+ * for callers that cannot call hipMalloc (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid allocation).  The work buffers of the
+ * solve (grid, spectrum, binning records, energy/virial partials) come from a
+ * static device array of 64 MiB per GPU instead of the handle; a solve whose
+ * buffers need more returns PME_ERR_BUDGET before any device work.  The
+ * handle supplies its plan, stream and energy/virial buffer (created
+ * beforehand), the caller inputs and outputs.  Results equal
+ * pme_recip_solve's.  Calls that overlap on one GPU (other streams or
+ * threads) race on the shared array; calls ordered on one stream do not.
+ * Afterwards the handle's stage state is reset (the next call starts with
+ * spread_charge or a solve). */
+pme_status pme_recip_solve_noalloc(pme_recip_t h, const void* d_xyzq, int ncoord,
+                                   const double recip[9], double kappa, int calc_energy,
+                                   int calc_virial, int stride, int force_type, void* d_force);
+
 /* Copy a grid view (device to device, on the handle's stream).
  * PME_GRID_REAL: charge grid after spread_charge, potential after c2r_fft,
  *   float[nfftz][nffty][nfftx].
