@@ -66,6 +66,12 @@ class ServerStats(C.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class Nlist(C.Structure):
+    """pme_nlist: the tile neighbour list of the direct-space term (device pointers)."""
+    _fields_ = [("n_ientry", C.c_int), ("ientry", C.c_void_p), ("n_tile", C.c_int), ("tile_indj", C.c_void_p),
+                ("tile_excl", C.c_void_p)]
+
+
 class RecipHeader(C.Structure):
     """pme_recip_header: the direct->recip request header (DomdecRecipComm.h:11-15)."""
     _fields_ = [("inv_boxx", C.c_double), ("inv_boxy", C.c_double), ("inv_boxz", C.c_double),
@@ -146,6 +152,16 @@ SIGNATURES = {
     "pme_server_stats_reset": [_vp],
     "pme_server_set_tuning": [_vp, _i, _i, _i],
     "pme_batch_get_timing": [_vp, C.POINTER(C.c_float)],
+    "pme_direct_create": [_vp, C.c_char_p, C.c_char_p, C.c_char_p, C.POINTER(_vp)],
+    "pme_direct_destroy": [_vp],
+    "pme_direct_setup": [_vp, _d, _d, _d, _d, _d, _d, _d, _i, _i],
+    "pme_direct_get_setup": [_vp] + [C.POINTER(C.c_float)] * 7 + [C.POINTER(_i)] * 2,
+    "pme_direct_set_box_size": [_vp, C.c_float, C.c_float, C.c_float],
+    "pme_direct_set_calc_vdw": [_vp, _i],
+    "pme_direct_set_calc_elec": [_vp, _i],
+    "pme_direct_set_14_list": [_vp, _i, _i, _vp, _vp, _vp],
+    "pme_direct_calc_14_force": [_vp, _vp, _i, _i, _i, _vp, _vp],
+    "pme_direct_calc_force": [_vp, _vp, C.POINTER(Nlist), _i, _i, _i, _vp, _vp],
 }
 _RESTYPE = {"pme_last_error": C.c_char_p, "pme_status_string": C.c_char_p}
 

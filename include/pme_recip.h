@@ -334,6 +334,65 @@ pme_status pme_server_set_tuning(pme_server_t s, int brick_target, int fft_tile,
 /* Host FP64 bsp_mod table (calc_prefac/dftmod, CudaPMERecip.cu:2481-2579). */
 pme_status pme_bsp_mod(int order, int n, double* out);
 
+/* ---------------- direct-space Ewald: CudaPMEDirectForce<long long, float> ----------------
+ * The real-space half of the Ewald sum over the reference's tile neighbour
+ * list (CudaPMEDirectForce.h:82-204; kernels CudaDirectForce_util.h:184-251,
+ * CudaDirectForceKernels.cu:352-760).  The reference's one instantiation has
+ * AT = long long: forces are ADDED to a device long long array as 2^40 fixed
+ * point, x at [i], y at [i + stride], z at [i + 2 stride], as the gradient
+ * +dE/dr.  Energies are ADDED to the three named terms of the energy/virial
+ * buffer, the shift forces of the virial to its sforce_dp[27][3].  Only the
+ * electrostatic models are built: vdw_model must be PME_VDW_NONE. */
+typedef struct pme_direct_s* pme_direct_t;
+/* models, CudaDirectForceTypes.h:62-65 (EWALD_LOOKUP is evaluated analytically, as EWALD) */
+enum { PME_VDW_NONE = 0, PME_ELEC_NONE = 0, PME_ELEC_EWALD = 101, PME_ELEC_EWALD_LOOKUP = 111 };
+/* ientry_t (CudaNeighborListBuild.h:24-29): the i tile of 32 atoms starting at
+ * iatomStart, shifted by box image ish = (sx+1) + 3(sy+1) + 9(sz+1), against the
+ * j tiles tile_indj[tileStart..tileEnd] (inclusive) */
+typedef struct { int iatomStart, ish, tileStart, tileEnd; } pme_ientry;
+/* tile_excl_t<32> (CudaNeighborListBuild.h:19-22): bit t of excl[l] excludes
+ * the pair (i = (l + t) mod 32, j = l) of the tile */
+typedef struct { unsigned int excl[32]; } pme_tile_excl;
+/* xx14list_t (Bonded_struct.h:122-124): pair (i, j), shift ishift added to xi - xj */
+typedef struct { int i, j, ishift; } pme_xx14;
+/* the tile list of CudaNeighborListBuild<32> (get_n_ientry, get_ientry,
+ * get_n_tile, get_tile_indj, get_tile_excl, :186-200), device pointers; every
+ * tile holds 32 atoms of the coordinate array (the reference pads to 32) */
+typedef struct {
+  int n_ientry;
+  const pme_ientry* ientry;
+  int n_tile;
+  const int* tile_indj;
+  const pme_tile_excl* tile_excl;
+} pme_nlist;
+
+/* CudaPMEDirectForce(energyVirial, nameVdw, nameElec, nameExcl), CudaPMEDirectForce.cu:62-118 */
+pme_status pme_direct_create(pme_ev_t ev, const char* name_vdw, const char* name_elec, const char* name_excl,
+                             pme_direct_t* out);
+pme_status pme_direct_destroy(pme_direct_t h);                              /* :123-134 */
+/* setup(boxx, boxy, boxz, kappa, roff, ron, e14fac, vdw_model, elec_model), :151-258 */
+pme_status pme_direct_setup(pme_direct_t h, double boxx, double boxy, double boxz, double kappa, double roff,
+                            double ron, double e14fac, int vdw_model, int elec_model);
+pme_status pme_direct_get_setup(pme_direct_t h, float* boxx, float* boxy, float* boxz, float* kappa, float* roff,
+                                float* ron, float* e14fac, int* vdw_model, int* elec_model); /* :263-276 */
+pme_status pme_direct_set_box_size(pme_direct_t h, float boxx, float boxy, float boxz); /* :293-298 */
+pme_status pme_direct_set_calc_vdw(pme_direct_t h, int calc_vdw);          /* :304-306 */
+pme_status pme_direct_set_calc_elec(pme_direct_t h, int calc_elec);        /* :312-314 */
+/* set_14_list(nin14list, nex14list, h_in14list, h_ex14list, stream), :575-593
+ * (host lists, copied before the call returns) */
+pme_status pme_direct_set_14_list(pme_direct_t h, int nin14, int nex14, const pme_xx14* h_in14,
+                                  const pme_xx14* h_ex14, pme_stream_t stream);
+/* calc_14_force(xyzq, calc_energy, calc_virial, stride, force, stream), :671-716:
+ * 1-4 interactions, qq (erfc + e14fac - 1)/r into the elec term, and 1-4
+ * exclusions, qq (erfc - 1)/r into the excl term */
+pme_status pme_direct_calc_14_force(pme_direct_t h, const void* d_xyzq, int calc_energy, int calc_virial,
+                                    int stride, long long* d_force, pme_stream_t stream);
+/* calc_force(xyzq, nlist, calc_energy, calc_virial, stride, force, stream),
+ * :719-770: qq erfc(kappa r)/r over the list's pairs closer than roff into
+ * the elec term */
+pme_status pme_direct_calc_force(pme_direct_t h, const void* d_xyzq, const pme_nlist* nlist, int calc_energy,
+                                 int calc_virial, int stride, long long* d_force, pme_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

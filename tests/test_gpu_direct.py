@@ -1,0 +1,160 @@
+"""GPU tests of the direct-space Ewald term (pme_direct_*, the
+CudaPMEDirectForce<long long, float> surface) against the FP64 oracle:
+non-bonded pairs on the reference's tile list with water exclusions, 1-4
+interactions and exclusions, energies per term, forces, the virial through
+the shift forces, and real + reciprocal + self = the exact Ewald energy."""
+import numpy as np
+import pytest
+
+import oracle as O
+from gpu_recip_amd import nlist, systems
+from gpu_recip_amd.direct import CudaPMEDirectForce, DeviceTileList
+from gpu_recip_amd.pme import CudaEnergyVirial, CudaPMERecip, force_stride
+from gpu_util import rel_err, rel_fro, rel_rms, say, to_dev, torch_dev
+
+pytestmark = pytest.mark.gpu
+
+E14FAC = 0.4
+
+
+def _system(ntrip, boxl, roff, seed, n14=24):
+    box = (boxl, boxl + 1.0, boxl + 2.0)
+    x = systems.triplets(ntrip, box, seed=seed)
+    x[:, :3] = np.mod(x[:, :3].astype(np.float64), box).astype(np.float32)  # the list's wrapped coordinates
+    n = len(x)
+    water = nlist.water_exclusions(n)
+    # a few 1-4 interactions between close atoms of different molecules
+    rng = np.random.default_rng(seed)
+    in14 = []
+    while len(in14) < n14:
+        i, j = (int(v) for v in rng.integers(0, n, 2))
+        if i // 3 == j // 3:
+            continue
+        d = x[i, :3].astype(np.float64) - x[j, :3]
+        d -= np.rint(d / box) * box
+        if np.linalg.norm(d) < 4.0 and (i, j) not in in14 and (j, i) not in in14:
+            in14.append((i, j))
+    in14 = np.array(in14, np.int64)
+    excl = np.concatenate([water, in14])
+    tl = nlist.build(x, box, roff, excl_pairs=excl)
+    return x, box, water, in14, excl, tl
+
+
+def _run(x, box, tl, kappa, roff, l_in14, l_ex14, ce=True, cv=True):
+    torch, dev = torch_dev()
+    inv = tl.sorted_index()
+    ev = CudaEnergyVirial()
+    dr = CudaPMEDirectForce(ev, "vdw", "elec", "excl")
+    dr.setup(*box, kappa, roff, roff - 1.0, E14FAC)
+    # the 1-4 lists in the list's (sorted) atom order
+    s_in = np.stack([inv[l_in14[:, 0]], inv[l_in14[:, 1]], l_in14[:, 2]], 1)
+    s_ex = np.stack([inv[l_ex14[:, 0]], inv[l_ex14[:, 1]], l_ex14[:, 2]], 1)
+    dr.set_14_list(s_in, s_ex)
+    dl = DeviceTileList(tl, dev)
+    stride = force_stride(tl.npad, 8)
+    f = torch.zeros(3 * stride, dtype=torch.int64, device=dev)
+    ev.clear()
+    dr.calc_force(dl.xyzq, dl, ce, cv, stride, f)
+    dr.calc_14_force(dl.xyzq, ce, cv, stride, f)
+    ev.copyToHost()
+    torch.cuda.synchronize()
+    fh = f.cpu().numpy()
+    fs = np.stack([fh[:tl.npad], fh[stride:stride + tl.npad], fh[2 * stride:2 * stride + tl.npad]], 1) / 2.0 ** 40
+    force = np.zeros((tl.natom, 3))
+    m = tl.perm >= 0
+    force[tl.perm[m]] = fs[m]
+    return {"force": force, "force_sorted": fs, "elec": ev.getEnergy("elec"), "excl": ev.getEnergy("excl"),
+            "vdw": ev.getEnergy("vdw"), "sforce": np.array(ev.getSforce()).reshape(27, 3), "raw": fh}
+
+
+@pytest.mark.parametrize("ntrip,boxl,roff,kappa", [(300, 21.0, 9.0, 0.34), (1000, 31.0, 12.0, 0.32)])
+def test_direct_against_oracle(ntrip, boxl, roff, kappa):
+    x, box, water, in14, excl, tl = _system(ntrip, boxl, roff, seed=100 + ntrip)
+    say(f"  {len(x)} atoms, {len(tl.ientry)} i entries, {len(tl.tile_indj)} j tiles, roff {roff}")
+    l_ex14 = nlist.ex14_list(x, box, water)
+    l_in14 = nlist.ex14_list(x, box, in14)
+    g = _run(x, box, tl, kappa, roff, l_in14, l_ex14)
+    od = O.direct_ewald(x, box, kappa, roff, excl=excl)
+    oi = O.pair14(x, box, kappa, E14FAC, l_in14)
+    oe = O.pair14(x, box, kappa, 0.0, l_ex14)
+    e_elec, e_excl = od["energy"] + oi["energy"], oe["energy"]
+    f_o = od["force"] + oi["force"] + oe["force"]
+    frms = rel_rms(g["force"], f_o)
+    say(f"  elec {g['elec']:.6f} / {e_elec:.6f}  excl {g['excl']:.6f} / {e_excl:.6f}  force rel-RMS {frms:.2e}")
+    assert g["vdw"] == 0.0
+    assert rel_err(g["elec"], e_elec) < 1e-5
+    assert rel_err(g["excl"], e_excl) < 1e-5
+    assert frms < 1e-5
+    # virial: sum_i x_i (x) g_i + sum_s shift_s (x) sforce_s = the pair virial
+    xs = tl.xyzq[:, :3].astype(np.float64)
+    w = xs.T @ g["force_sorted"]
+    shifts = nlist.SHIFTS * np.asarray(box)
+    w += shifts.T @ g["sforce"]
+    w_o = (od["virial"] + oi["virial"] + oe["virial"]).reshape(3, 3)
+    say(f"  virial rel {rel_fro(w, w_o):.2e}")
+    assert rel_fro(w, w_o) < 1e-4
+    assert not np.any(g["sforce"][13])
+
+
+def test_direct_is_deterministic_and_energy_flags_leave_forces():
+    x, box, water, in14, excl, tl = _system(300, 21.0, 9.0, seed=7)
+    l_ex14 = nlist.ex14_list(x, box, water)
+    l_in14 = nlist.ex14_list(x, box, in14)
+    a = _run(x, box, tl, 0.34, 9.0, l_in14, l_ex14)
+    b = _run(x, box, tl, 0.34, 9.0, l_in14, l_ex14)
+    c = _run(x, box, tl, 0.34, 9.0, l_in14, l_ex14, ce=False, cv=False)
+    assert np.array_equal(a["raw"], b["raw"])
+    assert a["elec"] == b["elec"] and a["excl"] == b["excl"]
+    assert np.array_equal(a["sforce"], b["sforce"])
+    assert np.array_equal(a["raw"], c["raw"])
+    assert c["elec"] == 0.0 and c["excl"] == 0.0 and not np.any(c["sforce"])
+
+
+def test_real_plus_recip_plus_self_is_the_ewald_energy():
+    """Eight charges: direct on the GPU + PME on the GPU + self equals the
+    exact Ewald energy (FP64 direct + brute-force reciprocal + self)."""
+    torch, dev = torch_dev()
+    box = (20.0, 20.0, 20.0)
+    rng = np.random.default_rng(3)
+    x = np.zeros((8, 4), np.float32)
+    x[:, :3] = rng.uniform(0.0, 20.0, (8, 3))
+    q = rng.choice([-0.75, -0.5, 0.5, 0.75], 8)
+    q[-1] -= q.sum()
+    x[:, 3] = q
+    kappa, roff = 0.45, 9.9
+    tl = nlist.build(x, box, roff)
+    g = _run(x, box, tl, kappa, roff, np.zeros((0, 3), np.int64), np.zeros((0, 3), np.int64))
+    ev = CudaEnergyVirial()
+    pme = CudaPMERecip(64, 64, 64, 8, ev)
+    xd = to_dev(x)
+    st = force_stride(8)
+    fr = torch.zeros(3 * st, dtype=torch.float32, device=dev)
+    recip = [1 / 20.0, 0, 0, 0, 1 / 20.0, 0, 0, 0, 1 / 20.0]
+    ev.clear()
+    pme.spread_charge(xd, 8, recip)
+    pme.r2c_fft()
+    pme.scalar_sum(recip, kappa, True, True)
+    pme.c2r_fft()
+    pme.gather_force(xd, 8, recip, st, fr)
+    pme.calc_self_energy(xd, 8, kappa)
+    ev.copyToHost()
+    torch.cuda.synchronize()
+    e_gpu = g["elec"] + ev.getEnergy("recip") + ev.getEnergy("self")
+    e_ref = (O.direct_ewald(x, box, kappa, roff)["energy"] + O.ewald_recip(x, box, kappa, (24, 24, 24))["energy"]
+             + O.self_energy(x, kappa))
+    say(f"  Ewald total {e_gpu:.8f} / {e_ref:.8f}")
+    assert rel_err(e_gpu, e_ref) < 1e-4
+
+
+def test_direct_setup_checks():
+    ev = CudaEnergyVirial()
+    dr = CudaPMEDirectForce(ev)
+    from gpu_recip_amd import _lib as L
+
+    with pytest.raises(L.PmeError):
+        dr.setup(30.0, 30.0, 30.0, 0.34, 9.0, 8.0, 1.0, 1, 101)  # a vdW model: not built
+    with pytest.raises(L.PmeError):
+        dr.setup(30.0, 30.0, 30.0, 0.34, 9.0, 8.0, 1.0, 0, 102)  # CSHIFT: not built
+    dr.setup(30.0, 31.0, 32.0, 0.34, 9.0, 8.0, 0.5, 0, 101)
+    s = dr.get_setup()
+    assert s[:3] == (30.0, 31.0, 32.0) and abs(s[6] - 0.5) < 1e-7 and s[7:] == (0, 101)
