@@ -27,6 +27,7 @@ from .oracle import (  # noqa: F401
     self_energy,
     solve,
     ewald_recip,
+    ewald_recip9,
     direct_ewald,
     pair14,
 )

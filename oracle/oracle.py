@@ -93,6 +93,7 @@ def lib():
     L.orc_solve.argtypes = [P, C.c_int, _fp, _dp, C.POINTER(C.c_double), _dp, C.c_void_p, C.c_void_p]
     L.orc_solve.restype = C.c_int
     L.orc_ewald_recip.argtypes = [C.c_int, _fp, _dp, C.c_double, _ip, C.POINTER(C.c_double), C.c_void_p, _dp, C.c_int]
+    L.orc_ewald_recip9.argtypes = [C.c_int, _fp, _dp, C.c_double, _ip, C.POINTER(C.c_double), C.c_void_p, _dp, C.c_int]
     L.orc_direct_ewald.argtypes = [C.c_int, _fp, _dp, C.c_double, C.c_double, C.c_int, C.c_void_p, _dp, _dp, C.c_int]
     L.orc_direct_ewald.restype = C.c_double
     L.orc_pair14.argtypes = [C.c_int, _fp, _dp, C.c_double, C.c_double, C.c_int, C.c_void_p, _dp, _dp]
@@ -244,3 +245,15 @@ def pair14(xyzq, box, kappa, fac, pairs):
     e = lib().orc_pair14(len(x), x, np.asarray(box, np.float64), float(kappa), float(fac), len(pl),
                          pl.ctypes.data if len(pl) else None, f, v)
     return {"energy": e, "force": f, "virial": v}
+
+
+def ewald_recip9(xyzq, recip9, kappa, kmax, want_force=True, nthreads=0):
+    """Brute-force Ewald reciprocal sum for any (triclinic) recip (orc_ewald_recip9)."""
+    x = _xyzq(xyzq)
+    e = C.c_double(0.0)
+    v = np.zeros(9)
+    f = np.zeros((len(x), 3)) if want_force else None
+    lib().orc_ewald_recip9(len(x), x, np.ascontiguousarray(recip9, np.float64).reshape(9), float(kappa),
+                           np.asarray(kmax, np.int32), C.byref(e), f.ctypes.data if want_force else None, v,
+                           nthreads)
+    return {"energy": e.value, "force": f, "virial": v}

@@ -70,17 +70,53 @@ void orc_index_f64(int natom, const float* xyzq, double r1, double r2, double r3
   }
 }
 
-/* Per-atom index + fractional part in the mode the params ask for. */
+/* Per-atom index + fractional part in the mode the params ask for.  The
+ * fractional coordinate of axis a is row a of recip times (x, y, z), so a
+ * triclinic box only adds the off-diagonal terms; for an orthorhombic recip
+ * they are exact zeros and the result is the reference's formula bit for bit.
+ * FP32 mode: the off-diagonal terms of the two other axes (in increasing axis
+ * order, the larger one innermost) are fused into the +2 term first, the
+ * diagonal term last: t = fmaf(x_a, R_aa, fmaf(x_b, R_ab, fmaf(x_c, R_ac, 2))). */
+static int is_tric(const double* r) {
+  return r[1] != 0.0 || r[2] != 0.0 || r[3] != 0.0 || r[5] != 0.0 || r[6] != 0.0 || r[7] != 0.0;
+}
+
+static void index_t_f32(float t, int n, int* i_out, float* w_out) {
+  float c = floorf(t + 0.5f) - 0.5f;
+  float fr = (float)n * (t - c);
+  int i = (int)fr;
+  *i_out = i;
+  *w_out = fr - (float)i;
+}
+
 static void atom_index(const orc_params* p, const float* q4, int ix[3], double w[3]) {
-  if (p->index_mode == ORC_INDEX_F32) {
-    float f;
-    index_one_f32(q4[0], (float)p->recip[0], p->nx, &ix[0], &f); w[0] = f;
-    index_one_f32(q4[1], (float)p->recip[4], p->ny, &ix[1], &f); w[1] = f;
-    index_one_f32(q4[2], (float)p->recip[8], p->nz, &ix[2], &f); w[2] = f;
-  } else {
-    index_one_f64(q4[0], p->recip[0], p->nx, &ix[0], &w[0]);
-    index_one_f64(q4[1], p->recip[4], p->ny, &ix[1], &w[1]);
-    index_one_f64(q4[2], p->recip[8], p->nz, &ix[2], &w[2]);
+  const int n[3] = {p->nx, p->ny, p->nz};
+  if (!is_tric(p->recip)) {
+    if (p->index_mode == ORC_INDEX_F32) {
+      float f;
+      for (int a = 0; a < 3; a++) {
+        index_one_f32(q4[a], (float)p->recip[4 * a], n[a], &ix[a], &f);
+        w[a] = f;
+      }
+    } else {
+      for (int a = 0; a < 3; a++) index_one_f64(q4[a], p->recip[4 * a], n[a], &ix[a], &w[a]);
+    }
+    return;
+  }
+  for (int a = 0; a < 3; a++) {
+    const int b = a == 0 ? 1 : 0, c = a == 2 ? 1 : 2;  /* the other two axes, b < c */
+    const double* R = &p->recip[3 * a];
+    if (p->index_mode == ORC_INDEX_F32) {
+      const float t = fmaf(q4[a], (float)R[a], fmaf(q4[b], (float)R[b], fmaf(q4[c], (float)R[c], 2.0f)));
+      float f;
+      index_t_f32(t, n[a], &ix[a], &f);
+      w[a] = f;
+    } else {
+      const double t = ((double)q4[a] * R[a] + (double)q4[b] * R[b] + (double)q4[c] * R[c]) + 2.0;
+      const double fr = (double)n[a] * (t - (floor(t + 0.5) - 0.5));
+      ix[a] = (int)fr;
+      w[a] = fr - (double)ix[a];
+    }
   }
 }
 
@@ -367,9 +403,12 @@ void orc_scalar_sum(const orc_params* p, double* cplx_io, double* energy, double
   const int n1 = p->nx, n2 = p->ny, n3 = p->nz;
   const int s1 = n1 / 2 + 1;
   const int nf1 = n1 / 2 + (n1 % 2), nf2 = n2 / 2 + (n2 % 2), nf3 = n3 / 2 + (n3 % 2);
-  const double r1 = p->recip[0], r2 = p->recip[4], r3 = p->recip[8];
+  const double* R = p->recip;  /* m-hat = m1 R[0][:] + m2 R[1][:] + m3 R[2][:] */
   const double fac = PI * PI / (p->kappa * p->kappa);
-  const double piv_inv = (r1 * r2 * r3) / PI;
+  /* 1/V = det(recip): r1 r2 r3 for an orthorhombic box (CudaPMERecip.cu:2024-2026) */
+  const double det = R[0] * (R[4] * R[8] - R[5] * R[7]) - R[1] * (R[3] * R[8] - R[5] * R[6]) +
+                     R[2] * (R[3] * R[7] - R[4] * R[6]);
+  const double piv_inv = is_tric(R) ? det / PI : (R[0] * R[4] * R[8]) / PI;
   double* b1 = (double*)malloc(sizeof(double) * n1);
   double* b2 = (double*)malloc(sizeof(double) * n2);
   double* b3 = (double*)malloc(sizeof(double) * n3);
@@ -389,7 +428,9 @@ void orc_scalar_sum(const orc_params* p, double* cplx_io, double* energy, double
         int m1 = k1 >= nf1 ? k1 - n1 : k1;
         int m2 = k2 >= nf2 ? k2 - n2 : k2;
         int m3 = k3 >= nf3 ? k3 - n3 : k3;
-        double mh1 = r1 * m1, mh2 = r2 * m2, mh3 = r3 * m3;
+        double mh1 = R[0] * m1 + R[3] * m2 + R[6] * m3;
+        double mh2 = R[1] * m1 + R[4] * m2 + R[7] * m3;
+        double mh3 = R[2] * m1 + R[5] * m2 + R[8] * m3;
         double msq = mh1 * mh1 + mh2 * mh2 + mh3 * mh3;
         double msq_inv = 1.0 / msq;
         double eterm = exp(-fac * msq) * piv_inv * b1[k1] * b2[k2] * b3[k3] * msq_inv;
@@ -404,7 +445,9 @@ void orc_scalar_sum(const orc_params* p, double* cplx_io, double* energy, double
           int m1s = k1s >= nf1 ? k1s - n1 : k1s;
           int m2s = k2s >= nf2 ? k2s - n2 : k2s;
           int m3s = k3s >= nf3 ? k3s - n3 : k3s;
-          double ms1 = r1 * m1s, ms2 = r2 * m2s, ms3 = r3 * m3s;
+          double ms1 = R[0] * m1s + R[3] * m2s + R[6] * m3s;
+          double ms2 = R[1] * m1s + R[4] * m2s + R[7] * m3s;
+          double ms3 = R[2] * m1s + R[5] * m2s + R[8] * m3s;
           double msqs = ms1 * ms1 + ms2 * ms2 + ms3 * ms3;
           double msqs_inv = 1.0 / msqs;
           double eterms = exp(-fac * msqs) * piv_inv * b1[k1s] * b2[k2s] * b3[k3s] * msqs_inv;
@@ -467,9 +510,19 @@ void orc_gather(const orc_params* p, int natom, const float* xyzq, const double*
       }
     }
     double q = (double)xyzq[4 * a + 3] * ORC_CCELEC;
-    force[3 * a + 0] = q * p->recip[0] * f1 * nx;
-    force[3 * a + 1] = q * p->recip[4] * f2 * ny;
-    force[3 * a + 2] = q * p->recip[8] * f3 * nz;
+    /* dE/dr_b = sum_a dE/ds_a R[a][b], dE/ds_a = q n_a f_a (CudaPMERecip.cu:1238-1241
+     * for the diagonal recip) */
+    const double* R = p->recip;
+    const double g1 = q * f1 * nx, g2 = q * f2 * ny, g3 = q * f3 * nz;
+    if (is_tric(R)) {
+      force[3 * a + 0] = g1 * R[0] + g2 * R[3] + g3 * R[6];
+      force[3 * a + 1] = g1 * R[1] + g2 * R[4] + g3 * R[7];
+      force[3 * a + 2] = g1 * R[2] + g2 * R[5] + g3 * R[8];
+    } else {
+      force[3 * a + 0] = q * R[0] * f1 * nx;
+      force[3 * a + 1] = q * R[4] * f2 * ny;
+      force[3 * a + 2] = q * R[8] * f3 * nz;
+    }
   }
 }
 
@@ -519,7 +572,16 @@ int orc_solve(const orc_params* p, int natom, const float* xyzq, double* force,
 void orc_ewald_recip(int natom, const float* xyzq, const double box[3], double kappa,
                      const int kmax[3], double* energy, double* force, double* virial9,
                      int nthreads) {
-  const double V = box[0] * box[1] * box[2];
+  const double r9[9] = {1.0 / box[0], 0, 0, 0, 1.0 / box[1], 0, 0, 0, 1.0 / box[2]};
+  orc_ewald_recip9(natom, xyzq, r9, kappa, kmax, energy, force, virial9, nthreads);
+}
+
+void orc_ewald_recip9(int natom, const float* xyzq, const double recip9[9], double kappa,
+                      const int kmax[3], double* energy, double* force, double* virial9,
+                      int nthreads) {
+  const double* R = recip9;
+  const double V = 1.0 / (R[0] * (R[4] * R[8] - R[5] * R[7]) - R[1] * (R[3] * R[8] - R[5] * R[6]) +
+                          R[2] * (R[3] * R[7] - R[4] * R[6]));
   const double fac = PI * PI / (kappa * kappa);
   const int K1 = 2 * kmax[0] + 1, K2 = 2 * kmax[1] + 1, K3 = 2 * kmax[2] + 1;
   const long long nk = (long long)K1 * K2 * K3;
@@ -538,7 +600,9 @@ void orc_ewald_recip(int natom, const float* xyzq, const double box[3], double k
       int m2 = (int)((t / K1) % K2) - kmax[1];
       int m3 = (int)(t / ((long long)K1 * K2)) - kmax[2];
       if (m1 == 0 && m2 == 0 && m3 == 0) continue;
-      double h1 = m1 / box[0], h2 = m2 / box[1], h3 = m3 / box[2];
+      double h1 = R[0] * m1 + R[3] * m2 + R[6] * m3;
+      double h2 = R[1] * m1 + R[4] * m2 + R[7] * m3;
+      double h3 = R[2] * m1 + R[5] * m2 + R[8] * m3;
       double msq = h1 * h1 + h2 * h2 + h3 * h3;
       double g = exp(-fac * msq) / msq;
       double sr = 0.0, si = 0.0;

@@ -35,7 +35,8 @@ typedef struct orc_params {
   int nx, ny, nz;       /* grid */
   int order;            /* B-spline order p (any p >= 3) */
   double kappa;         /* Ewald splitting parameter, 1/A */
-  double recip[9];      /* row-major reciprocal box; only the diagonal is used */
+  double recip[9];      /* row-major reciprocal box: fractional coordinate a = row a . (x, y, z);
+                           off-diagonal entries make the box triclinic */
   int index_mode;       /* ORC_INDEX_F64 or ORC_INDEX_F32 */
   int nyquist_mode;     /* ORC_NYQUIST_REFERENCE or ORC_NYQUIST_EXACT */
   int nthreads;         /* OpenMP threads (<= 0: library default) */
@@ -83,6 +84,12 @@ int orc_solve(const orc_params* p, int natom, const float* xyzq, double* force,
 void orc_ewald_recip(int natom, const float* xyzq, const double box[3], double kappa,
                      const int kmax[3], double* energy, double* force, double* virial9,
                      int nthreads);
+
+/* The same for any (triclinic) recip: reciprocal vectors m1 R[0][:] + m2 R[1][:] + m3 R[2][:],
+ * volume 1/det(recip). */
+void orc_ewald_recip9(int natom, const float* xyzq, const double recip9[9], double kappa,
+                      const int kmax[3], double* energy, double* force, double* virial9,
+                      int nthreads);
 
 /* f3: direct-space Ewald, FP64 brute force over every pair i < j and every
  * periodic image (27 shifts of an orthorhombic box) closer than roff, except

@@ -315,3 +315,81 @@ def test_nyquist_weight_difference_small():
     e_ex = O.solve(p, d["xyzq"])["energy"]
     assert e_ref >= e_ex
     assert abs(e_ref - e_ex) < 1e-8 * abs(e_ex)
+
+
+# ---------------------------------------------------------------- triclinic --
+def _tric_recip(H):
+    """recip (row-major) of a box whose columns are the box vectors: H^-1."""
+    return np.linalg.inv(np.asarray(H, np.float64)).reshape(9)
+
+
+def test_brute_ewald_is_lattice_basis_independent():
+    """A sheared basis of the same cubic lattice (c' = c + a) gives the same
+    brute-force reciprocal energy: the general m-hat of orc_ewald_recip9."""
+    rng = np.random.default_rng(21)
+    x = np.zeros((6, 4), np.float32)
+    x[:, :3] = rng.uniform(0, 15.0, (6, 3))
+    x[:, 3] = [0.5, -0.5, 0.25, -0.25, 0.75, -0.75]
+    cube = O.ewald_recip(x, (15.0, 15.0, 15.0), 0.35, (10, 10, 10))
+    sheared = O.ewald_recip9(x, _tric_recip([[15.0, 0.0, 15.0], [0.0, 15.0, 0.0], [0.0, 0.0, 15.0]]), 0.35,
+                             (20, 10, 10))
+    assert abs(sheared["energy"] - cube["energy"]) <= 1e-10 * abs(cube["energy"])
+    assert np.allclose(sheared["force"], cube["force"], rtol=0, atol=1e-9)
+    assert np.allclose(sheared["virial"], cube["virial"], rtol=0, atol=1e-6)  # k-space truncations differ
+
+
+def _tric_params(H, n, order, kappa, mode=O.INDEX_F64):
+    p = O.OracleParams(n[0], n[1], n[2], order, kappa, (1.0, 1.0, 1.0), index_mode=mode,
+                       nyquist_mode=O.NYQUIST_EXACT, recip=_tric_recip(H))
+    return p
+
+
+H_TRIC = [[24.0, 3.0, -2.0], [0.0, 22.0, 4.0], [0.0, 0.0, 20.0]]
+
+
+def test_triclinic_pme_matches_brute_ewald():
+    rng = np.random.default_rng(22)
+    x = np.zeros((10, 4), np.float32)
+    frac = rng.uniform(0, 1, (10, 3))
+    x[:, :3] = frac @ np.asarray(H_TRIC).T
+    q = rng.choice([-0.75, -0.5, 0.5, 0.75], 10)
+    q[-1] -= q.sum()
+    x[:, 3] = q
+    kappa = 0.3
+    o = O.solve(_tric_params(H_TRIC, (64, 64, 60), 8, kappa), x)
+    b = O.ewald_recip9(x, _tric_recip(H_TRIC), kappa, (14, 14, 14))
+    assert abs(o["energy"] - b["energy"]) <= 1e-6 * abs(b["energy"])
+    assert np.sqrt(((o["force"] - b["force"]) ** 2).sum() / (b["force"] ** 2).sum()) < 1e-5
+    assert np.linalg.norm(o["virial"] - b["virial"]) <= 1e-5 * np.linalg.norm(b["virial"])
+
+
+def test_triclinic_pme_forces_are_its_energy_gradient():
+    rng = np.random.default_rng(23)
+    x = np.zeros((6, 4), np.float32)
+    x[:, :3] = rng.uniform(0, 1, (6, 3)) @ np.asarray(H_TRIC).T
+    x[:, 3] = [0.5, -0.5, 0.25, -0.25, 0.75, -0.75]
+    p = _tric_params(H_TRIC, (32, 30, 30), 6, 0.35)
+    o = O.solve(p, x)
+    h = 1e-3
+    for i in range(6):
+        for a in range(3):
+            xp, xm = x.copy(), x.copy()
+            xp[i, a] += h
+            xm[i, a] -= h
+            dx = float(xp[i, a]) - float(xm[i, a])
+            fd = (O.solve(p, xp)["energy"] - O.solve(p, xm)["energy"]) / dx
+            assert abs(fd - o["force"][i, a]) <= 1e-3 * max(1.0, abs(o["force"][i, a]))
+
+
+def test_orthorhombic_recip_takes_the_reference_path():
+    """A diagonal recip given as a general matrix: the same indices and results."""
+    rng = np.random.default_rng(24)
+    x = np.zeros((50, 4), np.float32)
+    x[:, :3] = rng.uniform(-5, 35, (50, 3))
+    x[:, 3] = np.tile([0.5, -0.5], 25)
+    p = O.OracleParams(32, 32, 32, 4, 0.34, (30.0, 31.0, 32.0), index_mode=O.INDEX_F32)
+    o1 = O.solve(p, x)
+    p2 = O.OracleParams(32, 32, 32, 4, 0.34, (1.0, 1.0, 1.0), index_mode=O.INDEX_F32,
+                        recip=np.array([1 / 30.0, 0, 0, 0, 1 / 31.0, 0, 0, 0, 1 / 32.0]))
+    o2 = O.solve(p2, x)
+    assert np.array_equal(o1["force"], o2["force"]) and o1["energy"] == o2["energy"]
