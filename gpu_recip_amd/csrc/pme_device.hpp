@@ -14,12 +14,17 @@ constexpr float kForceScale = 1099511627776.0f;  // 2^40, gpu_utils.h:9
 // built with -fmad=true, which fuses x*recip + 2.0f into one FMA; nothing else
 // in the expression can fuse.  Contraction is switched off here so the
 // compiler cannot fuse anything beyond the explicit fmaf.
-__device__ __forceinline__ void grid_index(float x, float r, int n, int& i, float& w) {
+// base: 2 for a diagonal recip; a triclinic one brings the off-diagonal terms
+// of the row in it (atom_grid_index).
+__device__ __forceinline__ void grid_index_at(float x, float r, float base, int n, int& i, float& w) {
 #pragma clang fp contract(off)
-  const float t = __builtin_fmaf(x, r, 2.0f);
+  const float t = __builtin_fmaf(x, r, base);
   const float fr = (float)n * (t - (floorf(t + 0.5f) - 0.5f));
   i = (int)fr;
   w = fr - (float)i;
+}
+__device__ __forceinline__ void grid_index(float x, float r, int n, int& i, float& w) {
+  grid_index_at(x, r, 2.0f, n, i, w);
 }
 
 // Grid index brought into [0, n) before it addresses memory.  For finite

@@ -494,6 +494,61 @@ __device__ __forceinline__ float2 solve_point(const SolveArgs& s, float2 q, int 
   return cscale(q, eterm);
 }
 
+// eterm of a triclinic box at mapped wave numbers (m1, m2, m3), k3 = its z
+// index: m-hat = m1 R[0][:] + m2 R[1][:] + m3 R[2][:] (rows of recip).
+__device__ __forceinline__ float eterm_tric(const SolveArgs& s, int k1, int k2, int k3, int m1, int m2, int m3,
+                                            float* mh, float& msq_inv) {
+  const float f1 = (float)m1, f2 = (float)m2, f3 = (float)m3;
+  mh[0] = fmaf(s.rt[6], f3, fmaf(s.rt[3], f2, s.rt[0] * f1));
+  mh[1] = fmaf(s.rt[7], f3, fmaf(s.rt[4], f2, s.rt[1] * f1));
+  mh[2] = fmaf(s.rt[8], f3, fmaf(s.rt[5], f2, s.rt[2] * f1));
+  const float msq = fmaf(mh[2], mh[2], fmaf(mh[1], mh[1], mh[0] * mh[0]));
+  msq_inv = __builtin_amdgcn_rcpf(msq);
+  return __expf(-s.fac * msq) * (s.piv_inv * s.pf1[k1] * s.pf2[k2]) * s.pf3[k3] * msq_inv;
+}
+
+// energy and virial terms of one point (weight 1), the order of acc as in solve_point
+__device__ __forceinline__ void ev_terms(const SolveArgs& s, float eterm, float q2, const float* mh, float msq_inv,
+                                         double* acc) {
+  const float tmp1 = eterm * q2;
+  const float vterm = 2.0f * (s.fac + msq_inv);
+  const float tmp2 = tmp1 * vterm;
+  acc[0] += (double)tmp1;
+  acc[1] += (double)(tmp1 * (vterm * mh[0] * mh[0] - 1.0f));
+  acc[2] += (double)(tmp2 * mh[0] * mh[1]);
+  acc[3] += (double)(tmp2 * mh[0] * mh[2]);
+  acc[4] += (double)(tmp1 * (vterm * mh[1] * mh[1] - 1.0f));
+  acc[5] += (double)(tmp2 * mh[1] * mh[2]);
+  acc[6] += (double)(tmp1 * (vterm * mh[2] * mh[2] - 1.0f));
+}
+
+// solve_point for a triclinic box: the same weights (the point, plus its
+// mirror (n1-k1, -k2, -k3) for every k1 >= 1), but the mirror's m-hat is not
+// minus the point's on a Nyquist plane, so its eterm and virial terms are
+// evaluated from its own wave numbers.
+__device__ __forceinline__ float2 solve_point_tric(const SolveArgs& s, float2 q, int k1, int k2, int k3, double* acc) {
+  if ((k1 | k2 | k3) == 0) return make_float2(0.f, 0.f);
+  const int m1 = k1 >= s.nf1 ? k1 - s.n1 : k1;
+  const int m2 = k2 >= s.nf2 ? k2 - s.n2 : k2;
+  const int m3 = k3 >= s.nf3 ? k3 - s.n3 : k3;
+  float mh[3], msq_inv;
+  const float eterm = eterm_tric(s, k1, k2, k3, m1, m2, m3, mh, msq_inv);
+  if (s.calc_ev) {
+    const float q2 = fmaf(q.y, q.y, q.x * q.x);
+    ev_terms(s, eterm, q2, mh, msq_inv, acc);
+    if (k1 >= 1) {
+      const int k1s = s.n1 - k1, k2s = k2 ? s.n2 - k2 : 0, k3s = k3 ? s.n3 - k3 : 0;
+      const int m1s = k1s >= s.nf1 ? k1s - s.n1 : k1s;
+      const int m2s = k2s >= s.nf2 ? k2s - s.n2 : k2s;
+      const int m3s = k3s >= s.nf3 ? k3s - s.n3 : k3s;
+      float ms[3], msq_inv_s;
+      const float eterm_s = eterm_tric(s, k1s, k2s, k3s, m1s, m2s, m3s, ms, msq_inv_s);
+      ev_terms(s, eterm_s, q2, ms, msq_inv_s, acc);
+    }
+  }
+  return cscale(q, eterm);
+}
+
 // The influence function over the z lines of a column tile (in LDS, point j
 // of line l at res[j * TX + l]; lines l < nl are kx = kx0 + l at ky = k2).
 // Thread t owns line t mod TX and visits k3 = t / TX, t / TX + kBlock / TX,
@@ -842,7 +897,8 @@ __global__ __launch_bounds__(kBlock) void k_fft_cols(float2* __restrict__ cg, in
 }
 
 // Stand-alone scalar sum over the whole half spectrum (used when the spectrum
-// has been materialised, e.g. scalar_sum called twice).
+// has been materialised, e.g. scalar_sum called twice, and for triclinic
+// boxes).
 __global__ __launch_bounds__(kBlock) void k_scalar_sum(float2* __restrict__ cg, int hx, int total, SolveArgs sa) {
   __shared__ double sh_red[7 * (kBlock / 64)];
   double acc[7] = {0, 0, 0, 0, 0, 0, 0};
@@ -852,7 +908,7 @@ __global__ __launch_bounds__(kBlock) void k_scalar_sum(float2* __restrict__ cg, 
     const int t = idx / hx;
     const int k2 = t % sa.n2;
     const int k3 = t / sa.n2;
-    cg[idx] = solve_point(sa, cg[idx], k1, k2, k3, acc);
+    cg[idx] = sa.tric ? solve_point_tric(sa, cg[idx], k1, k2, k3, acc) : solve_point(sa, cg[idx], k1, k2, k3, acc);
   }
   if (sa.calc_ev) ev_partial(sa, acc, sh_red, blockIdx.x);
 }

@@ -77,9 +77,7 @@ __device__ __forceinline__ void gather_atom(const BinArgs& a, const float4& p, i
                                             void* force) {
   int ix, iy, iz;
   float fx, fy, fz;
-  grid_index(p.x, a.r1, a.nx, ix, fx);
-  grid_index(p.y, a.r2, a.ny, iy, fy);
-  grid_index(p.z, a.r3, a.nz, iz, fz);
+  atom_grid_index(a, p, ix, iy, iz, fx, fy, fz);
   ix = safe_index(ix, a.nx);
   iy = safe_index(iy, a.ny);
   iz = safe_index(iz, a.nz);
@@ -108,10 +106,17 @@ __device__ __forceinline__ void gather_atom(const BinArgs& a, const float4& p, i
                    },
                    f1, f2, f3);
   }
-  // CudaPMERecip.cu:1231-1241: q*ccelec, then q*recip*f*nfft in FP32
+  // CudaPMERecip.cu:1231-1241: q*ccelec, then q*recip*f*nfft in FP32; a
+  // triclinic box: dE/dr_b = sum_a (q f_a n_a) recip[a][b]
   const float q = p.w * ccelec_f;
-  store_force(ft, force, at, stride, q * a.r1 * f1 * (float)a.nx, q * a.r2 * f2 * (float)a.ny,
-              q * a.r3 * f3 * (float)a.nz);
+  if (!a.tric) {
+    store_force(ft, force, at, stride, q * a.r1 * f1 * (float)a.nx, q * a.r2 * f2 * (float)a.ny,
+                q * a.r3 * f3 * (float)a.nz);
+  } else {
+    const float g1 = q * f1 * (float)a.nx, g2 = q * f2 * (float)a.ny, g3 = q * f3 * (float)a.nz;
+    store_force(ft, force, at, stride, fmaf(g3, a.rzx, fmaf(g2, a.ryx, g1 * a.r1)),
+                fmaf(g3, a.rzy, fmaf(g2, a.r2, g1 * a.rxy)), fmaf(g3, a.r3, fmaf(g2, a.ryz, g1 * a.rxz)));
+  }
 }
 
 // Brick b of one grid (one kSpreadBlock-thread workgroup); smem: the

@@ -264,7 +264,7 @@ struct pme_recip_s {
   int state = ST_EMPTY;
   const void* bin_key_ptr = nullptr;
   int bin_key_n = -1;
-  float bin_key_r[3] = {0, 0, 0};
+  float bin_key_r[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};  // (float) recip of the last binning
 
   bool timing = false;
   hipEvent_t tev[9] = {};
@@ -381,6 +381,19 @@ static pme_status ensure_bins(pme_recip_t h, int n) {
   return PME_OK;
 }
 
+static bool is_ortho(const double* r) {
+  return r[1] == 0.0 && r[2] == 0.0 && r[3] == 0.0 && r[5] == 0.0 && r[6] == 0.0 && r[7] == 0.0;
+}
+
+// det(recip) = 1/V; a usable box has a finite, positive one
+static double recip_det(const double* r) {
+  return r[0] * (r[4] * r[8] - r[5] * r[7]) - r[1] * (r[3] * r[8] - r[5] * r[6]) + r[2] * (r[3] * r[7] - r[4] * r[6]);
+}
+static bool box_ok(const double* r) {
+  const double d = recip_det(r);
+  return std::isfinite(d) && d > 0.0;
+}
+
 static BinArgs bin_args(pme_recip_t h, const double* recip) {
   BinArgs a;
   a.nx = h->nx;
@@ -389,6 +402,13 @@ static BinArgs bin_args(pme_recip_t h, const double* recip) {
   a.r1 = (float)recip[0];
   a.r2 = (float)recip[4];
   a.r3 = (float)recip[8];
+  a.rxy = (float)recip[1];
+  a.rxz = (float)recip[2];
+  a.ryx = (float)recip[3];
+  a.ryz = (float)recip[5];
+  a.rzx = (float)recip[6];
+  a.rzy = (float)recip[7];
+  a.tric = is_ortho(recip) ? 0 : 1;
   a.nbx = h->nb[0];
   a.nby = h->nb[1];
   a.nbz = h->nb[2];
@@ -414,9 +434,6 @@ static AxisArgs axis_args(pme_recip_t h, int a) {
   return ax;
 }
 
-static bool is_ortho(const double* r) {
-  return r[1] == 0.0 && r[2] == 0.0 && r[3] == 0.0 && r[5] == 0.0 && r[6] == 0.0 && r[7] == 0.0;
-}
 
 static SolveArgs solve_args(pme_recip_t h, const double* recip, double kappa, bool calc_ev) {
   SolveArgs s;
@@ -429,9 +446,11 @@ static SolveArgs solve_args(pme_recip_t h, const double* recip, double kappa, bo
   s.r1 = (float)recip[0];
   s.r2 = (float)recip[4];
   s.r3 = (float)recip[8];
-  // CudaPMERecip.cu:2024-2026: inv_vol in FP64, then cast
-  const double inv_vol = recip[0] * recip[4] * recip[8];
+  // CudaPMERecip.cu:2024-2026: inv_vol in FP64, then cast; det(recip) for a triclinic box
+  s.tric = is_ortho(recip) ? 0 : 1;
+  const double inv_vol = s.tric ? recip_det(recip) : recip[0] * recip[4] * recip[8];
   s.piv_inv = (float)(inv_vol / kPi);
+  for (int i = 0; i < 9; i++) s.rt[i] = (float)recip[i];
   s.fac = (float)(kPi * kPi / (kappa * kappa));
   s.pf1 = h->prefac[0];
   s.pf2 = h->prefac[1];
@@ -602,15 +621,14 @@ static pme_status do_bin(pme_recip_t h, const void* xyzq, int n, const double* r
   HIP_TRY(launch_bin_atoms((const float4*)xyzq, n, a, h->stream));
   h->bin_key_ptr = xyzq;
   h->bin_key_n = n;
-  h->bin_key_r[0] = a.r1;
-  h->bin_key_r[1] = a.r2;
-  h->bin_key_r[2] = a.r3;
+  for (int i = 0; i < 9; i++) h->bin_key_r[i] = (float)recip[i];
   return PME_OK;
 }
 
 extern "C" pme_status pme_recip_spread_charge(pme_recip_t h, const void* d_xyzq, int ncoord, const double recip[9]) {
   if (!h || !recip || (ncoord > 0 && !d_xyzq)) return fail(PME_ERR_INVALID_ARG, "pme_recip_spread_charge: null argument");
   if (ncoord < 0) return fail(PME_ERR_INVALID_ARG, "pme_recip_spread_charge: ncoord %d", ncoord);
+  if (!box_ok(recip)) return fail(PME_ERR_INVALID_ARG, "pme_recip_spread_charge: recip has no positive determinant");
   mark(h, 0);
   PME_TRY(do_bin(h, d_xyzq, ncoord, recip));
   mark(h, 1);
@@ -650,8 +668,7 @@ static pme_status z_pass(pme_recip_t h, int mode, const SolveArgs& sa) {
 extern "C" pme_status pme_recip_scalar_sum(pme_recip_t h, const double recip[9], double kappa, int calc_energy,
                                            int calc_virial) {
   if (!h || !recip) return fail(PME_ERR_INVALID_ARG, "pme_recip_scalar_sum: null argument");
-  if (!is_ortho(recip))
-    return fail(PME_ERR_UNSUPPORTED, "pme_recip_scalar_sum: only orthorhombic boxes are supported");
+  if (!box_ok(recip)) return fail(PME_ERR_INVALID_ARG, "pme_recip_scalar_sum: recip has no positive determinant");
   if (!(kappa > 0.0)) return fail(PME_ERR_INVALID_ARG, "pme_recip_scalar_sum: kappa %g", kappa);
   if (h->state != ST_XY && h->state != ST_SPECTRUM && h->state != ST_SOLVED_ZINV)
     return fail(PME_ERR_STATE, "pme_recip_scalar_sum: no spectrum (call r2c_fft first)");
@@ -661,12 +678,14 @@ extern "C" pme_status pme_recip_scalar_sum(pme_recip_t h, const double recip[9],
   SolveArgs sa = solve_args(h, recip, kappa, ev);
   const int hx = h->nx / 2 + 1;
   mark(h, 4);
-  if (h->state == ST_XY) {
+  if (h->state == ST_XY && !sa.tric) {
     PME_TRY(z_pass(h, COL_FWD_SOLVE_INV, sa));
     if (ev) h->pend_solve = cols_blocks(hx, h->TXz, h->ny);
     h->state = ST_SOLVED_ZINV;
   } else {
-    if (h->state == ST_SOLVED_ZINV) PME_TRY(z_pass(h, COL_FWD, sa));
+    // the spectrum in full first (a triclinic box: the fused z pass holds
+    // the orthorhombic influence function only)
+    if (h->state == ST_SOLVED_ZINV || h->state == ST_XY) PME_TRY(z_pass(h, COL_FWD, sa));
     HIP_TRY(launch_scalar_sum(h->cgrid, hx, sa, h->stream));
     if (ev) h->pend_solve = scalar_sum_blocks(hx * h->ny * h->nz);
     h->state = ST_SPECTRUM;
@@ -713,16 +732,15 @@ extern "C" pme_status pme_recip_gather_force(pme_recip_t h, const void* d_xyzq, 
     return fail(PME_ERR_INVALID_ARG, "pme_recip_gather_force: force_type %d", force_type);
   if (force_type != PME_FORCE_FLOAT3 && stride < ncoord)
     return fail(PME_ERR_INVALID_ARG, "pme_recip_gather_force: stride %d < ncoord %d", stride, ncoord);
-  if (!is_ortho(recip))
-    return fail(PME_ERR_UNSUPPORTED, "pme_recip_gather_force: only orthorhombic boxes are supported");
+  if (!box_ok(recip)) return fail(PME_ERR_INVALID_ARG, "pme_recip_gather_force: recip has no positive determinant");
   if (h->state != ST_POTENTIAL) return fail(PME_ERR_STATE, "pme_recip_gather_force: no potential (call c2r_fft first)");
   if (ncoord == 0) return PME_OK;
-  const float r[3] = {(float)recip[0], (float)recip[4], (float)recip[8]};
+  bool same_box = true;
+  for (int i = 0; i < 9; i++) same_box = same_box && (float)recip[i] == h->bin_key_r[i];
   // the records of the last binning say which brick gathers which atom; the
   // gather itself reads the current coordinates, so stale records only cost
   // a detour through HBM.  Different atoms or box: bin again.
-  if (d_xyzq != h->bin_key_ptr || ncoord != h->bin_key_n || r[0] != h->bin_key_r[0] || r[1] != h->bin_key_r[1] ||
-      r[2] != h->bin_key_r[2]) {
+  if (d_xyzq != h->bin_key_ptr || ncoord != h->bin_key_n || !same_box) {
     PME_TRY(do_bin(h, d_xyzq, ncoord, recip));
   }
   mark(h, 7);
@@ -748,7 +766,7 @@ extern "C" pme_status pme_recip_solve(pme_recip_t h, const void* d_xyzq, int nco
                                       double kappa, int calc_energy, int calc_virial, int stride, int force_type,
                                       void* d_force) {
   if (!h || !recip) return fail(PME_ERR_INVALID_ARG, "pme_recip_solve: null argument");
-  if (!is_ortho(recip)) return fail(PME_ERR_UNSUPPORTED, "pme_recip_solve: only orthorhombic boxes are supported");
+  if (!box_ok(recip)) return fail(PME_ERR_INVALID_ARG, "pme_recip_solve: recip has no positive determinant");
   PME_TRY(pme_recip_spread_charge(h, d_xyzq, ncoord, recip));
   PME_TRY(pme_recip_r2c_fft(h));
   PME_TRY(pme_recip_scalar_sum(h, recip, kappa, calc_energy, calc_virial));
@@ -826,8 +844,8 @@ extern "C" pme_status pme_recip_solve_noalloc(pme_recip_t h, const void* d_xyzq,
                                               int force_type, void* d_force) {
   if (!h || !recip) return fail(PME_ERR_INVALID_ARG, "pme_recip_solve_noalloc: null argument");
   if (ncoord < 0) return fail(PME_ERR_INVALID_ARG, "pme_recip_solve_noalloc: ncoord %d", ncoord);
-  if (!is_ortho(recip))
-    return fail(PME_ERR_UNSUPPORTED, "pme_recip_solve_noalloc: only orthorhombic boxes are supported");
+  if (!box_ok(recip))
+    return fail(PME_ERR_INVALID_ARG, "pme_recip_solve_noalloc: recip has no positive determinant");
   const ScratchLayout L = scratch_layout(h, ncoord);
   if (L.total > kScratchBytes)
     return fail(PME_ERR_BUDGET,

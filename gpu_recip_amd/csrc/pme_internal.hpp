@@ -22,6 +22,8 @@ __host__ __device__ constexpr int seg_stride(int nbins) { return 2 * nbins + 1; 
 struct BinArgs {
   int nx, ny, nz;          // grid
   float r1, r2, r3;        // (float) recip diagonal, as the reference casts it
+  float rxy, rxz, ryx, ryz, rzx, rzy;  // (float) off-diagonal recip[0][1], [0][2], [1][0], [1][2], [2][0], [2][1]
+  int tric;                // 1: some off-diagonal entry is non-zero (triclinic box)
   int nbx, nby, nbz;       // bricks per axis
   int order;               // B-spline order (stencil width)
   int chunk;               // atoms per binning workgroup
@@ -31,6 +33,26 @@ struct BinArgs {
   float4* rec_pos;         // [nchunk * rec_stride] (x, y, z, q) copies
   int* rec_idx;            // [nchunk * rec_stride] atom index (home records)
 };
+
+// Grid indices and fractions of one atom on the three axes
+// (CudaPMERecip.cu:512-529 per axis).  The fractional coordinate of axis a is
+// row a of recip times (x, y, z): a triclinic recip first fuses the two
+// off-diagonal terms of the row into the +2 shift (the larger axis innermost)
+// and adds the diagonal term last, so a diagonal recip gives the reference's
+// single-FMA formula exactly.
+__device__ __forceinline__ void atom_grid_index(const BinArgs& a, const float4& p, int& ix, int& iy, int& iz,
+                                                float& fx, float& fy, float& fz) {
+#pragma clang fp contract(off)
+  float bx = 2.0f, by = 2.0f, bz = 2.0f;
+  if (a.tric) {
+    bx = __builtin_fmaf(p.y, a.rxy, __builtin_fmaf(p.z, a.rxz, 2.0f));
+    by = __builtin_fmaf(p.x, a.ryx, __builtin_fmaf(p.z, a.ryz, 2.0f));
+    bz = __builtin_fmaf(p.x, a.rzx, __builtin_fmaf(p.y, a.rzy, 2.0f));
+  }
+  grid_index_at(p.x, a.r1, bx, a.nx, ix, fx);
+  grid_index_at(p.y, a.r2, by, a.ny, iy, fy);
+  grid_index_at(p.z, a.r3, bz, a.nz, iz, fz);
+}
 
 // Deferred energy/virial reduction: per-workgroup partials written by the
 // scalar-sum and self-energy kernels are summed in fixed order by one
@@ -66,6 +88,8 @@ struct SolveArgs {
   const float* pf3;
   int calc_ev;             // accumulate energy and virial
   double* partial;         // [nblocks * 7] per-workgroup partial sums
+  float rt[9];             // (float) recip, row-major, for a triclinic box
+  int tric;                // 1: m-hat from the full recip (stand-alone scalar sum only)
 };
 
 enum ColMode { COL_FWD = 0, COL_INV = 1, COL_FWD_SOLVE_INV = 2, COL_FWD_INV = 3 };

@@ -48,10 +48,8 @@ struct AtomBricks {
 
 __device__ __forceinline__ AtomBricks atom_bricks(const float4& p, const BinArgs& a, const BrickDivs& dv) {
   int ix, iy, iz;
-  float w;
-  grid_index(p.x, a.r1, a.nx, ix, w);
-  grid_index(p.y, a.r2, a.ny, iy, w);
-  grid_index(p.z, a.r3, a.nz, iz, w);
+  float wx, wy, wz;
+  atom_grid_index(a, p, ix, iy, iz, wx, wy, wz);
   ix = safe_index(ix, a.nx);
   iy = safe_index(iy, a.ny);
   iz = safe_index(iz, a.nz);
@@ -173,9 +171,7 @@ __device__ __forceinline__ void spread_record(const BinArgs& a, float4 p, int x0
                                               unsigned long long* __restrict__ acc) {
   int ix, iy, iz;
   float fx, fy, fz;
-  grid_index(p.x, a.r1, a.nx, ix, fx);
-  grid_index(p.y, a.r2, a.ny, iy, fy);
-  grid_index(p.z, a.r3, a.nz, iz, fz);
+  atom_grid_index(a, p, ix, iy, iz, fx, fy, fz);
   float tx[P], ty[P], tz[P];
   bspline_theta<P>(fx, tx);
   bspline_theta<P>(fy, ty);
