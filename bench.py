@@ -170,6 +170,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-split", action="store_true", help="skip the split-precision (144^3) leg")
+    ap.add_argument("--no-direct", action="store_true", help="skip the direct-space Ewald leg")
     ap.add_argument("--serve-seconds", type=float, default=60.0,
                     help="seconds of open-loop arrivals (300 requests/s) for the served leg; 0 skips it")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -361,9 +362,83 @@ def main():
             split["modes"][pname] = {"solves_per_s": k4 / dt4, "ms_per_solve": dt4 / k4 * 1e3,
                                      "energy": ev4.getEnergy("recip"),
                                      "stage_ms": {STAGES[i]: float(ms4[i]) for i in range(len(STAGES))}}
+            if prec != L.PREC_FP32:
+                # matrix-core work issued: every radix-16 column of 16 complex
+                # points is 12 v_mfma_f32_32x32x8 per 32 columns (3 hi-lo
+                # products x 4 K steps), 2*32*32*8 flops each: 384 flops per
+                # complex point per pass; the y passes run one pass, the fused
+                # z pass two (forward and inverse)
+                gc4 = (s4.nfft[0] // 2 + 1) * s4.nfft[1] * s4.nfft[2]
+                passes = {"fft_fwd_y": 1, "fft_z_solve": 2, "fft_inv_y_or_xy": 1}
+                mf = {}
+                for st_name, npass in passes.items():
+                    fl = 384.0 * gc4 * npass
+                    t_s = float(ms4[STAGES.index(st_name)]) * 1e-3
+                    mf[st_name] = {"mfma_flops": fl, "tflops": fl / t_s / 1e12,
+                                   "utilisation_vs_2.5_pflops": fl / t_s / 2.5e15}
+                split["modes"][pname]["mfma"] = mf
         split["_forces"] = f4
         split["_stride"] = st4
         split["_system"] = s4
+
+    # ---------------- direct-space Ewald on the production system -------------
+    direct = None
+    if not args.no_direct:
+        from gpu_recip_amd import nlist
+        from gpu_recip_amd.direct import CudaPMEDirectForce, DeviceTileList
+
+        roff = 9.0
+        tl, ex14 = nlist.water_box(s2.xyzq, s2.box, roff)
+        checks, inter = nlist.count_pairs(tl, s2.box, roff)
+        evd = CudaEnergyVirial()
+        dr = CudaPMEDirectForce(evd, "vdw", "elec", "excl")
+        dr.setup(*s2.box, s2.kappa, roff, roff - 1.0, 1.0)
+        dr.set_14_list(np.zeros((0, 3), np.int32), ex14)
+        dl = DeviceTileList(tl, dev)
+        std = force_stride(tl.npad, 8)
+        fd = torch.zeros(3 * std, dtype=torch.int64, device=dev)
+
+        def dstep():
+            evd.clear()
+            dr.calc_force(dl.xyzq, dl, True, True, std, fd)
+            dr.calc_14_force(dl.xyzq, True, True, std, fd)
+            evd.copyToHost()
+            torch.cuda.synchronize()
+
+        for _ in range(max(3, args.warmup)):
+            dstep()
+        kd = max(10, args.steps)
+        t0 = time.perf_counter()
+        for _ in range(kd):
+            dstep()
+        dt_d = (time.perf_counter() - t0) / kd
+        # the pair kernel alone, HIP events on the stream it runs on (the null stream)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(kd):
+            dr.calc_force(dl.xyzq, dl, True, True, std, fd)
+        e1.record()
+        torch.cuda.synchronize()
+        ms_pair = e0.elapsed_time(e1) / kd
+        # FP32 work: 6 operations per pair check (3 differences, 3 multiply-adds
+        # for r^2), 28 per interacting pair (the Ewald term with erfc, exp and
+        # rsqrt counted as one operation each, force components and sums)
+        flops = 6.0 * checks + 28.0 * inter
+        direct = {
+            "workload": f"{s2.name}: direct-space Ewald, roff {roff} A, kappa {s2.kappa}, water exclusions "
+                        f"as 1-4 exclusion corrections; reference-format 32-atom tile list built on the host",
+            "natom": s2.natom, "tiles": tl.npad // 32, "i_entries": int(len(tl.ientry)),
+            "j_tiles": int(len(tl.tile_indj)), "pair_checks": checks, "interacting_pairs": inter,
+            "ms_per_call": dt_d * 1e3, "calls_per_s": 1.0 / dt_d,
+            "step": "calc_force + calc_14_force with energy and virial, energy/virial readback, sync",
+            "pair_kernel_ms": ms_pair, "interactions_per_s": inter / (ms_pair * 1e-3),
+            "roofline": {"bound": "valu", "kernel": "k_direct_force<true, true>",
+                         "achieved": flops / (ms_pair * 1e-3) / 1e12, "peak": 157.3, "unit": "TFLOP/s",
+                         "frac": flops / (ms_pair * 1e-3) / 1e12 / 157.3,
+                         "flops_per_launch": flops,
+                         "counting": "6 per pair check + 28 per interacting pair"},
+            "energy_elec": evd.getEnergy("elec"), "energy_excl": evd.getEnergy("excl"),
+        }
 
     # ---------------- served: open-loop Poisson arrivals, continuous batching --
     served = None
@@ -494,6 +569,7 @@ def main():
                      "algorithmic_bytes_per_launch": sb[kname], "avg_launch_ms": float(stage_ms[k])},
         "served": served,
         "split_precision": split,
+        "direct_space": direct,
         "accuracy": acc,
     }
     if not args.no_cpu_baseline:

@@ -385,18 +385,17 @@ __device__ __forceinline__ v16f32 mfma32x32x8(v4s16 a, v4s16 b, v16f32 c) {
   return __builtin_amdgcn_mfma_f32_32x32x8f16(__builtin_bit_cast(v4f16, a), __builtin_bit_cast(v4f16, b), c, 0, 0, 0);
 }
 
-// First Stockham stage with radix 16 (p = 1, no twiddles) as a matrix
-// product on the matrix cores: for every column c = i*nlines + l
-// (i < T = n/16), y[r] = sum_m W16^(S r m) x[m] with x[m] = src[(i + m T) ls + l]
-// and y[r] -> dst[(16 i + r) ls + l].  In real arithmetic this is the 32x32
-// matrix [[Fr, -Fi], [Fi, Fr]] times the 32-row column [Re x; Im x], done per
-// 32 columns by one wave with v_mfma_f32_32x32x8 (K = 8, four K steps).
-// Operand layouts of the 32x32x8 instruction: lane L = 32h + j holds
-// A[j][8q + 4h + t] and B[8q + 4h + t][j] (t < 4) and returns D[8(r/4) + 4h +
-// r%4][j] in accumulator register r.  hi-lo: D = Ah Bh + Ah Bl + Al Bh.
-template <int S, int PREC>
-__device__ __forceinline__ void mfma_dft16_stage(const float2* __restrict__ src, float2* __restrict__ dst, int n,
-                                                 int nlines, int ls, int tid, int nthr) {
+// 16-point DFTs of ncols columns as matrix products on the matrix cores: for
+// column c, y[r] = sum_m W16^(S r m) x[m], x[m] = load(ctx, m), each y[r]
+// handed to store(ctx, r, y[r]), ctx = prep(c) (the column's decomposition,
+// formed once).  In real arithmetic this is the 32x32 matrix
+// [[Fr, -Fi], [Fi, Fr]] times the 32-row column [Re x; Im x], done per 32
+// columns by one wave with v_mfma_f32_32x32x8 (K = 8, four K steps).  Operand
+// layouts of the 32x32x8 instruction: lane L = 32h + j holds A[j][8q + 4h + t]
+// and B[8q + 4h + t][j] (t < 4) and returns D[8(r/4) + 4h + r%4][j] in
+// accumulator register r.  hi-lo: D = Ah Bh + Ah Bl + Al Bh.
+template <int S, int PREC, typename Prep, typename Load, typename Store>
+__device__ __forceinline__ void mfma_dft16_cols(int ncols, Prep prep, Load load, Store store, int tid, int nthr) {
   const int lane = tid & 63;
   const int wave = tid >> 6;
   const int nwave = nthr >> 6;
@@ -405,8 +404,6 @@ __device__ __forceinline__ void mfma_dft16_stage(const float2* __restrict__ src,
   // opaque to the optimiser: keeps the operand set-up below inside this
   // stage instead of hoisted (and held in registers) across callers' loops
   asm volatile("" : "+v"(j));
-  const int T = n >> 4;
-  const int ncols = T * nlines;
   // A operand (the DFT matrix in real form) for the four K steps
   v4s16 ah[4], al[4];
 #pragma unroll
@@ -431,13 +428,12 @@ __device__ __forceinline__ void mfma_dft16_stage(const float2* __restrict__ src,
   for (int blk = wave; blk < nblk; blk += nwave) {
     const int c = blk * 32 + j;
     const bool valid = c < ncols;
-    const int i = valid ? sdiv(c, small_div(nlines)) : 0;
-    const int l = valid ? c - i * nlines : 0;
+    const auto ctx = prep(valid ? c : 0);
     float2 x[8];  // x[4h + t] and x[8 + 4h + t]
 #pragma unroll
     for (int t = 0; t < 4; t++) {
-      x[t] = valid ? src[(i + (4 * h + t) * T) * ls + l] : make_float2(0.f, 0.f);
-      x[4 + t] = valid ? src[(i + (8 + 4 * h + t) * T) * ls + l] : make_float2(0.f, 0.f);
+      x[t] = valid ? load(ctx, 4 * h + t) : make_float2(0.f, 0.f);
+      x[4 + t] = valid ? load(ctx, 8 + 4 * h + t) : make_float2(0.f, 0.f);
     }
     float scale = 1.0f, unscale = 1.0f;
     if (PREC == PREC_FP16X3) {
@@ -479,10 +475,52 @@ __device__ __forceinline__ void mfma_dft16_stage(const float2* __restrict__ src,
 #pragma unroll
       for (int r = 0; r < 8; r++) {
         const int row = 8 * (r >> 2) + 4 * h + (r & 3);  // 0..15: output point
-        dst[(16 * i + row) * ls + l] = make_float2(acc[r] * unscale, acc[r + 8] * unscale);
+        store(ctx, row, make_float2(acc[r] * unscale, acc[r + 8] * unscale));
       }
     }
   }
+}
+
+// First Stockham stage with radix 16 (p = 1, no twiddles) on the matrix
+// cores: column c = i*nlines + l (i < T = n/16) is x[m] = src[(i + m T) ls + l],
+// y[r] -> dst[(16 i + r) ls + l].
+template <int S, int PREC>
+__device__ __forceinline__ void mfma_dft16_stage(const float2* __restrict__ src, float2* __restrict__ dst, int n,
+                                                 int nlines, int ls, int tid, int nthr) {
+  const int T = n >> 4;
+  const SmallDiv dl = small_div(nlines);
+  mfma_dft16_cols<S, PREC>(
+      T * nlines,
+      [&](int c) {
+        const int i = sdiv(c, dl);
+        return make_int2(i, c - i * nlines);
+      },
+      [&](int2 ctx, int m) { return src[(ctx.x + m * T) * ls + ctx.y]; },
+      [&](int2 ctx, int r, float2 y) { dst[(16 * ctx.x + r) * ls + ctx.y] = y; }, tid, nthr);
+}
+
+// Pass A of a four-step transform n = 16 B on the matrix cores (the DFT_16 of
+// four_step_pass_a<16>): unit (line l, j2) takes the 16 points j2 + B j1,
+// multiplies output k1 by w_n^{S j2 k1} and writes it to (j2 + B k1).
+template <int S, int PREC>
+__device__ __forceinline__ void mfma_four_step_pass_a(const float2* __restrict__ src, float2* __restrict__ dst,
+                                                      const float2* __restrict__ tw, int B, int tstride, int nlines,
+                                                      int ls, int tid, int nthr) {
+  const SmallDiv dl = small_div(nlines);
+  const int Bls = mul24(B, ls);
+  mfma_dft16_cols<S, PREC>(
+      B * nlines,
+      [&](int c) {
+        const int j2 = sdiv(c, dl);
+        return make_int2(j2, c - mul24(j2, nlines));
+      },
+      [&](int2 ctx, int m) { return src[mul24(ctx.x, ls) + ctx.y + m * Bls]; },
+      [&](int2 ctx, int k1, float2 y) {
+        float2 w = tw[mul24(mul24(ctx.x, k1), tstride)];
+        if (S > 0) w.y = -w.y;
+        dst[mul24(ctx.x, ls) + ctx.y + k1 * Bls] = k1 == 0 ? y : cmul(y, w);
+      },
+      tid, nthr);
 }
 
 // ---- four-step transform of lines in LDS ----------------------------------
@@ -537,15 +575,24 @@ __device__ __forceinline__ void four_step_pass_b(const float2* __restrict__ src,
 }
 // codelet lengths of the four-step passes
 #define PME_F4_LENGTHS(X) X(2) X(3) X(4) X(5) X(6) X(8) X(9) X(10) X(12) X(16)
-template <int S>
+template <int S, bool SPLIT = false>
 __device__ __forceinline__ float2* fft_lines_four_step(float2* buf0, float2* buf1, const float2* tw,
-                                                       const RadixPlan& rp, int nlines, int ls, int tid, int nthr) {
-  switch (rp.f4a) {
+                                                       const RadixPlan& rp, int nlines, int ls, int tid, int nthr,
+                                                       int prec = PREC_FP32) {
+  if (SPLIT && prec != PREC_FP32 && rp.f4a == 16) {
+    // the 16-point DFTs as hi-lo split matrix products
+    if (prec == PREC_BF16X3)
+      mfma_four_step_pass_a<S, PREC_BF16X3>(buf0, buf1, tw, rp.f4b, rp.tstride, nlines, ls, tid, nthr);
+    else
+      mfma_four_step_pass_a<S, PREC_FP16X3>(buf0, buf1, tw, rp.f4b, rp.tstride, nlines, ls, tid, nthr);
+  } else {
+    switch (rp.f4a) {
 #define PME_F4A(L) \
   case L: four_step_pass_a<L, S>(buf0, buf1, tw, rp.f4b, rp.tstride, nlines, ls, tid, nthr); break;
-    PME_F4_LENGTHS(PME_F4A)
+      PME_F4_LENGTHS(PME_F4A)
 #undef PME_F4A
-    default: break;
+      default: break;
+    }
   }
   __syncthreads();
   switch (rp.f4b) {
@@ -574,8 +621,8 @@ __device__ __forceinline__ float2* fft_lines(float2* buf0, float2* buf1, const f
                              int nlines, int ls, int tid, int nthr, int prec = PREC_FP32) {
   float2* src = buf0;
   float2* dst = buf1;
-  if (FOUR_STEP && rp.f4a > 0 && !(SPLIT && prec != PREC_FP32))
-    return fft_lines_four_step<S>(buf0, buf1, tw, rp, nlines, ls, tid, nthr);
+  if (FOUR_STEP && rp.f4a > 0 && (!(SPLIT && prec != PREC_FP32) || rp.f4a == 16))
+    return fft_lines_four_step<S, SPLIT>(buf0, buf1, tw, rp, nlines, ls, tid, nthr, prec);
   int p = 1;
   int s0 = 0;
   if (SPLIT && prec != PREC_FP32 && rp.nrad > 0 && plan_radix(rp, 0) == 16) {

@@ -95,7 +95,15 @@ bool radix_plan_split(int n, int tstride, RadixPlan* rp) {
   }
   rp->n = n;
   rp->tstride = tstride;
-  rp->f4a = rp->f4b = 0;  // the matrix-core stage runs through the Stockham path
+  // a register four-step with the 16-point DFTs on the matrix cores when n/16
+  // is a codelet length; otherwise the matrix-core stage opens the Stockham
+  // stages
+  rp->f4a = rp->f4b = 0;
+  for (int b : {2, 3, 4, 5, 6, 8, 9, 10, 12, 16})
+    if (n == 16 * b) {
+      rp->f4a = 16;
+      rp->f4b = b;
+    }
   rp->nrad = rest.nrad + 1;
   rp->rad[0] = 16;
   for (int s = 0; s < rest.nrad; s++) rp->rad[s + 1] = rest.rad[s];

@@ -181,3 +181,39 @@ def ex14_list(xyzq, box, pairs) -> np.ndarray:
     s = np.rint((x[pairs[:, 1]] - x[pairs[:, 0]]) / box).astype(np.int64)
     ish = (s[:, 0] + 1) + 3 * (s[:, 1] + 1) + 9 * (s[:, 2] + 1)
     return np.stack([pairs[:, 0], pairs[:, 1], ish], 1).astype(np.int32)
+
+
+def count_pairs(tl: TileList, box, roff, chunk=2048):
+    """(pair checks, interacting pairs) of the list: every (i, j) of every
+    (i tile, shift, j tile) is a check; a pair is interacting when it is not
+    excluded and closer than roff (what the direct kernel evaluates)."""
+    box = np.asarray(box, np.float32)
+    x = tl.xyzq[:, :3].astype(np.float32)
+    ntp = len(tl.tile_indj)
+    counts = tl.ientry[:, 3] - tl.ientry[:, 2] + 1
+    ent = np.repeat(np.arange(len(tl.ientry)), counts)
+    i0 = tl.ientry[ent, 0]
+    sh = (SHIFTS[tl.ientry[ent, 1]] * box).astype(np.float32)
+    lane = np.arange(32)
+    rot = (lane[None, :] - lane[:, None]) % 32  # [j, i] -> bit (i - j) mod 32
+    inter = 0
+    for a in range(0, ntp, chunk):
+        b = min(a + chunk, ntp)
+        xi = x[i0[a:b, None] + lane[None, :]] + sh[a:b, None, :]       # (c, 32 i, 3)
+        xj = x[tl.tile_indj[a:b, None] + lane[None, :]]                 # (c, 32 j, 3)
+        d = xi[:, None, :, :] - xj[:, :, None, :]                      # (c, j, i, 3)
+        r2 = (d * d).sum(-1)
+        ex = (tl.tile_excl[a:b, :, None] >> rot[None, :, :].astype(np.uint32)) & 1  # (c, j, i)
+        inter += int(((ex == 0) & (r2 < roff * roff)).sum())
+    return ntp * 1024, inter
+
+
+def water_box(xyzq, box, roff):
+    """The direct-space inputs of a box of O, H, H triplets: the tile list with
+    the intramolecular pairs excluded, and those pairs as 1-4 exclusion
+    corrections in the list's sorted atom order (the reciprocal sum includes
+    them, so the direct term subtracts them)."""
+    water = water_exclusions(len(xyzq))
+    tl = build(xyzq, box, roff, excl_pairs=water)
+    ex14 = ex14_list(tl.xyzq, box, tl.sorted_index()[water])
+    return tl, ex14

@@ -5,6 +5,7 @@
     python -m gpu_recip_amd.workloads batched    [--solves 20] [--tuning brick,tile,wg_per_cu[,chains]]
     python -m gpu_recip_amd.workloads small      [--solves 20]
     python -m gpu_recip_amd.workloads served     [--solves 1000]
+    python -m gpu_recip_amd.workloads direct     [--solves 20]
 
 production: 96,000 atoms, 108x100x96, order 6 (plain five-call path)
 split:      250,002 atoms, 144^3, order 4 (plain path, chosen FFT precision)
@@ -14,6 +15,9 @@ small:      8,001 atoms, 48^3, order 4 (plain path)
 served:     the first --solves requests of the serving schedule (5k-400k
             atoms, grids 40-160, order 4 or 6), submitted at once to a
             2-lane server (saturation), after a short warm-up burst
+direct:     the direct-space Ewald term on the production system (roff 9 A,
+            water exclusions as 1-4 exclusion corrections), calc_force +
+            calc_14_force with energy and virial per solve
 Inputs are uploaded once; every solve ends with the energy/virial readback.
 """
 from __future__ import annotations
@@ -26,7 +30,7 @@ PRECS = {"fp32": 0, "bf16x3": 1, "fp16x3": 2}
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("workload", choices=["production", "split", "batched", "small", "served"])
+    ap.add_argument("workload", choices=["production", "split", "batched", "small", "served", "direct"])
     ap.add_argument("--solves", type=int, default=20)
     ap.add_argument("--precision", choices=sorted(PRECS), default="fp32")
     ap.add_argument("--tuning", default=None,
@@ -83,6 +87,31 @@ def main(argv=None):
             b.solve()
             b.results(ids)
         print(f"batched: {args.solves} x 64 solves in {time.perf_counter() - t0:.3f} s", flush=True)
+        return
+    if args.workload == "direct":
+        import numpy as np
+
+        from . import nlist
+        from .direct import CudaPMEDirectForce, DeviceTileList
+
+        s = systems.config_production()
+        tl, ex14 = nlist.water_box(s.xyzq, s.box, 9.0)
+        ev = CudaEnergyVirial()
+        dr = CudaPMEDirectForce(ev, "vdw", "elec", "excl")
+        dr.setup(*s.box, s.kappa, 9.0, 8.0, 1.0)
+        dr.set_14_list(np.zeros((0, 3), np.int32), ex14)
+        dl = DeviceTileList(tl, dev)
+        st = force_stride(tl.npad, 8)
+        f = torch.zeros(3 * st, dtype=torch.int64, device=dev)
+        t0 = time.perf_counter()
+        for _ in range(args.solves):
+            ev.clear()
+            dr.calc_force(dl.xyzq, dl, True, True, st, f)
+            dr.calc_14_force(dl.xyzq, True, True, st, f)
+            ev.copyToHost()
+            torch.cuda.synchronize()
+        print(f"direct ({len(tl.ientry)} i entries, {len(tl.tile_indj)} j tiles): {args.solves} calls in "
+              f"{time.perf_counter() - t0:.3f} s, elec {ev.getEnergy('elec'):.6f}", flush=True)
         return
     s = {"production": systems.config_production, "split": systems.config_split_precision,
          "small": systems.config_plain_small}[args.workload]()
