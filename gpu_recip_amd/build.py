@@ -73,16 +73,19 @@ def build(verbose: bool = False) -> str:
 
 DROPIN_SRC = os.path.join(ROOT, "tests", "cpp", "gpu_recip_dropin.cpp")
 DROPIN_BIN = os.path.join(ROOT, "tests", "cpp", "gpu_recip_dropin")
+DIRECT_DROPIN_SRC = os.path.join(ROOT, "tests", "cpp", "direct_dropin.cpp")
+DIRECT_DROPIN_BIN = os.path.join(ROOT, "tests", "cpp", "direct_dropin")
 
 
 def build_dropin_test() -> str:
-    """Compile the C++ drop-in test program (host code over include/pme_recip.hpp,
+    """Compile the C++ drop-in test programs (host code over include/pme_recip.hpp,
     linked against libpme_hip.so with an rpath to this package)."""
-    cmd = [HIPCC, "-std=c++17", "-O2", f"-I{INCLUDE}", DROPIN_SRC, "-o", DROPIN_BIN, f"-L{HERE}", "-lpme_hip",
-           f"-Wl,-rpath,{HERE}", "-Wl,-rpath,$ORIGIN/../../gpu_recip_amd"]
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        raise RuntimeError(f"drop-in test build failed:\n{r.stdout}\n{r.stderr}")
+    for src, out in ((DROPIN_SRC, DROPIN_BIN), (DIRECT_DROPIN_SRC, DIRECT_DROPIN_BIN)):
+        cmd = [HIPCC, "-std=c++17", "-O2", f"-I{INCLUDE}", src, "-o", out, f"-L{HERE}", "-lpme_hip",
+               f"-Wl,-rpath,{HERE}", "-Wl,-rpath,$ORIGIN/../../gpu_recip_amd"]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"drop-in test build failed ({os.path.basename(src)}):\n{r.stdout}\n{r.stderr}")
     return DROPIN_BIN
 
 

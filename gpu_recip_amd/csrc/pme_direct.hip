@@ -4,17 +4,29 @@
 // CudaPMEDirectForce<long long, float> (CudaDirectForce_util.h:184-251,
 // CudaDirectForceKernels.cu:352-400, 506-716).
 //
-// Non-bonded pairs: one wave64 per i-tile entry (32 i atoms, one periodic
-// shift); each half of the wave takes one of two consecutive j tiles, one j
-// atom per lane.  The 32 shifted i atoms sit in LDS and are visited in the
-// rotation i = (lane + t) mod 32, so the 32 lanes of a half touch 32
-// different i atoms at every step: their force sums are LDS float adds with
-// no two lanes on one word, in a fixed order (deterministic).  Pair forces
-// are summed in FP32 registers (j) and LDS (i) and added to the caller's
-// force array once per tile as 2^40 fixed point (integer adds: the result
-// does not depend on the order of the tiles).  Energies and the shift forces
-// of the virial are per-workgroup FP64 partials summed in fixed order by a
-// one-workgroup reduction into the energy/virial buffer.
+// Non-bonded pairs.  The j tiles of the list are split into equal
+// contiguous ranges, one per wave, so the waves carry equal work whatever the
+// spread of the entries' lengths (central-image entries hold tens of tiles,
+// the others a few); a small kernel first maps every j tile to its entry.
+// Each half of a wave walks its half of the range one tile per pass, one j
+// atom per lane, holding the i tile of the tile's entry (32 atoms, moved by
+// the entry's periodic shift on the j side) in LDS and flushing the i-force
+// sums whenever the entry changes.  A pass has two phases.  The check phase
+// tests the 32 x 64 pairs against the cutoff, each lane reading the i atoms
+// at LDS addresses common to its half (broadcast reads) and building a 32-bit
+// mask of its interacting i atoms; the tile's exclusion words, rotated into
+// the same bit order, clear the excluded pairs.  The dense phase packs the
+// wave's interacting pairs into an LDS queue (a wave prefix sum of the mask
+// counts gives each lane its slots) and evaluates them 64 at a time, so the
+// Ewald term runs only for pairs inside the cutoff (about one check in eight
+// for a 32-atom tile list).  Pair forces are rounded once to 2^40 fixed
+// point and summed with integer LDS atomics per i and per j atom; the sums
+// go to the caller's force array as integer atomics, the j sums after every
+// pass and the i sums when the entry changes, and the shift forces of the
+// virial are summed the same way: none of these depends on the order of the
+// pairs.  Energies are per-lane FP64 sums over a fixed set of pairs, summed
+// per workgroup in wave order; a parallel fixed-order reduction adds the
+// workgroups' partials to the energy/virial buffer.
 #include "pme_internal.hpp"
 
 namespace pme {
@@ -51,22 +63,46 @@ __device__ __forceinline__ void add_fixed(unsigned long long* force, int i, int 
   atomicAdd(&force[(size_t)i + 2 * (size_t)stride], (unsigned long long)(long long)rintf(kForceScale * fz));
 }
 
-__device__ __forceinline__ void lds_fadd(float* p, float v) {
-  __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+// A pair-force component as 2^40 fixed point, rounded once to the nearest
+// integer.  Below 2^11 in magnitude (every pair outside close contact) one
+// FMA does it (to_fixed_product: f * 2^40 is exact in FP64 and below 2^51);
+// larger f * 2^40 are integers already and convert exactly, held inside the
+// format's range of +-2^23.
+__device__ __forceinline__ unsigned long long pair_fixed(float f) {
+  if (fabsf(f) < 2048.0f) return to_fixed_product((double)f, 1099511627776.0);
+  const float c = fminf(fmaxf(f, -8388607.0f), 8388607.0f);
+  return (unsigned long long)(long long)((double)c * 1099511627776.0);
+}
+
+// Orders the LDS accesses of the lanes of one wave (all of them issue from
+// the same instruction stream, so a compiler barrier and the LDS counter wait
+// are enough; no workgroup barrier).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 // The workgroup's partials: elec, excl, then sforce[27][3] (the shift forces
 // of the virial, CudaEnergyVirial.h:14).  Each thread brings its energies and
-// one shift-force contribution (ish < 0: none); the shift forces are summed
-// per shift in thread order and the energies with block_sum, both in fixed
-// order.
+// one shift-force contribution (ish < 0: none); the energies are summed with
+// block_sum in fixed order, the shift forces per shift as 2^40 fixed point
+// with integer LDS atomics (exact, so the order does not matter).
 template <int NT>
-__device__ __forceinline__ void write_direct_partial(double elec, double excl, int ish, double sfx, double sfy,
-                                                     double sfz, bool energy, bool virial, double* out) {
+__device__ __forceinline__ void write_direct_partial(double elec, double excl, int ish, float sfx, float sfy,
+                                                     float sfz, bool energy, bool virial, double* out) {
   __shared__ double s_red[2 * (NT / 64)];
-  __shared__ double s_sf[NT][3];
-  __shared__ int s_ish[NT];
+  __shared__ unsigned long long s_sf[81];
   const int tid = threadIdx.x;
+  if (virial) {
+    for (int k = tid; k < 81; k += NT) s_sf[k] = 0ull;
+    __syncthreads();
+    if (ish >= 0) {
+      atomicAdd(&s_sf[3 * ish + 0], pair_fixed(sfx));
+      atomicAdd(&s_sf[3 * ish + 1], pair_fixed(sfy));
+      atomicAdd(&s_sf[3 * ish + 2], pair_fixed(sfz));
+    }
+  }
   if (energy) {
     double v[2] = {elec, excl};
     block_sum<2, NT>(v, s_red);
@@ -76,100 +112,193 @@ __device__ __forceinline__ void write_direct_partial(double elec, double excl, i
     }
   }
   if (virial) {
-    s_ish[tid] = ish;
-    s_sf[tid][0] = sfx;
-    s_sf[tid][1] = sfy;
-    s_sf[tid][2] = sfz;
     __syncthreads();
-    for (int k = tid; k < 81; k += NT) {
-      const int s = k / 3, c = k - 3 * s;
-      double acc = 0.0;
-      for (int t = 0; t < NT; t++)
-        if (s_ish[t] == s) acc += s_sf[t][c];
-      out[2 + k] = acc;
+    for (int k = tid; k < 81; k += NT) out[2 + k] = (double)(long long)s_sf[k] * (1.0 / 1099511627776.0);
+  }
+}
+
+// Every j tile's entry: tile_entry[t] = e for tileStart <= t <= tileEnd of
+// entry e, one wave per entry.  Tiles of no entry keep what they held; the
+// pair kernel checks every mapping against the entry's range.
+__global__ __launch_bounds__(256) void k_direct_tile_map(int n_ientry, const pme_ientry* __restrict__ ientry,
+                                                         int n_tile, int* __restrict__ tile_entry) {
+  const int e = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (e >= n_ientry) return;
+  const pme_ientry en = ientry[e];
+  const int hi = min(en.tileEnd, n_tile - 1);
+  for (int t = max(en.tileStart, 0) + lane; t <= hi; t += 64) tile_entry[t] = e;
+}
+
+// LDS of one wave of the pair kernel
+struct DirWave {
+  float4 xi[2][32];                 // i tile of each half's entry, q * ccelec
+  float4 xj[64];                    // j atoms of the pass moved by minus their shift, q
+  unsigned long long fi[2][3][32];  // i force sums of each half's entry, 2^40 fixed point
+  unsigned long long fj[3][64];     // j force sums of the pass
+  unsigned short pairs[32 * 64];    // interacting pairs of the pass: i | (j lane << 5)
+};
+
+// Adds the i-force sums of a half's entry to the force array and, for a
+// shifted image, to the shift forces; clears them.  Lanes of the half only.
+template <bool VIRIAL>
+__device__ __forceinline__ void flush_entry(DirWave& W, int h, int jl, int iatom, int ish, int stride,
+                                            unsigned long long* __restrict__ force, unsigned long long* s_sf) {
+  unsigned long long g[3];
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    g[c] = W.fi[h][c][jl];
+    W.fi[h][c][jl] = 0ull;
+  }
+  if (g[0] | g[1] | g[2]) {
+    atomicAdd(&force[iatom + jl], g[0]);
+    atomicAdd(&force[(size_t)(iatom + jl) + stride], g[1]);
+    atomicAdd(&force[(size_t)(iatom + jl) + 2 * (size_t)stride], g[2]);
+    // the entry's shift force (CudaDirectForce_util.h:321-347): the sum of
+    // its i forces, except for the central image
+    if (VIRIAL && ish != 13) {
+#pragma unroll
+      for (int c = 0; c < 3; c++) atomicAdd(&s_sf[3 * ish + c], g[c]);
     }
   }
 }
 
-// One i-tile entry per wave; kDirWaves waves per workgroup.
+// nwave waves over the list's n_tile j tiles, kDirWaves waves per workgroup.
 template <bool ENERGY, bool VIRIAL>
-__global__ __launch_bounds__(kDirWaves * 64) void k_direct_force(DirectParams d, int n_ientry,
+__global__ __launch_bounds__(kDirWaves * 64) void k_direct_force(DirectParams d, int n_ientry, int n_tile, int nwave,
                                                                  const pme_ientry* __restrict__ ientry,
+                                                                 const int* __restrict__ tile_entry,
                                                                  const int* __restrict__ tile_indj,
                                                                  const pme_tile_excl* __restrict__ tile_excl,
                                                                  const float4* __restrict__ xyzq, int stride,
                                                                  unsigned long long* __restrict__ force,
                                                                  double* __restrict__ partial) {
-  __shared__ float4 s_xi[kDirWaves][32];     // shifted i atoms, q * ccelec
-  __shared__ float s_fi[kDirWaves][2][3][32];  // i forces per half of the wave
+  __shared__ DirWave s_w[kDirWaves];
+  __shared__ unsigned long long s_sf[81];  // shift forces, 2^40 fixed point
+  __shared__ double s_pot[kDirWaves];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, h = lane >> 5, jl = lane & 31;
-  const int e = blockIdx.x * kDirWaves + wv;
-  const bool active = e < n_ientry;
-  pme_ientry en{0, 13, 1, 0};
-  if (active) en = ientry[e];
-  float sx, sy, sz;
-  box_shift(en.ish, d, sx, sy, sz);
-  if (active && h == 0) {
-    const float4 p = xyzq[en.iatomStart + jl];
-    s_xi[wv][jl] = make_float4(p.x + sx, p.y + sy, p.z + sz, p.w * (float)kCcelec);
+  DirWave& W = s_w[wv];
+  if (threadIdx.x < 81) s_sf[threadIdx.x] = 0ull;
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    W.fi[h][c][jl] = 0ull;
+    W.fj[c][lane] = 0ull;
   }
-  s_fi[wv][h][0][jl] = 0.f;
-  s_fi[wv][h][1][jl] = 0.f;
-  s_fi[wv][h][2][jl] = 0.f;
   __syncthreads();
+  // this wave's tiles [a, b): the first half takes [a, a + npass), the second the rest
+  const int w = blockIdx.x * kDirWaves + wv;
+  const int a = (int)((long long)w * n_tile / nwave), b = (int)((long long)(w + 1) * n_tile / nwave);
+  const int npass = w < nwave ? (b - a + 1) >> 1 : 0;  // the last workgroup may hold waves past nwave
+  const int t0 = h ? a + npass : a, t_end = h ? b : a + npass;
+  // the half's current entry
+  int cur = -1, cur_lo = 0, cur_hi = -1, cur_ish = 13, cur_i0 = 0;
+  float sx = 0.f, sy = 0.f, sz = 0.f;
   double pot = 0.0;
-  for (int base = en.tileStart; base <= en.tileEnd; base += 2) {
-    const int jt = base + h;
-    const bool valid = jt <= en.tileEnd;
+  for (int k = 0; k < npass; k++) {
+    const int t = t0 + k;
+    const int e = t < t_end ? tile_entry[t] : -1;
+    if (e != cur) {
+      if (cur >= 0) flush_entry<VIRIAL>(W, h, jl, cur_i0, cur_ish, stride, force, s_sf);
+      cur = -1;
+      if (e >= 0 && e < n_ientry) {
+        const pme_ientry en = ientry[e];
+        if (t >= en.tileStart && t <= en.tileEnd) {
+          cur = e;
+          cur_lo = en.tileStart;
+          cur_hi = en.tileEnd;
+          cur_ish = en.ish;
+          cur_i0 = en.iatomStart;
+          box_shift(cur_ish, d, sx, sy, sz);
+          const float4 p = xyzq[cur_i0 + jl];
+          W.xi[h][jl] = make_float4(p.x, p.y, p.z, p.w * (float)kCcelec);
+        }
+      }
+    }
+    wave_lds_sync();
+    const bool ok = cur >= 0 && t >= cur_lo && t <= cur_hi && t < t_end;
     unsigned excl = 0xffffffffu;
     float4 pj = make_float4(0.f, 0.f, 0.f, 0.f);
     int indj = 0;
-    if (valid) {
-      indj = tile_indj[jt];
-      excl = tile_excl[jt].excl[jl];
+    if (ok) {
+      indj = tile_indj[t];
+      excl = tile_excl[t].excl[jl];
       pj = xyzq[indj + jl];
     }
-    float fjx = 0.f, fjy = 0.f, fjz = 0.f;
-#pragma unroll 4
-    for (int t = 0; t < 32; t++) {
-      const int ii = (jl + t) & 31;
-      const float4 pi = s_xi[wv][ii];
-      const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-      const float r2 = dx * dx + dy * dy + dz * dz;
-      if (!(excl & 1u) && r2 < d.roff2) {
-        float pe;
-        const float fij = ewald_fij(r2, rsqrtf(r2), pi.w * pj.w, 0.0f, d, pe);
-        if (ENERGY) pot += (double)pe;
-        const float fx = fij * dx, fy = fij * dy, fz = fij * dz;
-        fjx -= fx;
-        fjy -= fy;
-        fjz -= fz;
-        lds_fadd(&s_fi[wv][h][0][ii], fx);
-        lds_fadd(&s_fi[wv][h][1][ii], fy);
-        lds_fadd(&s_fi[wv][h][2][ii], fz);
-      }
-      excl >>= 1;
-    }
-    if (valid && (fjx != 0.f || fjy != 0.f || fjz != 0.f)) add_fixed(force, indj + jl, stride, fjx, fjy, fjz);
-  }
-  __syncthreads();
-  // i forces: the two halves' sums, one i atom per lane of the first half
-  float fix = 0.f, fiy = 0.f, fiz = 0.f;
-  if (active && h == 0) {
-    fix = s_fi[wv][0][0][jl] + s_fi[wv][1][0][jl];
-    fiy = s_fi[wv][0][1][jl] + s_fi[wv][1][1][jl];
-    fiz = s_fi[wv][0][2][jl] + s_fi[wv][1][2][jl];
-    if (fix != 0.f || fiy != 0.f || fiz != 0.f) add_fixed(force, en.iatomStart + jl, stride, fix, fiy, fiz);
-  }
-  if (ENERGY || VIRIAL) {
-    // shift force of the entry (CudaDirectForce_util.h:321-347): the sum of
-    // its i forces, except for the central image
-    double sf[3] = {(double)fix, (double)fiy, (double)fiz};
+    // xi - (xj - shift) is the separation of i's image from j
+    const float xjx = pj.x - sx, xjy = pj.y - sy, xjz = pj.z - sz;
+    W.xj[lane] = make_float4(xjx, xjy, xjz, pj.w);
+    // check phase: bit i of m = pair (i, j) inside the cutoff
+    unsigned m = 0u;
 #pragma unroll
-    for (int c = 0; c < 3; c++) sf[c] = wave_sum(sf[c]);
-    const int ish = (VIRIAL && active && lane == 0 && en.ish != 13) ? en.ish : -1;
-    write_direct_partial<kDirWaves * 64>(pot, 0.0, ish, sf[0], sf[1], sf[2], ENERGY, VIRIAL,
-                                         partial + (size_t)blockIdx.x * kDirPartial);
+    for (int i = 31; i >= 0; i--) {
+      const float4 pi = W.xi[h][i];
+      const float dx = pi.x - xjx, dy = pi.y - xjy, dz = pi.z - xjz;
+      const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+      m = (m << 1) | (r2 < d.roff2 ? 1u : 0u);
+    }
+    // bit t of excl[l] excludes i = (l + t) mod 32: rotated left by l it is bit i
+    m &= ~__builtin_amdgcn_alignbit(excl, excl, (32 - jl) & 31);
+    if (!ok) m = 0u;
+    // dense phase: the wave's pairs packed in lane order
+    const int cnt = __popc(m);
+    int incl = cnt;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const int v = __shfl_up(incl, off, 64);
+      if (lane >= off) incl += v;
+    }
+    const int total = __shfl(incl, 63, 64);
+    int slot = incl - cnt;
+    while (m) {
+      const int i = __ffs(m) - 1;
+      m &= m - 1u;
+      W.pairs[slot++] = (unsigned short)(i | (lane << 5));
+    }
+    wave_lds_sync();
+    for (int q = lane; q < total; q += 64) {
+      const unsigned pr = W.pairs[q];
+      const int i = pr & 31, l = pr >> 5, hl = l >> 5;
+      const float4 pi = W.xi[hl][i], pjl = W.xj[l];
+      const float dx = pi.x - pjl.x, dy = pi.y - pjl.y, dz = pi.z - pjl.z;
+      const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+      float pe;
+      const float fij = ewald_fij(r2, rsqrtf(r2), pi.w * pjl.w, 0.0f, d, pe);
+      if (ENERGY) pot += (double)pe;
+      const float fx = fij * dx, fy = fij * dy, fz = fij * dz;
+      atomicAdd(&W.fi[hl][0][i], pair_fixed(fx));
+      atomicAdd(&W.fi[hl][1][i], pair_fixed(fy));
+      atomicAdd(&W.fi[hl][2][i], pair_fixed(fz));
+      atomicAdd(&W.fj[0][l], pair_fixed(-fx));
+      atomicAdd(&W.fj[1][l], pair_fixed(-fy));
+      atomicAdd(&W.fj[2][l], pair_fixed(-fz));
+    }
+    wave_lds_sync();
+    unsigned long long g[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      g[c] = W.fj[c][lane];
+      W.fj[c][lane] = 0ull;
+    }
+    if (ok && (g[0] | g[1] | g[2])) {
+      atomicAdd(&force[indj + jl], g[0]);
+      atomicAdd(&force[(size_t)(indj + jl) + stride], g[1]);
+      atomicAdd(&force[(size_t)(indj + jl) + 2 * (size_t)stride], g[2]);
+    }
+    wave_lds_sync();
+  }
+  if (cur >= 0) flush_entry<VIRIAL>(W, h, jl, cur_i0, cur_ish, stride, force, s_sf);
+  if (ENERGY || VIRIAL) {
+    pot = wave_sum(pot);
+    if (lane == 0) s_pot[wv] = pot;
+    __syncthreads();
+    double* out = partial + (size_t)blockIdx.x * kDirPartial;
+    const int tid = threadIdx.x;
+    if (ENERGY && tid == 0) {
+      double acc = 0.0;
+      for (int v = 0; v < kDirWaves; v++) acc += s_pot[v];
+      out[0] = acc;
+      out[1] = 0.0;
+    }
+    if (VIRIAL && tid < 81) out[2 + tid] = (double)(long long)s_sf[tid] * (1.0 / 1099511627776.0);
   }
 }
 
@@ -183,7 +312,8 @@ __global__ __launch_bounds__(kDir14Block) void k_direct_14(DirectParams d, int n
                                                            unsigned long long* __restrict__ force,
                                                            double* __restrict__ partial) {
   const int k = blockIdx.x * kDir14Block + threadIdx.x;
-  double elec = 0.0, excl = 0.0, sfx = 0.0, sfy = 0.0, sfz = 0.0;
+  double elec = 0.0, excl = 0.0;
+  float sfx = 0.f, sfy = 0.f, sfz = 0.f;
   int ish_out = -1;
   if (k < nin14 + nex14) {
     const bool is_in = k < nin14;
@@ -204,9 +334,9 @@ __global__ __launch_bounds__(kDir14Block) void k_direct_14(DirectParams d, int n
     add_fixed(force, p.j, stride, -fx, -fy, -fz);
     if (p.ishift != 13) {
       ish_out = p.ishift;
-      sfx = (double)fx;
-      sfy = (double)fy;
-      sfz = (double)fz;
+      sfx = fx;
+      sfy = fy;
+      sfz = fz;
     }
   }
   if (ENERGY || VIRIAL)
@@ -214,41 +344,47 @@ __global__ __launch_bounds__(kDir14Block) void k_direct_14(DirectParams d, int n
                                       partial + (size_t)blockIdx.x * kDirPartial);
 }
 
-// Sum the partials of nblk workgroups in fixed order and add them into the
-// energy/virial buffer: the elec and excl slots and sforce_dp[27][3].
-__global__ __launch_bounds__(128) void k_direct_reduce(const double* __restrict__ partial, int nblk, int energy,
+// Sum the partials of nblk workgroups and add them into the energy/virial
+// buffer: the elec and excl slots and sforce_dp[27][3].  One workgroup per
+// slot; thread k sums workgroups k, k + 256, ... and block_sum adds the
+// threads in fixed order (the same result on every run).
+__global__ __launch_bounds__(256) void k_direct_reduce(const double* __restrict__ partial, int nblk, int energy,
                                                        int virial, double* e_elec, double* e_excl,
                                                        double* sforce_dp) {
-  const int c = threadIdx.x;
-  if (c >= kDirPartial) return;
-  if (c < 2 && !energy) return;
-  if (c >= 2 && !virial) return;
+  __shared__ double s_red[4];
+  const int c = blockIdx.x;
+  if ((c < 2 && !energy) || (c >= 2 && !virial)) return;
   double acc = 0.0;
-  for (int b = 0; b < nblk; b++) acc += partial[(size_t)b * kDirPartial + c];
-  if (c == 0) e_elec[0] += acc;
-  else if (c == 1) e_excl[0] += acc;
-  else sforce_dp[c - 2] += acc;
+  for (int b = threadIdx.x; b < nblk; b += 256) acc += partial[(size_t)b * kDirPartial + c];
+  block_sum<1, 256>(&acc, s_red);
+  if (threadIdx.x == 0) {
+    if (c == 0) e_elec[0] += acc;
+    else if (c == 1) e_excl[0] += acc;
+    else sforce_dp[c - 2] += acc;
+  }
 }
 
-hipError_t launch_direct_force(const DirectParams& d, int n_ientry, const pme_ientry* ientry, const int* tile_indj,
+hipError_t launch_direct_force(const DirectParams& d, int n_ientry, int n_tile, int nwave,
+                               const pme_ientry* ientry, int* tile_entry, const int* tile_indj,
                                const pme_tile_excl* tile_excl, const float4* xyzq, int stride,
                                unsigned long long* force, bool energy, bool virial, double* partial,
                                hipStream_t s) {
-  const int nblk = direct_force_blocks(n_ientry);
-  if (nblk == 0) return hipSuccess;
-  const dim3 g(nblk), b(kDirWaves * 64);
+  if (n_ientry <= 0 || n_tile <= 0 || nwave <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_direct_tile_map, dim3((n_ientry + 3) / 4), dim3(256), 0, s, n_ientry, ientry, n_tile,
+                     tile_entry);
+  const dim3 g(direct_force_blocks(nwave)), b(kDirWaves * 64);
   if (energy && virial)
-    hipLaunchKernelGGL((k_direct_force<true, true>), g, b, 0, s, d, n_ientry, ientry, tile_indj, tile_excl, xyzq, stride,
-                       force, partial);
+    hipLaunchKernelGGL((k_direct_force<true, true>), g, b, 0, s, d, n_ientry, n_tile, nwave, ientry, tile_entry,
+                       tile_indj, tile_excl, xyzq, stride, force, partial);
   else if (energy)
-    hipLaunchKernelGGL((k_direct_force<true, false>), g, b, 0, s, d, n_ientry, ientry, tile_indj, tile_excl, xyzq,
-                       stride, force, partial);
+    hipLaunchKernelGGL((k_direct_force<true, false>), g, b, 0, s, d, n_ientry, n_tile, nwave, ientry, tile_entry,
+                       tile_indj, tile_excl, xyzq, stride, force, partial);
   else if (virial)
-    hipLaunchKernelGGL((k_direct_force<false, true>), g, b, 0, s, d, n_ientry, ientry, tile_indj, tile_excl, xyzq,
-                       stride, force, partial);
+    hipLaunchKernelGGL((k_direct_force<false, true>), g, b, 0, s, d, n_ientry, n_tile, nwave, ientry, tile_entry,
+                       tile_indj, tile_excl, xyzq, stride, force, partial);
   else
-    hipLaunchKernelGGL((k_direct_force<false, false>), g, b, 0, s, d, n_ientry, ientry, tile_indj, tile_excl, xyzq,
-                       stride, force, partial);
+    hipLaunchKernelGGL((k_direct_force<false, false>), g, b, 0, s, d, n_ientry, n_tile, nwave, ientry, tile_entry,
+                       tile_indj, tile_excl, xyzq, stride, force, partial);
   return hipGetLastError();
 }
 
@@ -273,12 +409,17 @@ hipError_t launch_direct_14(const DirectParams& d, int nin14, int nex14, const p
 hipError_t launch_direct_reduce(const double* partial, int nblk, bool energy, bool virial, double* e_elec,
                                 double* e_excl, double* sforce_dp, hipStream_t s) {
   if (nblk == 0 || !(energy || virial)) return hipSuccess;
-  hipLaunchKernelGGL(k_direct_reduce, dim3(1), dim3(128), 0, s, partial, nblk, energy ? 1 : 0, virial ? 1 : 0, e_elec,
-                     e_excl, sforce_dp);
+  hipLaunchKernelGGL(k_direct_reduce, dim3(kDirPartial), dim3(256), 0, s, partial, nblk, energy ? 1 : 0,
+                     virial ? 1 : 0, e_elec, e_excl, sforce_dp);
   return hipGetLastError();
 }
 
-int direct_force_blocks(int n_ientry) { return (n_ientry + kDirWaves - 1) / kDirWaves; }
+int direct_force_blocks(int nwave) { return (nwave + kDirWaves - 1) / kDirWaves; }
+
+int direct_force_waves(int n_tile, int ncu) {
+  // 16 waves per CU (four workgroups of kDirWaves), at least four j tiles each
+  return std::max(1, std::min(ncu * 16, (n_tile + 3) / 4));
+}
 int direct_14_blocks(int n) { return (n + kDir14Block - 1) / kDir14Block; }
 
 }  // namespace pme

@@ -46,6 +46,9 @@ struct pme_direct_s {
   int in14_cap = 0, ex14_cap = 0;
   double* partial = nullptr;    // [partial_cap * kDirPartial] per-workgroup partials
   int partial_cap = 0;
+  int* tile_entry = nullptr;    // [tile_cap] entry of every j tile of the list
+  int tile_cap = 0;
+  int ncu = 0;                  // compute units of the device
 };
 
 extern "C" pme_status pme_direct_create(pme_ev_t ev, const char* name_vdw, const char* name_elec,
@@ -69,6 +72,12 @@ extern "C" pme_status pme_direct_create(pme_ev_t ev, const char* name_vdw, const
     return st;
   }
   h->sforce_dp = blob;  // sforce_dp[27][3] opens the Virial_t blob
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&h->ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || h->ncu <= 0) {
+    delete h;
+    return fail(PME_ERR_DEVICE, "pme_direct_create: cannot query the device");
+  }
   *out = h;
   return PME_OK;
 }
@@ -79,6 +88,7 @@ extern "C" pme_status pme_direct_destroy(pme_direct_t h) {
   (void)hipFree(h->in14);
   (void)hipFree(h->ex14);
   (void)hipFree(h->partial);
+  (void)hipFree(h->tile_entry);
   delete h;
   return PME_OK;
 }
@@ -223,13 +233,23 @@ extern "C" pme_status pme_direct_calc_force(pme_direct_t h, const void* d_xyzq, 
   if (!nlist->ientry || !nlist->tile_indj || !nlist->tile_excl || !d_xyzq || !d_force)
     return fail(PME_ERR_INVALID_ARG, "pme_direct_calc_force: null device pointer");
   if (stride < 0) return fail(PME_ERR_INVALID_ARG, "pme_direct_calc_force: stride %d", stride);
+  if (nlist->n_tile == 0) return PME_OK;
   hipStream_t s = (hipStream_t)stream;
   const bool ce = calc_energy != 0, cv = calc_virial != 0;
-  const int nblk = direct_force_blocks(nlist->n_ientry);
+  const int nwave = direct_force_waves(nlist->n_tile, h->ncu);
+  const int nblk = direct_force_blocks(nwave);
   if (ce || cv) PME_TRY(ensure_partial(h, nblk, s));
-  HIP_TRY(launch_direct_force(h->d, nlist->n_ientry, nlist->ientry, nlist->tile_indj, nlist->tile_excl,
-                              (const float4*)d_xyzq, stride, reinterpret_cast<unsigned long long*>(d_force), ce, cv,
-                              h->partial, s));
+  if (nlist->n_tile > h->tile_cap) {
+    HIP_TRY(hipStreamSynchronize(s));
+    (void)hipFree(h->tile_entry);
+    h->tile_entry = nullptr;
+    h->tile_cap = 0;
+    HIP_TRY(hipMalloc(&h->tile_entry, sizeof(int) * (size_t)nlist->n_tile));
+    h->tile_cap = nlist->n_tile;
+  }
+  HIP_TRY(launch_direct_force(h->d, nlist->n_ientry, nlist->n_tile, nwave, nlist->ientry, h->tile_entry,
+                              nlist->tile_indj, nlist->tile_excl, (const float4*)d_xyzq, stride,
+                              reinterpret_cast<unsigned long long*>(d_force), ce, cv, h->partial, s));
   HIP_TRY(launch_direct_reduce(h->partial, nblk, ce, cv, h->e_elec, h->e_excl, h->sforce_dp, s));
   return PME_OK;
 }

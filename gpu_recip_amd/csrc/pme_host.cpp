@@ -228,6 +228,21 @@ enum GridState {
 };
 
 
+// Everything the launches of one pme_recip_solve depend on: the arguments,
+// the handle's stream, plan and buffers, and the self-energy partials the
+// inverse transform folds in.  Equal keys give the same launch sequence.
+struct SolveKey {
+  const void* xyzq;
+  void* force;
+  hipStream_t stream;
+  const void* buf[6];
+  double recip[9];
+  double kappa, pend_kcs;
+  int n, calc_energy, calc_virial, stride, force_type;
+  int order, prec, pend_self, nbins, chunk, nchunk, nchunk_cap, bins_for, partial_cap;
+  int nb[3], wmax[3];
+};
+
 struct pme_recip_s {
   int nx = 0, ny = 0, nz = 0, order = 0, fft_type = PME_FFT_BOX, accum = PME_ACCUM_FLOAT;
   hipStream_t stream = nullptr;
@@ -268,6 +283,11 @@ struct pme_recip_s {
 
   bool timing = false;
   hipEvent_t tev[9] = {};
+
+  // pme_recip_solve replayed as a graph (solve_graph)
+  SolveKey gkey{};
+  bool gkey_valid = false;
+  hipGraphExec_t gexec = nullptr;
 };
 
 static void free_handle(pme_recip_t h) {
@@ -292,6 +312,7 @@ static void free_handle(pme_recip_t h) {
   (void)hipFree(h->partial);
   (void)hipFree(h->self_partial);
   if (h->flush_ev) (void)hipEventDestroy(h->flush_ev);
+  if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
   for (auto& e : h->tev)
     if (e) (void)hipEventDestroy(e);
   delete h;
@@ -762,17 +783,119 @@ extern "C" pme_status pme_recip_calc_self_energy(pme_recip_t h, const void* d_xy
   return PME_OK;
 }
 
-extern "C" pme_status pme_recip_solve(pme_recip_t h, const void* d_xyzq, int ncoord, const double recip[9],
-                                      double kappa, int calc_energy, int calc_virial, int stride, int force_type,
-                                      void* d_force) {
-  if (!h || !recip) return fail(PME_ERR_INVALID_ARG, "pme_recip_solve: null argument");
-  if (!box_ok(recip)) return fail(PME_ERR_INVALID_ARG, "pme_recip_solve: recip has no positive determinant");
+static pme_status solve_stages(pme_recip_t h, const void* d_xyzq, int ncoord, const double recip[9], double kappa,
+                               int calc_energy, int calc_virial, int stride, int force_type, void* d_force) {
   PME_TRY(pme_recip_spread_charge(h, d_xyzq, ncoord, recip));
   PME_TRY(pme_recip_r2c_fft(h));
   PME_TRY(pme_recip_scalar_sum(h, recip, kappa, calc_energy, calc_virial));
   PME_TRY(pme_recip_c2r_fft(h));
   PME_TRY(pme_recip_gather_force(h, d_xyzq, ncoord, recip, stride, force_type, d_force));
   return PME_OK;
+}
+
+static SolveKey solve_key(pme_recip_t h, const void* d_xyzq, int ncoord, const double recip[9], double kappa,
+                          int calc_energy, int calc_virial, int stride, int force_type, void* d_force, int pend_self,
+                          double pend_kcs) {
+  SolveKey k;
+  std::memset(&k, 0, sizeof k);  // padding too: keys compare bytewise
+  k.xyzq = d_xyzq;
+  k.force = d_force;
+  k.stream = h->stream;
+  const void* b[6] = {h->grid, h->cgrid, h->seg_off, h->rec_pos, h->rec_idx, h->partial};
+  for (int i = 0; i < 6; i++) k.buf[i] = b[i];
+  for (int i = 0; i < 9; i++) k.recip[i] = recip[i];
+  k.kappa = kappa;
+  k.pend_kcs = pend_kcs;
+  k.n = ncoord;
+  k.calc_energy = calc_energy != 0;
+  k.calc_virial = calc_virial != 0;
+  k.stride = stride;
+  k.force_type = force_type;
+  k.order = h->order;
+  k.prec = h->prec;
+  k.pend_self = pend_self;
+  k.nbins = h->nbins;
+  k.chunk = h->chunk;
+  k.nchunk = h->nchunk;
+  k.nchunk_cap = h->nchunk_cap;
+  k.bins_for = h->bins_for;
+  k.partial_cap = h->partial_cap;
+  for (int a = 0; a < 3; a++) {
+    k.nb[a] = h->nb[a];
+    k.wmax[a] = h->wmax[a];
+  }
+  return k;
+}
+
+// The one-shot solve as a hipGraph on the handle's stream: the first call
+// with a new key runs the stages eagerly (allocating what they need), the
+// second captures them into a graph and launches it, later ones replay it.
+// A replay leaves the handle in the state the stages leave it in.  The null
+// stream cannot be captured, nor are timed solves; those run eagerly.
+static pme_status solve_graph(pme_recip_t h, const void* d_xyzq, int ncoord, const double recip[9], double kappa,
+                              int calc_energy, int calc_virial, int stride, int force_type, void* d_force) {
+  const int pend_self = h->pend_self;
+  const double pend_kcs = h->pend_kcs;
+  if (h->gkey_valid && h->gexec) {
+    const SolveKey k = solve_key(h, d_xyzq, ncoord, recip, kappa, calc_energy, calc_virial, stride, force_type,
+                                 d_force, pend_self, pend_kcs);
+    if (std::memcmp(&k, &h->gkey, sizeof k) == 0) {
+      HIP_TRY(hipGraphLaunch(h->gexec, h->stream));
+      h->state = ST_POTENTIAL;
+      h->pend_solve = h->pend_self = 0;
+      h->bin_key_ptr = d_xyzq;
+      h->bin_key_n = ncoord;
+      for (int i = 0; i < 9; i++) h->bin_key_r[i] = (float)recip[i];
+      return PME_OK;
+    }
+  }
+  if (h->gexec) {
+    (void)hipGraphExecDestroy(h->gexec);
+    h->gexec = nullptr;
+  }
+  const bool seen = h->gkey_valid;
+  const SolveKey before = solve_key(h, d_xyzq, ncoord, recip, kappa, calc_energy, calc_virial, stride, force_type,
+                                    d_force, pend_self, pend_kcs);
+  if (seen && std::memcmp(&before, &h->gkey, sizeof before) == 0) {
+    // the same solve as last time, with nothing left to allocate: capture it
+    HIP_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    const pme_status st =
+        solve_stages(h, d_xyzq, ncoord, recip, kappa, calc_energy, calc_virial, stride, force_type, d_force);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(h->stream, &g);
+    if (st != PME_OK || e != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      h->gkey_valid = false;
+      if (st != PME_OK) return st;
+      HIP_TRY(e);
+    }
+    const hipError_t ei = hipGraphInstantiate(&h->gexec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ei != hipSuccess) {
+      h->gexec = nullptr;
+      h->gkey_valid = false;
+      HIP_TRY(ei);
+    }
+    HIP_TRY(hipGraphLaunch(h->gexec, h->stream));
+    return PME_OK;
+  }
+  PME_TRY(solve_stages(h, d_xyzq, ncoord, recip, kappa, calc_energy, calc_virial, stride, force_type, d_force));
+  // the key as the next identical call will see it (buffers as allocated now)
+  h->gkey = solve_key(h, d_xyzq, ncoord, recip, kappa, calc_energy, calc_virial, stride, force_type, d_force,
+                      pend_self, pend_kcs);
+  h->gkey_valid = true;
+  return PME_OK;
+}
+
+extern "C" pme_status pme_recip_solve(pme_recip_t h, const void* d_xyzq, int ncoord, const double recip[9],
+                                      double kappa, int calc_energy, int calc_virial, int stride, int force_type,
+                                      void* d_force) {
+  if (!h || !recip) return fail(PME_ERR_INVALID_ARG, "pme_recip_solve: null argument");
+  if (!box_ok(recip)) return fail(PME_ERR_INVALID_ARG, "pme_recip_solve: recip has no positive determinant");
+  // a scalar-sum reduction still pending would be flushed inside the sequence
+  if (h->timing || h->stream == nullptr || h->pend_solve != 0)
+    return solve_stages(h, d_xyzq, ncoord, recip, kappa, calc_energy, calc_virial, stride, force_type, d_force);
+  return solve_graph(h, d_xyzq, ncoord, recip, kappa, calc_energy, calc_virial, stride, force_type, d_force);
 }
 
 // ------------------------------------------------- allocation-free solve --
@@ -875,8 +998,9 @@ extern "C" pme_status pme_recip_solve_noalloc(pme_recip_t h, const void* d_xyzq,
   h->nchunk = h->nchunk_cap = L.nchunk;
   h->bins_for = std::max(ncoord, 1);
   h->bin_key_n = -1;
-  pme_status st = pme_recip_solve(h, d_xyzq, ncoord, recip, kappa, calc_energy, calc_virial, stride, force_type,
-                                  d_force);
+  // the stages run eagerly: instantiating a graph may allocate
+  pme_status st = solve_stages(h, d_xyzq, ncoord, recip, kappa, calc_energy, calc_virial, stride, force_type,
+                               d_force);
   // a reduction still pending reads the scratch partials: launch it now
   if (st == PME_OK) st = flush_pending(h, h->stream, false);
   h->grid = grid;

@@ -362,19 +362,22 @@ struct DirectParams {
   float roff2;
   float e14fac;
 };
-constexpr int kDirWaves = 4;         // i-tile entries (waves) per non-bonded workgroup
+constexpr int kDirWaves = 4;         // waves per non-bonded workgroup
 constexpr int kDir14Block = 256;     // 1-4 entries per workgroup
 constexpr int kDirPartial = 2 + 81;  // doubles per workgroup partial: elec, excl, sforce[27][3]
-hipError_t launch_direct_force(const DirectParams& d, int n_ientry, const pme_ientry* ientry, const int* tile_indj,
+hipError_t launch_direct_force(const DirectParams& d, int n_ientry, int n_tile, int nwave,
+                               const pme_ientry* ientry, int* tile_entry, const int* tile_indj,
                                const pme_tile_excl* tile_excl, const float4* xyzq, int stride,
-                               unsigned long long* force, bool energy, bool virial, double* partial, hipStream_t s);
+                               unsigned long long* force, bool energy, bool virial, double* partial,
+                               hipStream_t s);
 hipError_t launch_direct_14(const DirectParams& d, int nin14, int nex14, const pme_xx14* in14, const pme_xx14* ex14,
                             const float4* xyzq, int stride, unsigned long long* force, bool energy, bool virial,
                             double* partial, hipStream_t s);
 // adds the nblk partials (fixed order) to the elec / excl terms and sforce_dp[27][3]
 hipError_t launch_direct_reduce(const double* partial, int nblk, bool energy, bool virial, double* e_elec,
                                 double* e_excl, double* sforce_dp, hipStream_t s);
-int direct_force_blocks(int n_ientry);
+int direct_force_blocks(int nwave);
+int direct_force_waves(int n_tile, int ncu);
 int direct_14_blocks(int n);
 
 }  // namespace pme

@@ -1,6 +1,7 @@
 // pme_recip.hpp -- C++ drop-in for the reference's CudaPMERecip /
-// CudaEnergyVirial classes (ap-hynninen/gpu_recip CudaPMERecip.h:37-205,
-// CudaEnergyVirial.h:20-53), header-only over the C ABI of pme_recip.h.
+// CudaEnergyVirial / CudaPMEDirectForce classes (ap-hynninen/gpu_recip
+// CudaPMERecip.h:37-205, CudaEnergyVirial.h:20-53, CudaPMEDirectForce.h:80-204),
+// header-only over the C ABI of pme_recip.h.
 //
 // Same class names, member names and argument meaning:
 //   CudaEnergyVirial ev;
@@ -160,6 +161,80 @@ class CudaPMERecip {
 
   pme_recip_t h_ = nullptr;
   int nfftx_, nffty_, nfftz_, order_;
+};
+
+// Models of the direct-space term, CudaDirectForceTypes.h:62-65 (only the
+// electrostatic ones are built)
+enum { NONE = PME_VDW_NONE, EWALD = PME_ELEC_EWALD, EWALD_LOOKUP = PME_ELEC_EWALD_LOOKUP };
+typedef pme_xx14 xx14list_t;  // Bonded_struct.h:122-124
+
+// CudaPMEDirectForce<AT, CT> (CudaPMEDirectForce.h:80-204): the direct-space
+// Ewald term.  The reference instantiates AT = long long, CT = float only.
+// calc_force takes the tile list as a pme_nlist of device pointers in place of
+// CudaNeighborListBuild<32> (whose get_ientry/get_tile_indj/get_tile_excl
+// fill it).
+template <typename AT, typename CT>
+class CudaPMEDirectForce {
+  static_assert(std::is_same<AT, long long>::value && std::is_same<CT, float>::value,
+                "CudaPMEDirectForce<long long, float> is the one instantiation");
+
+ public:
+  CudaPMEDirectForce(CudaEnergyVirial& energyVirial, const char* nameVdw, const char* nameElec,
+                     const char* nameExcl) {
+    pme::check(pme_direct_create(energyVirial.handle(), nameVdw, nameElec, nameExcl, &h_), "CudaPMEDirectForce");
+  }
+  ~CudaPMEDirectForce() { pme_direct_destroy(h_); }
+  CudaPMEDirectForce(const CudaPMEDirectForce&) = delete;
+  CudaPMEDirectForce& operator=(const CudaPMEDirectForce&) = delete;
+
+  void setup(double boxx, double boxy, double boxz, double kappa, double roff, double ron, double e14fac,
+             int vdw_model, int elec_model) {
+    pme::check(pme_direct_setup(h_, boxx, boxy, boxz, kappa, roff, ron, e14fac, vdw_model, elec_model), "setup");
+    calc_vdw_ = calc_elec_ = true;  // setup turns both terms on, CudaPMEDirectForce.cu:255-256
+  }
+  void get_setup(float& boxx, float& boxy, float& boxz, float& kappa, float& roff, float& ron, float& e14fac,
+                 int& vdw_model, int& elec_model) {
+    pme::check(pme_direct_get_setup(h_, &boxx, &boxy, &boxz, &kappa, &roff, &ron, &e14fac, &vdw_model, &elec_model),
+               "get_setup");
+  }
+  void get_box_size(CT& boxx, CT& boxy, CT& boxz) {
+    float kappa, roff, ron, e14fac;
+    int vdw, elec;
+    get_setup(boxx, boxy, boxz, kappa, roff, ron, e14fac, vdw, elec);
+  }
+  void set_box_size(const CT boxx, const CT boxy, const CT boxz) {
+    pme::check(pme_direct_set_box_size(h_, boxx, boxy, boxz), "set_box_size");
+  }
+  void set_calc_vdw(const bool calc_vdw) {
+    pme::check(pme_direct_set_calc_vdw(h_, calc_vdw), "set_calc_vdw");
+    calc_vdw_ = calc_vdw;
+  }
+  void set_calc_elec(const bool calc_elec) {
+    pme::check(pme_direct_set_calc_elec(h_, calc_elec), "set_calc_elec");
+    calc_elec_ = calc_elec;
+  }
+  bool get_calc_vdw() { return calc_vdw_; }
+  bool get_calc_elec() { return calc_elec_; }
+
+  void set_14_list(int nin14list, int nex14list, xx14list_t* h_in14list, xx14list_t* h_ex14list,
+                   hipStream_t stream = 0) {
+    pme::check(pme_direct_set_14_list(h_, nin14list, nex14list, h_in14list, h_ex14list, stream), "set_14_list");
+  }
+  void calc_14_force(const float4* xyzq, const bool calc_energy, const bool calc_virial, const int stride, AT* force,
+                     hipStream_t stream = 0) {
+    pme::check(pme_direct_calc_14_force(h_, xyzq, calc_energy, calc_virial, stride, force, stream), "calc_14_force");
+  }
+  void calc_force(const float4* xyzq, const pme_nlist& nlist, const bool calc_energy, const bool calc_virial,
+                  const int stride, AT* force, hipStream_t stream = 0) {
+    pme::check(pme_direct_calc_force(h_, xyzq, &nlist, calc_energy, calc_virial, stride, force, stream),
+               "calc_force");
+  }
+
+  pme_direct_t handle() const { return h_; }
+
+ private:
+  pme_direct_t h_ = nullptr;
+  bool calc_vdw_ = true, calc_elec_ = true;
 };
 
 #endif  // PME_RECIP_HPP
