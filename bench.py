@@ -316,6 +316,30 @@ def main():
         st_ms2 += np.array(pme2.get_timing())
     pme2.set_timing(False)
     st_ms2 /= nt
+    sb2 = stage_bytes(s2)
+    stage_frac2 = {STAGES[i]: sb2[STAGES[i]] / (st_ms2[i] * 1e-3) / (HBM_PEAK_GBS * 1e9)
+                   for i in range(len(STAGES)) if sb2[STAGES[i]] > 0 and st_ms2[i] > 0}
+    # the same solve through pme_recip_solve on a stream of its own: replayed
+    # as one hipGraph after the first two calls
+    side2 = torch.cuda.Stream()
+    ev2g = CudaEnergyVirial()
+    pme2g = CudaPMERecip(*s2.nfft, s2.order, ev2g)
+    pme2g.set_stream(side2)
+    f2g = torch.zeros(3 * st2, dtype=torch.float32, device=dev)
+
+    def p2gstep():
+        ev2g.clear(side2)
+        pme2g.solve(x2, s2.natom, s2.recip, s2.kappa, True, True, st2, f2g, L.FORCE_FLOAT)
+        ev2g.copyToHost(side2)
+        side2.synchronize()
+
+    for _ in range(max(3, args.warmup)):
+        p2gstep()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        p2gstep()
+    dt_2g = time.perf_counter() - t0
+    assert np.array_equal(f2g.cpu().numpy(), f2.cpu().numpy()), "graph solve differs from the five calls"
 
     # ---------------- split precision: 250,002 atoms, 144^3, order 4 ---------
     split = None
@@ -559,7 +583,15 @@ def main():
                              "atom_steps_per_s": s2.natom * args.steps / dt_2,
                              "ms_per_solve": dt_2 / args.steps * 1e3,
                              "stage_ms": {STAGES[i]: float(st_ms2[i]) for i in range(len(STAGES))},
-                             "solve_roofline_frac": s2.algorithmic_bytes() / (dt_2 / args.steps) / (HBM_PEAK_GBS * 1e9)},
+                             "stage_roofline_frac": stage_frac2,
+                             "stage_timing_note": "HIP events per stage on the handle's stream (each slot carries "
+                                                  "a few us of event overhead); rocprofv3 kernel times in profiles/",
+                             "solve_roofline_frac": s2.algorithmic_bytes() / (dt_2 / args.steps) / (HBM_PEAK_GBS * 1e9),
+                             "graph": {"step": "pme_recip_solve on a stream of its own (hipGraph replay), "
+                                               "energy/virial clear and readback, sync",
+                                       "solves_per_s": args.steps / dt_2g, "ms_per_solve": dt_2g / args.steps * 1e3,
+                                       "solve_roofline_frac": s2.algorithmic_bytes() / (dt_2g / args.steps)
+                                       / (HBM_PEAK_GBS * 1e9)}},
         "stage_ms": {STAGES[i]: float(stage_ms[i]) for i in range(len(STAGES))},
         "solve_algorithmic_bytes": bytes_b,
         "solve_roofline_frac": bytes_b / (ms_b * 1e-3) / (HBM_PEAK_GBS * 1e9),

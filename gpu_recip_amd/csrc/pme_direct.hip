@@ -39,16 +39,35 @@ __device__ __forceinline__ void box_shift(int ish, const DirectParams& d, float&
   sz = (float)(iz - 1) * d.boxz;
 }
 
+// erfc(x) for x >= 0 from e^{-x^2}: erfc(x) e^{x^2} = t P(t) with
+// t = 1 / (1 + p x), the form of Hastings' approximation (Abramowitz &
+// Stegun 7.1.26) with ten terms whose coefficients were fitted here by
+// minimax least squares on the relative error of erfc(x) e^{x^2} over
+// [0, 12].  Evaluated in FP32 with the hardware reciprocal its relative error
+// is below 4e-7 on [0, 6] (about three ulp, checked against an FP64 erfc).
+// The pair term needs e^{-x^2} for the force anyway, so erfc costs a
+// reciprocal and ten FMAs.
+constexpr float kErfcP = 0.57f;
+constexpr float kErfcC[10] = {0.3215866983f,  0.3216636181f, 0.2678903937f, 0.1791002899f,  -0.0476588681f,
+                              0.2127946466f,  -0.7263579965f, 0.7648586631f, -0.3605679274f, 0.0666905344f};
+__device__ __forceinline__ float erfc_from_exp(float x, float exp_neg_x2) {
+  const float t = __builtin_amdgcn_rcpf(fmaf(kErfcP, x, 1.0f));
+  float acc = kErfcC[9];
+#pragma unroll
+  for (int k = 8; k >= 0; k--) acc = fmaf(acc, t, kErfcC[k]);
+  return acc * t * exp_neg_x2;
+}
+
 // The EWALD pair term (CudaDirectForceKernels.cu:352-390) for charges already
 // multiplied by ccelec: pot = qq (erfc(kappa r) + fac_m1) / r with fac_m1 = 0
 // for non-bonded pairs, e14fac - 1 for 1-4 interactions and -1 for 1-4
 // exclusions.  Returns fij = (r dU/dr) / r^2, so that fij * (xi - xj) is the
-// gradient on i.  erfcf and expf are the ROCm device library's.
+// gradient on i.  expf is the ROCm device library's.
 __device__ __forceinline__ float ewald_fij(float r2, float rinv, float qq, float fac_m1, const DirectParams& d,
                                            float& pot) {
   const float r = r2 * rinv;
-  const float erfc_val = erfcf(d.kappa * r);
   const float exp_val = expf(-d.kappa2 * r2);
+  const float erfc_val = erfc_from_exp(d.kappa * r, exp_val);
   const float qq_efac_rinv = qq * (erfc_val + fac_m1) * rinv;
   pot = qq_efac_rinv;
   const float two_sqrtpi = 1.12837916709551f;  // 2/sqrt(pi)
@@ -72,6 +91,27 @@ __device__ __forceinline__ unsigned long long pair_fixed(float f) {
   if (fabsf(f) < 2048.0f) return to_fixed_product((double)f, 1099511627776.0);
   const float c = fminf(fmaxf(f, -8388607.0f), 8388607.0f);
   return (unsigned long long)(long long)((double)c * 1099511627776.0);
+}
+
+// A pair energy as 2^32 fixed point (integer sums do not depend on which wave
+// takes which pairs); |e| < 2^19 by one FMA, larger ones held inside +-2^30.
+__device__ __forceinline__ long long energy_fixed(float e) {
+  if (fabsf(e) < 524288.0f) return (long long)to_fixed_product((double)e, 4294967296.0);
+  const float c = fminf(fmaxf(e, -1073741824.0f), 1073741824.0f);
+  return (long long)((double)c * 4294967296.0);
+}
+
+// Inclusive prefix sum over the 64 lanes of a wave with DPP moves: shifts by
+// 1, 2, 4 and 8 lanes inside each row of 16, then row 0's and row 2's last
+// lanes into rows 1 and 3, and lane 31 into rows 2 and 3 (all VALU, no LDS).
+__device__ __forceinline__ int wave_inclusive_scan(int v) {
+  v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false);  // row_shr:1
+  v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false);  // row_shr:2
+  v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false);  // row_shr:4
+  v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false);  // row_shr:8
+  v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false);  // row_bcast:15
+  v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false);  // row_bcast:31
+  return v;
 }
 
 // Orders the LDS accesses of the lanes of one wave (all of them issue from
@@ -117,9 +157,15 @@ __device__ __forceinline__ void write_direct_partial(double elec, double excl, i
   }
 }
 
+// tile_entry[t] = -1 for every tile (the map kernel then fills in the tiles
+// of the entries).
+__global__ __launch_bounds__(256) void k_direct_tile_clear(int n_tile, int* __restrict__ tile_entry) {
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  if (t < n_tile) tile_entry[t] = -1;
+}
+
 // Every j tile's entry: tile_entry[t] = e for tileStart <= t <= tileEnd of
-// entry e, one wave per entry.  Tiles of no entry keep what they held; the
-// pair kernel checks every mapping against the entry's range.
+// entry e, one wave per entry; tiles of no entry stay -1.
 __global__ __launch_bounds__(256) void k_direct_tile_map(int n_ientry, const pme_ientry* __restrict__ ientry,
                                                          int n_tile, int* __restrict__ tile_entry) {
   const int e = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
@@ -131,7 +177,7 @@ __global__ __launch_bounds__(256) void k_direct_tile_map(int n_ientry, const pme
 
 // LDS of one wave of the pair kernel
 struct DirWave {
-  float4 xi[2][32];                 // i tile of each half's entry, q * ccelec
+  float4 xi[2][33];                 // i tile of each half's entry, q * ccelec (rows padded: other banks)
   float4 xj[64];                    // j atoms of the pass moved by minus their shift, q
   unsigned long long fi[2][3][32];  // i force sums of each half's entry, 2^40 fixed point
   unsigned long long fj[3][64];     // j force sums of the pass
@@ -162,9 +208,16 @@ __device__ __forceinline__ void flush_entry(DirWave& W, int h, int jl, int iatom
   }
 }
 
-// nwave waves over the list's n_tile j tiles, kDirWaves waves per workgroup.
+// The list's j tiles in chunks of kDirChunk, dealt to the waves in turn
+// (chunk c to wave c mod nwave): every wave sees tiles from the whole list,
+// whose pair density varies tenfold from tile to tile, so the waves' loads
+// even out without a shared counter.  Within a chunk each half of the wave
+// takes kDirChunk / 2 consecutive tiles, one per pass.  The j atoms of a
+// chunk are loaded together at its start, and the tile records (entry, atom
+// index, exclusions) of the wave's next chunk while the current one is
+// processed: one memory round trip per chunk, not a chain of them per pass.
 template <bool ENERGY, bool VIRIAL>
-__global__ __launch_bounds__(kDirWaves * 64) void k_direct_force(DirectParams d, int n_ientry, int n_tile, int nwave,
+__global__ __launch_bounds__(kDirWaves * 64) void k_direct_force(DirectParams d, int n_ientry, int n_tile,
                                                                  const pme_ientry* __restrict__ ientry,
                                                                  const int* __restrict__ tile_entry,
                                                                  const int* __restrict__ tile_indj,
@@ -174,128 +227,153 @@ __global__ __launch_bounds__(kDirWaves * 64) void k_direct_force(DirectParams d,
                                                                  double* __restrict__ partial) {
   __shared__ DirWave s_w[kDirWaves];
   __shared__ unsigned long long s_sf[81];  // shift forces, 2^40 fixed point
-  __shared__ double s_pot[kDirWaves];
+  __shared__ unsigned long long s_pot;     // energy, 2^32 fixed point
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, h = lane >> 5, jl = lane & 31;
   DirWave& W = s_w[wv];
   if (threadIdx.x < 81) s_sf[threadIdx.x] = 0ull;
+  if (threadIdx.x == 0) s_pot = 0ull;
 #pragma unroll
   for (int c = 0; c < 3; c++) {
     W.fi[h][c][jl] = 0ull;
     W.fj[c][lane] = 0ull;
   }
   __syncthreads();
-  // this wave's tiles [a, b): the first half takes [a, a + npass), the second the rest
-  const int w = blockIdx.x * kDirWaves + wv;
-  const int a = (int)((long long)w * n_tile / nwave), b = (int)((long long)(w + 1) * n_tile / nwave);
-  const int npass = w < nwave ? (b - a + 1) >> 1 : 0;  // the last workgroup may hold waves past nwave
-  const int t0 = h ? a + npass : a, t_end = h ? b : a + npass;
+  const int nchunk = (n_tile + kDirChunk - 1) / kDirChunk;
+  const int wtotal = (int)gridDim.x * kDirWaves;
+  int chunk = (int)blockIdx.x * kDirWaves + wv;
   // the half's current entry
   int cur = -1, cur_lo = 0, cur_hi = -1, cur_ish = 13, cur_i0 = 0;
   float sx = 0.f, sy = 0.f, sz = 0.f;
-  double pot = 0.0;
-  for (int k = 0; k < npass; k++) {
-    const int t = t0 + k;
-    const int e = t < t_end ? tile_entry[t] : -1;
-    if (e != cur) {
-      if (cur >= 0) flush_entry<VIRIAL>(W, h, jl, cur_i0, cur_ish, stride, force, s_sf);
-      cur = -1;
-      if (e >= 0 && e < n_ientry) {
-        const pme_ientry en = ientry[e];
-        if (t >= en.tileStart && t <= en.tileEnd) {
-          cur = e;
-          cur_lo = en.tileStart;
-          cur_hi = en.tileEnd;
-          cur_ish = en.ish;
-          cur_i0 = en.iatomStart;
-          box_shift(cur_ish, d, sx, sy, sz);
-          const float4 p = xyzq[cur_i0 + jl];
-          W.xi[h][jl] = make_float4(p.x, p.y, p.z, p.w * (float)kCcelec);
+  long long pot = 0;
+  constexpr int NP = kDirChunk / 2;  // passes per chunk
+  // tile records of the half's NP tiles of a chunk
+  int ne[NP], nindj[NP];
+  unsigned nexcl[NP];
+  auto load_records = [&](int c) {
+#pragma unroll
+    for (int k = 0; k < NP; k++) {
+      const int t = c * kDirChunk + h * NP + k;
+      const bool in = c < nchunk && t < n_tile;
+      ne[k] = in ? tile_entry[t] : -1;
+      nindj[k] = in ? tile_indj[t] : 0;
+      nexcl[k] = in ? tile_excl[t].excl[jl] : 0xffffffffu;
+    }
+  };
+  load_records(chunk);
+  while (chunk < nchunk) {
+    const int tbase = chunk * kDirChunk + h * NP;
+    int ce[NP], cindj[NP];
+    unsigned cexcl[NP];
+    float4 cpj[NP];
+#pragma unroll
+    for (int k = 0; k < NP; k++) {
+      ce[k] = ne[k];
+      cindj[k] = nindj[k];
+      cexcl[k] = nexcl[k];
+    }
+    // the j atoms of this chunk, then the records of the next one
+#pragma unroll
+    for (int k = 0; k < NP; k++) cpj[k] = ce[k] >= 0 ? xyzq[cindj[k] + jl] : make_float4(0.f, 0.f, 0.f, 0.f);
+    load_records(chunk + wtotal);
+#pragma unroll
+    for (int k = 0; k < NP; k++) {
+      const int t = tbase + k;
+      const int e = ce[k];
+      if (e != cur) {
+        if (cur >= 0) flush_entry<VIRIAL>(W, h, jl, cur_i0, cur_ish, stride, force, s_sf);
+        cur = -1;
+        if (e >= 0 && e < n_ientry) {
+          const pme_ientry en = ientry[e];
+          if (t >= en.tileStart && t <= en.tileEnd) {
+            cur = e;
+            cur_lo = en.tileStart;
+            cur_hi = en.tileEnd;
+            cur_ish = en.ish;
+            cur_i0 = en.iatomStart;
+            box_shift(cur_ish, d, sx, sy, sz);
+            const float4 p = xyzq[cur_i0 + jl];
+            W.xi[h][jl] = make_float4(p.x, p.y, p.z, p.w * (float)kCcelec);
+          }
         }
       }
-    }
-    wave_lds_sync();
-    const bool ok = cur >= 0 && t >= cur_lo && t <= cur_hi && t < t_end;
-    unsigned excl = 0xffffffffu;
-    float4 pj = make_float4(0.f, 0.f, 0.f, 0.f);
-    int indj = 0;
-    if (ok) {
-      indj = tile_indj[t];
-      excl = tile_excl[t].excl[jl];
-      pj = xyzq[indj + jl];
-    }
-    // xi - (xj - shift) is the separation of i's image from j
-    const float xjx = pj.x - sx, xjy = pj.y - sy, xjz = pj.z - sz;
-    W.xj[lane] = make_float4(xjx, xjy, xjz, pj.w);
-    // check phase: bit i of m = pair (i, j) inside the cutoff
-    unsigned m = 0u;
+      wave_lds_sync();
+      const bool ok = cur >= 0 && t >= cur_lo && t <= cur_hi && t < n_tile;
+      const unsigned excl = cexcl[k];
+      const float4 pj = cpj[k];
+      const int indj = cindj[k];
+      // xi - (xj - shift) is the separation of i's image from j
+      const float xjx = pj.x - sx, xjy = pj.y - sy, xjz = pj.z - sz;
+      W.xj[lane] = make_float4(xjx, xjy, xjz, pj.w);
+      // check phase: bit i of m = pair (i, j) inside the cutoff
+      unsigned m = 0u;
 #pragma unroll
-    for (int i = 31; i >= 0; i--) {
-      const float4 pi = W.xi[h][i];
-      const float dx = pi.x - xjx, dy = pi.y - xjy, dz = pi.z - xjz;
-      const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-      m = (m << 1) | (r2 < d.roff2 ? 1u : 0u);
-    }
-    // bit t of excl[l] excludes i = (l + t) mod 32: rotated left by l it is bit i
-    m &= ~__builtin_amdgcn_alignbit(excl, excl, (32 - jl) & 31);
-    if (!ok) m = 0u;
-    // dense phase: the wave's pairs packed in lane order
-    const int cnt = __popc(m);
-    int incl = cnt;
+      for (int g = 3; g >= 0; g--) {
+        float4 pg[8];  // eight i atoms read together, then tested
 #pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-      const int v = __shfl_up(incl, off, 64);
-      if (lane >= off) incl += v;
-    }
-    const int total = __shfl(incl, 63, 64);
-    int slot = incl - cnt;
-    while (m) {
-      const int i = __ffs(m) - 1;
-      m &= m - 1u;
-      W.pairs[slot++] = (unsigned short)(i | (lane << 5));
-    }
-    wave_lds_sync();
-    for (int q = lane; q < total; q += 64) {
-      const unsigned pr = W.pairs[q];
-      const int i = pr & 31, l = pr >> 5, hl = l >> 5;
-      const float4 pi = W.xi[hl][i], pjl = W.xj[l];
-      const float dx = pi.x - pjl.x, dy = pi.y - pjl.y, dz = pi.z - pjl.z;
-      const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-      float pe;
-      const float fij = ewald_fij(r2, rsqrtf(r2), pi.w * pjl.w, 0.0f, d, pe);
-      if (ENERGY) pot += (double)pe;
-      const float fx = fij * dx, fy = fij * dy, fz = fij * dz;
-      atomicAdd(&W.fi[hl][0][i], pair_fixed(fx));
-      atomicAdd(&W.fi[hl][1][i], pair_fixed(fy));
-      atomicAdd(&W.fi[hl][2][i], pair_fixed(fz));
-      atomicAdd(&W.fj[0][l], pair_fixed(-fx));
-      atomicAdd(&W.fj[1][l], pair_fixed(-fy));
-      atomicAdd(&W.fj[2][l], pair_fixed(-fz));
-    }
-    wave_lds_sync();
-    unsigned long long g[3];
+        for (int u = 0; u < 8; u++) pg[u] = W.xi[h][8 * g + u];
 #pragma unroll
-    for (int c = 0; c < 3; c++) {
-      g[c] = W.fj[c][lane];
-      W.fj[c][lane] = 0ull;
+        for (int u = 7; u >= 0; u--) {
+          const float dx = pg[u].x - xjx, dy = pg[u].y - xjy, dz = pg[u].z - xjz;
+          const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+          m = (m << 1) | (r2 < d.roff2 ? 1u : 0u);
+        }
+      }
+      // bit t of excl[l] excludes i = (l + t) mod 32: rotated left by l it is bit i
+      m &= ~__builtin_amdgcn_alignbit(excl, excl, (32 - jl) & 31);
+      if (!ok) m = 0u;
+      // dense phase: the wave's pairs packed in lane order
+      const int cnt = __popc(m);
+      const int incl = wave_inclusive_scan(cnt);
+      const int total = __builtin_amdgcn_readlane(incl, 63);
+      int slot = incl - cnt;
+      while (m) {
+        const int i = __ffs(m) - 1;
+        m &= m - 1u;
+        if (slot < 32 * 64) W.pairs[slot] = (unsigned short)(i | (lane << 5));
+        slot++;
+      }
+      wave_lds_sync();
+      for (int q = lane; q < total; q += 64) {
+        const unsigned pr = W.pairs[q];
+        const int i = pr & 31, l = pr >> 5, hl = l >> 5;
+        const float4 pi = W.xi[hl][i], pjl = W.xj[l];
+        const float dx = pi.x - pjl.x, dy = pi.y - pjl.y, dz = pi.z - pjl.z;
+        const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+        float pe;
+        const float fij = ewald_fij(r2, rsqrtf(r2), pi.w * pjl.w, 0.0f, d, pe);
+        if (ENERGY) pot += energy_fixed(pe);
+        const float fx = fij * dx, fy = fij * dy, fz = fij * dz;
+        atomicAdd(&W.fi[hl][0][i], pair_fixed(fx));
+        atomicAdd(&W.fi[hl][1][i], pair_fixed(fy));
+        atomicAdd(&W.fi[hl][2][i], pair_fixed(fz));
+        atomicAdd(&W.fj[0][l], pair_fixed(-fx));
+        atomicAdd(&W.fj[1][l], pair_fixed(-fy));
+        atomicAdd(&W.fj[2][l], pair_fixed(-fz));
+      }
+      wave_lds_sync();
+      unsigned long long g[3];
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        g[c] = W.fj[c][lane];
+        W.fj[c][lane] = 0ull;
+      }
+      if (ok && (g[0] | g[1] | g[2])) {
+        atomicAdd(&force[indj + jl], g[0]);
+        atomicAdd(&force[(size_t)(indj + jl) + stride], g[1]);
+        atomicAdd(&force[(size_t)(indj + jl) + 2 * (size_t)stride], g[2]);
+      }
+      wave_lds_sync();
     }
-    if (ok && (g[0] | g[1] | g[2])) {
-      atomicAdd(&force[indj + jl], g[0]);
-      atomicAdd(&force[(size_t)(indj + jl) + stride], g[1]);
-      atomicAdd(&force[(size_t)(indj + jl) + 2 * (size_t)stride], g[2]);
-    }
-    wave_lds_sync();
+    chunk += wtotal;
   }
   if (cur >= 0) flush_entry<VIRIAL>(W, h, jl, cur_i0, cur_ish, stride, force, s_sf);
   if (ENERGY || VIRIAL) {
-    pot = wave_sum(pot);
-    if (lane == 0) s_pot[wv] = pot;
+    if (ENERGY && pot != 0) atomicAdd(&s_pot, (unsigned long long)pot);
     __syncthreads();
     double* out = partial + (size_t)blockIdx.x * kDirPartial;
     const int tid = threadIdx.x;
     if (ENERGY && tid == 0) {
-      double acc = 0.0;
-      for (int v = 0; v < kDirWaves; v++) acc += s_pot[v];
-      out[0] = acc;
+      out[0] = (double)(long long)s_pot * (1.0 / 4294967296.0);
       out[1] = 0.0;
     }
     if (VIRIAL && tid < 81) out[2 + tid] = (double)(long long)s_sf[tid] * (1.0 / 1099511627776.0);
@@ -370,21 +448,22 @@ hipError_t launch_direct_force(const DirectParams& d, int n_ientry, int n_tile, 
                                unsigned long long* force, bool energy, bool virial, double* partial,
                                hipStream_t s) {
   if (n_ientry <= 0 || n_tile <= 0 || nwave <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_direct_tile_clear, dim3((n_tile + 255) / 256), dim3(256), 0, s, n_tile, tile_entry);
   hipLaunchKernelGGL(k_direct_tile_map, dim3((n_ientry + 3) / 4), dim3(256), 0, s, n_ientry, ientry, n_tile,
                      tile_entry);
   const dim3 g(direct_force_blocks(nwave)), b(kDirWaves * 64);
   if (energy && virial)
-    hipLaunchKernelGGL((k_direct_force<true, true>), g, b, 0, s, d, n_ientry, n_tile, nwave, ientry, tile_entry,
-                       tile_indj, tile_excl, xyzq, stride, force, partial);
+    hipLaunchKernelGGL((k_direct_force<true, true>), g, b, 0, s, d, n_ientry, n_tile, ientry, tile_entry, tile_indj,
+                       tile_excl, xyzq, stride, force, partial);
   else if (energy)
-    hipLaunchKernelGGL((k_direct_force<true, false>), g, b, 0, s, d, n_ientry, n_tile, nwave, ientry, tile_entry,
-                       tile_indj, tile_excl, xyzq, stride, force, partial);
+    hipLaunchKernelGGL((k_direct_force<true, false>), g, b, 0, s, d, n_ientry, n_tile, ientry, tile_entry, tile_indj,
+                       tile_excl, xyzq, stride, force, partial);
   else if (virial)
-    hipLaunchKernelGGL((k_direct_force<false, true>), g, b, 0, s, d, n_ientry, n_tile, nwave, ientry, tile_entry,
-                       tile_indj, tile_excl, xyzq, stride, force, partial);
+    hipLaunchKernelGGL((k_direct_force<false, true>), g, b, 0, s, d, n_ientry, n_tile, ientry, tile_entry, tile_indj,
+                       tile_excl, xyzq, stride, force, partial);
   else
-    hipLaunchKernelGGL((k_direct_force<false, false>), g, b, 0, s, d, n_ientry, n_tile, nwave, ientry, tile_entry,
-                       tile_indj, tile_excl, xyzq, stride, force, partial);
+    hipLaunchKernelGGL((k_direct_force<false, false>), g, b, 0, s, d, n_ientry, n_tile, ientry, tile_entry, tile_indj,
+                       tile_excl, xyzq, stride, force, partial);
   return hipGetLastError();
 }
 
@@ -417,8 +496,8 @@ hipError_t launch_direct_reduce(const double* partial, int nblk, bool energy, bo
 int direct_force_blocks(int nwave) { return (nwave + kDirWaves - 1) / kDirWaves; }
 
 int direct_force_waves(int n_tile, int ncu) {
-  // 16 waves per CU (four workgroups of kDirWaves), at least four j tiles each
-  return std::max(1, std::min(ncu * 16, (n_tile + 3) / 4));
+  // 16 waves per CU (four workgroups of kDirWaves), at most one per chunk
+  return std::max(1, std::min(ncu * 16, (n_tile + kDirChunk - 1) / kDirChunk));
 }
 int direct_14_blocks(int n) { return (n + kDir14Block - 1) / kDir14Block; }
 

@@ -363,6 +363,7 @@ struct DirectParams {
   float e14fac;
 };
 constexpr int kDirWaves = 4;         // waves per non-bonded workgroup
+constexpr int kDirChunk = 8;         // j tiles per chunk of the pair kernel (kDirChunk / 2 per half-wave)
 constexpr int kDir14Block = 256;     // 1-4 entries per workgroup
 constexpr int kDirPartial = 2 + 81;  // doubles per workgroup partial: elec, excl, sforce[27][3]
 hipError_t launch_direct_force(const DirectParams& d, int n_ientry, int n_tile, int nwave,
