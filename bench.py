@@ -325,6 +325,7 @@ def main():
     ev2g = CudaEnergyVirial()
     pme2g = CudaPMERecip(*s2.nfft, s2.order, ev2g)
     pme2g.set_stream(side2)
+    pme2g.set_graph(True)
     f2g = torch.zeros(3 * st2, dtype=torch.float32, device=dev)
 
     def p2gstep():

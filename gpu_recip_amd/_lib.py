@@ -109,6 +109,7 @@ SIGNATURES = {
     "pme_recip_set_stream": [_vp, _vp],
     "pme_recip_set_order": [_vp, _i],
     "pme_recip_set_precision": [_vp, _i],
+    "pme_recip_set_graph": [_vp, _i],
     "pme_recip_get_info": [_vp, C.POINTER(_i), C.POINTER(_i), C.POINTER(_i), C.POINTER(_i)],
     "pme_recip_spread_charge": [_vp, _vp, _i, _pd],
     "pme_recip_r2c_fft": [_vp],

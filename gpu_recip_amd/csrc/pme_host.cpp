@@ -284,7 +284,8 @@ struct pme_recip_s {
   bool timing = false;
   hipEvent_t tev[9] = {};
 
-  // pme_recip_solve replayed as a graph (solve_graph)
+  // pme_recip_solve replayed as a graph (solve_graph), when use_graph is set
+  bool use_graph = false;
   SolveKey gkey{};
   bool gkey_valid = false;
   hipGraphExec_t gexec = nullptr;
@@ -887,13 +888,24 @@ static pme_status solve_graph(pme_recip_t h, const void* d_xyzq, int ncoord, con
   return PME_OK;
 }
 
+extern "C" pme_status pme_recip_set_graph(pme_recip_t h, int on) {
+  if (!h) return fail(PME_ERR_INVALID_ARG, "pme_recip_set_graph: null handle");
+  h->use_graph = on != 0;
+  if (!h->use_graph && h->gexec) {
+    (void)hipGraphExecDestroy(h->gexec);
+    h->gexec = nullptr;
+  }
+  h->gkey_valid = false;
+  return PME_OK;
+}
+
 extern "C" pme_status pme_recip_solve(pme_recip_t h, const void* d_xyzq, int ncoord, const double recip[9],
                                       double kappa, int calc_energy, int calc_virial, int stride, int force_type,
                                       void* d_force) {
   if (!h || !recip) return fail(PME_ERR_INVALID_ARG, "pme_recip_solve: null argument");
   if (!box_ok(recip)) return fail(PME_ERR_INVALID_ARG, "pme_recip_solve: recip has no positive determinant");
   // a scalar-sum reduction still pending would be flushed inside the sequence
-  if (h->timing || h->stream == nullptr || h->pend_solve != 0)
+  if (!h->use_graph || h->timing || h->stream == nullptr || h->pend_solve != 0)
     return solve_stages(h, d_xyzq, ncoord, recip, kappa, calc_energy, calc_virial, stride, force_type, d_force);
   return solve_graph(h, d_xyzq, ncoord, recip, kappa, calc_energy, calc_virial, stride, force_type, d_force);
 }

@@ -116,6 +116,10 @@ class CudaPMERecip:
         """PREC_FP32, PREC_BF16X3 or PREC_FP16X3 for the y/z FFT passes."""
         L.call("pme_recip_set_precision", self.h, int(precision))
 
+    def set_graph(self, on: bool) -> None:
+        """solve() as a hipGraph replay on the handle's own stream (off by default)."""
+        L.call("pme_recip_set_graph", self.h, int(bool(on)))
+
     def get_info(self):
         v = [C.c_int() for _ in range(4)]
         L.call("pme_recip_get_info", self.h, *[C.byref(x) for x in v])

@@ -1,6 +1,7 @@
 """Run one benchmark workload for a fixed number of solves (for profilers).
 
     python -m gpu_recip_amd.workloads production [--solves 20] [--precision fp32|bf16x3|fp16x3]
+                                                 [--side-stream] [--one-shot] [--graph]
     python -m gpu_recip_amd.workloads split      [--solves 20] [--precision ...]
     python -m gpu_recip_amd.workloads batched    [--solves 20] [--tuning brick,tile,wg_per_cu[,chains]]
     python -m gpu_recip_amd.workloads small      [--solves 20]
@@ -19,6 +20,9 @@ direct:     the direct-space Ewald term on the production system (roff 9 A,
             water exclusions as 1-4 exclusion corrections), calc_force +
             calc_14_force with energy and virial per solve
 Inputs are uploaded once; every solve ends with the energy/virial readback.
+The plain workloads run the five stage calls on the null stream; --side-stream
+runs them on a stream of the handle's own, --one-shot through pme_recip_solve,
+--graph through pme_recip_solve replayed as a hipGraph (implies both).
 """
 from __future__ import annotations
 
@@ -33,6 +37,9 @@ def main(argv=None):
     ap.add_argument("workload", choices=["production", "split", "batched", "small", "served", "direct"])
     ap.add_argument("--solves", type=int, default=20)
     ap.add_argument("--precision", choices=sorted(PRECS), default="fp32")
+    ap.add_argument("--side-stream", action="store_true", help="plain: run on a stream of the handle's own")
+    ap.add_argument("--one-shot", action="store_true", help="plain: pme_recip_solve instead of the five calls")
+    ap.add_argument("--graph", action="store_true", help="plain: pme_recip_solve replayed as a hipGraph")
     ap.add_argument("--tuning", default=None,
                     help="batched: brick,tile,wg_per_cu[,chains] to use instead of autotuning (e.g. 24,32,4,2)")
     args = ap.parse_args(argv)
@@ -118,21 +125,40 @@ def main(argv=None):
     ev = CudaEnergyVirial()
     pme = CudaPMERecip(*s.nfft, s.order, ev)
     pme.set_precision(PRECS[args.precision])
+    one_shot = args.one_shot or args.graph
+    stream = torch.cuda.Stream() if (args.side_stream or args.graph) else None
+    if stream is not None:
+        pme.set_stream(stream)
+    pme.set_graph(args.graph)
     x = torch.from_numpy(s.xyzq).to(dev)
     st = force_stride(s.natom)
     f = torch.zeros(3 * st, dtype=torch.float32, device=dev)
+
+    def step():
+        ev.clear(stream)
+        if one_shot:
+            pme.solve(x, s.natom, s.recip, s.kappa, True, True, st, f, L.FORCE_FLOAT)
+        else:
+            pme.spread_charge(x, s.natom, s.recip)
+            pme.r2c_fft()
+            pme.scalar_sum(s.recip, s.kappa, True, True)
+            pme.c2r_fft()
+            pme.gather_force(x, s.natom, s.recip, st, f, L.FORCE_FLOAT)
+        ev.copyToHost(stream)
+        if stream is not None:
+            stream.synchronize()
+        else:
+            torch.cuda.synchronize()
+
+    for _ in range(3):
+        step()
     t0 = time.perf_counter()
     for _ in range(args.solves):
-        ev.clear()
-        pme.spread_charge(x, s.natom, s.recip)
-        pme.r2c_fft()
-        pme.scalar_sum(s.recip, s.kappa, True, True)
-        pme.c2r_fft()
-        pme.gather_force(x, s.natom, s.recip, st, f, L.FORCE_FLOAT)
-        ev.copyToHost()
-        torch.cuda.synchronize()
-    print(f"{s.name} ({args.precision}): {args.solves} solves in {time.perf_counter() - t0:.3f} s, "
-          f"energy {ev.getEnergy('recip'):.6f}", flush=True)
+        step()
+    dt = time.perf_counter() - t0
+    how = ("graph" if args.graph else "one-shot" if one_shot else "five calls") + (", own stream" if stream else "")
+    print(f"{s.name} ({args.precision}, {how}): {args.solves} solves in {dt:.3f} s, {dt / args.solves * 1e6:.1f} us "
+          f"per solve, energy {ev.getEnergy('recip'):.6f}", flush=True)
 
 
 if __name__ == "__main__":

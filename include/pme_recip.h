@@ -136,6 +136,14 @@ pme_status pme_recip_solve(pme_recip_t h, const void* d_xyzq, int ncoord,
                            const double recip[9], double kappa, int calc_energy,
                            int calc_virial, int stride, int force_type, void* d_force);
 
+/* pme_recip_solve as a hipGraph replay (on = 1; default 0).  No counterpart in
+ * the reference.  On a handle with a stream of its own (not the null stream)
+ * the first solve with new arguments runs the stages, the second captures
+ * them and later ones replay the graph; results are identical.  On ROCm 7 the
+ * replay measured slower than the six launches it replaces (117.5 against
+ * 108.3 us per solve on a 96,000-atom, 108x100x96 system), hence off by default. */
+pme_status pme_recip_set_graph(pme_recip_t h, int on);
+
 /* pme_recip_solve that allocates nothing.  No counterpart in the reference:
  * for callers that cannot call hipMalloc (code under HIP graph capture, tight
  * serving loops, harnesses that forbid allocation).  The work buffers of the

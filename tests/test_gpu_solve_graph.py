@@ -1,6 +1,6 @@
-"""pme_recip_solve on a caller's stream runs as a hipGraph: the first call
-with new arguments runs the stages, the second captures them, later calls
-replay the graph.  Every replay must equal the five-call sequence bit for bit
+"""pme_recip_solve with pme_recip_set_graph(1) on a caller's stream runs as a
+hipGraph: the first call with new arguments runs the stages, the second
+captures them, later calls replay the graph.  Every replay must equal the five-call sequence bit for bit
 (forces, energy, virial), also after the coordinates move in place, with
 other work and synchronisation between replays, after the arguments change
 and back, and with the allocation-free solve interleaved."""
@@ -41,6 +41,7 @@ def test_graph_replays_equal_the_stages():
     ev = CudaEnergyVirial()
     pme = CudaPMERecip(*nfft, order, ev)
     pme.set_stream(side)
+    pme.set_graph(True)
     f = torch.zeros(3 * st, dtype=torch.float32, device=dev)
 
     def solve(recip, kap=kappa):
