@@ -28,6 +28,7 @@ from .oracle import (  # noqa: F401
     solve,
     ewald_recip,
     ewald_recip9,
+    direct_tiles,
     direct_ewald,
     pair14,
 )

@@ -98,6 +98,9 @@ def lib():
     L.orc_direct_ewald.restype = C.c_double
     L.orc_pair14.argtypes = [C.c_int, _fp, _dp, C.c_double, C.c_double, C.c_int, C.c_void_p, _dp, _dp]
     L.orc_pair14.restype = C.c_double
+    L.orc_direct_tiles.argtypes = [C.c_int, _fp, _dp, C.c_double, C.c_double, C.c_int, C.c_void_p, C.c_void_p,
+                                   C.c_void_p, _dp, _dp, _dp, C.c_int]
+    L.orc_direct_tiles.restype = C.c_double
     _lib = L
     return L
 
@@ -233,6 +236,22 @@ def direct_ewald(xyzq, box, kappa, roff, excl=None, nthreads=0):
     e = lib().orc_direct_ewald(len(x), x, np.asarray(box, np.float64), float(kappa), float(roff),
                                0 if ex is None else len(ex), None if ex is None else ex.ctypes.data, f, v, nthreads)
     return {"energy": e, "force": f, "virial": v}
+
+
+def direct_tiles(xyzq_sorted, box, kappa, roff, ientry, tile_indj, tile_excl, nthreads=0):
+    """Direct-space sum over a 32-atom tile list (orc_direct_tiles), in the
+    list's atom order.  Returns dict(energy, force (npad,3) = +dE/dr, virial
+    (9,), sforce (27,3))."""
+    x = _xyzq(xyzq_sorted)
+    f = np.zeros((len(x), 3))
+    v = np.zeros(9)
+    sf = np.zeros(81)
+    ie = np.ascontiguousarray(ientry, np.int32).reshape(-1, 4)
+    tj = np.ascontiguousarray(tile_indj, np.int32)
+    te = np.ascontiguousarray(tile_excl, np.uint32).reshape(-1, 32)
+    e = lib().orc_direct_tiles(len(x), x, np.asarray(box, np.float64), float(kappa), float(roff), len(ie),
+                               ie.ctypes.data, tj.ctypes.data, te.ctypes.data, f, v, sf, nthreads)
+    return {"energy": e, "force": f, "virial": v, "sforce": sf.reshape(27, 3)}
 
 
 def pair14(xyzq, box, kappa, fac, pairs):

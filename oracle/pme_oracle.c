@@ -763,3 +763,72 @@ double orc_pair14(int natom, const float* xyzq, const double box[3], double kapp
   }
   return e;
 }
+
+double orc_direct_tiles(int npad, const float* xyzq, const double box[3], double kappa, double roff, int n_ientry,
+                        const int* ientry, const int* tile_indj, const unsigned* tile_excl, double* force,
+                        double* virial9, double* sforce, int nthreads) {
+  const int nt = pick_threads(nthreads);
+  const double roff2 = roff * roff;
+  double e_tot = 0.0, vir[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, sf[81];
+  for (int k = 0; k < 81; k++) sf[k] = 0.0;
+#pragma omp parallel num_threads(nt)
+  {
+    double e_loc = 0.0, v_loc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, sf_loc[81];
+    for (int k = 0; k < 81; k++) sf_loc[k] = 0.0;
+    double* f_loc = force ? (double*)calloc((size_t)3 * npad, sizeof(double)) : NULL;
+#pragma omp for schedule(dynamic, 4)
+    for (int e = 0; e < n_ientry; e++) {
+      const int i0 = ientry[4 * e], ish = ientry[4 * e + 1], t0 = ientry[4 * e + 2], t1 = ientry[4 * e + 3];
+      /* calc_box_shift, gpu_utils.h:312-320 */
+      const double sx = (ish % 3 - 1) * box[0], sy = ((ish / 3) % 3 - 1) * box[1], sz = (ish / 9 - 1) * box[2];
+      double gi[3] = {0.0, 0.0, 0.0};
+      for (int t = t0; t <= t1; t++) {
+        const int j0 = tile_indj[t];
+        for (int l = 0; l < 32; l++) {
+          const unsigned w = tile_excl[32 * (size_t)t + l];
+          const int j = j0 + l;
+          for (int b = 0; b < 32; b++) {
+            if ((w >> b) & 1u) continue;
+            const int i = i0 + (l + b) % 32;
+            if (i < 0 || j < 0 || i >= npad || j >= npad) continue;
+            const double dx = (double)xyzq[4 * i] + sx - (double)xyzq[4 * j];
+            const double dy = (double)xyzq[4 * i + 1] + sy - (double)xyzq[4 * j + 1];
+            const double dz = (double)xyzq[4 * i + 2] + sz - (double)xyzq[4 * j + 2];
+            if (dx * dx + dy * dy + dz * dz >= roff2) continue;
+            const double qq = ORC_CCELEC * (double)xyzq[4 * i + 3] * (double)xyzq[4 * j + 3];
+            double u, g[3];
+            pair_ewald(dx, dy, dz, qq, kappa, 1.0, &u, g);
+            e_loc += u;
+            const double d[3] = {dx, dy, dz};
+            for (int a = 0; a < 3; a++) {
+              gi[a] += g[a];
+              for (int c = 0; c < 3; c++) v_loc[3 * a + c] += d[a] * g[c];
+            }
+            if (f_loc)
+              for (int a = 0; a < 3; a++) {
+                f_loc[3 * i + a] += g[a];
+                f_loc[3 * j + a] -= g[a];
+              }
+          }
+        }
+      }
+      if (ish != 13)
+        for (int a = 0; a < 3; a++) sf_loc[3 * ish + a] += gi[a];
+    }
+#pragma omp critical
+    {
+      e_tot += e_loc;
+      for (int k = 0; k < 9; k++) vir[k] += v_loc[k];
+      for (int k = 0; k < 81; k++) sf[k] += sf_loc[k];
+      if (f_loc)
+        for (int k = 0; k < 3 * npad; k++) force[k] += f_loc[k];
+    }
+    free(f_loc);
+  }
+  if (virial9)
+    for (int k = 0; k < 9; k++) virial9[k] += vir[k];
+  if (sforce)
+    for (int k = 0; k < 81; k++) sforce[k] += sf[k];
+  return e_tot;
+}
+

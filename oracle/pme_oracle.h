@@ -107,6 +107,20 @@ double orc_direct_ewald(int natom, const float* xyzq, const double box[3], doubl
  * Gradient and pair virial ADDED as above.  Returns the energy. */
 double orc_pair14(int natom, const float* xyzq, const double box[3], double kappa, double fac, int npair,
                   const int* list, double* force, double* virial9);
+/* The direct-space sum over a 32-atom tile list, the traversal of
+ * CudaDirectForce_util.h:184-251: for every entry ientry[4e..4e+3] =
+ * (iatomStart, ish, tileStart, tileEnd), every j tile t in [tileStart,
+ * tileEnd] starting at atom tile_indj[t], every j = lane l of the tile and
+ * every bit t' of tile_excl[32 t + l] that is clear, the pair
+ * (i = iatomStart + (l + t') mod 32, j) with the shift of ish added to xi,
+ * when closer than roff: ccelec qi qj erfc(kappa r)/r.  Gradient ADDED to
+ * force (npad*3, may be NULL), pair virial to virial9 (may be NULL); sforce
+ * (27*3, may be NULL) gets, per shift other than 13, the sum of the i-side
+ * gradients of its entries (the shift forces of CudaDirectForce_util.h:321-347).
+ * Returns the energy. */
+double orc_direct_tiles(int npad, const float* xyzq, const double box[3], double kappa, double roff, int n_ientry,
+                        const int* ientry, const int* tile_indj, const unsigned* tile_excl, double* force,
+                        double* virial9, double* sforce, int nthreads);
 
 #ifdef __cplusplus
 }

@@ -158,3 +158,35 @@ def test_direct_setup_checks():
     dr.setup(30.0, 31.0, 32.0, 0.34, 9.0, 8.0, 0.5, 0, 101)
     s = dr.get_setup()
     assert s[:3] == (30.0, 31.0, 32.0) and abs(s[6] - 0.5) < 1e-7 and s[7:] == (0, 101)
+
+
+def test_direct_at_production_size():
+    """The 96,000-atom production system with roff 9 A (135,955 j tiles, so
+    every wave takes several chunks of the list): calc_force against the FP64
+    sum over the same tile list (orc_direct_tiles), energies, forces and the
+    shift forces of the virial."""
+    torch, dev = torch_dev()
+    s = systems.config_production()
+    roff = 9.0
+    tl, _ = nlist.water_box(s.xyzq, s.box, roff)
+    say(f"  {s.natom} atoms, {len(tl.ientry)} i entries, {len(tl.tile_indj)} j tiles")
+    ev = CudaEnergyVirial()
+    dr = CudaPMEDirectForce(ev, "vdw", "elec", "excl")
+    dr.setup(*s.box, s.kappa, roff, roff - 1.0, 1.0)
+    dl = DeviceTileList(tl, dev)
+    stride = force_stride(tl.npad, 8)
+    f = torch.zeros(3 * stride, dtype=torch.int64, device=dev)
+    ev.clear()
+    dr.calc_force(dl.xyzq, dl, True, True, stride, f)
+    ev.copyToHost()
+    torch.cuda.synchronize()
+    fh = f.cpu().numpy()
+    g = np.stack([fh[:tl.npad], fh[stride:stride + tl.npad], fh[2 * stride:2 * stride + tl.npad]], 1) / 2.0 ** 40
+    o = O.direct_tiles(tl.xyzq, s.box, s.kappa, roff, tl.ientry, tl.tile_indj, tl.tile_excl)
+    sf = np.array(ev.getSforce()).reshape(27, 3)
+    e = ev.getEnergy("elec")
+    say(f"  elec {e:.6f} / {o['energy']:.6f}  force rel-RMS {rel_rms(g, o['force']):.2e}  "
+        f"shift forces {rel_fro(sf, o['sforce']):.2e}")
+    assert rel_err(e, o["energy"]) < 1e-5
+    assert rel_rms(g, o["force"]) < 1e-5
+    assert rel_fro(sf, o["sforce"]) < 1e-4

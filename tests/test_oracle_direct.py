@@ -118,3 +118,37 @@ def test_exclusion_correction_removes_the_full_coulomb_pair():
     in14 = O.pair14(x, box, kappa, 1.0, [(0, 1, 13)])
     assert abs(excl["energy"] + in14["energy"] - full["energy"]) <= 1e-9 * abs(full["energy"])
     assert np.allclose(excl["force"] + in14["force"], full["force"], rtol=0, atol=1e-9)
+
+
+def test_tile_list_sum_equals_the_brute_force_sum():
+    """orc_direct_tiles (the reference's traversal of the 32-atom tile list:
+    entries, shifted i tiles, j tiles, exclusion words) equals the
+    brute-force pair sum over all images with the same exclusions, for a list
+    built by gpu_recip_amd.nlist (complete by test_nlist.py)."""
+    from gpu_recip_amd import nlist, systems
+
+    box = (22.0, 23.0, 24.0)
+    x = systems.triplets(300, box, seed=21)
+    x[:, :3] = np.mod(x[:, :3].astype(np.float64), box).astype(np.float32)
+    water = nlist.water_exclusions(len(x))
+    extra = np.array([[0, 10], [33, 70], [5, 200]], np.int64)
+    excl = np.concatenate([water, extra])
+    kappa, roff = 0.34, 9.0
+    tl = nlist.build(x, box, roff, excl_pairs=excl)
+    t = O.direct_tiles(tl.xyzq, box, kappa, roff, tl.ientry, tl.tile_indj, tl.tile_excl)
+    b = O.direct_ewald(x, box, kappa, roff, excl=excl)
+    fb = np.zeros((tl.npad, 3))
+    fb[tl.sorted_index()] = b["force"]
+    assert abs(t["energy"] - b["energy"]) <= 1e-10 * abs(b["energy"])
+    assert np.abs(t["force"] - fb).max() <= 1e-9 * np.abs(fb).max()
+    assert np.allclose(t["virial"], b["virial"], rtol=1e-10, atol=1e-8)
+    # the shift forces reproduce the virial: sum_i x_i (x) g_i + sum_s s (x) sforce_s
+    xs = tl.xyzq[:, :3].astype(np.float64)
+    w = xs.T @ t["force"] + (nlist.SHIFTS * np.asarray(box)).T @ t["sforce"]
+    assert np.allclose(w.reshape(9), b["virial"], rtol=1e-9, atol=1e-7)
+    # dropping one exclusion word's bits changes the sum (the words are read)
+    te = tl.tile_excl.copy()
+    k = int(np.argmax((te != 0xFFFFFFFF).any(1) & (te != 0).any(1)))
+    te[k] = 0
+    t2 = O.direct_tiles(tl.xyzq, box, kappa, roff, tl.ientry, tl.tile_indj, te)
+    assert t2["energy"] != t["energy"]
