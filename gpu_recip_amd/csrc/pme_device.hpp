@@ -10,6 +10,14 @@ constexpr int kStageU = 8;           // global loads kept in flight per thread w
 constexpr double kCcelec = 332.0716; // CudaPMERecip.h:27
 constexpr float kForceScale = 1099511627776.0f;  // 2^40, gpu_utils.h:9
 
+// Plane pitch of the half spectrum in complex elements: the hx * ny values of
+// a z plane (rows of hx = nx/2 + 1, ky outer) rounded up to a multiple of 32,
+// so that every plane starts on a 256-byte boundary and the z pass, which
+// reads aligned runs of each plane, touches whole cache lines only.
+__host__ __device__ __forceinline__ long long spec_pitch(int hx, int ny) {
+  return ((long long)hx * ny + 31) & ~31LL;
+}
+
 // Grid index of one coordinate, CudaPMERecip.cu:512-529.  The reference is
 // built with -fmad=true, which fuses x*recip + 2.0f into one FMA; nothing else
 // in the expression can fuse.  Contraction is switched off here so the

@@ -20,8 +20,14 @@ namespace pme {
 // ---------------------------------------------------------------- x pass ----
 
 // Row tile `tile` (NR rows) of one grid; sm: dynamic LDS (fft_x_lds_bytes).
+// Offset of spectrum row r (= ky + ny kz) in the padded half spectrum.
+__device__ __forceinline__ size_t spec_row(int r, int ny, int hx, long long pz) {
+  const int kz = sdiv(r, small_div(ny));
+  return (size_t)kz * (size_t)pz + (size_t)(r - kz * ny) * hx;
+}
+
 __device__ __forceinline__ void fft_x_r2c_body(const float* __restrict__ grid, float2* __restrict__ cg, int nx,
-                                               int nrows, int NR, const AxisArgs& ax, int tile, float2* sm) {
+                                               int nrows, int ny, int NR, const AxisArgs& ax, int tile, float2* sm) {
   const int tid = threadIdx.x;
   const int nf = ax.rp.n;
   const int LS = NR + 1;
@@ -86,25 +92,25 @@ __device__ __forceinline__ void fft_x_r2c_body(const float* __restrict__ grid, f
       const float2 E = cscale(cadd(zk, zc), 0.5f);
       const float2 D = csub(zk, zc);
       const float2 O = make_float2(0.5f * D.y, -0.5f * D.x);
-      cg[(size_t)(row0 + row) * hx + k] = cadd(E, cmul(tw[k], O));
+      cg[spec_row(row0 + row, ny, hx, spec_pitch(hx, ny)) + k] = cadd(E, cmul(tw[k], O));
     }
   } else {
     for (int e = tid; e < nr * hx; e += kBlock) {
       const int row = sdiv(e, small_div(hx));
       const int k = e - row * hx;
-      cg[(size_t)(row0 + row) * hx + k] = res[k * LS + row];
+      cg[spec_row(row0 + row, ny, hx, spec_pitch(hx, ny)) + k] = res[k * LS + row];
     }
   }
 }
 
 __global__ __launch_bounds__(kBlock) void k_fft_x_r2c(const float* __restrict__ grid, float2* __restrict__ cg, int nx,
-                                                      int nrows, int NR, AxisArgs ax) {
+                                                      int nrows, int ny, int NR, AxisArgs ax) {
   extern __shared__ __align__(16) float2 sm[];
-  fft_x_r2c_body(grid, cg, nx, nrows, NR, ax, blockIdx.x, sm);
+  fft_x_r2c_body(grid, cg, nx, nrows, ny, NR, ax, blockIdx.x, sm);
 }
 
 __device__ __forceinline__ void fft_x_c2r_body(const float2* __restrict__ cg, float* __restrict__ grid, int nx,
-                                               int nrows, int NR, const AxisArgs& ax, int tile, float2* sm) {
+                                               int nrows, int ny, int NR, const AxisArgs& ax, int tile, float2* sm) {
   const int tid = threadIdx.x;
   const int nf = ax.rp.n;
   const int LS = NR + 1;
@@ -129,7 +135,7 @@ __device__ __forceinline__ void fft_x_c2r_body(const float2* __restrict__ cg, fl
         const int k = e - row * h;
         xk[u] = xc[u] = make_float2(0.f, 0.f);
         if (row < nr) {
-          const float2* x = cg + (size_t)(row0 + row) * hx;
+          const float2* x = cg + spec_row(row0 + row, ny, hx, spec_pitch(hx, ny));
           xk[u] = x[k];
           xc[u] = cconj(x[h - k]);
         }
@@ -152,7 +158,7 @@ __device__ __forceinline__ void fft_x_c2r_body(const float2* __restrict__ cg, fl
       const int k = e - row * nx;
       float2 z = make_float2(0.f, 0.f);
       if (row < nr) {
-        const float2* x = cg + (size_t)(row0 + row) * hx;
+        const float2* x = cg + spec_row(row0 + row, ny, hx, spec_pitch(hx, ny));
         z = (k < hx) ? x[k] : cconj(x[nx - k]);
       }
       b0[k * LS + row] = z;
@@ -177,9 +183,9 @@ __device__ __forceinline__ void fft_x_c2r_body(const float2* __restrict__ cg, fl
 }
 
 __global__ __launch_bounds__(kBlock) void k_fft_x_c2r(const float2* __restrict__ cg, float* __restrict__ grid, int nx,
-                                                      int nrows, int NR, AxisArgs ax) {
+                                                      int nrows, int ny, int NR, AxisArgs ax) {
   extern __shared__ __align__(16) float2 sm[];
-  fft_x_c2r_body(cg, grid, nx, nrows, NR, ax, blockIdx.x, sm);
+  fft_x_c2r_body(cg, grid, nx, nrows, ny, NR, ax, blockIdx.x, sm);
 }
 
 // ------------------------------------------------------------- xy planes --
@@ -309,11 +315,12 @@ __device__ __forceinline__ void fft_plane_fwd_body(const float* __restrict__ gri
   }
   __syncthreads();
   float2* r2 = fft_lines<-1, SPLIT>(q, res, twy, axy.rp, ncol, LSY, tid, kPlaneBlock, axy.prec);
-  float2* dst = cg + (size_t)z * ny * hx;
+  const long long pz = spec_pitch(hx, ny);
+  float2* dst = cg + (size_t)z * (size_t)pz;
   for (int e = tid; e < nrow * hx; e += kPlaneBlock) {
     const int r = sdiv(e, small_div(hx));
     const int pl = sdiv(r, small_div(ny));
-    dst[e] = r2[(r - pl * ny) * LSY + pl * hx + (e - r * hx)];
+    dst[(size_t)pl * (size_t)pz + (e - pl * ny * hx)] = r2[(r - pl * ny) * LSY + pl * hx + (e - r * hx)];
   }
 }
 
@@ -335,14 +342,17 @@ __device__ __forceinline__ void fft_plane_inv_body(const float2* __restrict__ cg
   float2* b1 = b0 + bsz;
   for (int i = tid; i < nx; i += kPlaneBlock) twx[i] = axx.tw[i];
   for (int i = tid; i < ny; i += kPlaneBlock) twy[i] = axy.tw[i];
-  const float2* src = cg + (size_t)z * ny * hx;
+  const long long pz = spec_pitch(hx, ny);
+  const float2* src = cg + (size_t)z * (size_t)pz;
   const int ne = nrow * hx;
+  const SmallDiv dpl = small_div(ny * hx);
   for (int e0 = tid; e0 < ne; e0 += kStageU * kPlaneBlock) {
     float2 v[kStageU];
 #pragma unroll
     for (int u = 0; u < kStageU; u++) {
       const int e = e0 + u * kPlaneBlock;
-      v[u] = e < ne ? src[e] : make_float2(0.f, 0.f);
+      const int pl = sdiv(e, dpl);
+      v[u] = e < ne ? src[(size_t)pl * (size_t)pz + (e - pl * ny * hx)] : make_float2(0.f, 0.f);
     }
 #pragma unroll
     for (int u = 0; u < kStageU; u++) {
@@ -564,7 +574,7 @@ __device__ __forceinline__ float2 solve_point_tric(const SolveArgs& s, float2 q,
 // (the off-diagonal mirror weights are those of solve_point; w13 and w23
 // depend on k3 through the zero/Nyquist test of m3).
 __device__ __forceinline__ void solve_lines(const SolveArgs& s, float2* __restrict__ res, int n, int TX, int nl,
-                                            int kx0, int k2, double* acc) {
+                                            int c0, int hxd, double* acc) {
   const int tid = threadIdx.x;
   const SmallDiv dtx = small_div(TX);
   const int j0 = sdiv(tid, dtx);
@@ -572,7 +582,10 @@ __device__ __forceinline__ void solve_lines(const SolveArgs& s, float2* __restri
   const int jstep = sdiv(kBlock, dtx);  // >= 4: column tiles are at most 64 wide
   static_assert(kBlock >= 4 * 64, "solve_lines: every line needs a thread");
   if (j0 >= jstep || l >= nl) return;
-  const int k1 = kx0 + l;
+  // line c0 + l of the z pass: flattened (ky, kx) of the z plane, rows of hxd
+  const int c = c0 + l;
+  const int k2 = sdiv(c, small_div(hxd));
+  const int k1 = c - k2 * hxd;
   const int m1 = k1 >= s.nf1 ? k1 - s.n1 : k1;
   const int m2 = k2 >= s.nf2 ? k2 - s.n2 : k2;
   const float mh1 = s.r1 * (float)m1;
@@ -723,12 +736,13 @@ __device__ __forceinline__ void fold_line_sums(const LineFactors& f, const doubl
 }
 
 // One column tile of the z pass (as fft_cols_body<COL_FWD_SOLVE_INV>) with
-// n = N1 N2: TX lines kx0 .. kx0+TX-1 at ky = ty, point j of line l at
-// base + j pt_stride + l.  sm: twiddles (ntw) and two n*TX buffers.
+// n = N1 N2: TX consecutive lines c0 .. c0+TX-1 of the flattened (ky, kx)
+// index of a z plane (rows of hxd), point j of line l at base + j pt_stride
+// + l.  sm: twiddles (ntw) and two n*TX buffers.
 template <int N1, int N2>
 __device__ __forceinline__ void zsolve_four_step(float2* __restrict__ cg, int hx, int TX, long long pt_stride,
                                                  long long outer_stride, const AxisArgs& ax, const SolveArgs& sa,
-                                                 int tx, int ty, int blk, float2* sm) {
+                                                 int tx, int ty, int blk, float2* sm, int hxd) {
   constexpr int n = N1 * N2;
   __shared__ double sh_red[7 * (kBlock / 64)];
   const int tid = threadIdx.x;
@@ -782,7 +796,8 @@ __device__ __forceinline__ void zsolve_four_step(float2* __restrict__ cg, int hx
     for (int j2 = 0; j2 < N2; j2++) X[j2] = b1[(k1 * N2 + j2) * TX + l];
     DftN<N2, -1>::run(X);  // X[k2]: point k3 = k1 + N1 k2
     if (l < nl) {
-      const LineFactors f = line_factors(sa, kx0 + l, ty);
+      const int ky = sdiv(kx0 + l, small_div(hxd));
+      const LineFactors f = line_factors(sa, kx0 + l - ky * hxd, ky);
       double sums[5] = {0, 0, 0, 0, 0};
 #pragma unroll
       for (int k2 = 0; k2 < N2; k2++) X[k2] = solve_line_point(sa, f, k1 + N1 * k2, X[k2], sums);
@@ -816,10 +831,10 @@ __device__ __forceinline__ void zsolve_four_step(float2* __restrict__ cg, int hx
 __device__ __forceinline__ void zsolve_four_step_dispatch(int code, float2* __restrict__ cg, int hx, int TX,
                                                           long long pt_stride, long long outer_stride,
                                                           const AxisArgs& ax, const SolveArgs& sa, int tx, int ty,
-                                                          int blk, float2* sm) {
+                                                          int blk, float2* sm, int hxd) {
   switch (code) {
 #define PME_F4_CASE(c, a, b) \
-  case c: zsolve_four_step<a, b>(cg, hx, TX, pt_stride, outer_stride, ax, sa, tx, ty, blk, sm); break;
+  case c: zsolve_four_step<a, b>(cg, hx, TX, pt_stride, outer_stride, ax, sa, tx, ty, blk, sm, hxd); break;
     PME_F4_PAIRS(PME_F4_CASE)
 #undef PME_F4_CASE
     default: break;
@@ -829,14 +844,17 @@ __device__ __forceinline__ void zsolve_four_step_dispatch(int code, float2* __re
 
 // ------------------------------------------------------------ y / z passes --
 
-// Column tile (tx, ty) of one grid: lines kx0 = tx*TX ..., outer index ty
-// (ky for the z pass); blk indexes the energy/virial partial of the tile.
+// Column tile (tx, ty) of one grid: lines kx0 = tx*TX ... of the hx lines at
+// outer index ty.  The y pass: hx = nx/2 + 1 columns per (kz = ty).  The z
+// pass: hx = the hxd * ny lines of a z plane, flattened (ky, kx) with rows of
+// hxd = nx/2 + 1, and ty = 0.  blk indexes the energy/virial partial of the
+// tile.
 template <int MODE, bool SPLIT>
 __device__ __forceinline__ void fft_cols_body(float2* __restrict__ cg, int n, int hx, int TX, long long pt_stride,
                                               long long outer_stride, const AxisArgs& ax, const SolveArgs& sa,
-                                              int tx, int ty, int blk, float2* sm) {
+                                              int tx, int ty, int blk, float2* sm, int hxd) {
   if (MODE == COL_FWD_SOLVE_INV && !SPLIT && ax.f4 >= 0) {
-    zsolve_four_step_dispatch(ax.f4, cg, hx, TX, pt_stride, outer_stride, ax, sa, tx, ty, blk, sm);
+    zsolve_four_step_dispatch(ax.f4, cg, hx, TX, pt_stride, outer_stride, ax, sa, tx, ty, blk, sm, hxd);
     return;
   }
   __shared__ double sh_red[7 * (kBlock / 64)];
@@ -872,7 +890,7 @@ __device__ __forceinline__ void fft_cols_body(float2* __restrict__ cg, int n, in
   }
   double acc[7] = {0, 0, 0, 0, 0, 0, 0};
   if (MODE == COL_FWD_SOLVE_INV) {
-    solve_lines(sa, res, n, TX, nl, kx0, ty, acc);
+    solve_lines(sa, res, n, TX, nl, kx0, hxd, acc);
     __syncthreads();
   }
   if (MODE == COL_FWD_SOLVE_INV || MODE == COL_FWD_INV) {
@@ -889,11 +907,11 @@ __device__ __forceinline__ void fft_cols_body(float2* __restrict__ cg, int n, in
 template <int MODE, bool SPLIT>
 __global__ __launch_bounds__(kBlock) void k_fft_cols(float2* __restrict__ cg, int n, int hx, int TX,
                                                      long long pt_stride, long long outer_stride, AxisArgs ax,
-                                                     SolveArgs sa, EvReduce red) {
+                                                     SolveArgs sa, EvReduce red, int hxd) {
   extern __shared__ __align__(16) float2 sm[];
   if ((red.nsolve > 0 || red.nself > 0) && blockIdx.x == 0 && blockIdx.y == 0) ev_reduce_block<kBlock>(red);
   fft_cols_body<MODE, SPLIT>(cg, n, hx, TX, pt_stride, outer_stride, ax, sa, blockIdx.x, blockIdx.y,
-                      blockIdx.x + gridDim.x * blockIdx.y, sm);
+                      blockIdx.x + gridDim.x * blockIdx.y, sm, hxd);
 }
 
 // Stand-alone scalar sum over the whole half spectrum (used when the spectrum
@@ -908,7 +926,8 @@ __global__ __launch_bounds__(kBlock) void k_scalar_sum(float2* __restrict__ cg, 
     const int t = idx / hx;
     const int k2 = t % sa.n2;
     const int k3 = t / sa.n2;
-    cg[idx] = sa.tric ? solve_point_tric(sa, cg[idx], k1, k2, k3, acc) : solve_point(sa, cg[idx], k1, k2, k3, acc);
+    const size_t a = (size_t)k3 * (size_t)spec_pitch(hx, sa.n2) + (size_t)k2 * hx + k1;
+    cg[a] = sa.tric ? solve_point_tric(sa, cg[a], k1, k2, k3, acc) : solve_point(sa, cg[a], k1, k2, k3, acc);
   }
   if (sa.calc_ev) ev_partial(sa, acc, sh_red, blockIdx.x);
 }
@@ -918,17 +937,21 @@ __global__ __launch_bounds__(kBlock) void k_scalar_sum(float2* __restrict__ cg, 
 size_t fft_x_lds_bytes(int nf, int NR, int ntw) { return sizeof(float2) * ((size_t)ntw + 2 * (size_t)nf * (NR + 1)); }
 size_t fft_cols_lds_bytes(int n, int TX) { return sizeof(float2) * ((size_t)n + 2 * (size_t)n * TX); }
 
-hipError_t launch_fft_x_r2c(const float* grid, float2* cg, int nx, int nrows, int NR, const AxisArgs& ax,
+hipError_t launch_fft_x_r2c(const float* grid, float2* cg, int nx, int ny, int nz, int NR, const AxisArgs& ax,
                             hipStream_t s) {
   const size_t lds = fft_x_lds_bytes(ax.rp.n, NR, ax.ntw);
-  hipLaunchKernelGGL(k_fft_x_r2c, dim3((nrows + NR - 1) / NR), dim3(kBlock), lds, s, grid, cg, nx, nrows, NR, ax);
+  const int nrows = ny * nz;
+  hipLaunchKernelGGL(k_fft_x_r2c, dim3((nrows + NR - 1) / NR), dim3(kBlock), lds, s, grid, cg, nx, nrows, ny, NR,
+                     ax);
   return hipGetLastError();
 }
 
-hipError_t launch_fft_x_c2r(const float2* cg, float* grid, int nx, int nrows, int NR, const AxisArgs& ax,
+hipError_t launch_fft_x_c2r(const float2* cg, float* grid, int nx, int ny, int nz, int NR, const AxisArgs& ax,
                             hipStream_t s) {
   const size_t lds = fft_x_lds_bytes(ax.rp.n, NR, ax.ntw);
-  hipLaunchKernelGGL(k_fft_x_c2r, dim3((nrows + NR - 1) / NR), dim3(kBlock), lds, s, cg, grid, nx, nrows, NR, ax);
+  const int nrows = ny * nz;
+  hipLaunchKernelGGL(k_fft_x_c2r, dim3((nrows + NR - 1) / NR), dim3(kBlock), lds, s, cg, grid, nx, nrows, ny, NR,
+                     ax);
   return hipGetLastError();
 }
 
@@ -936,15 +959,15 @@ int cols_blocks(int hx, int TX, int nouter) { return ((hx + TX - 1) / TX) * nout
 
 hipError_t launch_fft_cols(int mode, float2* cg, int n, int hx, int TX, long long pt_stride, int nouter,
                            long long outer_stride, const AxisArgs& ax, const SolveArgs& sa, const EvReduce& red,
-                           hipStream_t s) {
+                           int hxd, hipStream_t s) {
   const size_t lds = fft_cols_lds_bytes(n, TX);
   const dim3 grid((hx + TX - 1) / TX, nouter);
   const bool split = ax.prec != PREC_FP32;
 #define PME_COLS(M)                                                                                         \
   if (split)                                                                                              \
-    hipLaunchKernelGGL((k_fft_cols<M, true>), grid, dim3(kBlock), lds, s, cg, n, hx, TX, pt_stride, outer_stride, ax, sa, red); \
+    hipLaunchKernelGGL((k_fft_cols<M, true>), grid, dim3(kBlock), lds, s, cg, n, hx, TX, pt_stride, outer_stride, ax, sa, red, hxd); \
   else                                                                                                    \
-    hipLaunchKernelGGL((k_fft_cols<M, false>), grid, dim3(kBlock), lds, s, cg, n, hx, TX, pt_stride, outer_stride, ax, sa, red);
+    hipLaunchKernelGGL((k_fft_cols<M, false>), grid, dim3(kBlock), lds, s, cg, n, hx, TX, pt_stride, outer_stride, ax, sa, red, hxd);
   switch (mode) {
     case COL_FWD: PME_COLS(COL_FWD); break;
     case COL_INV: PME_COLS(COL_INV); break;
@@ -976,9 +999,9 @@ __global__ __launch_bounds__(kBlock) void k_batch_fft_x(const SysDesc* __restric
     const SysDesc& d = sys[w.sys];
     const int nrows = d.bin.ny * d.bin.nz;
     if (INV)
-      fft_x_c2r_body(d.cgrid, d.grid, d.bin.nx, nrows, d.NR, d.ax[0], w.idx, sm);
+      fft_x_c2r_body(d.cgrid, d.grid, d.bin.nx, nrows, d.bin.ny, d.NR, d.ax[0], w.idx, sm);
     else
-      fft_x_r2c_body(d.grid, d.cgrid, d.bin.nx, nrows, d.NR, d.ax[0], w.idx, sm);
+      fft_x_r2c_body(d.grid, d.cgrid, d.bin.nx, nrows, d.bin.ny, d.NR, d.ax[0], w.idx, sm);
     __syncthreads();
   }
 }
@@ -1023,12 +1046,15 @@ __global__ __launch_bounds__(kBlock) void k_batch_cols(const SysDesc* __restrict
     const SysDesc& d = sys[w.sys];
     if (reduce && w.idx == 0 && (d.sa.calc_ev || d.calc_self)) batch_ev_reduce<kBlock>(d);
     const int hx = d.bin.nx / 2 + 1;
-    const int n = AXIS == 1 ? d.bin.ny : d.bin.nz;
-    const int TX = AXIS == 1 ? d.TXy : d.TXz;
-    const long long pt = AXIS == 1 ? (long long)hx : (long long)hx * d.bin.ny;
-    const long long outer = AXIS == 1 ? (long long)hx * d.bin.ny : (long long)hx;
-    const int ntx = (hx + TX - 1) / TX;
-    fft_cols_body<MODE, SPLIT>(d.cgrid, n, hx, TX, pt, outer, d.ax[AXIS], d.sa, w.idx % ntx, w.idx / ntx, w.idx, sm);
+    const long long pz = spec_pitch(hx, d.bin.ny);
+    if (AXIS == 1) {  // y: tiles of kx per kz
+      const int ntx = (hx + d.TXy - 1) / d.TXy;
+      fft_cols_body<MODE, SPLIT>(d.cgrid, d.bin.ny, hx, d.TXy, hx, pz, d.ax[1], d.sa, w.idx % ntx, w.idx / ntx,
+                                 w.idx, sm, hx);
+    } else {  // z: tiles of the flattened (ky, kx) of a plane
+      fft_cols_body<MODE, SPLIT>(d.cgrid, d.bin.nz, hx * d.bin.ny, d.TXz, pz, 0, d.ax[2], d.sa, w.idx, 0, w.idx, sm,
+                                 hx);
+    }
     __syncthreads();
   }
 }

@@ -532,7 +532,7 @@ extern "C" pme_status pme_recip_create(int nfftx, int nffty, int nfftz, int orde
     const size_t G = (size_t)h->nx * h->ny * h->nz;
     const size_t Gc = (size_t)(h->nx / 2 + 1) * h->ny * h->nz;
     HIP_TRY(hipMalloc(&h->grid, sizeof(float) * G));
-    HIP_TRY(hipMalloc(&h->cgrid, sizeof(float2) * Gc));
+    HIP_TRY(hipMalloc(&h->cgrid, sizeof(float2) * (size_t)spec_pitch(h->nx / 2 + 1, h->ny) * h->nz));
     for (int a = 0; a < 3; a++) {
       HIP_TRY(hipMalloc(&h->prefac[a], sizeof(float) * n[a]));
       HIP_TRY(hipMalloc(&h->tw[a], sizeof(float2) * n[a]));
@@ -549,7 +549,7 @@ extern "C" pme_status pme_recip_create(int nfftx, int nffty, int nfftz, int orde
     h->plane = fft_plane_lds_bytes(h->nx, h->ny, h->rp[0].n, 1) != 0;
     PME_TRY(upload_prefac(h));
     const int hx = h->nx / 2 + 1;
-    h->partial_cap = std::max(cols_blocks(hx, h->TXz, h->ny), scalar_sum_blocks((int)Gc));
+    h->partial_cap = std::max(cols_blocks(hx * h->ny, h->TXz, 1), scalar_sum_blocks((int)Gc));
     HIP_TRY(hipMalloc(&h->partial, sizeof(double) * 7 * (size_t)h->partial_cap));
     HIP_TRY(hipMalloc(&h->self_partial, sizeof(double) * 1024));
     HIP_TRY(hipEventCreateWithFlags(&h->flush_ev, hipEventDisableTiming));
@@ -671,10 +671,10 @@ extern "C" pme_status pme_recip_r2c_fft(pme_recip_t h) {
     HIP_TRY(launch_fft_plane_fwd(h->grid, h->cgrid, h->nx, h->ny, h->nz, axis_args(h, 0), axis_args(h, 1), h->stream));
     mark(h, 3);
   } else {
-    HIP_TRY(launch_fft_x_r2c(h->grid, h->cgrid, h->nx, h->ny * h->nz, h->NR, axis_args(h, 0), h->stream));
+    HIP_TRY(launch_fft_x_r2c(h->grid, h->cgrid, h->nx, h->ny, h->nz, h->NR, axis_args(h, 0), h->stream));
     mark(h, 3);
-    HIP_TRY(launch_fft_cols(COL_FWD, h->cgrid, h->ny, hx, h->TXy, hx, h->nz, (long long)hx * h->ny, axis_args(h, 1),
-                            dummy, no_reduce(), h->stream));
+    HIP_TRY(launch_fft_cols(COL_FWD, h->cgrid, h->ny, hx, h->TXy, hx, h->nz, spec_pitch(hx, h->ny), axis_args(h, 1),
+                            dummy, no_reduce(), hx, h->stream));
   }
   h->state = ST_XY;
   return PME_OK;
@@ -682,8 +682,8 @@ extern "C" pme_status pme_recip_r2c_fft(pme_recip_t h) {
 
 static pme_status z_pass(pme_recip_t h, int mode, const SolveArgs& sa) {
   const int hx = h->nx / 2 + 1;
-  HIP_TRY(launch_fft_cols(mode, h->cgrid, h->nz, hx, h->TXz, (long long)hx * h->ny, h->ny, hx, axis_args(h, 2), sa,
-                          no_reduce(), h->stream));
+  HIP_TRY(launch_fft_cols(mode, h->cgrid, h->nz, hx * h->ny, h->TXz, spec_pitch(hx, h->ny), 1, 0, axis_args(h, 2), sa,
+                          no_reduce(), hx, h->stream));
   return PME_OK;
 }
 
@@ -702,7 +702,7 @@ extern "C" pme_status pme_recip_scalar_sum(pme_recip_t h, const double recip[9],
   mark(h, 4);
   if (h->state == ST_XY && !sa.tric) {
     PME_TRY(z_pass(h, COL_FWD_SOLVE_INV, sa));
-    if (ev) h->pend_solve = cols_blocks(hx, h->TXz, h->ny);
+    if (ev) h->pend_solve = cols_blocks(hx * h->ny, h->TXz, 1);
     h->state = ST_SOLVED_ZINV;
   } else {
     // the spectrum in full first (a triclinic box: the fused z pass holds
@@ -735,11 +735,11 @@ extern "C" pme_status pme_recip_c2r_fft(pme_recip_t h) {
     h->pend_solve = h->pend_self = 0;
     mark(h, 6);
   } else {
-    HIP_TRY(launch_fft_cols(COL_INV, h->cgrid, h->ny, hx, h->TXy, hx, h->nz, (long long)hx * h->ny, axis_args(h, 1),
-                            dummy, pending_reduce(h), h->stream));
+    HIP_TRY(launch_fft_cols(COL_INV, h->cgrid, h->ny, hx, h->TXy, hx, h->nz, spec_pitch(hx, h->ny), axis_args(h, 1),
+                            dummy, pending_reduce(h), hx, h->stream));
     h->pend_solve = h->pend_self = 0;
     mark(h, 6);
-    HIP_TRY(launch_fft_x_c2r(h->cgrid, h->grid, h->nx, h->ny * h->nz, h->NR, axis_args(h, 0), h->stream));
+    HIP_TRY(launch_fft_x_c2r(h->cgrid, h->grid, h->nx, h->ny, h->nz, h->NR, axis_args(h, 0), h->stream));
   }
   h->state = ST_POTENTIAL;
   return PME_OK;
@@ -956,7 +956,6 @@ static ScratchLayout scratch_layout(pme_recip_t h, int n) {
   L.chunk = cp.chunk;
   L.nchunk = cp.nchunk;
   const size_t G = (size_t)h->nx * h->ny * h->nz;
-  const size_t Gc = (size_t)(h->nx / 2 + 1) * h->ny * h->nz;
   const size_t rs = 8 * (size_t)cp.chunk * cp.nchunk;
   size_t off = 0;
   auto take = [&](size_t bytes) {
@@ -965,7 +964,7 @@ static ScratchLayout scratch_layout(pme_recip_t h, int n) {
     return at;
   };
   L.grid = take(sizeof(float) * G);
-  L.cgrid = take(sizeof(float2) * Gc);
+  L.cgrid = take(sizeof(float2) * (size_t)spec_pitch(h->nx / 2 + 1, h->ny) * h->nz);
   L.seg_off = take(sizeof(int) * (size_t)cp.nchunk * seg_stride(h->nbins));
   L.rec_pos = take(sizeof(float4) * rs);
   L.rec_idx = take(sizeof(int) * rs);
@@ -1049,7 +1048,10 @@ extern "C" pme_status pme_recip_copy_grid(pme_recip_t h, int which, void* d_dst,
     }
     if (h->state != ST_SPECTRUM) return fail(PME_ERR_STATE, "pme_recip_copy_grid: no spectrum");
     if (bytes < Gc * sizeof(float2)) return fail(PME_ERR_INVALID_ARG, "pme_recip_copy_grid: buffer too small");
-    HIP_TRY(hipMemcpyAsync(d_dst, h->cgrid, Gc * sizeof(float2), hipMemcpyDeviceToDevice, h->stream));
+    // the reference's BOX layout, without the plane padding of the pitch
+    const size_t row = sizeof(float2) * (size_t)(h->nx / 2 + 1) * h->ny;
+    HIP_TRY(hipMemcpy2DAsync(d_dst, row, h->cgrid, sizeof(float2) * (size_t)spec_pitch(h->nx / 2 + 1, h->ny), row,
+                             h->nz, hipMemcpyDeviceToDevice, h->stream));
   } else {
     return fail(PME_ERR_INVALID_ARG, "pme_recip_copy_grid: which %d", which);
   }

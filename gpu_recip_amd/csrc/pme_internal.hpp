@@ -238,19 +238,23 @@ hipError_t launch_gather(int order, int force_type, const float4* xyzq, const Bi
                          const int wmax[3], const float* phi, int stride, void* force,
                          hipStream_t s);
 
-hipError_t launch_fft_x_r2c(const float* grid, float2* cg, int nx, int nrows, int NR,
+// The half spectrum holds nz planes of pitch spec_pitch(hx, ny) complex
+// values, each plane ny rows of hx = nx/2 + 1 (kx fastest).
+hipError_t launch_fft_x_r2c(const float* grid, float2* cg, int nx, int ny, int nz, int NR,
                             const AxisArgs& ax, hipStream_t s);
-hipError_t launch_fft_x_c2r(const float2* cg, float* grid, int nx, int nrows, int NR,
+hipError_t launch_fft_x_c2r(const float2* cg, float* grid, int nx, int ny, int nz, int NR,
                             const AxisArgs& ax, hipStream_t s);
-// lines along one axis of the half spectrum: tile of TX consecutive kx, grid
-// (ceil(hx/TX), nouter); element (line l, point j) at base + j*pt_stride + l,
-// base = kx0 + outer*outer_stride.  For the z pass outer is ky (needed by
+// lines along one axis of the half spectrum: tile of TX consecutive lines,
+// grid (ceil(hx/TX), nouter); element (line l, point j) at base + j*pt_stride
+// + l, base = l0 + outer*outer_stride.  The y pass: hx = nx/2 + 1 lines per
+// kz (outer).  The z pass: hx = (nx/2 + 1) ny lines, the flattened (ky, kx)
+// of a plane, nouter = 1, hxd = nx/2 + 1 to recover kx and ky (needed by
 // COL_FWD_SOLVE_INV, which writes per-workgroup energy/virial partials).
 // Workgroup (0,0) also performs a pending energy/virial reduction if red
 // has one.
 hipError_t launch_fft_cols(int mode, float2* cg, int n, int hx, int TX, long long pt_stride,
                            int nouter, long long outer_stride, const AxisArgs& ax,
-                           const SolveArgs& sa, const EvReduce& red, hipStream_t s);
+                           const SolveArgs& sa, const EvReduce& red, int hxd, hipStream_t s);
 int cols_blocks(int hx, int TX, int nouter);
 hipError_t launch_scalar_sum(float2* cg, int hx, const SolveArgs& sa, hipStream_t s);
 int scalar_sum_blocks(int total);

@@ -158,7 +158,13 @@ FftTiles pick_fft_tiles(int nx, int ny, int nz, int nf_x, int tile_target) {
     return tx;
   };
   t.TXy = tile(ny);
-  t.TXz = tile(nz);
+  // z: the lines are the flattened (ky, kx) of a plane, so the tile width
+  // need not divide hx; a power of two keeps every tile on whole, aligned
+  // cache lines of the plane
+  int tz = 1;
+  while (tz * 2 <= tile_target) tz *= 2;
+  while (tz > 1 && fft_cols_lds_bytes(nz, tz) > kLdsBudget) tz /= 2;
+  t.TXz = tz;
   return t;
 }
 

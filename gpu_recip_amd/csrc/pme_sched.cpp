@@ -57,9 +57,9 @@ pme_status plan_system(const pme_system_desc& in, int cap_atoms, const Tuning& t
   p->ft = pick_fft_tiles(n[0], n[1], n[2], p->rp[0].n, t.fft_tile);
   p->plane = fft_plane_lds_bytes(n[0], n[1], p->rp[0].n, 1) != 0 ? 1 : 0;
   const int hx = n[0] / 2 + 1;
-  p->nzblk = cols_blocks(hx, p->ft.TXz, n[1]);
+  p->nzblk = cols_blocks(hx * n[1], p->ft.TXz, 1);
   const size_t G = (size_t)n[0] * n[1] * n[2];
-  const size_t Gc = (size_t)hx * n[1] * n[2];
+  const size_t Gc = (size_t)spec_pitch(hx, n[1]) * n[2];
   const size_t rs = 8 * (size_t)p->cp.chunk * p->cp.nchunk;
   const size_t sz[7] = {sizeof(float) * G,
                         sizeof(float2) * Gc,
@@ -249,7 +249,7 @@ void build_items(const SysDesc* const* descs, const SystemPlan* const* plans, co
           break;
         }
         case BS_FFTZ:
-          cnt = cols_blocks(hx, d.TXz, d.bin.ny);
+          cnt = cols_blocks(hx * d.bin.ny, d.TXz, 1);
           lds = fft_cols_lds_bytes(d.bin.nz, d.TXz);
           break;
       }
