@@ -75,18 +75,18 @@ struct pme_batch_s {
   double* h_ev = nullptr;    // pinned mirror
   int ev_copied = 0;         // slots the solve's own d_ev -> h_ev copy covers
   TableCache tables;
-  // work lists of the current subset: with two chains the systems are split
-  // in two halves of about equal cost whose stage sequences run on `stream`
-  // and `aux` side by side (one chain's kernel tails and launch gaps are
-  // filled by the other's work)
+  // work lists of the current subset: with several chains the systems are
+  // dealt into parts of about equal cost whose stage sequences run on
+  // `stream` and the auxiliary streams side by side (one chain's kernel tails
+  // and launch gaps are filled by the others' work)
   std::vector<int> cur_ids;
   bool lists_valid = false;
   int nchains = kBatchChains;
-  std::vector<Launch> launches;   // chain on `stream`
-  std::vector<Launch> launches2;  // chain on `aux` (empty: one chain)
-  hipStream_t aux = nullptr;
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
-  int* d_ctr = nullptr;  // work-queue counters: 2 per launch, chain 2 after chain 1's kMaxLaunches
+  std::vector<Launch> launches;                        // chain 0, on `stream`
+  std::vector<Launch> launches_aux[kMaxBatchChains - 1];  // chain c on aux[c - 1] (empty: unused)
+  hipStream_t aux[kMaxBatchChains - 1] = {};
+  hipEvent_t ev_fork = nullptr, ev_join[kMaxBatchChains - 1] = {};
+  int* d_ctr = nullptr;  // work-queue counters: 2 per launch, chain c at c * 2 * kMaxLaunches
   WorkItem* d_items = nullptr;
   size_t items_cap = 0;
   // graph of the current subset
@@ -189,13 +189,15 @@ extern "C" pme_status pme_batch_create(size_t hbm_budget_bytes, pme_stream_t str
     }
     b->own_stream = true;
   }
-  if (hipStreamCreateWithFlags(&b->aux, hipStreamNonBlocking) != hipSuccess ||
-      hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&b->ev_join, hipEventDisableTiming) != hipSuccess ||
-      hipMalloc(&b->d_ctr, sizeof(int) * 4 * kMaxLaunches) != hipSuccess ||
-      hipMemset(b->d_ctr, 0, sizeof(int) * 4 * kMaxLaunches) != hipSuccess) {
+  bool ok = hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming) == hipSuccess &&
+            hipMalloc(&b->d_ctr, sizeof(int) * 2 * kMaxLaunches * kMaxBatchChains) == hipSuccess &&
+            hipMemset(b->d_ctr, 0, sizeof(int) * 2 * kMaxLaunches * kMaxBatchChains) == hipSuccess;
+  for (int c = 0; ok && c < kMaxBatchChains - 1; c++)
+    ok = hipStreamCreateWithFlags(&b->aux[c], hipStreamNonBlocking) == hipSuccess &&
+         hipEventCreateWithFlags(&b->ev_join[c], hipEventDisableTiming) == hipSuccess;
+  if (!ok) {
     pme_batch_destroy(b);
-    return set_error(PME_ERR_DEVICE, "pme_batch_create: second stream, events or work-queue counters failed");
+    return set_error(PME_ERR_DEVICE, "pme_batch_create: auxiliary streams, events or work-queue counters failed");
   }
   int dev = 0;
   hipDeviceProp_t prop;
@@ -215,7 +217,8 @@ extern "C" pme_status pme_batch_create(size_t hbm_budget_bytes, pme_stream_t str
 extern "C" pme_status pme_batch_destroy(pme_batch_t b) {
   if (!b) return PME_OK;
   (void)hipStreamSynchronize(b->stream);
-  if (b->aux) (void)hipStreamSynchronize(b->aux);
+  for (hipStream_t a : b->aux)
+    if (a) (void)hipStreamSynchronize(a);
   drop_graph(b);
   for (auto& s : b->slots)
     if (s.used) free_slot_state(b, s);
@@ -230,8 +233,10 @@ extern "C" pme_status pme_batch_destroy(pme_batch_t b) {
   if (b->ev_in) (void)hipEventDestroy(b->ev_in);
   if (b->ev_done) (void)hipEventDestroy(b->ev_done);
   if (b->ev_fork) (void)hipEventDestroy(b->ev_fork);
-  if (b->ev_join) (void)hipEventDestroy(b->ev_join);
-  if (b->aux) (void)hipStreamDestroy(b->aux);
+  for (hipEvent_t e : b->ev_join)
+    if (e) (void)hipEventDestroy(e);
+  for (hipStream_t a : b->aux)
+    if (a) (void)hipStreamDestroy(a);
   if (b->own_stream) (void)hipStreamDestroy(b->stream);
   delete b;
   return PME_OK;
@@ -317,14 +322,14 @@ static pme_status build_lists(pme_batch_t b, const std::vector<int>& ids) {
     const double ca = system_cost(b->slots[x].in), cc = system_cost(b->slots[y].in);
     return ca > cc || (ca == cc && x < y);
   });
-  // two chains (not while timing stages): systems dealt alternately in
-  // decreasing cost, each chain planned for half of the CUs
-  const bool two = b->nchains > 1 && !b->timing && order_ids.size() >= 2;
-  std::vector<int> part[2];
-  for (size_t k = 0; k < order_ids.size(); k++) part[two ? (k & 1) : 0].push_back(order_ids[k]);
+  // several chains (not while timing stages): systems dealt in turn in
+  // decreasing cost, each chain planned for its share of the CUs
+  const int nc = b->timing ? 1 : std::max(1, std::min(b->nchains, (int)order_ids.size()));
+  std::vector<int> part[kMaxBatchChains];
+  for (size_t k = 0; k < order_ids.size(); k++) part[k % nc].push_back(order_ids[k]);
   std::vector<WorkItem> items;
-  b->launches2.clear();
-  for (int c = 0; c < (two ? 2 : 1); c++) {
+  for (auto& l : b->launches_aux) l.clear();
+  for (int c = 0; c < nc; c++) {
     std::vector<const SysDesc*> descs;
     std::vector<const SystemPlan*> plans;
     std::vector<int> orders;
@@ -334,9 +339,8 @@ static pme_status build_lists(pme_batch_t b, const std::vector<int>& ids) {
       orders.push_back(b->slots[id].in.order);
     }
     std::vector<WorkItem> it;
-    std::vector<Launch>& L = c == 0 ? b->launches : b->launches2;
-    build_items(descs.data(), plans.data(), orders.data(), (int)part[c].size(), two ? std::max(1, b->ncu / 2) : b->ncu,
-                it, L);
+    std::vector<Launch>& L = c == 0 ? b->launches : b->launches_aux[c - 1];
+    build_items(descs.data(), plans.data(), orders.data(), (int)part[c].size(), std::max(1, b->ncu / nc), it, L);
     for (Launch& l : L) l.offset += (int)items.size();
     for (WorkItem& w : it) w.sys = part[c][w.sys];
     items.insert(items.end(), it.begin(), it.end());
@@ -359,14 +363,18 @@ static pme_status build_lists(pme_batch_t b, const std::vector<int>& ids) {
 // mirror (part of the captured graph), so reading results needs no further
 // device work.
 static pme_status enqueue(pme_batch_t b) {
-  if (!b->launches2.empty()) {  // fork the second chain onto aux, join before the copy
+  if (!b->launches_aux[0].empty()) {  // fork the other chains onto aux, join before the copy
     BHIP(hipEventRecord(b->ev_fork, b->stream));
-    BHIP(hipStreamWaitEvent(b->aux, b->ev_fork, 0));
+    for (int c = 1; c < kMaxBatchChains; c++)
+      if (!b->launches_aux[c - 1].empty()) BHIP(hipStreamWaitEvent(b->aux[c - 1], b->ev_fork, 0));
     BTRY(enqueue_stages(b->launches, b->d_sys, b->d_items, b->d_ctr, b->tune, b->ncu, b->stream, nullptr));
-    BTRY(enqueue_stages(b->launches2, b->d_sys, b->d_items, b->d_ctr + 2 * kMaxLaunches, b->tune, b->ncu, b->aux,
-                        nullptr));
-    BHIP(hipEventRecord(b->ev_join, b->aux));
-    BHIP(hipStreamWaitEvent(b->stream, b->ev_join, 0));
+    for (int c = 1; c < kMaxBatchChains; c++) {
+      if (b->launches_aux[c - 1].empty()) continue;
+      BTRY(enqueue_stages(b->launches_aux[c - 1], b->d_sys, b->d_items, b->d_ctr + 2 * kMaxLaunches * c, b->tune,
+                          b->ncu, b->aux[c - 1], nullptr));
+      BHIP(hipEventRecord(b->ev_join[c - 1], b->aux[c - 1]));
+      BHIP(hipStreamWaitEvent(b->stream, b->ev_join[c - 1], 0));
+    }
   } else {
     BTRY(enqueue_stages(b->launches, b->d_sys, b->d_items, b->d_ctr, b->tune, b->ncu, b->stream,
                         b->timing ? b->tev : nullptr));
@@ -565,7 +573,8 @@ extern "C" pme_status pme_batch_set_tuning(pme_batch_t b, int brick_target, int 
 
 extern "C" pme_status pme_batch_set_chains(pme_batch_t b, int nchains) {
   if (!b) return set_error(PME_ERR_INVALID_ARG, "pme_batch_set_chains: null batch");
-  if (nchains != 1 && nchains != 2) return set_error(PME_ERR_INVALID_ARG, "pme_batch_set_chains: %d", nchains);
+  if (nchains < 1 || nchains > kMaxBatchChains)
+    return set_error(PME_ERR_INVALID_ARG, "pme_batch_set_chains: %d (1 to %d)", nchains, kMaxBatchChains);
   if (nchains != b->nchains) {
     BHIP(hipStreamSynchronize(b->stream));
     b->nchains = nchains;

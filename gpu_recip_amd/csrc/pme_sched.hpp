@@ -30,7 +30,8 @@ struct SystemPlan {
   size_t total;         // bytes of the region
 };
 
-constexpr int kBatchChains = 2;  // default stage chains of a batched solve (1 or 2)
+constexpr int kBatchChains = 2;     // default stage chains of a batched solve
+constexpr int kMaxBatchChains = 4;  // most stage chains (streams) of a batched solve
 
 struct Tuning {
   int brick_target = kDefaultBrick;

@@ -291,7 +291,7 @@ class PmeBatch:
         return tuple(x.value for x in v)
 
     def set_chains(self, nchains: int) -> None:
-        """1 or 2 stage chains per solve (pme_batch_set_chains)."""
+        """1 to 4 stage chains per solve (pme_batch_set_chains)."""
         L.call("pme_batch_set_chains", self.h, int(nchains))
 
     def get_chains(self) -> int:

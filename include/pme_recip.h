@@ -241,11 +241,12 @@ pme_status pme_batch_set_graph(pme_batch_t b, int enable);
  * workgroup per work item).  Re-plans every resident system. */
 pme_status pme_batch_set_tuning(pme_batch_t b, int brick_target, int fft_tile, int wg_per_cu);
 pme_status pme_batch_get_tuning(pme_batch_t b, int* brick_target, int* fft_tile, int* wg_per_cu);
-/* Stage chains of a solve: 2 (default) splits the systems into two halves of
- * about equal cost whose stage sequences run on two streams side by side
- * (joined before the energy/virial copy), so one half's kernel tails and
- * launch gaps are filled by the other's work; 1 runs one sequence.  Results
- * do not depend on it.  Solves with stage timing on run one chain. */
+/* Stage chains of a solve, 1 to 4 (default 2): the systems are dealt in turn,
+ * in decreasing cost, into that many parts whose stage sequences run on as
+ * many streams side by side (joined before the energy/virial copy), so one
+ * part's kernel tails and launch gaps are filled by the others' work; 1 runs
+ * one sequence.  Results do not depend on it.  Solves with stage timing on
+ * run one chain. */
 pme_status pme_batch_set_chains(pme_batch_t b, int nchains);
 pme_status pme_batch_get_chains(pme_batch_t b, int* nchains);
 /* Autotuner: time the current resident set over a grid of tuning choices

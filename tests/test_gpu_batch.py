@@ -194,9 +194,10 @@ def test_batch_self_energy():
 
 @pytest.mark.parametrize("wg_per_cu", [0, 4])
 def test_two_chains_match_one(wg_per_cu):
-    """A solve split into two stage chains on two streams gives the forces,
-    energies, virials and self energies of the one-chain solve bit for bit,
-    graph replays and eager solves alike (the same kernels per system)."""
+    """A solve split into two, three or four stage chains on as many streams
+    gives the forces, energies, virials and self energies of the one-chain
+    solve bit for bit, graph replays and eager solves alike (the same kernels
+    per system)."""
     torch, dev = torch_dev()
     sl = systems.config_batched(9)
     b = PmeBatch(8 << 30)
@@ -204,7 +205,7 @@ def test_two_chains_match_one(wg_per_cu):
     ids = [r[0] for r in reg]
     b.set_tuning(24, 32, wg_per_cu)
     out = {}
-    for chains in (1, 2):
+    for chains in (1, 2, 3, 4):
         b.set_chains(chains)
         assert b.get_chains() == chains
         for graph in (True, False):
@@ -221,7 +222,9 @@ def test_two_chains_match_one(wg_per_cu):
         assert all(np.array_equal(a, c) for a, c in zip(fs, ref[0])), key
         assert np.array_equal(e, ref[1]) and np.array_equal(v, ref[2]) and np.array_equal(es, ref[3]), key
     with pytest.raises(L.PmeError):
-        b.set_chains(3)
+        b.set_chains(5)
+    with pytest.raises(L.PmeError):
+        b.set_chains(0)
 
 
 def test_batched_config_vs_oracle():
