@@ -754,13 +754,13 @@ __device__ __forceinline__ void zsolve_four_step(float2* __restrict__ cg, int hx
   const int kx0 = tx * TX;
   const int nl = min(TX, hx - kx0);
   const size_t base = (size_t)kx0 + (size_t)ty * (size_t)outer_stride;
-  const SmallDiv dtx = small_div(TX);
+  const int tsh = __builtin_ctz(TX);  // TX is a power of two (pick_fft_tiles)
   for (int e0 = tid; e0 < n * TX; e0 += kStageU * kBlock) {
     float2 v[kStageU];
 #pragma unroll
     for (int u = 0; u < kStageU; u++) {
       const int e = e0 + u * kBlock;
-      const int j = sdiv(e, dtx);
+      const int j = (e >> tsh);
       const int l = e - j * TX;
       v[u] = (e < n * TX && l < nl) ? cg[base + (size_t)j * pt_stride + l] : make_float2(0.f, 0.f);
     }
@@ -773,7 +773,7 @@ __device__ __forceinline__ void zsolve_four_step(float2* __restrict__ cg, int hx
   __syncthreads();
   // forward, first half: unit (l, j2)
   for (int u = tid; u < N2 * TX; u += kBlock) {
-    const int j2 = sdiv(u, dtx);
+    const int j2 = (u >> tsh);
     const int l = u - j2 * TX;
     float2 x[N1];
 #pragma unroll
@@ -789,7 +789,7 @@ __device__ __forceinline__ void zsolve_four_step(float2* __restrict__ cg, int hx
   // forward second half, influence function, inverse first half: unit (l, k1)
   double acc[7] = {0, 0, 0, 0, 0, 0, 0};
   for (int u = tid; u < N1 * TX; u += kBlock) {
-    const int k1 = sdiv(u, dtx);
+    const int k1 = (u >> tsh);
     const int l = u - k1 * TX;
     float2 X[N2];
 #pragma unroll
@@ -814,7 +814,7 @@ __device__ __forceinline__ void zsolve_four_step(float2* __restrict__ cg, int hx
   __syncthreads();
   // inverse second half: unit (l, j2), then the store
   for (int u = tid; u < N2 * TX; u += kBlock) {
-    const int j2 = sdiv(u, dtx);
+    const int j2 = (u >> tsh);
     const int l = u - j2 * TX;
     float2 z[N1];
 #pragma unroll
@@ -963,6 +963,8 @@ hipError_t launch_fft_cols(int mode, float2* cg, int n, int hx, int TX, long lon
   const size_t lds = fft_cols_lds_bytes(n, TX);
   const dim3 grid((hx + TX - 1) / TX, nouter);
   const bool split = ax.prec != PREC_FP32;
+  // the fused z pass indexes its tile with shifts
+  if (mode == COL_FWD_SOLVE_INV && (TX <= 0 || (TX & (TX - 1)) != 0)) return hipErrorInvalidValue;
 #define PME_COLS(M)                                                                                         \
   if (split)                                                                                              \
     hipLaunchKernelGGL((k_fft_cols<M, true>), grid, dim3(kBlock), lds, s, cg, n, hx, TX, pt_stride, outer_stride, ax, sa, red, hxd); \

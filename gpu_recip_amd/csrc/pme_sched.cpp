@@ -57,6 +57,8 @@ pme_status plan_system(const pme_system_desc& in, int cap_atoms, const Tuning& t
   p->ft = pick_fft_tiles(n[0], n[1], n[2], p->rp[0].n, t.fft_tile);
   p->plane = fft_plane_lds_bytes(n[0], n[1], p->rp[0].n, 1) != 0 ? 1 : 0;
   const int hx = n[0] / 2 + 1;
+  if (p->ft.TXz <= 0 || (p->ft.TXz & (p->ft.TXz - 1)) != 0)  // the z pass indexes its tiles with shifts
+    return set_error(PME_ERR_UNSUPPORTED, "system: z tile %d is not a power of two", p->ft.TXz);
   p->nzblk = cols_blocks(hx * n[1], p->ft.TXz, 1);
   const size_t G = (size_t)n[0] * n[1] * n[2];
   const size_t Gc = (size_t)spec_pitch(hx, n[1]) * n[2];
