@@ -231,6 +231,23 @@ int plane_group_host(int nx, int ny, int nf, int maxnp) { return plane_group(nx,
 
 // planes z .. z+np-1 of the real grid -> the same planes of the half spectrum
 // (SPLIT: the y pass may use the split-precision matrix-core stage)
+// Thread map of a row-major block of rows of w elements (w <= NT) for loops
+// without a division per element: thread t takes column t mod w of rows
+// t / w, t / w + step, ... with step = NT / w (threads past step * w idle).
+struct ColMap {
+  int c, r0, step;
+  bool on;
+};
+template <int NT>
+__device__ __forceinline__ ColMap col_map(int w) {
+  ColMap m;
+  m.step = NT / w;
+  m.c = (int)threadIdx.x % w;
+  m.r0 = (int)threadIdx.x / w;
+  m.on = m.r0 < m.step;
+  return m;
+}
+
 template <bool SPLIT>
 __device__ __forceinline__ void fft_plane_fwd_body(const float* __restrict__ grid, float2* __restrict__ cg, int nx,
                                                    int ny, const AxisArgs& axx, const AxisArgs& axy, int z, int np,
@@ -252,52 +269,49 @@ __device__ __forceinline__ void fft_plane_fwd_body(const float* __restrict__ gri
   const size_t plane = (size_t)z * nx * ny;
   if (even) {
     const float2* src = reinterpret_cast<const float2*>(grid + plane);
-    const int ne = nf * nrow;
-    for (int e0 = tid; e0 < ne; e0 += kStageU * kPlaneBlock) {
-      float2 v[kStageU];
+    const ColMap m = col_map<kPlaneBlock>(nf);
+    if (m.on)
+      for (int rb = m.r0; rb < nrow; rb += kStageU * m.step) {
+        float2 v[kStageU];
 #pragma unroll
-      for (int u = 0; u < kStageU; u++) {
-        const int e = e0 + u * kPlaneBlock;
-        v[u] = e < ne ? src[e] : make_float2(0.f, 0.f);
-      }
+        for (int u = 0; u < kStageU; u++) {
+          const int r = rb + u * m.step;
+          v[u] = r < nrow ? src[(size_t)r * nf + m.c] : make_float2(0.f, 0.f);
+        }
 #pragma unroll
-      for (int u = 0; u < kStageU; u++) {
-        const int e = e0 + u * kPlaneBlock;
-        if (e < ne) {
-          const int y = sdiv(e, small_div(nf));
-          b0[(e - y * nf) * LSX + y] = v[u];
+        for (int u = 0; u < kStageU; u++) {
+          const int r = rb + u * m.step;
+          if (r < nrow) b0[m.c * LSX + r] = v[u];
         }
       }
-    }
   } else {
     const float* src = grid + plane;
-    const int ne = nx * nrow;
-    for (int e0 = tid; e0 < ne; e0 += kStageU * kPlaneBlock) {
-      float v[kStageU];
+    const ColMap m = col_map<kPlaneBlock>(nx);
+    if (m.on)
+      for (int rb = m.r0; rb < nrow; rb += kStageU * m.step) {
+        float v[kStageU];
 #pragma unroll
-      for (int u = 0; u < kStageU; u++) {
-        const int e = e0 + u * kPlaneBlock;
-        v[u] = e < ne ? src[e] : 0.f;
-      }
+        for (int u = 0; u < kStageU; u++) {
+          const int r = rb + u * m.step;
+          v[u] = r < nrow ? src[(size_t)r * nx + m.c] : 0.f;
+        }
 #pragma unroll
-      for (int u = 0; u < kStageU; u++) {
-        const int e = e0 + u * kPlaneBlock;
-        if (e < ne) {
-          const int y = sdiv(e, small_div(nx));
-          b0[(e - y * nx) * LSX + y] = make_float2(v[u], 0.f);
+        for (int u = 0; u < kStageU; u++) {
+          const int r = rb + u * m.step;
+          if (r < nrow) b0[m.c * LSX + r] = make_float2(v[u], 0.f);
         }
       }
-    }
   }
   __syncthreads();
   float2* res = fft_lines<-1>(b0, b1, twx, axx.rp, nrow, LSX, tid, kPlaneBlock);
   float2* q = res == b0 ? b1 : b0;
   // R2C post-processing (even nx) and transpose to columns:
   // X[k] = E[k] + w^k O[k], E = (Z[k] + Z*[h-k])/2, O = (Z[k] - Z*[h-k])/(2i)
-  for (int e = tid; e < nrow * hx; e += kPlaneBlock) {
-    const int r = sdiv(e, small_div(hx));  // x line (row of the plane group)
-    const int k = e - r * hx;
-    const int pl = sdiv(r, small_div(ny));
+  const SmallDiv dny = small_div(ny);
+  const ColMap mh = col_map<kPlaneBlock>(hx);
+  for (int r = mh.r0; mh.on && r < nrow; r += mh.step) {  // x line r (row of the plane group), point k
+    const int k = mh.c;
+    const int pl = sdiv(r, dny);
     const int y = r - pl * ny;
     float2 X;
     if (even) {
@@ -317,10 +331,10 @@ __device__ __forceinline__ void fft_plane_fwd_body(const float* __restrict__ gri
   float2* r2 = fft_lines<-1, SPLIT>(q, res, twy, axy.rp, ncol, LSY, tid, kPlaneBlock, axy.prec);
   const long long pz = spec_pitch(hx, ny);
   float2* dst = cg + (size_t)z * (size_t)pz;
-  for (int e = tid; e < nrow * hx; e += kPlaneBlock) {
-    const int r = sdiv(e, small_div(hx));
-    const int pl = sdiv(r, small_div(ny));
-    dst[(size_t)pl * (size_t)pz + (e - pl * ny * hx)] = r2[(r - pl * ny) * LSY + pl * hx + (e - r * hx)];
+  for (int r = mh.r0; mh.on && r < nrow; r += mh.step) {
+    const int pl = sdiv(r, dny);
+    const int y = r - pl * ny;
+    dst[(size_t)pl * (size_t)pz + y * hx + mh.c] = r2[y * LSY + pl * hx + mh.c];
   }
 }
 
@@ -344,26 +358,24 @@ __device__ __forceinline__ void fft_plane_inv_body(const float2* __restrict__ cg
   for (int i = tid; i < ny; i += kPlaneBlock) twy[i] = axy.tw[i];
   const long long pz = spec_pitch(hx, ny);
   const float2* src = cg + (size_t)z * (size_t)pz;
-  const int ne = nrow * hx;
-  const SmallDiv dpl = small_div(ny * hx);
-  for (int e0 = tid; e0 < ne; e0 += kStageU * kPlaneBlock) {
-    float2 v[kStageU];
+  const SmallDiv dny = small_div(ny);
+  const ColMap mh = col_map<kPlaneBlock>(hx);
+  if (mh.on)
+    for (int rb = mh.r0; rb < nrow; rb += kStageU * mh.step) {
+      float2 v[kStageU];
+      int at[kStageU];  // LDS column offset of row r: y * LSY + pl * hx
 #pragma unroll
-    for (int u = 0; u < kStageU; u++) {
-      const int e = e0 + u * kPlaneBlock;
-      const int pl = sdiv(e, dpl);
-      v[u] = e < ne ? src[(size_t)pl * (size_t)pz + (e - pl * ny * hx)] : make_float2(0.f, 0.f);
-    }
-#pragma unroll
-    for (int u = 0; u < kStageU; u++) {
-      const int e = e0 + u * kPlaneBlock;
-      if (e < ne) {
-        const int r = sdiv(e, small_div(hx));
-        const int pl = sdiv(r, small_div(ny));
-        b0[(r - pl * ny) * LSY + pl * hx + (e - r * hx)] = v[u];
+      for (int u = 0; u < kStageU; u++) {
+        const int r = rb + u * mh.step;
+        const int pl = sdiv(r, dny);
+        const int y = r - pl * ny;
+        at[u] = y * LSY + pl * hx;
+        v[u] = r < nrow ? src[(size_t)pl * (size_t)pz + y * hx + mh.c] : make_float2(0.f, 0.f);
       }
+#pragma unroll
+      for (int u = 0; u < kStageU; u++)
+        if (rb + u * mh.step < nrow) b0[at[u] + mh.c] = v[u];
     }
-  }
   __syncthreads();
   float2* r = fft_lines<+1, SPLIT>(b0, b1, twy, axy.rp, ncol, LSY, tid, kPlaneBlock, axy.prec);
   float2* p = r == b0 ? b1 : b0;
@@ -371,10 +383,10 @@ __device__ __forceinline__ void fft_plane_inv_body(const float2* __restrict__ cg
   if (even) {
     // Z[k] = (X[k] + X*[h-k]) + i w^{-k} (X[k] - X*[h-k]),  k < h
     const int h = nf;
-    for (int e = tid; e < nrow * h; e += kPlaneBlock) {
-      const int row = sdiv(e, small_div(h));
-      const int k = e - row * h;
-      const int pl = sdiv(row, small_div(ny));
+    const ColMap m = col_map<kPlaneBlock>(h);
+    for (int row = m.r0; m.on && row < nrow; row += m.step) {
+      const int k = m.c;
+      const int pl = sdiv(row, dny);
       const float2* col = r + (row - pl * ny) * LSY + pl * hx;
       const float2 xk = col[k];
       const float2 xc = cconj(col[h - k]);
@@ -384,10 +396,10 @@ __device__ __forceinline__ void fft_plane_inv_body(const float2* __restrict__ cg
     }
   } else {
     // the full Hermitian spectrum of each row
-    for (int e = tid; e < nrow * nx; e += kPlaneBlock) {
-      const int row = sdiv(e, small_div(nx));
-      const int k = e - row * nx;
-      const int pl = sdiv(row, small_div(ny));
+    const ColMap m = col_map<kPlaneBlock>(nx);
+    for (int row = m.r0; m.on && row < nrow; row += m.step) {
+      const int k = m.c;
+      const int pl = sdiv(row, dny);
       const float2* col = r + (row - pl * ny) * LSY + pl * hx;
       p[k * LSX + row] = k < hx ? col[k] : cconj(col[nx - k]);
     }
@@ -397,16 +409,12 @@ __device__ __forceinline__ void fft_plane_inv_body(const float2* __restrict__ cg
   const size_t plane = (size_t)z * nx * ny;
   if (even) {
     float2* dst = reinterpret_cast<float2*>(grid + plane);
-    for (int e = tid; e < nrow * nf; e += kPlaneBlock) {
-      const int y = sdiv(e, small_div(nf));
-      dst[e] = s[(e - y * nf) * LSX + y];
-    }
+    const ColMap m = col_map<kPlaneBlock>(nf);
+    for (int row = m.r0; m.on && row < nrow; row += m.step) dst[(size_t)row * nf + m.c] = s[m.c * LSX + row];
   } else {
     float* dst = grid + plane;
-    for (int e = tid; e < nrow * nx; e += kPlaneBlock) {
-      const int y = sdiv(e, small_div(nx));
-      dst[e] = s[(e - y * nx) * LSX + y].x;
-    }
+    const ColMap m = col_map<kPlaneBlock>(nx);
+    for (int row = m.r0; m.on && row < nrow; row += m.step) dst[(size_t)row * nx + m.c] = s[m.c * LSX + row].x;
   }
 }
 
