@@ -84,11 +84,18 @@ def measured_traffic(kernel, workload="batched"):
     path = os.path.join(ROOT, "profiles", "hbm_traffic.md")
     if not os.path.exists(path):
         return None
+    col = None  # the "HBM bytes (calibrated)" column, found from the table header
     for line in open(path):
+        if not line.lstrip().startswith("|"):
+            continue
         cells = [c.strip().strip("`") for c in line.strip().strip("|").split("|")]
-        if len(cells) == 5 and cells[0] == kernel and cells[1] == workload:
+        if col is None:
+            if "HBM bytes (calibrated)" in cells:
+                col = cells.index("HBM bytes (calibrated)")
+            continue
+        if len(cells) > col and cells[0] == kernel and cells[1] == workload:
             try:
-                return int(float(cells[4]))
+                return int(float(cells[col].replace(",", "")))
             except ValueError:
                 return None
     return None
