@@ -70,12 +70,20 @@ def stage_bytes(s):
     }
 
 
-# kernel behind each timing slot of the batched solve (order-4 systems whose
-# xy planes fit one workgroup, as in the 64-system set)
-BATCH_KERNELS = {"bin": "pme::k_batch_bin", "spread": "void pme::k_batch_spread<4>",
-                 "fft_fwd_x_or_xy": "void pme::k_batch_plane<0, false>",
-                 "fft_z_solve": "void pme::k_batch_cols<2, 2, false>",
-                 "fft_inv_y_or_xy": "void pme::k_batch_plane<1, false>", "gather": "void pme::k_batch_gather<4>"}
+# kernels behind each timing slot of the batched solve (order-4 systems whose
+# xy planes fit one workgroup, as in the 64-system set; the z pass runs one
+# launch per four-step register class present: N2 <= 10, and 75 = 5 x 15)
+BATCH_KERNELS = {"bin": ("pme::k_batch_bin",), "spread": ("void pme::k_batch_spread<4>",),
+                 "fft_fwd_x_or_xy": ("void pme::k_batch_plane<0, false>",),
+                 "fft_z_solve": ("void pme::k_batch_zsolve<0>", "void pme::k_batch_zsolve<1>"),
+                 "fft_inv_y_or_xy": ("void pme::k_batch_plane<1, false>",), "gather": ("void pme::k_batch_gather<4>",)}
+
+
+def stage_traffic(kernels, workload="batched"):
+    """HBM bytes of one timing slot: the sum over its kernels' launches, or
+    None when one of them has no measurement."""
+    vals = [measured_traffic(k, workload) for k in kernels]
+    return None if not vals or any(v is None for v in vals) else sum(vals)
 
 
 def measured_traffic(kernel, workload="batched"):
@@ -604,7 +612,7 @@ def main():
         "solve_algorithmic_bytes": bytes_b,
         "solve_roofline_frac": bytes_b / (ms_b * 1e-3) / (HBM_PEAK_GBS * 1e9),
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(BATCH_KERNELS.get(kname, "")),
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": stage_traffic(BATCH_KERNELS.get(kname, ())),
                      "traffic_source": "profiles/hbm_traffic.md (rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-calibrated)",
                      "algorithmic_bytes_per_launch": sb[kname], "avg_launch_ms": float(stage_ms[k])},
         "served": served,

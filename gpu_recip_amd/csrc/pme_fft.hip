@@ -680,6 +680,21 @@ int four_step_code(int n) {
   return -1;
 }
 
+// Register class of a factor pair.  A kernel that dispatches over several
+// pairs is allocated the registers of their union, which grows well beyond
+// the largest single pair (all 31 pairs and the Stockham stages: 157 VGPRs,
+// three waves per SIMD; one pair: 72 to 118).  The fused z pass therefore
+// has one kernel per class: class 0 holds the pairs with N2 <= 10 (114 VGPRs,
+// four waves per SIMD), class 1 the rest.
+constexpr int f4_class_of(int n2) { return n2 <= 10 ? 0 : 1; }
+int four_step_class(int code) {
+#define PME_F4_CLS(c, a, b) \
+  if (code == (c)) return f4_class_of(b);
+  PME_F4_PAIRS(PME_F4_CLS)
+#undef PME_F4_CLS
+  return -1;
+}
+
 // eterm of one point of a z line; (k1, k2) factors precomputed (as in
 // solve_lines, so the value is the same)
 struct LineFactors {
@@ -746,62 +761,59 @@ __device__ __forceinline__ void fold_line_sums(const LineFactors& f, const doubl
 // One column tile of the z pass (as fft_cols_body<COL_FWD_SOLVE_INV>) with
 // n = N1 N2: TX consecutive lines c0 .. c0+TX-1 of the flattened (ky, kx)
 // index of a z plane (rows of hxd), point j of line l at base + j pt_stride
-// + l.  sm: twiddles (ntw) and two n*TX buffers.
+// + l.  sm: twiddles (ntw) and ONE n*TX buffer b: the first half reads its
+// columns straight from HBM into registers and writes b at (k1 N2 + j2); the
+// middle pass reads and rewrites the same N2 points of b per unit (in place);
+// the last half reads b and stores straight to HBM.  Lanes run along l, so
+// every HBM access of a wave is two runs of TX consecutive float2.
 template <int N1, int N2>
 __device__ __forceinline__ void zsolve_four_step(float2* __restrict__ cg, int hx, int TX, long long pt_stride,
                                                  long long outer_stride, const AxisArgs& ax, const SolveArgs& sa,
                                                  int tx, int ty, int blk, float2* sm, int hxd) {
-  constexpr int n = N1 * N2;
   __shared__ double sh_red[7 * (kBlock / 64)];
   const int tid = threadIdx.x;
   float2* tw = sm;
-  float2* b0 = sm + ax.ntw;
-  float2* b1 = b0 + n * TX;
+  float2* b = sm + ax.ntw;
   for (int i = tid; i < ax.ntw; i += kBlock) tw[i] = ax.tw[i];
   const int ts = ax.rp.tstride;
   const int kx0 = tx * TX;
   const int nl = min(TX, hx - kx0);
-  const size_t base = (size_t)kx0 + (size_t)ty * (size_t)outer_stride;
+  // the tile's first point (uniform) and 32-bit point offsets from it: a z
+  // line spans n * pt_stride < 2^27 points (edges <= 512)
+  float2* __restrict__ tile = cg + ((size_t)kx0 + (size_t)ty * (size_t)outer_stride);
+  const unsigned pst = (unsigned)pt_stride;
   const int tsh = __builtin_ctz(TX);  // TX is a power of two (pick_fft_tiles)
-  for (int e0 = tid; e0 < n * TX; e0 += kStageU * kBlock) {
-    float2 v[kStageU];
-#pragma unroll
-    for (int u = 0; u < kStageU; u++) {
-      const int e = e0 + u * kBlock;
-      const int j = (e >> tsh);
-      const int l = e - j * TX;
-      v[u] = (e < n * TX && l < nl) ? cg[base + (size_t)j * pt_stride + l] : make_float2(0.f, 0.f);
-    }
-#pragma unroll
-    for (int u = 0; u < kStageU; u++) {
-      const int e = e0 + u * kBlock;
-      if (e < n * TX) b0[e] = v[u];
-    }
-  }
-  __syncthreads();
-  // forward, first half: unit (l, j2)
-  for (int u = tid; u < N2 * TX; u += kBlock) {
+  // forward, first half: unit (l, j2), column j2 + N2 j1 loaded from HBM
+  for (int u0 = 0; u0 < N2 * TX; u0 += kBlock) {
+    const int u = u0 + tid;
     const int j2 = (u >> tsh);
     const int l = u - j2 * TX;
+    const bool live = u < N2 * TX;
     float2 x[N1];
+    const unsigned o = (unsigned)j2 * pst + (unsigned)l;
 #pragma unroll
-    for (int j1 = 0; j1 < N1; j1++) x[j1] = b0[(j2 + N2 * j1) * TX + l];
-    DftN<N1, -1>::run(x);
+    for (int j1 = 0; j1 < N1; j1++)
+      x[j1] = (live && l < nl) ? tile[o + (unsigned)(N2 * j1) * pst] : make_float2(0.f, 0.f);
+    if (u0 == 0) __syncthreads();  // twiddles in LDS (uniform branch)
+    if (live) {
+      DftN<N1, -1>::run(x);
 #pragma unroll
-    for (int k1 = 0; k1 < N1; k1++) {
-      const float2 y = k1 == 0 ? x[0] : cmul(x[k1], tw[mul24(j2 * k1, ts)]);
-      b1[(k1 * N2 + j2) * TX + l] = y;
+      for (int k1 = 0; k1 < N1; k1++) {
+        const float2 y = k1 == 0 ? x[0] : cmul(x[k1], tw[mul24(j2 * k1, ts)]);
+        b[(k1 * N2 + j2) * TX + l] = y;
+      }
     }
   }
   __syncthreads();
-  // forward second half, influence function, inverse first half: unit (l, k1)
+  // forward second half, influence function, inverse first half: unit (l, k1),
+  // in place on the N2 points (k1 N2 + j2) of b
   double acc[7] = {0, 0, 0, 0, 0, 0, 0};
   for (int u = tid; u < N1 * TX; u += kBlock) {
     const int k1 = (u >> tsh);
     const int l = u - k1 * TX;
     float2 X[N2];
 #pragma unroll
-    for (int j2 = 0; j2 < N2; j2++) X[j2] = b1[(k1 * N2 + j2) * TX + l];
+    for (int j2 = 0; j2 < N2; j2++) X[j2] = b[(k1 * N2 + j2) * TX + l];
     DftN<N2, -1>::run(X);  // X[k2]: point k3 = k1 + N1 k2
     if (l < nl) {
       const int ky = sdiv(kx0 + l, small_div(hxd));
@@ -816,37 +828,54 @@ __device__ __forceinline__ void zsolve_four_step(float2* __restrict__ cg, int hx
     for (int j2 = 0; j2 < N2; j2++) {
       const float2 t = tw[mul24(j2 * k1, ts)];
       const float2 z = j2 == 0 ? X[0] : cmul(X[j2], make_float2(t.x, -t.y));
-      b0[(k1 * N2 + j2) * TX + l] = z;
+      b[(k1 * N2 + j2) * TX + l] = z;
     }
   }
   __syncthreads();
-  // inverse second half: unit (l, j2), then the store
+  // inverse second half: unit (l, j2), stored straight to HBM
   for (int u = tid; u < N2 * TX; u += kBlock) {
     const int j2 = (u >> tsh);
     const int l = u - j2 * TX;
     float2 z[N1];
 #pragma unroll
-    for (int k1 = 0; k1 < N1; k1++) z[k1] = b0[(k1 * N2 + j2) * TX + l];
+    for (int k1 = 0; k1 < N1; k1++) z[k1] = b[(k1 * N2 + j2) * TX + l];
     DftN<N1, +1>::run(z);  // z[j1]: point j = j2 + N2 j1
     if (l < nl) {
+      const unsigned o = (unsigned)j2 * pst + (unsigned)l;
 #pragma unroll
-      for (int j1 = 0; j1 < N1; j1++) cg[base + (size_t)(j2 + N2 * j1) * pt_stride + l] = z[j1];
+      for (int j1 = 0; j1 < N1; j1++) tile[o + (unsigned)(N2 * j1) * pst] = z[j1];
     }
   }
   if (sa.calc_ev) ev_partial(sa, acc, sh_red, blk);
 }
 
+// the pairs of register class CLS only
+template <int CLS>
 __device__ __forceinline__ void zsolve_four_step_dispatch(int code, float2* __restrict__ cg, int hx, int TX,
                                                           long long pt_stride, long long outer_stride,
                                                           const AxisArgs& ax, const SolveArgs& sa, int tx, int ty,
                                                           int blk, float2* sm, int hxd) {
   switch (code) {
-#define PME_F4_CASE(c, a, b) \
-  case c: zsolve_four_step<a, b>(cg, hx, TX, pt_stride, outer_stride, ax, sa, tx, ty, blk, sm, hxd); break;
+#define PME_F4_CASE(c, a, b)                                                                          \
+  case c:                                                                                             \
+    if constexpr (f4_class_of(b) == CLS)                                                              \
+      zsolve_four_step<a, b>(cg, hx, TX, pt_stride, outer_stride, ax, sa, tx, ty, blk, sm, hxd);      \
+    break;
     PME_F4_PAIRS(PME_F4_CASE)
 #undef PME_F4_CASE
     default: break;
   }
+}
+
+// The fused z pass of one grid as a register four-step with n = N1 N2 (one
+// kernel per pair, so each gets the registers of its own pair only): column
+// tiles of TX lines of the flattened (ky, kx) of a z plane.
+template <int N1, int N2>
+__global__ __launch_bounds__(kBlock) void k_zsolve(float2* __restrict__ cg, int nlines, int TX, long long pt_stride,
+                                                   AxisArgs ax, SolveArgs sa, EvReduce red, int hxd) {
+  extern __shared__ __align__(16) float2 sm[];
+  if ((red.nsolve > 0 || red.nself > 0) && blockIdx.x == 0) ev_reduce_block<kBlock>(red);
+  zsolve_four_step<N1, N2>(cg, nlines, TX, pt_stride, 0, ax, sa, blockIdx.x, 0, blockIdx.x, sm, hxd);
 }
 
 
@@ -861,10 +890,6 @@ template <int MODE, bool SPLIT>
 __device__ __forceinline__ void fft_cols_body(float2* __restrict__ cg, int n, int hx, int TX, long long pt_stride,
                                               long long outer_stride, const AxisArgs& ax, const SolveArgs& sa,
                                               int tx, int ty, int blk, float2* sm, int hxd) {
-  if (MODE == COL_FWD_SOLVE_INV && !SPLIT && ax.f4 >= 0) {
-    zsolve_four_step_dispatch(ax.f4, cg, hx, TX, pt_stride, outer_stride, ax, sa, tx, ty, blk, sm, hxd);
-    return;
-  }
   __shared__ double sh_red[7 * (kBlock / 64)];
   const int tid = threadIdx.x;
   float2* tw = sm;
@@ -943,7 +968,7 @@ __global__ __launch_bounds__(kBlock) void k_scalar_sum(float2* __restrict__ cg, 
 // ----------------------------------------------------------------- launch --
 
 size_t fft_x_lds_bytes(int nf, int NR, int ntw) { return sizeof(float2) * ((size_t)ntw + 2 * (size_t)nf * (NR + 1)); }
-size_t fft_cols_lds_bytes(int n, int TX) { return sizeof(float2) * ((size_t)n + 2 * (size_t)n * TX); }
+size_t fft_cols_lds_bytes(int n, int TX, int nbuf) { return sizeof(float2) * ((size_t)n + (size_t)nbuf * n * TX); }
 
 hipError_t launch_fft_x_r2c(const float* grid, float2* cg, int nx, int ny, int nz, int NR, const AxisArgs& ax,
                             hipStream_t s) {
@@ -968,11 +993,25 @@ int cols_blocks(int hx, int TX, int nouter) { return ((hx + TX - 1) / TX) * nout
 hipError_t launch_fft_cols(int mode, float2* cg, int n, int hx, int TX, long long pt_stride, int nouter,
                            long long outer_stride, const AxisArgs& ax, const SolveArgs& sa, const EvReduce& red,
                            int hxd, hipStream_t s) {
-  const size_t lds = fft_cols_lds_bytes(n, TX);
-  const dim3 grid((hx + TX - 1) / TX, nouter);
   const bool split = ax.prec != PREC_FP32;
+  const bool four_step = mode == COL_FWD_SOLVE_INV && !split && ax.f4 >= 0;
+  const size_t lds = fft_cols_lds_bytes(n, TX, four_step ? 1 : 2);
+  const dim3 grid((hx + TX - 1) / TX, nouter);
   // the fused z pass indexes its tile with shifts
   if (mode == COL_FWD_SOLVE_INV && (TX <= 0 || (TX & (TX - 1)) != 0)) return hipErrorInvalidValue;
+  if (four_step) {
+    if (nouter != 1 || outer_stride != 0) return hipErrorInvalidValue;  // one flattened plane of lines
+    switch (ax.f4) {
+#define PME_ZS_CASE(c, a, b)                                                                                       \
+  case c:                                                                                                          \
+    hipLaunchKernelGGL((k_zsolve<a, b>), grid, dim3(kBlock), lds, s, cg, hx, TX, pt_stride, ax, sa, red, hxd); \
+    break;
+      PME_F4_PAIRS(PME_ZS_CASE)
+#undef PME_ZS_CASE
+      default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+  }
 #define PME_COLS(M)                                                                                         \
   if (split)                                                                                              \
     hipLaunchKernelGGL((k_fft_cols<M, true>), grid, dim3(kBlock), lds, s, cg, n, hx, TX, pt_stride, outer_stride, ax, sa, red, hxd); \
@@ -1069,6 +1108,22 @@ __global__ __launch_bounds__(kBlock) void k_batch_cols(const SysDesc* __restrict
   }
 }
 
+// The fused z pass of systems whose z length has a four-step pair of class
+// CLS (their items only): tiles of the flattened (ky, kx) of a z plane.
+template <int CLS>
+__global__ __launch_bounds__(kBlock) void k_batch_zsolve(const SysDesc* __restrict__ sys,
+                                                         const WorkItem* __restrict__ items, int nitems) {
+  extern __shared__ __align__(16) float2 sm[];
+  for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+    const WorkItem w = uniform_item(items[it]);
+    const SysDesc& d = sys[w.sys];
+    const int hx = d.bin.nx / 2 + 1;
+    zsolve_four_step_dispatch<CLS>(d.ax[2].f4, d.cgrid, hx * d.bin.ny, d.TXz, spec_pitch(hx, d.bin.ny), 0, d.ax[2],
+                                   d.sa, w.idx, 0, w.idx, sm, hx);
+    __syncthreads();
+  }
+}
+
 // items are (system, first plane z); each covers planes z .. z+g-1 with g =
 // plane_group(.., maxnp) of its system (fewer at the top of the grid).
 // SPLIT: every system of the launch may use the matrix-core stage in y.
@@ -1110,6 +1165,18 @@ hipError_t launch_batch_fft_x(int inverse, const SysDesc* sys, const WorkItem* i
     hipLaunchKernelGGL(k_batch_fft_x<1>, dim3(grid), dim3(kBlock), lds, s, sys, items, nitems);
   else
     hipLaunchKernelGGL(k_batch_fft_x<0>, dim3(grid), dim3(kBlock), lds, s, sys, items, nitems);
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_zsolve(int cls, const SysDesc* sys, const WorkItem* items, int nitems, int grid, size_t lds,
+                               hipStream_t s) {
+  if (nitems <= 0) return hipSuccess;
+  if (cls == 0)
+    hipLaunchKernelGGL(k_batch_zsolve<0>, dim3(grid), dim3(kBlock), lds, s, sys, items, nitems);
+  else if (cls == 1)
+    hipLaunchKernelGGL(k_batch_zsolve<1>, dim3(grid), dim3(kBlock), lds, s, sys, items, nitems);
+  else
+    return hipErrorInvalidValue;
   return hipGetLastError();
 }
 

@@ -483,7 +483,9 @@ static SolveArgs solve_args(pme_recip_t h, const double* recip, double kappa, bo
 }
 
 static void pick_tiles(pme_recip_t h) {
-  const FftTiles t = pick_fft_tiles(h->nx, h->ny, h->nz, h->rp[0].n, kDefaultFftTile);
+  // the z tile is sized for the FP32 four-step z pass (one LDS buffer); a split
+  // precision set later runs that tile in two buffers, within the 160 KB of LDS
+  const FftTiles t = pick_fft_tiles(h->nx, h->ny, h->nz, h->rp[0].n, kDefaultFftTile, four_step_code(h->nz) >= 0);
   h->NR = t.NR;
   h->TXy = t.TXy;
   h->TXz = t.TXz;

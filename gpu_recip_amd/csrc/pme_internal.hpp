@@ -308,6 +308,8 @@ hipError_t launch_fft_plane_inv(const float2* cg, float* grid, int nx, int ny, i
 // Factor pair code of the register four-step z pass for a z length n
 // (-1: none, the Stockham stages are used)
 int four_step_code(int n);
+// register class of a four-step pair (0: N2 <= 10, 1: larger), -1: none
+int four_step_class(int code);
 
 // LDS bytes of a plane kernel over np planes per workgroup (0 when they do
 // not fit one workgroup)
@@ -334,6 +336,9 @@ hipError_t launch_batch_fft_x(int inverse, const SysDesc* sys, const WorkItem* i
 // axis 1 (y) or 2 (z); mode COL_*; with reduce != 0 the first tile of every
 // system sums that system's energy/virial partials into ev_out; split as
 // for launch_batch_plane, for the transformed axis
+// fused z pass of the systems whose z length has a four-step pair of class cls
+hipError_t launch_batch_zsolve(int cls, const SysDesc* sys, const WorkItem* items, int nitems, int grid, size_t lds,
+                               hipStream_t s);
 hipError_t launch_batch_cols(int axis, int mode, int reduce, int split, const SysDesc* sys, const WorkItem* items,
                              int nitems, int grid, size_t lds, hipStream_t s);
 
@@ -342,7 +347,9 @@ size_t bin_lds_bytes(int nbins);
 size_t spread_lds_bytes(int order, const int wmax[3], int nchunk);
 size_t gather_lds_bytes(int order, const int wmax[3], int nchunk);
 size_t fft_x_lds_bytes(int nf, int NR, int ntw);
-size_t fft_cols_lds_bytes(int n, int TX);
+// column passes: twiddles and nbuf n*TX buffers (two for the Stockham stages,
+// one for the fused z pass as a register four-step)
+size_t fft_cols_lds_bytes(int n, int TX, int nbuf = 2);
 
 constexpr int kSpreadBlock = 1024;  // threads per spread / gather workgroup (16 waves per CU)
 constexpr int kRecPerThread = 2;    // gather: brick records read per thread per round

@@ -141,7 +141,7 @@ bool plan_bricks(const int n[3], int order, int target, BrickPlan* bp) {
   return bp->nbins <= kMaxBins && bin_lds_bytes(bp->nbins) <= kLdsBudget;
 }
 
-FftTiles pick_fft_tiles(int nx, int ny, int nz, int nf_x, int tile_target) {
+FftTiles pick_fft_tiles(int nx, int ny, int nz, int nf_x, int tile_target, bool z_four_step) {
   FftTiles t;
   const int hx = nx / 2 + 1;
   t.NR = nf_x <= 64 ? 16 : (nf_x <= 128 ? 8 : (nf_x <= 256 ? 4 : 2));
@@ -163,7 +163,7 @@ FftTiles pick_fft_tiles(int nx, int ny, int nz, int nf_x, int tile_target) {
   // cache lines of the plane
   int tz = 1;
   while (tz * 2 <= tile_target) tz *= 2;
-  while (tz > 1 && fft_cols_lds_bytes(nz, tz) > kLdsBudget) tz /= 2;
+  while (tz > 1 && fft_cols_lds_bytes(nz, tz, z_four_step ? 1 : 2) > kLdsBudget) tz /= 2;
   t.TXz = tz;
   return t;
 }

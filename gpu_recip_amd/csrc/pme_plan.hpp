@@ -47,7 +47,8 @@ bool plan_bricks(const int n[3], int order, int target, BrickPlan* bp);
 struct FftTiles {
   int NR, TXy, TXz;
 };
-FftTiles pick_fft_tiles(int nx, int ny, int nz, int nf_x, int tile_target);
+// z_four_step: the fused z pass runs as a register four-step (one LDS buffer)
+FftTiles pick_fft_tiles(int nx, int ny, int nz, int nf_x, int tile_target, bool z_four_step);
 
 struct ChunkPlan {
   int chunk, nchunk;

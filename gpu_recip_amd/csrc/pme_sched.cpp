@@ -54,7 +54,8 @@ pme_status plan_system(const pme_system_desc& in, int cap_atoms, const Tuning& t
       p->prec[a] = in.precision;
     }
   }
-  p->ft = pick_fft_tiles(n[0], n[1], n[2], p->rp[0].n, t.fft_tile);
+  p->ft = pick_fft_tiles(n[0], n[1], n[2], p->rp[0].n, t.fft_tile,
+                         p->prec[2] == PME_PREC_FP32 && four_step_code(n[2]) >= 0);
   p->plane = fft_plane_lds_bytes(n[0], n[1], p->rp[0].n, 1) != 0 ? 1 : 0;
   const int hx = n[0] / 2 + 1;
   if (p->ft.TXz <= 0 || (p->ft.TXz & (p->ft.TXz - 1)) != 0)  // the z pass indexes its tiles with shifts
@@ -216,6 +217,9 @@ void build_items(const SysDesc* const* descs, const SystemPlan* const* plans, co
         const int a = stage == BS_FFTZ ? 2 : 1;
         if ((d.ax[a].prec != PME_PREC_FP32) != (split != 0)) continue;
       }
+      // z: order 1 + c takes the systems with a four-step pair of register
+      // class c, order 0 the others (Stockham stages, split precision)
+      if (stage == BS_FFTZ && order != (d.ax[2].f4 >= 0 ? 1 + four_step_class(d.ax[2].f4) : 0)) continue;
       int cnt = 0;
       size_t lds = 0;
       const int hx = d.bin.nx / 2 + 1;
@@ -252,7 +256,7 @@ void build_items(const SysDesc* const* descs, const SystemPlan* const* plans, co
         }
         case BS_FFTZ:
           cnt = cols_blocks(hx * d.bin.ny, d.TXz, 1);
-          lds = fft_cols_lds_bytes(d.bin.nz, d.TXz);
+          lds = fft_cols_lds_bytes(d.bin.nz, d.TXz, d.ax[2].f4 >= 0 ? 1 : 2);
           break;
       }
       for (int i = 0; i < cnt; i++) items.push_back(WorkItem{k, i});
@@ -266,6 +270,7 @@ void build_items(const SysDesc* const* descs, const SystemPlan* const* plans, co
   add_stage(BS_FFTX, 0, 0);
   for (int sp : {0, 1}) add_stage(BS_FFTY, 0, sp);
   for (int sp : {0, 1}) add_stage(BS_PLANE, group, sp);
+  for (int c : {1, 2}) add_stage(BS_FFTZ, c, 0);
   for (int sp : {0, 1}) add_stage(BS_FFTZ, 0, sp);
   for (int o : {4, 6, 8}) add_stage(BS_GATHER, o, 0);
 }
@@ -324,7 +329,10 @@ pme_status enqueue_stages(const std::vector<Launch>& launches, const SysDesc* d_
                                          stream));
             break;
           case BS_FFTZ:
-            SCHED_HIP(launch_batch_cols(2, COL_FWD_SOLVE_INV, 0, L.split, d_sys, it, L.count, grid, L.lds, stream));
+            if (L.order > 0)
+              SCHED_HIP(launch_batch_zsolve(L.order - 1, d_sys, it, L.count, grid, L.lds, stream));
+            else
+              SCHED_HIP(launch_batch_cols(2, COL_FWD_SOLVE_INV, 0, L.split, d_sys, it, L.count, grid, L.lds, stream));
             break;
           case BS_GATHER: SCHED_HIP(launch_batch_gather(L.order, d_sys, it, L.count, ctr, grid, L.lds, stream)); break;
           default: break;

@@ -62,7 +62,8 @@ void fill_dynamic(SysDesc& d, const pme_system_desc& in, const SystemPlan& p);
 // One launch of a stage over the items [offset, offset + count).
 struct Launch {
   int stage;   // BatchStage
-  int order;   // spread/gather template order; largest plane group of a plane launch (0 otherwise)
+  int order;   // spread/gather template order; largest plane group of a plane launch; z pass: 1 + the
+               // four-step register class, 0 for the Stockham / split kernels (0 otherwise)
   int split;   // FFT launches: 1 for the kernels with the split-precision matrix-core stage
   int offset;
   int count;
