@@ -202,12 +202,21 @@ __global__ __launch_bounds__(kBlock) void k_fft_x_c2r(const float2* __restrict__
 // LDS: twiddles of x and y, two buffers of plane_buf() points.  x lines
 // have point stride LSX, y lines point stride LSY, both odd so that 8-byte
 // accesses of consecutive lines hit distinct bank pairs.
+// Odd nx: rows 2j and 2j+1 of a plane are the real and imaginary parts of
+// one x line (one complex FFT of length nx transforms two rows; an odd ny
+// pairs the plane's last row with zeros), so a plane has (ny + 1) / 2 x
+// lines; the pairs never cross planes, so a group of planes rounds exactly
+// as its planes one at a time.
 
 __host__ __device__ constexpr int odd_pad(int n) { return (n + 1) | 1; }
 
+// x lines of one plane
+__host__ __device__ constexpr int plane_xlines(int nx, int ny) { return (nx & 1) ? (ny + 1) >> 1 : ny; }
+
 __host__ __device__ constexpr int plane_buf(int nx, int ny, int nf, int np) {
-  return (nf * odd_pad(np * ny) > ny * odd_pad(np * (nx / 2 + 1))) ? nf * odd_pad(np * ny)
-                                                                    : ny * odd_pad(np * (nx / 2 + 1));
+  return (nf * odd_pad(np * plane_xlines(nx, ny)) > ny * odd_pad(np * (nx / 2 + 1)))
+             ? nf * odd_pad(np * plane_xlines(nx, ny))
+             : ny * odd_pad(np * (nx / 2 + 1));
 }
 
 __host__ __device__ constexpr size_t plane_lds_need(int nx, int ny, int nf, int np) {
@@ -219,15 +228,22 @@ size_t fft_plane_lds_bytes(int nx, int ny, int nf_x, int np) {
   return b <= kPlaneLdsBudget ? b : 0;
 }
 
+// LDS budget of one plane workgroup: kPlaneLdsBudget for the full-size
+// (kPlaneBlock-thread) workgroup, alone on its CU; half of it for the
+// half-size one, two of which share a CU
+__host__ __device__ constexpr size_t plane_budget(bool half) { return half ? kPlaneLdsBudget / 2 : kPlaneLdsBudget; }
+
 // planes per workgroup of a batched plane launch allowing up to maxnp: the
 // largest group whose buffers fit (the same on host and device)
-__host__ __device__ inline int plane_group(int nx, int ny, int nf, int maxnp) {
+__host__ __device__ inline int plane_group(int nx, int ny, int nf, int maxnp, bool half) {
   int g = maxnp;
-  while (g > 1 && plane_lds_need(nx, ny, nf, g) > kPlaneLdsBudget) g--;
+  while (g > 1 && plane_lds_need(nx, ny, nf, g) > plane_budget(half)) g--;
   return g;
 }
 
-int plane_group_host(int nx, int ny, int nf, int maxnp) { return plane_group(nx, ny, nf, maxnp); }
+int plane_group_host(int nx, int ny, int nf, int maxnp, bool half) { return plane_group(nx, ny, nf, maxnp, half); }
+
+bool plane_fits_half(int nx, int ny, int nf) { return plane_lds_need(nx, ny, nf, 1) <= plane_budget(true); }
 
 // planes z .. z+np-1 of the real grid -> the same planes of the half spectrum
 // (SPLIT: the y pass may use the split-precision matrix-core stage)
@@ -248,28 +264,30 @@ __device__ __forceinline__ ColMap col_map(int w) {
   return m;
 }
 
-template <bool SPLIT>
+template <bool SPLIT, int NT>
 __device__ __forceinline__ void fft_plane_fwd_body(const float* __restrict__ grid, float2* __restrict__ cg, int nx,
                                                    int ny, const AxisArgs& axx, const AxisArgs& axy, int z, int np,
                                                    float2* sm) {
   const int tid = threadIdx.x;
   const int nf = axx.rp.n;  // x line length: nx/2 (even nx) or nx
   const int hx = nx / 2 + 1;
-  const int nrow = np * ny;   // x lines
-  const int ncol = np * hx;   // y lines
-  const int LSX = odd_pad(nrow), LSY = odd_pad(ncol);
+  const int nrow = np * ny;                   // rows
+  const int hyl = plane_xlines(nx, ny);      // x lines per plane
+  const int nxl = np * hyl;                   // x lines
+  const int ncol = np * hx;                   // y lines
+  const int LSX = odd_pad(nxl), LSY = odd_pad(ncol);
   const int bsz = plane_buf(nx, ny, nf, np);
   float2* twx = sm;
   float2* twy = twx + nx;
   float2* b0 = twy + ny;
   float2* b1 = b0 + bsz;
-  for (int i = tid; i < nx; i += kPlaneBlock) twx[i] = axx.tw[i];
-  for (int i = tid; i < ny; i += kPlaneBlock) twy[i] = axy.tw[i];
+  for (int i = tid; i < nx; i += NT) twx[i] = axx.tw[i];
+  for (int i = tid; i < ny; i += NT) twy[i] = axy.tw[i];
   const bool even = (nx & 1) == 0;
   const size_t plane = (size_t)z * nx * ny;
   if (even) {
     const float2* src = reinterpret_cast<const float2*>(grid + plane);
-    const ColMap m = col_map<kPlaneBlock>(nf);
+    const ColMap m = col_map<NT>(nf);
     if (m.on)
       for (int rb = m.r0; rb < nrow; rb += kStageU * m.step) {
         float2 v[kStageU];
@@ -285,8 +303,11 @@ __device__ __forceinline__ void fft_plane_fwd_body(const float* __restrict__ gri
         }
       }
   } else {
+    // rows 2j, 2j+1 of plane pl -> real and imaginary parts of x line
+    // pl*hyl + j (an odd ny pairs the last row of each plane with zeros)
     const float* src = grid + plane;
-    const ColMap m = col_map<kPlaneBlock>(nx);
+    const SmallDiv dny = small_div(ny);
+    const ColMap m = col_map<NT>(nx);
     if (m.on)
       for (int rb = m.r0; rb < nrow; rb += kStageU * m.step) {
         float v[kStageU];
@@ -298,17 +319,21 @@ __device__ __forceinline__ void fft_plane_fwd_body(const float* __restrict__ gri
 #pragma unroll
         for (int u = 0; u < kStageU; u++) {
           const int r = rb + u * m.step;
-          if (r < nrow) b0[m.c * LSX + r] = make_float2(v[u], 0.f);
+          const int pl = sdiv(r, dny);
+          const int y = r - pl * ny;
+          if (r < nrow) reinterpret_cast<float*>(b0 + (m.c * LSX + pl * hyl + (y >> 1)))[y & 1] = v[u];
         }
       }
+    if ((ny & 1) && m.on)
+      for (int pl = m.r0; pl < np; pl += m.step) b0[m.c * LSX + pl * hyl + (ny >> 1)].y = 0.f;
   }
   __syncthreads();
-  float2* res = fft_lines<-1>(b0, b1, twx, axx.rp, nrow, LSX, tid, kPlaneBlock);
+  float2* res = fft_lines<-1>(b0, b1, twx, axx.rp, nxl, LSX, tid, NT);
   float2* q = res == b0 ? b1 : b0;
   // R2C post-processing (even nx) and transpose to columns:
   // X[k] = E[k] + w^k O[k], E = (Z[k] + Z*[h-k])/2, O = (Z[k] - Z*[h-k])/(2i)
   const SmallDiv dny = small_div(ny);
-  const ColMap mh = col_map<kPlaneBlock>(hx);
+  const ColMap mh = col_map<NT>(hx);
   for (int r = mh.r0; mh.on && r < nrow; r += mh.step) {  // x line r (row of the plane group), point k
     const int k = mh.c;
     const int pl = sdiv(r, dny);
@@ -323,12 +348,18 @@ __device__ __forceinline__ void fft_plane_fwd_body(const float* __restrict__ gri
       const float2 O = make_float2(0.5f * D.y, -0.5f * D.x);
       X = cadd(E, cmul(twx[k], O));
     } else {
-      X = res[k * LSX + r];
+      // Z = FFT(a + i b) of the row pair: A[k] = (Z[k] + Z*[nx-k]) / 2,
+      // B[k] = (Z[k] - Z*[nx-k]) / (2i); row y is a (y even) or b (y odd)
+      const int i = pl * hyl + (y >> 1);
+      const float2 zk = res[k * LSX + i];
+      const float2 zc = cconj(res[(k == 0 ? 0 : nx - k) * LSX + i]);
+      const float2 D = csub(zk, zc);
+      X = (y & 1) ? make_float2(0.5f * D.y, -0.5f * D.x) : cscale(cadd(zk, zc), 0.5f);
     }
     q[y * LSY + pl * hx + k] = X;
   }
   __syncthreads();
-  float2* r2 = fft_lines<-1, SPLIT>(q, res, twy, axy.rp, ncol, LSY, tid, kPlaneBlock, axy.prec);
+  float2* r2 = fft_lines<-1, SPLIT>(q, res, twy, axy.rp, ncol, LSY, tid, NT, axy.prec);
   const long long pz = spec_pitch(hx, ny);
   float2* dst = cg + (size_t)z * (size_t)pz;
   for (int r = mh.r0; mh.on && r < nrow; r += mh.step) {
@@ -339,27 +370,29 @@ __device__ __forceinline__ void fft_plane_fwd_body(const float* __restrict__ gri
 }
 
 // planes z .. z+np-1 of the half spectrum -> the same planes of the real grid
-template <bool SPLIT>
+template <bool SPLIT, int NT>
 __device__ __forceinline__ void fft_plane_inv_body(const float2* __restrict__ cg, float* __restrict__ grid, int nx,
                                                    int ny, const AxisArgs& axx, const AxisArgs& axy, int z, int np,
                                                    float2* sm) {
   const int tid = threadIdx.x;
   const int nf = axx.rp.n;
   const int hx = nx / 2 + 1;
-  const int nrow = np * ny;
-  const int ncol = np * hx;
-  const int LSX = odd_pad(nrow), LSY = odd_pad(ncol);
+  const int nrow = np * ny;                   // rows
+  const int hyl = plane_xlines(nx, ny);      // x lines per plane
+  const int nxl = np * hyl;                   // x lines
+  const int ncol = np * hx;                   // y lines
+  const int LSX = odd_pad(nxl), LSY = odd_pad(ncol);
   const int bsz = plane_buf(nx, ny, nf, np);
   float2* twx = sm;
   float2* twy = twx + nx;
   float2* b0 = twy + ny;
   float2* b1 = b0 + bsz;
-  for (int i = tid; i < nx; i += kPlaneBlock) twx[i] = axx.tw[i];
-  for (int i = tid; i < ny; i += kPlaneBlock) twy[i] = axy.tw[i];
+  for (int i = tid; i < nx; i += NT) twx[i] = axx.tw[i];
+  for (int i = tid; i < ny; i += NT) twy[i] = axy.tw[i];
   const long long pz = spec_pitch(hx, ny);
   const float2* src = cg + (size_t)z * (size_t)pz;
   const SmallDiv dny = small_div(ny);
-  const ColMap mh = col_map<kPlaneBlock>(hx);
+  const ColMap mh = col_map<NT>(hx);
   if (mh.on)
     for (int rb = mh.r0; rb < nrow; rb += kStageU * mh.step) {
       float2 v[kStageU];
@@ -377,13 +410,13 @@ __device__ __forceinline__ void fft_plane_inv_body(const float2* __restrict__ cg
         if (rb + u * mh.step < nrow) b0[at[u] + mh.c] = v[u];
     }
   __syncthreads();
-  float2* r = fft_lines<+1, SPLIT>(b0, b1, twy, axy.rp, ncol, LSY, tid, kPlaneBlock, axy.prec);
+  float2* r = fft_lines<+1, SPLIT>(b0, b1, twy, axy.rp, ncol, LSY, tid, NT, axy.prec);
   float2* p = r == b0 ? b1 : b0;
   const bool even = (nx & 1) == 0;
   if (even) {
     // Z[k] = (X[k] + X*[h-k]) + i w^{-k} (X[k] - X*[h-k]),  k < h
     const int h = nf;
-    const ColMap m = col_map<kPlaneBlock>(h);
+    const ColMap m = col_map<NT>(h);
     for (int row = m.r0; m.on && row < nrow; row += m.step) {
       const int k = m.c;
       const int pl = sdiv(row, dny);
@@ -395,26 +428,42 @@ __device__ __forceinline__ void fft_plane_inv_body(const float2* __restrict__ cg
       p[k * LSX + row] = make_float2(A.x - B.y, A.y + B.x);
     }
   } else {
-    // the full Hermitian spectrum of each row
-    const ColMap m = col_map<kPlaneBlock>(nx);
-    for (int row = m.r0; m.on && row < nrow; row += m.step) {
+    // x line pl*hyl + j: Z = A + i B from the full Hermitian spectra A, B of
+    // rows 2j and 2j+1 of plane pl (zero past its last row); its inverse FFT
+    // has row 2j in its real and row 2j+1 in its imaginary part
+    const SmallDiv dhyl = small_div(hyl);
+    const ColMap m = col_map<NT>(nx);
+    for (int i = m.r0; m.on && i < nxl; i += m.step) {
       const int k = m.c;
-      const int pl = sdiv(row, dny);
-      const float2* col = r + (row - pl * ny) * LSY + pl * hx;
-      p[k * LSX + row] = k < hx ? col[k] : cconj(col[nx - k]);
+      const int kk = k < hx ? k : nx - k;
+      const int pl = sdiv(i, dhyl);
+      const int ya = 2 * (i - pl * hyl);
+      const float2* col = r + pl * hx + kk;
+      float2 A = col[ya * LSY];
+      float2 B = make_float2(0.f, 0.f);
+      if (ya + 1 < ny) B = col[(ya + 1) * LSY];
+      if (k >= hx) {
+        A = cconj(A);
+        B = cconj(B);
+      }
+      p[k * LSX + i] = make_float2(A.x - B.y, A.y + B.x);
     }
   }
   __syncthreads();
-  float2* s = fft_lines<+1>(p, r, twx, axx.rp, nrow, LSX, tid, kPlaneBlock);
+  float2* s = fft_lines<+1>(p, r, twx, axx.rp, nxl, LSX, tid, NT);
   const size_t plane = (size_t)z * nx * ny;
   if (even) {
     float2* dst = reinterpret_cast<float2*>(grid + plane);
-    const ColMap m = col_map<kPlaneBlock>(nf);
+    const ColMap m = col_map<NT>(nf);
     for (int row = m.r0; m.on && row < nrow; row += m.step) dst[(size_t)row * nf + m.c] = s[m.c * LSX + row];
   } else {
     float* dst = grid + plane;
-    const ColMap m = col_map<kPlaneBlock>(nx);
-    for (int row = m.r0; m.on && row < nrow; row += m.step) dst[(size_t)row * nx + m.c] = s[m.c * LSX + row].x;
+    const ColMap m = col_map<NT>(nx);
+    for (int row = m.r0; m.on && row < nrow; row += m.step) {
+      const int pl = sdiv(row, dny);
+      const int y = row - pl * ny;
+      dst[(size_t)row * nx + m.c] = reinterpret_cast<const float*>(s + (m.c * LSX + pl * hyl + (y >> 1)))[y & 1];
+    }
   }
 }
 
@@ -422,7 +471,7 @@ template <bool SPLIT>
 __global__ __launch_bounds__(kPlaneBlock) void k_fft_plane_fwd(const float* __restrict__ grid, float2* __restrict__ cg,
                                                               int nx, int ny, AxisArgs axx, AxisArgs axy) {
   extern __shared__ __align__(16) float2 sm[];
-  fft_plane_fwd_body<SPLIT>(grid, cg, nx, ny, axx, axy, blockIdx.x, 1, sm);
+  fft_plane_fwd_body<SPLIT, kPlaneBlock>(grid, cg, nx, ny, axx, axy, blockIdx.x, 1, sm);
 }
 
 template <bool SPLIT>
@@ -431,7 +480,7 @@ __global__ __launch_bounds__(kPlaneBlock) void k_fft_plane_inv(const float2* __r
                                                               EvReduce red) {
   extern __shared__ __align__(16) float2 sm[];
   if ((red.nsolve > 0 || red.nself > 0) && blockIdx.x == 0) ev_reduce_block<kPlaneBlock>(red);
-  fft_plane_inv_body<SPLIT>(cg, grid, nx, ny, axx, axy, blockIdx.x, 1, sm);
+  fft_plane_inv_body<SPLIT, kPlaneBlock>(cg, grid, nx, ny, axx, axy, blockIdx.x, 1, sm);
 }
 
 hipError_t launch_fft_plane_fwd(const float* grid, float2* cg, int nx, int ny, int nz, const AxisArgs& ax,
@@ -1125,36 +1174,52 @@ __global__ __launch_bounds__(kBlock) void k_batch_zsolve(const SysDesc* __restri
 }
 
 // items are (system, first plane z); each covers planes z .. z+g-1 with g =
-// plane_group(.., maxnp) of its system (fewer at the top of the grid).
-// SPLIT: every system of the launch may use the matrix-core stage in y.
-template <int INV, bool SPLIT>
-__global__ __launch_bounds__(kPlaneBlock) void k_batch_plane(const SysDesc* __restrict__ sys,
-                                                             const WorkItem* __restrict__ items, int nitems,
-                                                             int reduce, int maxnp) {
+// plane_group(.., maxnp, NT < kPlaneBlock) of its system (fewer at the top of
+// the grid).  SPLIT: every system of the launch may use the matrix-core stage
+// in y.  NT = kPlaneBlock / 2: half-size workgroups, two per CU, for the
+// systems whose plane fits half the LDS budget: one workgroup's HBM loads and
+// stores then overlap the other's transforms.
+template <int INV, bool SPLIT, int NT>
+__global__ __launch_bounds__(NT) void k_batch_plane(const SysDesc* __restrict__ sys,
+                                                    const WorkItem* __restrict__ items, int nitems, int reduce,
+                                                    int maxnp) {
   extern __shared__ __align__(16) float2 sm[];
   for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
     const WorkItem w = uniform_item(items[it]);
     const SysDesc& d = sys[w.sys];
-    const int npl = min(plane_group(d.bin.nx, d.bin.ny, d.ax[0].rp.n, maxnp), d.bin.nz - w.idx);
+    const int npl = min(plane_group(d.bin.nx, d.bin.ny, d.ax[0].rp.n, maxnp, NT < kPlaneBlock), d.bin.nz - w.idx);
     if (INV) {
-      if (reduce && w.idx == 0 && (d.sa.calc_ev || d.calc_self)) batch_ev_reduce<kPlaneBlock>(d);
-      fft_plane_inv_body<SPLIT>(d.cgrid, d.grid, d.bin.nx, d.bin.ny, d.ax[0], d.ax[1], w.idx, npl, sm);
+      if (reduce && w.idx == 0 && (d.sa.calc_ev || d.calc_self)) batch_ev_reduce<NT>(d);
+      fft_plane_inv_body<SPLIT, NT>(d.cgrid, d.grid, d.bin.nx, d.bin.ny, d.ax[0], d.ax[1], w.idx, npl, sm);
     } else {
-      fft_plane_fwd_body<SPLIT>(d.grid, d.cgrid, d.bin.nx, d.bin.ny, d.ax[0], d.ax[1], w.idx, npl, sm);
+      fft_plane_fwd_body<SPLIT, NT>(d.grid, d.cgrid, d.bin.nx, d.bin.ny, d.ax[0], d.ax[1], w.idx, npl, sm);
     }
     __syncthreads();
   }
 }
 
-hipError_t launch_batch_plane(int inverse, int reduce, int maxnp, int split, const SysDesc* sys,
+hipError_t launch_batch_plane(int inverse, int reduce, int maxnp, int split, int half, const SysDesc* sys,
                               const WorkItem* items, int nitems, int grid, size_t lds, hipStream_t s) {
   if (nitems <= 0) return hipSuccess;
   if (maxnp < 1 || maxnp > kPlaneGroupMax) return hipErrorInvalidValue;
-  const dim3 g(grid), b(kPlaneBlock);
-  if (inverse && split) hipLaunchKernelGGL((k_batch_plane<1, true>), g, b, lds, s, sys, items, nitems, reduce, maxnp);
-  else if (inverse) hipLaunchKernelGGL((k_batch_plane<1, false>), g, b, lds, s, sys, items, nitems, reduce, maxnp);
-  else if (split) hipLaunchKernelGGL((k_batch_plane<0, true>), g, b, lds, s, sys, items, nitems, reduce, maxnp);
-  else hipLaunchKernelGGL((k_batch_plane<0, false>), g, b, lds, s, sys, items, nitems, reduce, maxnp);
+  const dim3 g(grid);
+#define PME_BPLANE(I, S)                                                                                        \
+  if (half)                                                                                                     \
+    hipLaunchKernelGGL((k_batch_plane<I, S, kPlaneBlock / 2>), g, dim3(kPlaneBlock / 2), lds, s, sys, items,   \
+                       nitems, reduce, maxnp);                                                                  \
+  else                                                                                                          \
+    hipLaunchKernelGGL((k_batch_plane<I, S, kPlaneBlock>), g, dim3(kPlaneBlock), lds, s, sys, items, nitems,    \
+                       reduce, maxnp);
+  if (inverse && split) {
+    PME_BPLANE(1, true)
+  } else if (inverse) {
+    PME_BPLANE(1, false)
+  } else if (split) {
+    PME_BPLANE(0, true)
+  } else {
+    PME_BPLANE(0, false)
+  }
+#undef PME_BPLANE
   return hipGetLastError();
 }
 

@@ -315,14 +315,18 @@ int four_step_class(int code);
 // not fit one workgroup)
 size_t fft_plane_lds_bytes(int nx, int ny, int nf_x, int np);
 // planes per workgroup of a system in a batched plane launch allowing up to
-// maxnp (the largest group that fits the LDS budget)
-int plane_group_host(int nx, int ny, int nf_x, int maxnp);
+// maxnp (the largest group that fits the LDS budget of a full-size or, with
+// half, a half-size workgroup)
+int plane_group_host(int nx, int ny, int nf_x, int maxnp, bool half);
+// one plane fits the LDS budget of a half-size plane workgroup
+bool plane_fits_half(int nx, int ny, int nf_x);
 // batched plane kernels: items are (system, first z plane), with
-// plane_group_host(.., maxnp) planes each; with reduce != 0 the plane-0 item
-// of every system assigns that system's energy/virial.  split != 0: the
+// plane_group_host(.., maxnp, half) planes each; with reduce != 0 the plane-0
+// item of every system assigns that system's energy/virial.  split != 0: the
 // kernel with the matrix-core y stage (systems whose y axis is not FP32);
-// split == 0 requires FP32 y axes.
-hipError_t launch_batch_plane(int inverse, int reduce, int maxnp, int split, const SysDesc* sys,
+// split == 0 requires FP32 y axes.  half != 0: kPlaneBlock / 2 threads per
+// workgroup (two workgroups per CU), for systems with plane_fits_half only.
+hipError_t launch_batch_plane(int inverse, int reduce, int maxnp, int split, int half, const SysDesc* sys,
                               const WorkItem* items, int nitems, int grid, size_t lds, hipStream_t s);
 
 // batched launchers: `grid` persistent workgroups loop over the work items

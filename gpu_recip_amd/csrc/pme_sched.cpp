@@ -207,8 +207,10 @@ void build_items(const SysDesc* const* descs, const SystemPlan* const* plans, co
   // split: the FFT launches of systems whose transformed axis uses the
   // split-precision matrix-core stage go to kernels compiled with it, the
   // others to FP32-only kernels
-  auto add_stage = [&](int stage, int order, int split) {
-    Launch L{stage, order, split, (int)items.size(), 0, 0};
+  // half: plane launches of the systems whose plane fits a half-size
+  // workgroup (half = 1) or does not (half = 0)
+  auto add_stage = [&](int stage, int order, int split, int half) {
+    Launch L{stage, order, split, half, (int)items.size(), 0, 0};
     for (int k = 0; k < n; k++) {
       const SysDesc& d = *descs[k];
       const SystemPlan& p = *plans[k];
@@ -247,8 +249,8 @@ void build_items(const SysDesc* const* descs, const SystemPlan* const* plans, co
           lds = fft_cols_lds_bytes(d.bin.ny, d.TXy);
           break;
         case BS_PLANE: {
-          if (!d.plane) break;
-          const int g = plane_group_host(d.bin.nx, d.bin.ny, d.ax[0].rp.n, order);
+          if (!d.plane || plane_fits_half(d.bin.nx, d.bin.ny, d.ax[0].rp.n) != (half != 0)) break;
+          const int g = plane_group_host(d.bin.nx, d.bin.ny, d.ax[0].rp.n, order, half != 0);
           lds = fft_plane_lds_bytes(d.bin.nx, d.bin.ny, d.ax[0].rp.n, g);
           for (int z = 0; z < d.bin.nz; z += g) items.push_back(WorkItem{k, z});
           L.count += (d.bin.nz + g - 1) / g;
@@ -265,14 +267,15 @@ void build_items(const SysDesc* const* descs, const SystemPlan* const* plans, co
     }
     if (L.count > 0) launches.push_back(L);
   };
-  add_stage(BS_BIN, 0, 0);
-  for (int o : {4, 6, 8}) add_stage(BS_SPREAD, o, 0);
-  add_stage(BS_FFTX, 0, 0);
-  for (int sp : {0, 1}) add_stage(BS_FFTY, 0, sp);
-  for (int sp : {0, 1}) add_stage(BS_PLANE, group, sp);
-  for (int c : {1, 2}) add_stage(BS_FFTZ, c, 0);
-  for (int sp : {0, 1}) add_stage(BS_FFTZ, 0, sp);
-  for (int o : {4, 6, 8}) add_stage(BS_GATHER, o, 0);
+  add_stage(BS_BIN, 0, 0, 0);
+  for (int o : {4, 6, 8}) add_stage(BS_SPREAD, o, 0, 0);
+  add_stage(BS_FFTX, 0, 0, 0);
+  for (int sp : {0, 1}) add_stage(BS_FFTY, 0, sp, 0);
+  for (int sp : {0, 1})
+    for (int h : {1, 0}) add_stage(BS_PLANE, group, sp, h);
+  for (int c : {1, 2}) add_stage(BS_FFTZ, c, 0, 0);
+  for (int sp : {0, 1}) add_stage(BS_FFTZ, 0, sp, 0);
+  for (int o : {4, 6, 8}) add_stage(BS_GATHER, o, 0, 0);
 }
 
 // Workgroups of a batched launch.  Spread and gather (queue != 0) always run
@@ -280,12 +283,12 @@ void build_items(const SysDesc* const* descs, const SystemPlan* const* plans, co
 // (their LDS brick leaves room for one workgroup per CU anyway, and a
 // persistent workgroup saves the start of a new one per brick).  The others
 // run one workgroup per item with wg_per_cu == 0, else wg_per_cu persistent
-// workgroups per CU (one for the LDS-filling plane kernels) striding over the
-// items.
-static int grid_for(const Tuning& t, int ncu, int count, bool big_block, bool queue) {
+// workgroups per CU striding over the items; the plane kernels (cap > 0)
+// cap per CU (one full-size or two half-size workgroups fill a CU).
+static int grid_for(const Tuning& t, int ncu, int count, int cap, bool queue) {
   if (queue) return std::max(1, std::min(count, ncu));
   if (t.wg_per_cu <= 0) return count;
-  const int per_cu = big_block ? 1 : t.wg_per_cu;
+  const int per_cu = cap > 0 ? cap : t.wg_per_cu;
   return std::max(1, std::min(count, ncu * per_cu));
 }
 
@@ -315,7 +318,7 @@ pme_status enqueue_stages(const std::vector<Launch>& launches, const SysDesc* d_
         const WorkItem* it = d_items + L.offset;
         int* ctr = d_ctr + 2 * li;  // this launch's work-queue counters (spread, gather)
         const bool queue = stage == BS_SPREAD || stage == BS_GATHER;
-        const int grid = grid_for(t, ncu, L.count, stage == BS_PLANE, queue);
+        const int grid = grid_for(t, ncu, L.count, stage == BS_PLANE ? (L.half ? 2 : 1) : 0, queue);
         switch (stage) {
           case BS_BIN: SCHED_HIP(launch_batch_bin(d_sys, it, L.count, grid, L.lds, stream)); break;
           case BS_SPREAD: SCHED_HIP(launch_batch_spread(L.order, d_sys, it, L.count, ctr, grid, L.lds, stream)); break;
@@ -325,7 +328,7 @@ pme_status enqueue_stages(const std::vector<Launch>& launches, const SysDesc* d_
                                         stream));
             break;
           case BS_PLANE:
-            SCHED_HIP(launch_batch_plane(inv ? 1 : 0, inv ? 1 : 0, L.order, L.split, d_sys, it, L.count, grid, L.lds,
+            SCHED_HIP(launch_batch_plane(inv ? 1 : 0, inv ? 1 : 0, L.order, L.split, L.half, d_sys, it, L.count, grid, L.lds,
                                          stream));
             break;
           case BS_FFTZ:

@@ -65,6 +65,7 @@ struct Launch {
   int order;   // spread/gather template order; largest plane group of a plane launch; z pass: 1 + the
                // four-step register class, 0 for the Stockham / split kernels (0 otherwise)
   int split;   // FFT launches: 1 for the kernels with the split-precision matrix-core stage
+  int half;    // plane launches: 1 for half-size workgroups (two per CU)
   int offset;
   int count;
   size_t lds;  // dynamic LDS bytes: max over the launch's systems
