@@ -233,6 +233,7 @@ def main():
     tune_ms = None if args.no_autotune else batch.autotune(None)
     tuning = batch.get_tuning()
     chains = batch.get_chains()
+    half_bricks = batch.get_half_bricks()
 
     def bstep():
         batch.solve()
@@ -590,6 +591,7 @@ def main():
                    "kappa": 0.34, "step": "one batched solve of all 64 systems (hipGraph replay) + energy/virial "
                                           "readback", "tuning": {"brick_target": tuning[0], "fft_tile": tuning[1],
                                                                  "wg_per_cu": tuning[2], "chains": chains,
+                                                                 "half_bricks": half_bricks,
                                                                  "autotune_ms": tune_ms}},
         "atom_steps_per_s": atoms_b * args.steps / dt_b,
         "plain_same_systems": {"solves_per_s": plain_c3, "atom_steps_per_s": atoms_b * psteps / dt_p,

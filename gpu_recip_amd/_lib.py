@@ -137,6 +137,8 @@ SIGNATURES = {
     "pme_batch_get_tuning": [_vp, _pi, _pi, _pi],
     "pme_batch_set_chains": [_vp, _i],
     "pme_batch_get_chains": [_vp, _pi],
+    "pme_batch_set_half_bricks": [_vp, _i],
+    "pme_batch_get_half_bricks": [_vp, _pi],
     "pme_batch_autotune": [_vp, C.c_char_p, C.POINTER(C.c_float)],
     "pme_batch_memory": [_vp, C.POINTER(_sz), C.POINTER(_sz)],
     "pme_batch_set_timing": [_vp, _i],

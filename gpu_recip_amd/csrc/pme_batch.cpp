@@ -571,6 +571,24 @@ extern "C" pme_status pme_batch_set_tuning(pme_batch_t b, int brick_target, int 
   return PME_OK;
 }
 
+extern "C" pme_status pme_batch_set_half_bricks(pme_batch_t b, int on) {
+  if (!b) return set_error(PME_ERR_INVALID_ARG, "pme_batch_set_half_bricks: null batch");
+  if (on != 0 && on != 1) return set_error(PME_ERR_INVALID_ARG, "pme_batch_set_half_bricks: %d (0 or 1)", on);
+  if (on == b->tune.half_bricks) return PME_OK;
+  const Tuning old = b->tune;
+  b->tune.half_bricks = on;
+  drop_graph(b);
+  pme_status st = replan_all(b);
+  if (st == PME_ERR_BUDGET || st == PME_ERR_UNSUPPORTED) b->tune = old;  // nothing was freed
+  return st;
+}
+
+extern "C" pme_status pme_batch_get_half_bricks(pme_batch_t b, int* on) {
+  if (!b || !on) return set_error(PME_ERR_INVALID_ARG, "pme_batch_get_half_bricks: null argument");
+  *on = b->tune.half_bricks;
+  return PME_OK;
+}
+
 extern "C" pme_status pme_batch_set_chains(pme_batch_t b, int nchains) {
   if (!b) return set_error(PME_ERR_INVALID_ARG, "pme_batch_set_chains: null batch");
   if (nchains < 1 || nchains > kMaxBatchChains)
@@ -631,7 +649,7 @@ static std::string signature(pme_batch_t b) {
   return key;
 }
 
-// cache file: one JSON object {"key": [brick, tile, wg_per_cu, ms], ...}
+// cache file: one JSON object {"key": [brick, tile, wg_per_cu, chains, half_bricks, ms], ...}
 static std::map<std::string, std::vector<double>> read_cache(const char* path) {
   std::map<std::string, std::vector<double>> m;
   if (!path) return m;
@@ -651,7 +669,7 @@ static std::map<std::string, std::vector<double>> read_cache(const char* path) {
     std::stringstream vs(t.substr(lb + 1, rb - lb - 1));
     std::string tok;
     while (std::getline(vs, tok, ',')) vals.push_back(std::atof(tok.c_str()));
-    if (vals.size() == 5) m[key] = vals;  // brick, tile, wg/cu, chains, ms
+    if (vals.size() == 6) m[key] = vals;  // brick, tile, wg/cu, chains, half bricks, ms
     i = rb + 1;
   }
   return m;
@@ -665,7 +683,8 @@ static void write_cache(const char* path, const std::map<std::string, std::vecto
   size_t k = 0;
   for (auto& kv : m) {
     f << "  \"" << kv.first << "\": [" << (int)kv.second[0] << ", " << (int)kv.second[1] << ", " << (int)kv.second[2]
-      << ", " << (int)kv.second[3] << ", " << kv.second[4] << "]" << (++k < m.size() ? "," : "") << "\n";
+      << ", " << (int)kv.second[3] << ", " << (int)kv.second[4] << ", " << kv.second[5] << "]"
+      << (++k < m.size() ? "," : "") << "\n";
   }
   f << "}\n";
 }
@@ -693,9 +712,10 @@ extern "C" pme_status pme_batch_autotune(pme_batch_t b, const char* cache_path, 
   auto cache = read_cache(cache_path);
   auto hit = cache.find(key);
   if (hit != cache.end()) {
+    BTRY(pme_batch_set_half_bricks(b, (int)hit->second[4]));
     BTRY(pme_batch_set_tuning(b, (int)hit->second[0], (int)hit->second[1], (int)hit->second[2]));
     BTRY(pme_batch_set_chains(b, (int)hit->second[3]));
-    if (best_ms) *best_ms = (float)hit->second[4];
+    if (best_ms) *best_ms = (float)hit->second[5];
     return PME_OK;
   }
   // the outputs of INT64 systems accumulate: tuning solves would add into them
@@ -704,24 +724,30 @@ extern "C" pme_status pme_batch_autotune(pme_batch_t b, const char* cache_path, 
       return set_error(PME_ERR_STATE, "pme_batch_autotune: resident INT64 (accumulating) outputs");
   float best = 1e30f;
   int bb = b->tune.brick_target, bt = b->tune.fft_tile, bw = b->tune.wg_per_cu, bc = b->nchains;
-  for (int brick : {12, 16, 20, 24, 32})
-    for (int tile : {8, 16, 32})
-      for (int wpc : {0, 4})
-        for (int ch : {1, 2}) {
-          if (pme_batch_set_tuning(b, brick, tile, wpc) != PME_OK || pme_batch_set_chains(b, ch) != PME_OK) continue;
-          float ms = 0.f;
-          if (time_solves(b, 3, 10, &ms) != PME_OK) continue;
-          if (ms < best) {
-            best = ms;
-            bb = brick;
-            bt = tile;
-            bw = wpc;
-            bc = ch;
+  int bh = b->tune.half_bricks;
+  for (int half : {0, 1})
+    for (int brick : {12, 16, 20, 24, 32})
+      for (int tile : {8, 16, 32})
+        for (int wpc : {0, 4})
+          for (int ch : {1, 2}) {
+            if (pme_batch_set_half_bricks(b, half) != PME_OK || pme_batch_set_tuning(b, brick, tile, wpc) != PME_OK ||
+                pme_batch_set_chains(b, ch) != PME_OK)
+              continue;
+            float ms = 0.f;
+            if (time_solves(b, 3, 10, &ms) != PME_OK) continue;
+            if (ms < best) {
+              best = ms;
+              bb = brick;
+              bt = tile;
+              bw = wpc;
+              bc = ch;
+              bh = half;
+            }
           }
-        }
+  BTRY(pme_batch_set_half_bricks(b, bh));
   BTRY(pme_batch_set_tuning(b, bb, bt, bw));
   BTRY(pme_batch_set_chains(b, bc));
-  cache[key] = {(double)bb, (double)bt, (double)bw, (double)bc, (double)best};
+  cache[key] = {(double)bb, (double)bt, (double)bw, (double)bc, (double)bh, (double)best};
   write_cache(cache_path, cache);
   if (best_ms) *best_ms = best;
   return PME_OK;

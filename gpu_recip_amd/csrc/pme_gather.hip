@@ -119,7 +119,7 @@ __device__ __forceinline__ void gather_atom(const BinArgs& a, const float4& p, i
   }
 }
 
-// Brick b of one grid (one kSpreadBlock-thread workgroup); smem: the
+// Brick b of one grid (one NT-thread workgroup); smem: the
 // workgroup's dynamic LDS (gather_lds_bytes).  ft: PME_FORCE_* store mode.
 // The brick's potential plus a (p-1) upper halo is staged in LDS; the home
 // records of the brick (the atoms whose base grid index it holds, first in
@@ -129,11 +129,11 @@ __device__ __forceinline__ void gather_atom(const BinArgs& a, const float4& p, i
 // that bin and gather the same coordinates (the batched path: one solve reads
 // the caller's array once).  Otherwise the current coordinates are read, so
 // an atom moved since the binning is still right (gather_atom).
-template <int P, bool FROM_REC>
+template <int P, bool FROM_REC, int NT>
 __device__ __forceinline__ void gather_brick_body(const float4* __restrict__ xyzq, const BinArgs& a, int b, int wxm,
                                                   int wym, int wzm, const float* __restrict__ phi, float ccelec_f,
                                                   int ft, int stride, void* force, unsigned char* smem) {
-  __shared__ int sh_scan[kSpreadBlock / 64];
+  __shared__ int sh_scan[NT / 64];
   const int hmax = (wxm + P - 1) * (wym + P - 1) * (wzm + P - 1);
   float* pl = reinterpret_cast<float*>(smem);                 // halo brick, [hz][hy][hx]
   int* s_pre = reinterpret_cast<int*>(pl + ((hmax + 3) & ~3));  // [nchunk + 1]
@@ -150,16 +150,16 @@ __device__ __forceinline__ void gather_brick_body(const float4* __restrict__ xyz
 
   // stage brick + upper halo (periodic) in LDS in this brick's own [hz][hy][hx]
   // layout, so LDS word i is halo point i; up to kGatherStageU loads in
-  // flight per thread (one round for halos up to kGatherStageU * kSpreadBlock)
+  // flight per thread (one round for halos up to kGatherStageU * NT)
   const int hx = wx + P - 1, hy = wy + P - 1, hz = wz + P - 1;
   const int nh = hx * hy * hz;
   const int nxny = mul24(a.nx, a.ny);  // grid offsets fit 32 bits (edges <= 512)
   const SmallDiv dhx = small_div(hx), dhy = small_div(hy);
-  for (int i0 = tid; i0 < nh; i0 += kGatherStageU * kSpreadBlock) {
+  for (int i0 = tid; i0 < nh; i0 += kGatherStageU * NT) {
     float v[kGatherStageU];
 #pragma unroll
     for (int u = 0; u < kGatherStageU; u++) {
-      const int i = i0 + u * kSpreadBlock;
+      const int i = i0 + u * NT;
       const int t = sdiv(i, dhx);
       const int lx = i - t * hx;
       const int lz = sdiv(t, dhy);
@@ -172,20 +172,20 @@ __device__ __forceinline__ void gather_brick_body(const float4* __restrict__ xyz
     }
 #pragma unroll
     for (int u = 0; u < kGatherStageU; u++) {
-      const int i = i0 + u * kSpreadBlock;
+      const int i = i0 + u * NT;
       if (i < nh) pl[i] = v[u];
     }
   }
-  const int total = brick_segments<kSpreadBlock>(a, sb, s_pre, s_base, sh_scan);  // ends with a barrier
+  const int total = brick_segments<NT>(a, sb, s_pre, s_base, sh_scan);  // ends with a barrier
 
   // up to kRecPerThread home records per thread per round, every load of the
   // round issued before the first is used
-  for (int base = 0; base < total; base += kGatherCap) {
+  for (int base = 0; base < total; base += kRecPerThread * NT) {
     int atom[kRecPerThread];
     float4 pos[kRecPerThread];
 #pragma unroll
     for (int r = 0; r < kRecPerThread; r++) {
-      const int v = base + r * kSpreadBlock + tid;
+      const int v = base + r * NT + tid;
       atom[r] = -1;
       pos[r] = make_float4(0.f, 0.f, 0.f, 0.f);
       if (v < total) {
@@ -211,7 +211,8 @@ __global__ __launch_bounds__(kSpreadBlock) void k_gather_brick(const float4* __r
                                                                int wym, int wzm, const float* __restrict__ phi,
                                                                float ccelec_f, int ft, int stride, void* force) {
   extern __shared__ __align__(16) unsigned char smem[];
-  gather_brick_body<P, false>(xyzq, a, blockIdx.x, wxm, wym, wzm, phi, ccelec_f, ft, stride, force, smem);
+  gather_brick_body<P, false, kSpreadBlock>(xyzq, a, blockIdx.x, wxm, wym, wzm, phi, ccelec_f, ft, stride, force,
+                                            smem);
 }
 
 size_t gather_lds_bytes(int order, const int wmax[3], int nchunk) {
@@ -241,30 +242,39 @@ hipError_t launch_gather(int order, int force_type, const float4* xyzq, const Bi
 
 // ---- batched: persistent workgroups over the work items of many systems ----
 
-template <int P>
-__global__ __launch_bounds__(kSpreadBlock) void k_batch_gather(const SysDesc* __restrict__ sys,
-                                                               const WorkItem* __restrict__ items, int nitems,
-                                                               int* ctr) {
+// NT = kSpreadBlock / 2: half-size workgroups, two per CU, for bricks
+// planned within half the LDS budget
+template <int P, int NT>
+__global__ __launch_bounds__(NT) void k_batch_gather(const SysDesc* __restrict__ sys,
+                                                     const WorkItem* __restrict__ items, int nitems, int* ctr) {
   extern __shared__ __align__(16) unsigned char smem[];
   for (int it = blockIdx.x; it < nitems; it = next_item(nitems, ctr)) {
     const WorkItem w = uniform_item(items[it]);
     const SysDesc& d = sys[w.sys];
-    gather_brick_body<P, true>(d.xyzq, d.bin, w.idx, d.wmax[0], d.wmax[1], d.wmax[2], d.grid, (float)kCcelec, d.force_type,
+    gather_brick_body<P, true, NT>(d.xyzq, d.bin, w.idx, d.wmax[0], d.wmax[1], d.wmax[2], d.grid, (float)kCcelec, d.force_type,
                          d.stride, d.force, smem);
     __syncthreads();
   }
   items_done(nitems, ctr);
 }
 
-hipError_t launch_batch_gather(int order, const SysDesc* sys, const WorkItem* items, int nitems, int* ctr, int grid,
-                               size_t lds, hipStream_t s) {
+hipError_t launch_batch_gather(int order, int half, const SysDesc* sys, const WorkItem* items, int nitems, int* ctr,
+                               int grid, size_t lds, hipStream_t s) {
   if (nitems <= 0) return hipSuccess;
+  constexpr int H = kSpreadBlock / 2;
+#define PME_BGATHER(P)                                                                                           \
+  if (half)                                                                                                      \
+    hipLaunchKernelGGL((k_batch_gather<P, H>), dim3(grid), dim3(H), lds, s, sys, items, nitems, ctr);            \
+  else                                                                                                           \
+    hipLaunchKernelGGL((k_batch_gather<P, kSpreadBlock>), dim3(grid), dim3(kSpreadBlock), lds, s, sys, items,    \
+                       nitems, ctr);
   switch (order) {
-    case 4: hipLaunchKernelGGL(k_batch_gather<4>, dim3(grid), dim3(kSpreadBlock), lds, s, sys, items, nitems, ctr); break;
-    case 6: hipLaunchKernelGGL(k_batch_gather<6>, dim3(grid), dim3(kSpreadBlock), lds, s, sys, items, nitems, ctr); break;
-    case 8: hipLaunchKernelGGL(k_batch_gather<8>, dim3(grid), dim3(kSpreadBlock), lds, s, sys, items, nitems, ctr); break;
+    case 4: PME_BGATHER(4) break;
+    case 6: PME_BGATHER(6) break;
+    case 8: PME_BGATHER(8) break;
     default: return hipErrorInvalidValue;
   }
+#undef PME_BGATHER
   return hipGetLastError();
 }
 

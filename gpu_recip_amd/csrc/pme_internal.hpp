@@ -331,10 +331,12 @@ hipError_t launch_batch_plane(int inverse, int reduce, int maxnp, int split, int
 
 // batched launchers: `grid` persistent workgroups loop over the work items
 hipError_t launch_batch_bin(const SysDesc* sys, const WorkItem* items, int nitems, int grid, size_t lds, hipStream_t s);
-hipError_t launch_batch_spread(int order, const SysDesc* sys, const WorkItem* items, int nitems, int* ctr, int grid, size_t lds,
-                               hipStream_t s);
-hipError_t launch_batch_gather(int order, const SysDesc* sys, const WorkItem* items, int nitems, int* ctr, int grid, size_t lds,
-                               hipStream_t s);
+// spread / gather: half != 0 runs kSpreadBlock / 2 threads per workgroup
+// (bricks planned within half the LDS budget, two workgroups per CU)
+hipError_t launch_batch_spread(int order, int half, const SysDesc* sys, const WorkItem* items, int nitems, int* ctr,
+                               int grid, size_t lds, hipStream_t s);
+hipError_t launch_batch_gather(int order, int half, const SysDesc* sys, const WorkItem* items, int nitems, int* ctr,
+                               int grid, size_t lds, hipStream_t s);
 hipError_t launch_batch_fft_x(int inverse, const SysDesc* sys, const WorkItem* items, int nitems, int grid, size_t lds,
                               hipStream_t s);
 // axis 1 (y) or 2 (z); mode COL_*; with reduce != 0 the first tile of every
@@ -357,7 +359,6 @@ size_t fft_cols_lds_bytes(int n, int TX, int nbuf = 2);
 
 constexpr int kSpreadBlock = 1024;  // threads per spread / gather workgroup (16 waves per CU)
 constexpr int kRecPerThread = 2;    // gather: brick records read per thread per round
-constexpr int kGatherCap = kRecPerThread * kSpreadBlock;  // gather: home atoms per round
 constexpr int kGatherStageU = 24;   // gather: halo loads in flight per thread (one round up to 24 K points)
 constexpr int kBinBlock = 512;     // threads per binning workgroup (one atom each at the default chunk)
 constexpr int kPlaneBlock = 1024;   // threads per xy plane FFT workgroup

@@ -120,7 +120,7 @@ std::vector<float2> twiddles(int n) {
   return t;
 }
 
-bool plan_bricks(const int n[3], int order, int target, BrickPlan* bp) {
+bool plan_bricks(const int n[3], int order, int target, BrickPlan* bp, size_t budget) {
   for (int a = 0; a < 3; a++) {
     int nb = (n[a] + target / 2) / target;
     if (nb < 1) nb = 1;
@@ -128,8 +128,7 @@ bool plan_bricks(const int n[3], int order, int target, BrickPlan* bp) {
     bp->nb[a] = nb;
     bp->wmax[a] = (n[a] + nb - 1) / nb;
   }
-  while (spread_lds_bytes(order, bp->wmax, kMaxChunks) > kBrickLdsBudget ||
-         gather_lds_bytes(order, bp->wmax, kMaxChunks) > kBrickLdsBudget) {
+  while (spread_lds_bytes(order, bp->wmax, kMaxChunks) > budget || gather_lds_bytes(order, bp->wmax, kMaxChunks) > budget) {
     int best = -1;
     for (int a = 0; a < 3; a++)
       if (n[a] / (bp->nb[a] + 1) >= order - 1 && (best < 0 || bp->wmax[a] > bp->wmax[best])) best = a;

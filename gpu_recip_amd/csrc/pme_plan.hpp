@@ -41,8 +41,8 @@ struct BrickPlan {
   int nbins;
 };
 // bricks of about `target` points per edge, each at least order-1 wide, LDS
-// footprints of spread and gather within kBrickLdsBudget
-bool plan_bricks(const int n[3], int order, int target, BrickPlan* bp);
+// footprints of spread and gather within `budget` bytes
+bool plan_bricks(const int n[3], int order, int target, BrickPlan* bp, size_t budget = kBrickLdsBudget);
 
 struct FftTiles {
   int NR, TXy, TXz;

@@ -38,7 +38,7 @@ pme_status validate_system(const pme_system_desc* s) {
 
 pme_status plan_system(const pme_system_desc& in, int cap_atoms, const Tuning& t, SystemPlan* p) {
   const int n[3] = {in.nfftx, in.nffty, in.nfftz};
-  if (!plan_bricks(n, in.order, t.brick_target, &p->bp))
+  if (!plan_bricks(n, in.order, t.brick_target, &p->bp, t.half_bricks ? kBrickLdsBudget / 2 : kBrickLdsBudget))
     return set_error(PME_ERR_UNSUPPORTED, "system: no brick plan fits the LDS budget");
   p->cp = plan_chunks(std::max(cap_atoms, 1));
   const bool even = (n[0] % 2) == 0;
@@ -279,14 +279,14 @@ void build_items(const SysDesc* const* descs, const SystemPlan* const* plans, co
 }
 
 // Workgroups of a batched launch.  Spread and gather (queue != 0) always run
-// one persistent workgroup per CU taking bricks from the launch's work queue
-// (their LDS brick leaves room for one workgroup per CU anyway, and a
-// persistent workgroup saves the start of a new one per brick).  The others
+// one persistent workgroup per CU (two half-size ones with half_bricks)
+// taking bricks from the launch's work queue (their LDS brick leaves room for
+// no more, and a persistent workgroup saves the start of a new one per brick).  The others
 // run one workgroup per item with wg_per_cu == 0, else wg_per_cu persistent
 // workgroups per CU striding over the items; the plane kernels (cap > 0)
 // cap per CU (one full-size or two half-size workgroups fill a CU).
 static int grid_for(const Tuning& t, int ncu, int count, int cap, bool queue) {
-  if (queue) return std::max(1, std::min(count, ncu));
+  if (queue) return std::max(1, std::min(count, ncu * (t.half_bricks ? 2 : 1)));
   if (t.wg_per_cu <= 0) return count;
   const int per_cu = cap > 0 ? cap : t.wg_per_cu;
   return std::max(1, std::min(count, ncu * per_cu));
@@ -321,7 +321,9 @@ pme_status enqueue_stages(const std::vector<Launch>& launches, const SysDesc* d_
         const int grid = grid_for(t, ncu, L.count, stage == BS_PLANE ? (L.half ? 2 : 1) : 0, queue);
         switch (stage) {
           case BS_BIN: SCHED_HIP(launch_batch_bin(d_sys, it, L.count, grid, L.lds, stream)); break;
-          case BS_SPREAD: SCHED_HIP(launch_batch_spread(L.order, d_sys, it, L.count, ctr, grid, L.lds, stream)); break;
+          case BS_SPREAD:
+            SCHED_HIP(launch_batch_spread(L.order, t.half_bricks, d_sys, it, L.count, ctr, grid, L.lds, stream));
+            break;
           case BS_FFTX: SCHED_HIP(launch_batch_fft_x(inv ? 1 : 0, d_sys, it, L.count, grid, L.lds, stream)); break;
           case BS_FFTY:
             SCHED_HIP(launch_batch_cols(1, inv ? COL_INV : COL_FWD, inv ? 1 : 0, L.split, d_sys, it, L.count, grid, L.lds,
@@ -337,7 +339,9 @@ pme_status enqueue_stages(const std::vector<Launch>& launches, const SysDesc* d_
             else
               SCHED_HIP(launch_batch_cols(2, COL_FWD_SOLVE_INV, 0, L.split, d_sys, it, L.count, grid, L.lds, stream));
             break;
-          case BS_GATHER: SCHED_HIP(launch_batch_gather(L.order, d_sys, it, L.count, ctr, grid, L.lds, stream)); break;
+          case BS_GATHER:
+            SCHED_HIP(launch_batch_gather(L.order, t.half_bricks, d_sys, it, L.count, ctr, grid, L.lds, stream));
+            break;
           default: break;
         }
       }

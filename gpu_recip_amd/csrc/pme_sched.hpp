@@ -37,6 +37,8 @@ struct Tuning {
   int brick_target = kDefaultBrick;
   int fft_tile = kDefaultFftTile;
   int wg_per_cu = 0;    // 0: one workgroup per work item
+  int half_bricks = 0;  // 1: bricks within half the LDS budget, spread and gather in half-size
+                        // workgroups, two per CU (one's HBM phases overlap the other's LDS work)
 };
 
 pme_status validate_system(const pme_system_desc* s);

@@ -221,13 +221,13 @@ __device__ __forceinline__ void spread_record(const BinArgs& a, float4 p, int x0
 // (one brick along x): the high margin is the periodic image of the row
 // start and is added in.  Products of small non-negative integers in 24 bits
 // (edges <= 512), grid offsets in 32 bits (nx*ny < 2^19, offsets < 2^27).
-template <int P, bool WRAP>
+template <int P, bool WRAP, int NT>
 __device__ __forceinline__ void store_brick_pairs(const unsigned long long* __restrict__ acc, float* __restrict__ grid,
                                                   int nx, int ny, int x0, int y0, int z0, int wx, int wy, int hw,
                                                   int npair, int bxs, int plane) {
   const int nxny = mul24(nx, ny);
   const SmallDiv dhw = small_div(hw), dwy = small_div(wy);
-  for (int i = threadIdx.x; i < npair; i += kSpreadBlock) {
+  for (int i = threadIdx.x; i < npair; i += NT) {
     const int t = sdiv(i, dhw);
     const int lx = 2 * (i - mul24(t, hw));
     const int lz = sdiv(t, dwy);
@@ -244,15 +244,15 @@ __device__ __forceinline__ void store_brick_pairs(const unsigned long long* __re
   }
 }
 
-// Brick b of one grid (one kSpreadBlock-thread workgroup); smem: the
-// workgroup's dynamic LDS (spread_lds_bytes).  Each thread takes one record
+// Brick b of one grid (one NT-thread workgroup); smem: the workgroup's
+// dynamic LDS (spread_lds_bytes).  Each thread takes one record
 // per batch (an atom whose stencil touches the brick), keeps q*theta_x,
 // theta_y, theta_z in registers and adds its p^3 stencil into the brick
 // (spread_record).
-template <int P>
+template <int P, int NT>
 __device__ __forceinline__ void spread_brick_body(const BinArgs& a, int b, int wxm, int wym, int wzm,
                                                   float* __restrict__ grid, unsigned char* smem) {
-  __shared__ int sh_scan[kSpreadBlock / 64];
+  __shared__ int sh_scan[NT / 64];
 
   const int bxs = spread_xstride(wxm, P);
   unsigned long long* acc = reinterpret_cast<unsigned long long*>(smem);
@@ -269,14 +269,14 @@ __device__ __forceinline__ void spread_brick_body(const BinArgs& a, int b, int w
   const int plane = mul24(wym, bxs);
 
   const SegBounds sb = brick_seg_load<false>(a, b);  // in flight while the brick is cleared
-  for (int i = tid; i < acc_n; i += kSpreadBlock) acc[i] = 0ull;
-  const int total = brick_segments<kSpreadBlock>(a, sb, s_pre, s_base, sh_scan);  // ends with a barrier
+  for (int i = tid; i < acc_n; i += NT) acc[i] = 0ull;
+  const int total = brick_segments<NT>(a, sb, s_pre, s_base, sh_scan);  // ends with a barrier
 
-  for (int base = 0; base < total; base += kSpreadBlock) {
+  for (int base = 0; base < total; base += NT) {
     // records of a batch in a strided order (stride 17, coprime with the
     // batch size): consecutive records are often atoms of one molecule with
     // the same stencil rows, and integer sums do not depend on the order
-    const int nbat = min(kSpreadBlock, total - base);
+    const int nbat = min(NT, total - base);
     if (tid >= nbat) break;
     const int stride = (nbat % 17) ? 17 : ((nbat % 19) ? 19 : 23);
     const int lt = tid * stride;
@@ -291,16 +291,16 @@ __device__ __forceinline__ void spread_brick_body(const BinArgs& a, int b, int w
   const int hw = (wx + 1) >> 1;
   const int npair = hw * wy * wz;
   if (wrapx)
-    store_brick_pairs<P, true>(acc, grid, a.nx, a.ny, x0, y0, z0, wx, wy, hw, npair, bxs, plane);
+    store_brick_pairs<P, true, NT>(acc, grid, a.nx, a.ny, x0, y0, z0, wx, wy, hw, npair, bxs, plane);
   else
-    store_brick_pairs<P, false>(acc, grid, a.nx, a.ny, x0, y0, z0, wx, wy, hw, npair, bxs, plane);
+    store_brick_pairs<P, false, NT>(acc, grid, a.nx, a.ny, x0, y0, z0, wx, wy, hw, npair, bxs, plane);
 }
 
 template <int P>
 __global__ __launch_bounds__(kSpreadBlock) void k_spread_brick(BinArgs a, int wxm, int wym, int wzm,
                                                                float* __restrict__ grid) {
   extern __shared__ __align__(16) unsigned char smem[];
-  spread_brick_body<P>(a, blockIdx.x, wxm, wym, wzm, grid, smem);
+  spread_brick_body<P, kSpreadBlock>(a, blockIdx.x, wxm, wym, wzm, grid, smem);
 }
 
 size_t spread_lds_bytes(int order, const int wmax[3], int nchunk) {
@@ -353,15 +353,16 @@ __global__ __launch_bounds__(kBinBlock) void k_batch_bin(const SysDesc* __restri
   }
 }
 
-template <int P>
-__global__ __launch_bounds__(kSpreadBlock) void k_batch_spread(const SysDesc* __restrict__ sys,
-                                                               const WorkItem* __restrict__ items, int nitems,
-                                                               int* ctr) {
+// NT = kSpreadBlock / 2: half-size workgroups, two per CU, for bricks
+// planned within half the LDS budget
+template <int P, int NT>
+__global__ __launch_bounds__(NT) void k_batch_spread(const SysDesc* __restrict__ sys,
+                                                     const WorkItem* __restrict__ items, int nitems, int* ctr) {
   extern __shared__ __align__(16) unsigned char smem[];
   for (int it = blockIdx.x; it < nitems; it = next_item(nitems, ctr)) {
     const WorkItem w = uniform_item(items[it]);
     const SysDesc& d = sys[w.sys];
-    spread_brick_body<P>(d.bin, w.idx, d.wmax[0], d.wmax[1], d.wmax[2], d.grid, smem);
+    spread_brick_body<P, NT>(d.bin, w.idx, d.wmax[0], d.wmax[1], d.wmax[2], d.grid, smem);
     __syncthreads();
   }
   items_done(nitems, ctr);
@@ -374,15 +375,23 @@ hipError_t launch_batch_bin(const SysDesc* sys, const WorkItem* items, int nitem
   return hipGetLastError();
 }
 
-hipError_t launch_batch_spread(int order, const SysDesc* sys, const WorkItem* items, int nitems, int* ctr, int grid,
-                               size_t lds, hipStream_t s) {
+hipError_t launch_batch_spread(int order, int half, const SysDesc* sys, const WorkItem* items, int nitems, int* ctr,
+                               int grid, size_t lds, hipStream_t s) {
   if (nitems <= 0) return hipSuccess;
+  constexpr int H = kSpreadBlock / 2;
+#define PME_BSPREAD(P)                                                                                           \
+  if (half)                                                                                                      \
+    hipLaunchKernelGGL((k_batch_spread<P, H>), dim3(grid), dim3(H), lds, s, sys, items, nitems, ctr);            \
+  else                                                                                                           \
+    hipLaunchKernelGGL((k_batch_spread<P, kSpreadBlock>), dim3(grid), dim3(kSpreadBlock), lds, s, sys, items,    \
+                       nitems, ctr);
   switch (order) {
-    case 4: hipLaunchKernelGGL(k_batch_spread<4>, dim3(grid), dim3(kSpreadBlock), lds, s, sys, items, nitems, ctr); break;
-    case 6: hipLaunchKernelGGL(k_batch_spread<6>, dim3(grid), dim3(kSpreadBlock), lds, s, sys, items, nitems, ctr); break;
-    case 8: hipLaunchKernelGGL(k_batch_spread<8>, dim3(grid), dim3(kSpreadBlock), lds, s, sys, items, nitems, ctr); break;
+    case 4: PME_BSPREAD(4) break;
+    case 6: PME_BSPREAD(6) break;
+    case 8: PME_BSPREAD(8) break;
     default: return hipErrorInvalidValue;
   }
+#undef PME_BSPREAD
   return hipGetLastError();
 }
 

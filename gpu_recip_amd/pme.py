@@ -299,6 +299,15 @@ class PmeBatch:
         L.call("pme_batch_get_chains", self.h, C.byref(v))
         return v.value
 
+    def set_half_bricks(self, on: bool) -> None:
+        """Half-size spread/gather bricks, two workgroups per CU (pme_batch_set_half_bricks)."""
+        L.call("pme_batch_set_half_bricks", self.h, 1 if on else 0)
+
+    def get_half_bricks(self) -> bool:
+        v = C.c_int()
+        L.call("pme_batch_get_half_bricks", self.h, C.byref(v))
+        return bool(v.value)
+
     def autotune(self, cache_path=None) -> float:
         ms = C.c_float()
         L.call("pme_batch_autotune", self.h, None if cache_path is None else str(cache_path).encode(), C.byref(ms))

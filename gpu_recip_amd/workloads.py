@@ -3,7 +3,7 @@
     python -m gpu_recip_amd.workloads production [--solves 20] [--precision fp32|bf16x3|fp16x3]
                                                  [--side-stream] [--one-shot] [--graph]
     python -m gpu_recip_amd.workloads split      [--solves 20] [--precision ...]
-    python -m gpu_recip_amd.workloads batched    [--solves 20] [--tuning brick,tile,wg_per_cu[,chains]]
+    python -m gpu_recip_amd.workloads batched    [--solves 20] [--tuning brick,tile,wg_per_cu[,chains[,half]]]
     python -m gpu_recip_amd.workloads small      [--solves 20]
     python -m gpu_recip_amd.workloads served     [--solves 1000]
     python -m gpu_recip_amd.workloads direct     [--solves 20]
@@ -41,7 +41,8 @@ def main(argv=None):
     ap.add_argument("--one-shot", action="store_true", help="plain: pme_recip_solve instead of the five calls")
     ap.add_argument("--graph", action="store_true", help="plain: pme_recip_solve replayed as a hipGraph")
     ap.add_argument("--tuning", default=None,
-                    help="batched: brick,tile,wg_per_cu[,chains] to use instead of autotuning (e.g. 24,32,4,2)")
+                    help="batched: brick,tile,wg_per_cu[,chains[,half bricks]] to use instead of autotuning "
+                         "(e.g. 24,32,4,2,1)")
     args = ap.parse_args(argv)
 
     import torch
@@ -81,14 +82,17 @@ def main(argv=None):
                                            precision=PRECS[args.precision]), keep=(x, f)))
         if args.tuning:
             v = [int(t) for t in args.tuning.split(",")]
+            if len(v) > 4:
+                b.set_half_bricks(bool(v[4]))
             b.set_tuning(*v[:3])
             if len(v) > 3:
                 b.set_chains(v[3])
-            print(f"batched: plan (brick, tile, wg/cu) = {b.get_tuning()}, {b.get_chains()} chain(s)", flush=True)
+            print(f"batched: plan (brick, tile, wg/cu) = {b.get_tuning()}, {b.get_chains()} chain(s), "
+                  f"half bricks {b.get_half_bricks()}", flush=True)
         else:
             ms = b.autotune(None)  # the plan the benchmark uses
             print(f"batched: autotuned (brick, tile, wg/cu) = {b.get_tuning()}, {b.get_chains()} chain(s), "
-                  f"{ms:.3f} ms per batched solve", flush=True)
+                  f"half bricks {b.get_half_bricks()}, {ms:.3f} ms per batched solve", flush=True)
         t0 = time.perf_counter()
         for _ in range(args.solves):
             b.solve()

@@ -249,8 +249,14 @@ pme_status pme_batch_get_tuning(pme_batch_t b, int* brick_target, int* fft_tile,
  * run one chain. */
 pme_status pme_batch_set_chains(pme_batch_t b, int nchains);
 pme_status pme_batch_get_chains(pme_batch_t b, int* nchains);
+/* Half bricks (on = 1; default 0): bricks planned within half the LDS of a
+ * spread/gather workgroup, run as half-size workgroups, two per CU, so one's
+ * HBM phases overlap the other's LDS work.  Re-plans every resident system;
+ * results do not depend on it. */
+pme_status pme_batch_set_half_bricks(pme_batch_t b, int on);
+pme_status pme_batch_get_half_bricks(pme_batch_t b, int* on);
 /* Autotuner: time the current resident set over a grid of tuning choices
- * (brick target, FFT tile, workgroups per CU, stage chains), keep the
+ * (half bricks, brick target, FFT tile, workgroups per CU, stage chains), keep the
  * fastest, and persist it in a JSON cache keyed by the set's signature and
  * the GPU architecture (cache_path may be NULL).  Returns the best time per
  * solve (ms). */

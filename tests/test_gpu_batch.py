@@ -227,6 +227,36 @@ def test_two_chains_match_one(wg_per_cu):
         b.set_chains(0)
 
 
+@pytest.mark.parametrize("order", [4, 6, 8])
+def test_half_bricks_match_full(order):
+    """Half bricks (smaller bricks, half-size spread/gather workgroups, two
+    per CU) give the forces, energies and virials of full bricks bit for bit:
+    the charge grid is an integer sum whatever the bricks, and each atom's
+    gather reads the same potential values in the same order."""
+    torch, dev = torch_dev()
+    sl = [systems.PmeSystem(s.name, s.xyzq, s.box, s.nfft, order, s.kappa) for s in systems.config_batched(6)]
+    b = PmeBatch(8 << 30)
+    reg = [_add(b, s, torch, dev) for s in sl]
+    ids = [r[0] for r in reg]
+    out = {}
+    for half in (False, True):
+        b.set_half_bricks(half)
+        assert b.get_half_bricks() == half
+        for graph in (True, False):
+            b.set_graph(graph)
+            for f in (r[2] for r in reg):
+                f.zero_()
+            b.solve()
+            e, v = b.results(ids)
+            out[(half, graph)] = ([_forces(r[2], s.natom, r[3]) for s, r in zip(sl, reg)], e, v)
+    ref = out[(False, False)]
+    for key, (fs, e, v) in out.items():
+        assert all(np.array_equal(a, c) for a, c in zip(fs, ref[0])), key
+        assert np.array_equal(e, ref[1]) and np.array_equal(v, ref[2]), key
+    with pytest.raises(L.PmeError):
+        L.call("pme_batch_set_half_bricks", b.h, 2)
+
+
 def test_batched_config_vs_oracle():
     """The 64-system batched workload: every system against the single-system
     solver, eight of them against the FP64 oracle (exact coordinates)."""
@@ -374,6 +404,7 @@ def test_autotune_cache(tmp_path):
     assert ms > 0 and cache.exists()
     entry = list(json.loads(cache.read_text()).values())[0]
     assert tuple(int(v) for v in entry[:3]) == tune
+    assert int(entry[3]) == b.get_chains() and bool(entry[4]) == b.get_half_bricks()
     # results with the tuned plan: forces to FP32 rounding (tiles change the
     # FFT's summation order), energies to 1e-6
     b.solve()
@@ -387,6 +418,7 @@ def test_autotune_cache(tmp_path):
         _add(b2, s, torch, dev)
     ms2 = b2.autotune(str(cache))
     assert b2.get_tuning() == tune and math.isclose(ms2, ms, rel_tol=1e-5)
+    assert b2.get_half_bricks() == b.get_half_bricks()
 
 
 def test_stage_timing():
