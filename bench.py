@@ -70,13 +70,15 @@ def stage_bytes(s):
     }
 
 
-# kernels behind each timing slot of the batched solve (order-4 systems whose
-# xy planes fit one workgroup, as in the 64-system set; the z pass runs one
-# launch per four-step register class present: N2 <= 10, and 75 = 5 x 15)
-BATCH_KERNELS = {"bin": ("pme::k_batch_bin",), "spread": ("void pme::k_batch_spread<4>",),
-                 "fft_fwd_x_or_xy": ("void pme::k_batch_plane<0, false>",),
+# kernels behind each timing slot of the batched solve of the 64-system set
+# (order 4, full bricks, every xy plane fitting a half-size plane workgroup;
+# the z pass runs one launch per four-step register class present: N2 <= 10,
+# and 75 = 5 x 15)
+BATCH_KERNELS = {"bin": ("pme::k_batch_bin",), "spread": ("void pme::k_batch_spread<4, 1024>",),
+                 "fft_fwd_x_or_xy": ("void pme::k_batch_plane<0, false, 512>",),
                  "fft_z_solve": ("void pme::k_batch_zsolve<0>", "void pme::k_batch_zsolve<1>"),
-                 "fft_inv_y_or_xy": ("void pme::k_batch_plane<1, false>",), "gather": ("void pme::k_batch_gather<4>",)}
+                 "fft_inv_y_or_xy": ("void pme::k_batch_plane<1, false, 512>",),
+                 "gather": ("void pme::k_batch_gather<4, 1024>",)}
 
 
 def stage_traffic(kernels, workload="batched"):
