@@ -613,6 +613,10 @@ def main():
                                        "solve_roofline_frac": s2.algorithmic_bytes() / (dt_2g / args.steps)
                                        / (HBM_PEAK_GBS * 1e9)}},
         "stage_ms": {STAGES[i]: float(stage_ms[i]) for i in range(len(STAGES))},
+        # each stage's algorithmic bytes over its mean time, against 8 TB/s (the
+        # empty slots of the plane layout carry only event overhead)
+        "stage_roofline_frac": {STAGES[i]: sb[STAGES[i]] / (stage_ms[i] * 1e-3) / (HBM_PEAK_GBS * 1e9)
+                                for i in range(len(STAGES)) if sb[STAGES[i]] > 0},
         "solve_algorithmic_bytes": bytes_b,
         "solve_roofline_frac": bytes_b / (ms_b * 1e-3) / (HBM_PEAK_GBS * 1e9),
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

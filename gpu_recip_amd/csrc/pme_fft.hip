@@ -1180,7 +1180,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_zsolve(const SysDesc* __restri
 // systems whose plane fits half the LDS budget: one workgroup's HBM loads and
 // stores then overlap the other's transforms.
 template <int INV, bool SPLIT, int NT>
-__global__ __launch_bounds__(NT) void k_batch_plane(const SysDesc* __restrict__ sys,
+__global__ __launch_bounds__(NT, 4) void k_batch_plane(const SysDesc* __restrict__ sys,
                                                     const WorkItem* __restrict__ items, int nitems, int reduce,
                                                     int maxnp) {
   extern __shared__ __align__(16) float2 sm[];

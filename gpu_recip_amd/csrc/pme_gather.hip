@@ -149,31 +149,32 @@ __device__ __forceinline__ void gather_brick_body(const float4* __restrict__ xyz
   const SegBounds sb = brick_seg_load<true>(a, b);
 
   // stage brick + upper halo (periodic) in LDS in this brick's own [hz][hy][hx]
-  // layout, so LDS word i is halo point i; up to kGatherStageU loads in
-  // flight per thread (one round for halos up to kGatherStageU * NT)
+  // layout: thread c takes column (lx, ly) = (c mod hx, c / hx) of the halo
+  // brick and walks it in z, kGatherStageU loads in flight (one division per
+  // thread; consecutive threads read consecutive x, as rows of the grid)
   const int hx = wx + P - 1, hy = wy + P - 1, hz = wz + P - 1;
-  const int nh = hx * hy * hz;
+  const int ncol = hx * hy;
   const int nxny = mul24(a.nx, a.ny);  // grid offsets fit 32 bits (edges <= 512)
-  const SmallDiv dhx = small_div(hx), dhy = small_div(hy);
-  for (int i0 = tid; i0 < nh; i0 += kGatherStageU * NT) {
-    float v[kGatherStageU];
+  for (int c = tid; c < ncol; c += NT) {
+    const int ly = c / hx;
+    const int lx = c - ly * hx;
+    int gx = x0 + lx, gy = y0 + ly;
+    if (gx >= a.nx) gx -= a.nx;
+    if (gy >= a.ny) gy -= a.ny;
+    const float* col = phi + (gx + mul24(a.nx, gy));
+    for (int lz0 = 0; lz0 < hz; lz0 += kGatherStageU) {
+      float v[kGatherStageU];
 #pragma unroll
-    for (int u = 0; u < kGatherStageU; u++) {
-      const int i = i0 + u * NT;
-      const int t = sdiv(i, dhx);
-      const int lx = i - t * hx;
-      const int lz = sdiv(t, dhy);
-      const int ly = t - lz * hy;
-      int gx = x0 + lx, gy = y0 + ly, gz = z0 + lz;
-      if (gx >= a.nx) gx -= a.nx;
-      if (gy >= a.ny) gy -= a.ny;
-      if (gz >= a.nz) gz -= a.nz;
-      v[u] = (i < nh) ? phi[gx + mul24(a.nx, gy) + mul24(nxny, gz)] : 0.f;
-    }
+      for (int u = 0; u < kGatherStageU; u++) {
+        // z0 + lz < nz + P - 1 < 2 nz for lz < hz: one wrap; 0 past the halo
+        const int lz = lz0 + u;
+        int gz = z0 + lz;
+        if (gz >= a.nz) gz -= a.nz;
+        v[u] = lz < hz ? col[mul24(nxny, lz < hz ? gz : 0)] : 0.f;
+      }
 #pragma unroll
-    for (int u = 0; u < kGatherStageU; u++) {
-      const int i = i0 + u * NT;
-      if (i < nh) pl[i] = v[u];
+      for (int u = 0; u < kGatherStageU; u++)
+        if (lz0 + u < hz) pl[mul24(lz0 + u, ncol) + c] = v[u];
     }
   }
   const int total = brick_segments<NT>(a, sb, s_pre, s_base, sh_scan);  // ends with a barrier
@@ -243,9 +244,9 @@ hipError_t launch_gather(int order, int force_type, const float4* xyzq, const Bi
 // ---- batched: persistent workgroups over the work items of many systems ----
 
 // NT = kSpreadBlock / 2: half-size workgroups, two per CU, for bricks
-// planned within half the LDS budget
+// planned within half the LDS budget (four waves per SIMD: at most 128 VGPRs)
 template <int P, int NT>
-__global__ __launch_bounds__(NT) void k_batch_gather(const SysDesc* __restrict__ sys,
+__global__ __launch_bounds__(NT, 4) void k_batch_gather(const SysDesc* __restrict__ sys,
                                                      const WorkItem* __restrict__ items, int nitems, int* ctr) {
   extern __shared__ __align__(16) unsigned char smem[];
   for (int it = blockIdx.x; it < nitems; it = next_item(nitems, ctr)) {

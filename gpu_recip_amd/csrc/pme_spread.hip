@@ -354,9 +354,9 @@ __global__ __launch_bounds__(kBinBlock) void k_batch_bin(const SysDesc* __restri
 }
 
 // NT = kSpreadBlock / 2: half-size workgroups, two per CU, for bricks
-// planned within half the LDS budget
+// planned within half the LDS budget (four waves per SIMD: at most 128 VGPRs)
 template <int P, int NT>
-__global__ __launch_bounds__(NT) void k_batch_spread(const SysDesc* __restrict__ sys,
+__global__ __launch_bounds__(NT, 4) void k_batch_spread(const SysDesc* __restrict__ sys,
                                                      const WorkItem* __restrict__ items, int nitems, int* ctr) {
   extern __shared__ __align__(16) unsigned char smem[];
   for (int it = blockIdx.x; it < nitems; it = next_item(nitems, ctr)) {
