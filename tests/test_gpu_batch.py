@@ -114,6 +114,33 @@ def test_plane_groups_match_single(reps):
             assert rel_fro(vk, one["virial"]) < 1e-12, s.name
 
 
+def test_full_and_half_plane_workgroups():
+    """A batch mixing planes that fit a half-size plane workgroup (two per CU)
+    with planes that need a full-size one (120 x 100 and 100 x 96: 100 and 81 KB
+    of LDS, above the 75 KB half budget; and an odd 81 x 80 plane, whose x
+    lines pair two real rows) runs two plane launches per direction; every
+    system's forces equal the single-system solver's bit for bit."""
+    torch, dev = torch_dev()
+    specs = [(4, (120, 100, 40), (110.0, 92.0, 36.0), 4000), (6, (100, 96, 36), (92.0, 88.0, 33.0), 3000),
+             (4, (81, 80, 30), (74.0, 73.0, 27.0), 2000), (4, (48, 48, 48), (40.0, 40.0, 40.0), 2667),
+             (8, (45, 36, 40), (41.0, 33.0, 37.0), 1500)]
+    sl = [systems.PmeSystem(f"pl{i}", systems.triplets(nt, box, systems.SEED_BASE + 900 + i), box, nfft, order, 0.34)
+          for i, (order, nfft, box, nt) in enumerate(specs)]
+    b = PmeBatch(8 << 30)
+    reg = [_add(b, s, torch, dev) for s in sl]
+    ids = [r[0] for r in reg]
+    for wpc in (0, 4):
+        b.set_tuning(24, 32, wpc)
+        for graph in (True, False):
+            b.set_graph(graph)
+            b.solve()
+            e, v = b.results(ids)
+            for s, (sid, x, f, stride), ek in zip(sl, reg, e):
+                one = run_solve(s.nfft, s.order, s.kappa, s.box, s.xyzq)
+                assert np.array_equal(_forces(f, s.natom, stride), one["force"]), (s.name, wpc, graph)
+                assert rel_err(ek, one["energy"]) < 1e-12, (s.name, wpc, graph)
+
+
 def test_batch_edge_systems():
     """A batch holding an empty system (no atoms, null pointers), atoms far
     outside the box (any periodic image) and a clustered system whose atoms
