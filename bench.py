@@ -71,12 +71,11 @@ def stage_bytes(s):
 
 
 # kernels behind each timing slot of the batched solve of the 64-system set
-# (order 4, full bricks, every xy plane fitting a half-size plane workgroup;
-# the z pass runs one launch per four-step register class present: N2 <= 10,
-# and 75 = 5 x 15)
+# (order 4, full bricks, every xy plane fitting a half-size plane workgroup,
+# every z length a four-step pair of register class 0)
 BATCH_KERNELS = {"bin": ("pme::k_batch_bin",), "spread": ("void pme::k_batch_spread<4, 1024>",),
                  "fft_fwd_x_or_xy": ("void pme::k_batch_plane<0, false, 512>",),
-                 "fft_z_solve": ("void pme::k_batch_zsolve<0>", "void pme::k_batch_zsolve<1>"),
+                 "fft_z_solve": ("void pme::k_batch_zsolve<0>",),
                  "fft_inv_y_or_xy": ("void pme::k_batch_plane<1, false, 512>",),
                  "gather": ("void pme::k_batch_gather<4, 1024>",)}
 

@@ -732,13 +732,14 @@ int four_step_code(int n) {
 // Register class of a factor pair.  A kernel that dispatches over several
 // pairs is allocated the registers of their union, which grows well beyond
 // the largest single pair (all 31 pairs and the Stockham stages: 157 VGPRs,
-// three waves per SIMD; one pair: 72 to 118).  The fused z pass therefore
-// has one kernel per class: class 0 holds the pairs with N2 <= 10 (114 VGPRs,
-// four waves per SIMD), class 1 the rest.
-constexpr int f4_class_of(int n2) { return n2 <= 10 ? 0 : 1; }
+// three waves per SIMD; one pair: 72 to 118).  The batched z pass therefore
+// has one kernel per class: class 0 holds the pairs with N2 <= 10 and 5 x 15
+// (75, a common edge; held to 128 VGPRs, four waves per SIMD, with three
+// spilled), class 1 the rest (125 VGPRs).
+constexpr int f4_class_of(int n1, int n2) { return (n2 <= 10 || (n1 == 5 && n2 == 15)) ? 0 : 1; }
 int four_step_class(int code) {
 #define PME_F4_CLS(c, a, b) \
-  if (code == (c)) return f4_class_of(b);
+  if (code == (c)) return f4_class_of(a, b);
   PME_F4_PAIRS(PME_F4_CLS)
 #undef PME_F4_CLS
   return -1;
@@ -907,7 +908,7 @@ __device__ __forceinline__ void zsolve_four_step_dispatch(int code, float2* __re
   switch (code) {
 #define PME_F4_CASE(c, a, b)                                                                          \
   case c:                                                                                             \
-    if constexpr (f4_class_of(b) == CLS)                                                              \
+    if constexpr (f4_class_of(a, b) == CLS)                                                              \
       zsolve_four_step<a, b>(cg, hx, TX, pt_stride, outer_stride, ax, sa, tx, ty, blk, sm, hxd);      \
     break;
     PME_F4_PAIRS(PME_F4_CASE)
@@ -1160,7 +1161,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_cols(const SysDesc* __restrict
 // The fused z pass of systems whose z length has a four-step pair of class
 // CLS (their items only): tiles of the flattened (ky, kx) of a z plane.
 template <int CLS>
-__global__ __launch_bounds__(kBlock) void k_batch_zsolve(const SysDesc* __restrict__ sys,
+__global__ __launch_bounds__(kBlock, 4) void k_batch_zsolve(const SysDesc* __restrict__ sys,
                                                          const WorkItem* __restrict__ items, int nitems) {
   extern __shared__ __align__(16) float2 sm[];
   for (int it = blockIdx.x; it < nitems; it += gridDim.x) {

@@ -308,7 +308,7 @@ hipError_t launch_fft_plane_inv(const float2* cg, float* grid, int nx, int ny, i
 // Factor pair code of the register four-step z pass for a z length n
 // (-1: none, the Stockham stages are used)
 int four_step_code(int n);
-// register class of a four-step pair (0: N2 <= 10, 1: larger), -1: none
+// register class of a four-step pair (0: N2 <= 10 and 5 x 15, 1: the others), -1: none
 int four_step_class(int code);
 
 // LDS bytes of a plane kernel over np planes per workgroup (0 when they do
