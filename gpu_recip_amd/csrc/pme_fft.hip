@@ -474,12 +474,18 @@ __global__ __launch_bounds__(kPlaneBlock) void k_fft_plane_fwd(const float* __re
   fft_plane_fwd_body<SPLIT, kPlaneBlock>(grid, cg, nx, ny, axx, axy, blockIdx.x, 1, sm);
 }
 
+// Workgroup z < nz transforms plane z; with a pending energy/virial
+// reduction the launch has one more workgroup that performs it alone (on a
+// CU the planes leave free), so no plane's workgroup waits for it.
 template <bool SPLIT>
 __global__ __launch_bounds__(kPlaneBlock) void k_fft_plane_inv(const float2* __restrict__ cg, float* __restrict__ grid,
-                                                              int nx, int ny, AxisArgs axx, AxisArgs axy,
+                                                              int nx, int ny, int nz, AxisArgs axx, AxisArgs axy,
                                                               EvReduce red) {
   extern __shared__ __align__(16) float2 sm[];
-  if ((red.nsolve > 0 || red.nself > 0) && blockIdx.x == 0) ev_reduce_block<kPlaneBlock>(red);
+  if ((int)blockIdx.x >= nz) {
+    ev_reduce_block<kPlaneBlock>(red);
+    return;
+  }
   fft_plane_inv_body<SPLIT, kPlaneBlock>(cg, grid, nx, ny, axx, axy, blockIdx.x, 1, sm);
 }
 
@@ -498,10 +504,11 @@ hipError_t launch_fft_plane_inv(const float2* cg, float* grid, int nx, int ny, i
                                 const AxisArgs& ay, const EvReduce& red, hipStream_t s) {
   const size_t lds = fft_plane_lds_bytes(nx, ny, ax.rp.n, 1);
   if (!lds) return hipErrorInvalidValue;
+  const dim3 g(nz + ((red.nsolve > 0 || red.nself > 0) ? 1 : 0));
   if (ay.prec != PREC_FP32)
-    hipLaunchKernelGGL(k_fft_plane_inv<true>, dim3(nz), dim3(kPlaneBlock), lds, s, cg, grid, nx, ny, ax, ay, red);
+    hipLaunchKernelGGL(k_fft_plane_inv<true>, g, dim3(kPlaneBlock), lds, s, cg, grid, nx, ny, nz, ax, ay, red);
   else
-    hipLaunchKernelGGL(k_fft_plane_inv<false>, dim3(nz), dim3(kPlaneBlock), lds, s, cg, grid, nx, ny, ax, ay, red);
+    hipLaunchKernelGGL(k_fft_plane_inv<false>, g, dim3(kPlaneBlock), lds, s, cg, grid, nx, ny, nz, ax, ay, red);
   return hipGetLastError();
 }
 

@@ -299,8 +299,8 @@ __device__ __forceinline__ void ev_reduce_block(const EvReduce& r) {
 
 // xy plane kernels (x and y passes of one z plane in one workgroup):
 // forward reads the real plane and writes its half-spectrum plane, inverse
-// the reverse; the inverse's workgroup 0 also performs a pending
-// energy/virial reduction.
+// the reverse; a pending energy/virial reduction runs in one extra
+// workgroup of the inverse launch.
 hipError_t launch_fft_plane_fwd(const float* grid, float2* cg, int nx, int ny, int nz, const AxisArgs& ax,
                                 const AxisArgs& ay, hipStream_t s);
 hipError_t launch_fft_plane_inv(const float2* cg, float* grid, int nx, int ny, int nz, const AxisArgs& ax,
