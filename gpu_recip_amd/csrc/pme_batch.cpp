@@ -689,18 +689,27 @@ static void write_cache(const char* path, const std::map<std::string, std::vecto
   f << "}\n";
 }
 
+// Mean device time of one solve, each solve timed alone (events around it,
+// then a wait), as a caller that reads every solve's results before the next
+// one runs it: back-to-back solves would overlap one solve's tail with the
+// next one's start and favour plans that only pipeline well.
 static pme_status time_solves(pme_batch_t b, int warm, int reps, float* ms) {
   hipEvent_t e0, e1;
   BHIP(hipEventCreate(&e0));
   BHIP(hipEventCreate(&e1));
   for (int i = 0; i < warm; i++) BTRY(pme_batch_solve(b, 0, nullptr));
-  BHIP(hipEventRecord(e0, b->stream));
-  for (int i = 0; i < reps; i++) BTRY(pme_batch_solve(b, 0, nullptr));
-  BHIP(hipEventRecord(e1, b->stream));
-  BHIP(hipEventSynchronize(e1));
-  float t = 0.f;
-  BHIP(hipEventElapsedTime(&t, e0, e1));
-  *ms = t / reps;
+  BHIP(hipStreamSynchronize(b->stream));
+  float sum = 0.f;
+  for (int i = 0; i < reps; i++) {
+    BHIP(hipEventRecord(e0, b->stream));
+    BTRY(pme_batch_solve(b, 0, nullptr));
+    BHIP(hipEventRecord(e1, b->stream));
+    BHIP(hipEventSynchronize(e1));
+    float t = 0.f;
+    BHIP(hipEventElapsedTime(&t, e0, e1));
+    sum += t;
+  }
+  *ms = sum / reps;
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   return PME_OK;

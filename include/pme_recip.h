@@ -256,8 +256,9 @@ pme_status pme_batch_get_chains(pme_batch_t b, int* nchains);
 pme_status pme_batch_set_half_bricks(pme_batch_t b, int on);
 pme_status pme_batch_get_half_bricks(pme_batch_t b, int* on);
 /* Autotuner: time the current resident set over a grid of tuning choices
- * (half bricks, brick target, FFT tile, workgroups per CU, stage chains), keep the
- * fastest, and persist it in a JSON cache keyed by the set's signature and
+ * (half bricks, brick target, FFT tile, workgroups per CU, stage chains), each
+ * solve timed alone (as a caller that waits for every solve's results), keep
+ * the fastest, and persist it in a JSON cache keyed by the set's signature and
  * the GPU architecture (cache_path may be NULL).  Returns the best time per
  * solve (ms). */
 pme_status pme_batch_autotune(pme_batch_t b, const char* cache_path, float* best_ms);
