@@ -3,8 +3,8 @@ the reference's PME reciprocal solve (same index formula, B-spline recursion
 and stage order, FP32 grids, FP64 energy/virial sums), multithreaded with
 OpenMP.
 
-BASELINE ONLY: bench.py times it as the CPU baseline and tests/ check it
-against the FP64 oracle.  The product package never loads it.
+For the CPU comparison only: bench.py times it as the CPU baseline and
+tests/ check it against the FP64 oracle.  The product package never loads it.
 """
 from __future__ import annotations
 
