@@ -33,6 +33,14 @@ struct SystemPlan {
 constexpr int kBatchChains = 2;     // default stage chains of a batched solve
 constexpr int kMaxBatchChains = 4;  // most stage chains (streams) of a batched solve
 
+// Default plan of the serving loop: its batches are many systems at once, so
+// larger bricks (fewer halo records and per-brick phases) and wider FFT tiles
+// keep the GPU full; measured on 3,000 requests of the serving schedule at
+// saturation: 29,405 solves/s against 22,859 with the single-system defaults
+// (brick 16, tile 16).
+constexpr int kServeBrick = 24;
+constexpr int kServeFftTile = 32;
+
 struct Tuning {
   int brick_target = kDefaultBrick;
   int fft_tile = kDefaultFftTile;

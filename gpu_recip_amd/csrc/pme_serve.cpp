@@ -102,7 +102,7 @@ struct pme_server_s {
   std::unordered_map<unsigned long long, Result> results;  // guarded by mu
   unsigned long long next_ticket = 1;
   unsigned long long outstanding = 0;                  // submitted, not completed (mu)
-  Tuning tune;                                         // guarded by mu
+  Tuning tune{kServeBrick, kServeFftTile, 0, 0};       // guarded by mu
   std::vector<int> release_keys;                       // guarded by mu
   // stats (guarded by mu)
   pme_server_stats st{};

@@ -42,7 +42,7 @@ def main(argv=None):
     ap.add_argument("--graph", action="store_true", help="plain: pme_recip_solve replayed as a hipGraph")
     ap.add_argument("--tuning", default=None,
                     help="batched: brick,tile,wg_per_cu[,chains[,half bricks]] to use instead of autotuning "
-                         "(e.g. 24,32,4,2,1)")
+                         "(e.g. 24,32,4,2,1); served: brick,tile,wg_per_cu of the server")
     args = ap.parse_args(argv)
 
     import torch
@@ -60,6 +60,8 @@ def main(argv=None):
         reqs = serving.schedule(1e9, max_requests=args.solves)
         serving.materialise(reqs, dev)
         srv = PmeServer(64 << 30, nlanes=2)
+        if args.tuning:
+            srv.set_tuning(*[int(t) for t in args.tuning.split(",")][:3])
         warm = serving.schedule(0.2, seed=serving.SERVE_SEED + 99)
         serving.materialise(warm, dev)
         serving.run_burst(srv, warm)

@@ -345,6 +345,8 @@ pme_status pme_server_drain(pme_server_t s);
 pme_status pme_server_release(pme_server_t s, int key);
 pme_status pme_server_stats_get(pme_server_t s, pme_server_stats* out);
 pme_status pme_server_stats_reset(pme_server_t s);
+/* Plan of the server's batches (as pme_batch_set_tuning); default brick 24,
+ * FFT tile 32, one workgroup per work item. */
 pme_status pme_server_set_tuning(pme_server_t s, int brick_target, int fft_tile, int wg_per_cu);
 
 /* Host FP64 bsp_mod table (calc_prefac/dftmod, CudaPMERecip.cu:2481-2579). */
