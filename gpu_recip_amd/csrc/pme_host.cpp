@@ -365,7 +365,11 @@ static pme_status upload_prefac(pme_recip_t h) {
 static bool plan_bricks(pme_recip_t h) {
   const int n[3] = {h->nx, h->ny, h->nz};
   BrickPlan bp;
-  if (!pme::plan_bricks(n, h->order, kDefaultBrick, &bp)) return false;
+  int dev = 0, ncu = 0;
+  if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      ncu <= 0)
+    ncu = 256;
+  if (!pme::plan_bricks_auto(n, h->order, ncu, &bp)) return false;
   for (int a = 0; a < 3; a++) {
     h->nb[a] = bp.nb[a];
     h->wmax[a] = bp.wmax[a];

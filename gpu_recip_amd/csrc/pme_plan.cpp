@@ -140,6 +140,42 @@ bool plan_bricks(const int n[3], int order, int target, BrickPlan* bp, size_t bu
   return bp->nbins <= kMaxBins && bin_lds_bytes(bp->nbins) <= kLdsBudget;
 }
 
+// Modelled spread time per CU of a brick plan (us): rounds of bricks over the
+// CUs times a per-brick fixed cost plus, per grid point of a brick, the store
+// and the stencil work of its records (halo factor prod(1 + (p-1)/w) records
+// per atom, p^3 stencil points each).  Constants from timing variants of the
+// spread at 96,000 atoms, 108x100x96, order 6, brick 16: 7.7 us fixed per brick
+// (start, clear, segment bounds and scan), 8.3e-4 us per point stored and
+// 1.59e-3 us per point and record of an order-6 stencil; the gather's phases
+// scale alike.  It ranks the measured plans right: at 96k atoms brick 16
+// (99.9 us per solve) ahead of 18/20/24 (105.8/116.9/124.1), at 250,002 atoms,
+// 144^3, order 4 brick 24 (145.2) ahead of 18/20/16 (149.1/157.3/161.2).
+static double brick_cost(const int n[3], int order, int ncu, const BrickPlan& bp) {
+  const double g = (double)n[0] * n[1] * n[2];
+  const double pts = g / bp.nbins;
+  double halo = 1.0;
+  for (int a = 0; a < 3; a++) halo *= 1.0 + (double)(order - 1) * bp.nb[a] / n[a];
+  const double rounds = (double)((bp.nbins + ncu - 1) / ncu);
+  const double p3 = (double)order * order * order / 216.0;
+  return rounds * (7.7 + pts * (8.3e-4 + 1.59e-3 * halo * p3));
+}
+
+bool plan_bricks_auto(const int n[3], int order, int ncu, BrickPlan* bp) {
+  bool any = false;
+  double best = 0.0;
+  for (int target : {16, 18, 20, 24, 28, 32}) {
+    BrickPlan t;
+    if (!plan_bricks(n, order, target, &t)) continue;
+    const double c = brick_cost(n, order, std::max(ncu, 1), t);
+    if (!any || c < best) {
+      best = c;
+      *bp = t;
+      any = true;
+    }
+  }
+  return any;
+}
+
 FftTiles pick_fft_tiles(int nx, int ny, int nz, int nf_x, int tile_target, bool z_four_step) {
   FftTiles t;
   const int hx = nx / 2 + 1;

@@ -43,6 +43,9 @@ struct BrickPlan {
 // bricks of about `target` points per edge, each at least order-1 wide, LDS
 // footprints of spread and gather within `budget` bytes
 bool plan_bricks(const int n[3], int order, int target, BrickPlan* bp, size_t budget = kBrickLdsBudget);
+// The brick plan of a single-system solve on ncu CUs: of the plans for brick
+// targets 16 to 32, the one with the least modelled spread time per CU
+bool plan_bricks_auto(const int n[3], int order, int ncu, BrickPlan* bp);
 
 struct FftTiles {
   int NR, TXy, TXz;
