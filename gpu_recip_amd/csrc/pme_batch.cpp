@@ -731,28 +731,40 @@ extern "C" pme_status pme_batch_autotune(pme_batch_t b, const char* cache_path, 
   for (const Slot& s : b->slots)
     if (s.used && s.in.force_type == PME_FORCE_INT64)
       return set_error(PME_ERR_STATE, "pme_batch_autotune: resident INT64 (accumulating) outputs");
-  float best = 1e30f;
-  int bb = b->tune.brick_target, bt = b->tune.fft_tile, bw = b->tune.wg_per_cu, bc = b->nchains;
-  int bh = b->tune.half_bricks;
+  // a sweep of 10 solves per plan, then the 4 fastest again with 40 each: the
+  // minimum over 120 short timings alone favours plans that were lucky once
+  struct Cand {
+    float ms;
+    int half, brick, tile, wpc, ch;
+  };
+  std::vector<Cand> cand;
+  auto apply = [&](const Cand& c) {
+    return pme_batch_set_half_bricks(b, c.half) == PME_OK && pme_batch_set_tuning(b, c.brick, c.tile, c.wpc) == PME_OK &&
+           pme_batch_set_chains(b, c.ch) == PME_OK;
+  };
   for (int half : {0, 1})
     for (int brick : {12, 16, 20, 24, 32})
       for (int tile : {8, 16, 32})
         for (int wpc : {0, 4})
           for (int ch : {1, 2}) {
-            if (pme_batch_set_half_bricks(b, half) != PME_OK || pme_batch_set_tuning(b, brick, tile, wpc) != PME_OK ||
-                pme_batch_set_chains(b, ch) != PME_OK)
-              continue;
-            float ms = 0.f;
-            if (time_solves(b, 3, 10, &ms) != PME_OK) continue;
-            if (ms < best) {
-              best = ms;
-              bb = brick;
-              bt = tile;
-              bw = wpc;
-              bc = ch;
-              bh = half;
-            }
+            Cand c{0.f, half, brick, tile, wpc, ch};
+            if (!apply(c) || time_solves(b, 3, 10, &c.ms) != PME_OK) continue;
+            cand.push_back(c);
           }
+  if (cand.empty()) return set_error(PME_ERR_UNSUPPORTED, "pme_batch_autotune: no plan could be timed");
+  std::sort(cand.begin(), cand.end(), [](const Cand& x, const Cand& y) { return x.ms < y.ms; });
+  float best = 1e30f;
+  Cand win = cand[0];
+  for (size_t i = 0; i < cand.size() && i < 4; i++) {
+    Cand c = cand[i];
+    if (!apply(c) || time_solves(b, 2, 40, &c.ms) != PME_OK) continue;
+    if (c.ms < best) {
+      best = c.ms;
+      win = c;
+    }
+  }
+  if (best >= 1e30f) best = win.ms;
+  const int bb = win.brick, bt = win.tile, bw = win.wpc, bc = win.ch, bh = win.half;
   BTRY(pme_batch_set_half_bricks(b, bh));
   BTRY(pme_batch_set_tuning(b, bb, bt, bw));
   BTRY(pme_batch_set_chains(b, bc));
