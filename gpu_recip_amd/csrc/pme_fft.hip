@@ -744,6 +744,24 @@ int four_step_code(int n) {
 // (75, a common edge; held to 128 VGPRs, four waves per SIMD, with three
 // spilled), class 1 the rest (125 VGPRs).
 constexpr int f4_class_of(int n1, int n2) { return (n2 <= 10 || (n1 == 5 && n2 == 15)) ? 0 : 1; }
+int four_step_code_for_plan(const RadixPlan& rp) {
+  const int code = four_step_code(rp.n);
+  int a = 0, b = 0;
+  return (rp.f4a > 0 && code >= 0 && four_step_pair(code, &a, &b) && a == rp.f4a && b == rp.f4b) ? code : -1;
+}
+
+bool four_step_pair(int code, int* n1, int* n2) {
+#define PME_F4_PAIR(c, a, b) \
+  if (code == (c)) {         \
+    *n1 = (a);               \
+    *n2 = (b);               \
+    return true;             \
+  }
+  PME_F4_PAIRS(PME_F4_PAIR)
+#undef PME_F4_PAIR
+  return false;
+}
+
 int four_step_class(int code) {
 #define PME_F4_CLS(c, a, b) \
   if (code == (c)) return f4_class_of(a, b);
@@ -924,6 +942,81 @@ __device__ __forceinline__ void zsolve_four_step_dispatch(int code, float2* __re
   }
 }
 
+// One tile of a y pass (S = -1 forward, +1 inverse) as a register four-step
+// with n = N1 N2 in ONE LDS buffer: the first half loads its columns
+// j2 + N2 j1 from HBM into registers, the second half reads its N2 points
+// (k1 N2 + j2) back and stores X[k1 + N1 k2] straight to HBM (one LDS round
+// trip instead of a staging copy and two).  Tile (tx, ty): lines
+// kx0 = tx*TX ... of the hx lines at outer index ty.
+template <int N1, int N2, int S>
+__device__ __forceinline__ void cols_four_step(float2* __restrict__ cg, int hx, int TX, long long pt_stride,
+                                               long long outer_stride, const AxisArgs& ax, int tx, int ty,
+                                               float2* sm) {
+  const int tid = threadIdx.x;
+  float2* tw = sm;
+  float2* b = sm + ax.ntw;
+  for (int i = tid; i < ax.ntw; i += kBlock) tw[i] = ax.tw[i];
+  const int ts = ax.rp.tstride;
+  const int kx0 = tx * TX;
+  const int nl = min(TX, hx - kx0);
+  // the tile's first point (uniform) and 32-bit point offsets from it: a y
+  // line spans n * pt_stride < 2^18 points (edges <= 512, pt_stride = hx)
+  float2* __restrict__ tile = cg + ((size_t)kx0 + (size_t)ty * (size_t)outer_stride);
+  const unsigned pst = (unsigned)pt_stride;
+  const SmallDiv dtx = small_div(TX);
+  for (int u0 = 0; u0 < N2 * TX; u0 += kBlock) {
+    const int u = u0 + tid;
+    const int j2 = sdiv(u, dtx);
+    const int l = u - j2 * TX;
+    const bool live = u < N2 * TX;
+    float2 x[N1];
+    const unsigned o = (unsigned)j2 * pst + (unsigned)l;
+#pragma unroll
+    for (int j1 = 0; j1 < N1; j1++)
+      x[j1] = (live && l < nl) ? tile[o + (unsigned)(N2 * j1) * pst] : make_float2(0.f, 0.f);
+    if (u0 == 0) __syncthreads();  // twiddles in LDS (uniform branch)
+    if (live) {
+      DftN<N1, S>::run(x);
+      const int tj = mul24(j2, ts);
+      b[j2 * TX + l] = x[0];
+#pragma unroll
+      for (int k1 = 1; k1 < N1; k1++) {
+        float2 w = tw[k1 * tj];
+        if (S > 0) w.y = -w.y;
+        b[(k1 * N2 + j2) * TX + l] = cmul(x[k1], w);
+      }
+    }
+  }
+  __syncthreads();
+  for (int u = tid; u < N1 * TX; u += kBlock) {
+    const int k1 = sdiv(u, dtx);
+    const int l = u - k1 * TX;
+    float2 X[N2];
+#pragma unroll
+    for (int j2 = 0; j2 < N2; j2++) X[j2] = b[(k1 * N2 + j2) * TX + l];
+    DftN<N2, S>::run(X);
+    if (l < nl) {
+      const unsigned o = (unsigned)k1 * pst + (unsigned)l;
+#pragma unroll
+      for (int k2 = 0; k2 < N2; k2++) tile[o + (unsigned)(N1 * k2) * pst] = X[k2];
+    }
+  }
+}
+
+// the y pass of a tile through cols_four_step for the pair of code
+template <int S>
+__device__ __forceinline__ void cols_four_step_dispatch(int code, float2* __restrict__ cg, int hx, int TX,
+                                                        long long pt_stride, long long outer_stride,
+                                                        const AxisArgs& ax, int tx, int ty, float2* sm) {
+  switch (code) {
+#define PME_YC_CASE(c, a, b) \
+  case c: cols_four_step<a, b, S>(cg, hx, TX, pt_stride, outer_stride, ax, tx, ty, sm); break;
+    PME_F4_PAIRS(PME_YC_CASE)
+#undef PME_YC_CASE
+    default: break;
+  }
+}
+
 // The fused z pass of one grid as a register four-step with n = N1 N2 (one
 // kernel per pair, so each gets the registers of its own pair only): column
 // tiles of TX lines of the flattened (ky, kx) of a z plane.
@@ -947,6 +1040,12 @@ template <int MODE, bool SPLIT>
 __device__ __forceinline__ void fft_cols_body(float2* __restrict__ cg, int n, int hx, int TX, long long pt_stride,
                                               long long outer_stride, const AxisArgs& ax, const SolveArgs& sa,
                                               int tx, int ty, int blk, float2* sm, int hxd) {
+  // y passes with a four-step pair (ax.f4col, set for the y axis when the pair
+  // is the radix plan's): one LDS buffer, columns straight from and to HBM
+  if ((MODE == COL_FWD || MODE == COL_INV) && !SPLIT && ax.f4col >= 0) {
+    cols_four_step_dispatch<MODE == COL_FWD ? -1 : 1>(ax.f4col, cg, hx, TX, pt_stride, outer_stride, ax, tx, ty, sm);
+    return;
+  }
   __shared__ double sh_red[7 * (kBlock / 64)];
   const int tid = threadIdx.x;
   float2* tw = sm;

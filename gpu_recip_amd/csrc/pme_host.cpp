@@ -455,8 +455,10 @@ static AxisArgs axis_args(pme_recip_t h, int a) {
   ax.tw = h->tw[a];
   ax.ntw = a == 0 ? h->nx : (a == 1 ? h->ny : h->nz);
   ax.prec = split ? h->prec : PME_PREC_FP32;
-  // the fused z pass runs as a register four-step FFT where the length has a factor pair
+  // the fused z pass runs as a register four-step FFT where the length has a
+  // factor pair, the y passes where their radix plan's pair is in the table
   ax.f4 = (a == 2 && !split) ? four_step_code(h->nz) : -1;
+  ax.f4col = (a == 1 && !split) ? four_step_code_for_plan(ax.rp) : -1;
   return ax;
 }
 

@@ -74,6 +74,7 @@ struct AxisArgs {
   int ntw;                 // table length (the grid edge)
   int prec;                // PREC_*: a radix-16 first stage runs on the matrix cores
   int f4;                  // z pass: four-step factor pair (four_step_code), -1: Stockham stages
+  int f4col;               // y passes: four-step pair of the one-buffer column pass, -1: generic stages
 };
 
 // Influence function + energy/virial (scalar_sum), CudaPMERecip.cu:594-901
@@ -308,6 +309,11 @@ hipError_t launch_fft_plane_inv(const float2* cg, float* grid, int nx, int ny, i
 // Factor pair code of the register four-step z pass for a z length n
 // (-1: none, the Stockham stages are used)
 int four_step_code(int n);
+// the factor pair (n1, n2) of a four_step_code
+bool four_step_pair(int code, int* n1, int* n2);
+// four_step_code of a radix plan's four-step pair when the table holds that
+// pair, else -1 (the y passes then run the generic stages)
+int four_step_code_for_plan(const RadixPlan& rp);
 // register class of a four-step pair (0: N2 <= 10 and 5 x 15, 1: the others), -1: none
 int four_step_class(int code);
 

@@ -169,6 +169,7 @@ pme_status fill_desc(SysDesc& d, const pme_system_desc& in, const SystemPlan& p,
     d.ax[a].ntw = n[a];
     d.ax[a].prec = p.prec[a];
     d.ax[a].f4 = (a == 2 && p.prec[a] == PME_PREC_FP32) ? four_step_code(n[a]) : -1;
+    d.ax[a].f4col = (a == 1 && p.prec[a] == PME_PREC_FP32) ? four_step_code_for_plan(p.rp[1]) : -1;
     if (a == 0) d.sa.pf1 = pf;
     if (a == 1) d.sa.pf2 = pf;
     if (a == 2) d.sa.pf3 = pf;
