@@ -622,8 +622,7 @@ __device__ __forceinline__ float2* fft_lines_four_step(float2* buf0, float2* buf
 // of FP32-only kernels at what their Stockham stages need.  Lengths with a
 // four-step factor pair (rp.f4a > 0) run fft_lines_four_step instead of
 // the Stockham stages (except the matrix-core variant); FOUR_STEP = false
-// keeps the Stockham stages (the x pass over a few long rows per workgroup,
-// where the four-step units leave most threads idle).
+// keeps the Stockham stages.
 template <int S, bool SPLIT = false, bool FOUR_STEP = true>
 __device__ __forceinline__ float2* fft_lines(float2* buf0, float2* buf1, const float2* tw, const RadixPlan& rp,
                              int nlines, int ls, int tid, int nthr, int prec = PREC_FP32) {

@@ -80,7 +80,7 @@ __device__ __forceinline__ void fft_x_r2c_body(const float* __restrict__ grid, f
     }
   }
   __syncthreads();
-  float2* res = fft_lines<-1, false, false>(b0, b1, tw, ax.rp, NR, LS, tid, kBlock);
+  float2* res = fft_lines<-1>(b0, b1, tw, ax.rp, NR, LS, tid, kBlock);
   if (even) {
     // X[k] = E[k] + w^k O[k], E = (Z[k] + Z*[h-k])/2, O = (Z[k] - Z*[h-k])/(2i)
     const int h = nf;
@@ -165,7 +165,7 @@ __device__ __forceinline__ void fft_x_c2r_body(const float2* __restrict__ cg, fl
     }
   }
   __syncthreads();
-  float2* res = fft_lines<+1, false, false>(b0, b1, tw, ax.rp, NR, LS, tid, kBlock);
+  float2* res = fft_lines<+1>(b0, b1, tw, ax.rp, NR, LS, tid, kBlock);
   if (even) {
     float2* dst = reinterpret_cast<float2*>(grid + (size_t)row0 * nx);
     for (int e = tid; e < nr * nf; e += kBlock) {
